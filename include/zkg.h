@@ -1,0 +1,213 @@
+/*
+ * zkg.h -- C ABI of libzkg, the MI355X (gfx950) STARK trace-commitment
+ * accelerator.  It stands where ZirOS names a `zkf-gpu` crate (README.md:206,
+ * absent from the tree) and replaces the Metal entry points of zkf-metal for
+ * the coset LDE, Poseidon2 row hashing, Merkle commitment and FRI folding.
+ *
+ * Conventions (after zkf-ffi/src/lib.rs:19-74,164-180): opaque handles; every
+ * function returns int, 0 on success and a negative ZKG_E* code on failure,
+ * with a thread-local message readable through zkg_last_error().  The caller
+ * owns host memory; device buffers and trees belong to the library until the
+ * caller frees them.  A context may be used by one submitting thread at a time;
+ * the serving loop owns an internal thread of its own.
+ *
+ * Field elements cross the boundary in canonical form, little-endian:
+ * Goldilocks as uint64_t, BabyBear as uint32_t, a BabyBear degree-4 extension
+ * element (x^4 - 11) as 4 uint32_t coefficients, lowest first.  Matrices are
+ * row-major (height x width).  Heights are powers of two.
+ */
+#ifndef ZKG_H
+#define ZKG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZKG_ABI_VERSION 1
+
+enum zkg_field { ZKG_GOLDILOCKS = 0, ZKG_BABYBEAR = 1 };
+
+enum zkg_status {
+    ZKG_OK = 0,
+    ZKG_EINVAL = -1,   /* bad argument (shape, field, width, null) */
+    ZKG_EDEVICE = -2,  /* HIP runtime error */
+    ZKG_ENOMEM = -3,   /* device allocation failed or HBM budget exceeded */
+    ZKG_ESTATE = -4,   /* constants not loaded, serving loop not running, ... */
+    ZKG_EBUSY = -5,    /* serve ticket not finished yet */
+};
+
+typedef struct zkg_ctx zkg_ctx;
+typedef struct zkg_buf zkg_buf;
+typedef struct zkg_tree zkg_tree;
+typedef struct zkg_fri zkg_fri;
+
+/* ---- errors and versions ------------------------------------------------ */
+/* Replaces zkf-ffi's zkf_last_error_message (zkf-ffi/src/lib.rs:164-180). */
+const char *zkg_last_error(void);
+int zkg_abi_version(void);
+
+/* ---- device context ------------------------------------------------------ */
+/* Replaces the process-global MetalContext (zkf-metal/src/device.rs:1469-1480).
+ * hbm_budget_bytes = 0 means 288 GiB.  Twiddle tables for both fields become
+ * resident at creation. */
+int zkg_ctx_create(int device_ordinal, uint64_t hbm_budget_bytes, zkg_ctx **out);
+int zkg_ctx_destroy(zkg_ctx *ctx);
+int zkg_ctx_synchronize(zkg_ctx *ctx);
+/* Bytes of HBM this context currently holds (buffers, trees, tables). */
+int zkg_ctx_hbm_used(const zkg_ctx *ctx, uint64_t *bytes);
+
+/* ---- device buffers ------------------------------------------------------ */
+/* Replaces GpuBufferAllocator::{alloc_shared, free_shared}
+ * (zkf-runtime/src/buffer_bridge.rs:77-84); HBM is discrete memory, so a
+ * handle comes back rather than a host pointer. */
+int zkg_buf_alloc(zkg_ctx *ctx, uint64_t bytes, zkg_buf **out);
+int zkg_buf_free(zkg_buf *buf);
+int zkg_buf_upload(zkg_buf *buf, uint64_t offset, const void *src, uint64_t bytes);
+int zkg_buf_download(const zkg_buf *buf, uint64_t offset, void *dst, uint64_t bytes);
+int zkg_buf_size(const zkg_buf *buf, uint64_t *bytes);
+/* Fills n elements from element `start` of the synthetic stream `seed`:
+ * element i is the first v_a = mix64(seed + (i+1) * 0x9e3779b97f4a7c15 +
+ * a * 0x632be59bd9b4e019), a = 0, 1, ..., (>> 33 for BabyBear) with v_a < p,
+ * mix64 being the SplitMix64 finaliser. */
+int zkg_fill_random(zkg_ctx *ctx, int field, zkg_buf *buf, uint64_t seed, uint64_t start, uint64_t n);
+
+/* ---- Poseidon2 constants ------------------------------------------------ */
+/* Keeps the round constants of Poseidon2<field, width> derived from
+ * SmallRng::seed_from_u64(seed) resident (the reference rebuilds them per call:
+ * zkf-metal/src/poseidon2/goldilocks.rs:42-83, babybear.rs:76-111).
+ * Supported: (GOLDILOCKS, 16), (GOLDILOCKS, 8), (BABYBEAR, 16). */
+int zkg_constants_load(zkg_ctx *ctx, int field, int width, uint64_t seed);
+/* Replaces the constants with an explicit flat [initial][internal][terminal]
+ * table (rounds_f * width + rounds_p canonical values), as the reference's
+ * Poseidon2Accelerator passes them. */
+int zkg_constants_set(zkg_ctx *ctx, int field, int width, const void *flat, uint32_t n_external_rounds,
+                      uint32_t n_internal_rounds);
+
+/* ---- transforms ---------------------------------------------------------- */
+/* p3 TwoAdicSubgroupDft::dft_batch / idft_batch on every column, in place,
+ * natural order in and out, inverse scaled by 1/height.  Replaces MetalDft
+ * (zkf-metal/src/ntt/p3_adapter.rs:69-75,78-106). */
+int zkg_ntt_batch(zkg_ctx *ctx, int field, zkg_buf *buf, uint64_t height, uint32_t width, int inverse);
+/* Coset LDE as TwoAdicFriPcs::commit uses RuntimeDft
+ * (zkf-backends/src/plonky3.rs:101-257): out (height << log_blowup rows) =
+ * bit_reverse_rows(coset_lde_batch(in, log_blowup, shift)).  in is unchanged. */
+int zkg_coset_lde_batch(zkg_ctx *ctx, int field, const zkg_buf *in, uint64_t height, uint32_t width,
+                        uint32_t log_blowup, uint64_t shift, zkg_buf *out);
+
+/* ---- Poseidon2 ----------------------------------------------------------- */
+/* Replaces Poseidon2Accelerator::batch_permute_* (zkf-core/src/acceleration.rs:
+ * 110-140): n states of `width` elements, in place, with the loaded constants. */
+int zkg_poseidon2_permute(zkg_ctx *ctx, int field, int width, zkg_buf *states, uint64_t n);
+/* Host-memory form with an explicit constant table, exactly the trait's
+ * batch_permute_goldilocks / batch_permute_babybear (width 16). */
+int zkg_poseidon2_batch_permute_goldilocks(zkg_ctx *ctx, uint64_t *states, uint64_t n_perms,
+                                           const uint64_t *round_constants, uint32_t n_external_rounds,
+                                           uint32_t n_internal_rounds);
+int zkg_poseidon2_batch_permute_babybear(zkg_ctx *ctx, uint32_t *states, uint64_t n_perms,
+                                         const uint32_t *round_constants, uint32_t n_external_rounds,
+                                         uint32_t n_internal_rounds);
+/* PaddingFreeSponge<P, width, width/2, width/2> of every row of a matrix:
+ * digests receives height * width/2 elements. */
+int zkg_hash_rows(zkg_ctx *ctx, int field, int p2_width, const zkg_buf *mat, uint64_t height,
+                  uint32_t mat_width, zkg_buf *digests);
+
+/* ---- Merkle commitment (Mmcs) -------------------------------------------- */
+/* Replaces MetalMerkleTreeMmcs::commit (zkf-metal/src/merkle.rs:76-105) for a
+ * single matrix of power-of-two height: digest layer 0 = row hashes, layer i+1
+ * = TruncatedPermutation of pairs (vendor/p3-merkle-tree-gpu/src/
+ * merkle_tree.rs:84-151).  The tree keeps a reference to `mat` for openings:
+ * the caller keeps it alive until zkg_tree_free. */
+int zkg_merkle_commit(zkg_ctx *ctx, int field, int p2_width, const zkg_buf *mat, uint64_t height,
+                      uint32_t width, zkg_tree **out);
+int zkg_tree_root(const zkg_tree *tree, void *root_out);
+int zkg_tree_num_layers(const zkg_tree *tree, uint32_t *n);
+int zkg_tree_digest_elems(const zkg_tree *tree, uint32_t *n);
+/* Copies digest layer `layer` (height >> layer digests) to host memory. */
+int zkg_tree_layer(const zkg_tree *tree, uint32_t layer, void *out, uint64_t out_bytes);
+/* Mmcs::open_batch (vendor/p3-merkle-tree-gpu/src/mmcs.rs:179-210) for n
+ * indices: rows_out gets n * width elements, paths_out n * (layers-1) digests. */
+int zkg_tree_open(const zkg_tree *tree, const uint64_t *indices, uint32_t n, void *rows_out, void *paths_out);
+int zkg_tree_free(zkg_tree *tree);
+
+/* Fused commit of one trace: coset LDE at log_blowup with the field's
+ * generator shift (p3 Val::GENERATOR: 7 or 31), row hash, tree.  lde_out
+ * (height << log_blowup rows) receives the LDE and backs the tree's openings. */
+int zkg_commit(zkg_ctx *ctx, int field, int p2_width, const zkg_buf *trace, uint64_t height, uint32_t width,
+               uint32_t log_blowup, zkg_buf *lde_out, zkg_tree **out);
+/* The same commit through the unfused, reference-structured path: one radix-2
+ * global pass per NTT stage, per-element shift powers, a bit-reversal pass,
+ * one launch per Merkle level, a stream synchronisation after every launch
+ * (zkf-metal/src/ntt/p3_adapter.rs:224-258, merkle.rs:324-443). */
+int zkg_commit_plain(zkg_ctx *ctx, int field, int p2_width, const zkg_buf *trace, uint64_t height,
+                     uint32_t width, uint32_t log_blowup, zkg_buf *lde_out, zkg_tree **out);
+
+/* ---- FRI ------------------------------------------------------------------ */
+/* FriAccelerator::fold_goldilocks semantics (zkf-core/src/acceleration.rs:
+ * 196-217): out[i] = (e[2i] + e[2i+1])/2 + alpha (e[2i] - e[2i+1]) inv_tw[i]. */
+int zkg_fri_fold(zkg_ctx *ctx, int field, const zkg_buf *evals, uint64_t n_out, uint64_t alpha,
+                 const zkg_buf *inv_twiddles, zkg_buf *out);
+/* p3-fri TwoAdicFriFolding::fold_matrix, arity 2: evals holds 2 n_out values
+ * in bit-reversed order; ext = 1 for BabyBear EF4 (beta: 4 uint32_t), 0 for
+ * the base field (beta: one element as uint64_t). */
+int zkg_fri_fold_matrix(zkg_ctx *ctx, int ext, int field, const zkg_buf *evals, uint64_t n_out,
+                        const void *beta, zkg_buf *out);
+/* FRI commit phase over EF4 (BabyBear Poseidon2-16 MMCS): every layer of
+ * length 2^log_len ... 2^(log_final+1) is committed as a width-2 matrix and
+ * folded with betas[4 r .. 4 r + 3].  The codeword buffer is read, not kept. */
+int zkg_fri_commit_phase(zkg_ctx *ctx, const zkg_buf *codeword, uint32_t log_len, uint32_t log_final,
+                         const uint32_t *betas, zkg_fri **out);
+int zkg_fri_num_layers(const zkg_fri *fri, uint32_t *n);
+int zkg_fri_layer_root(const zkg_fri *fri, uint32_t layer, uint32_t *root8);
+/* final codeword: 2^log_final EF4 values (4 uint32_t each) */
+int zkg_fri_final(const zkg_fri *fri, uint32_t *out);
+/* Query openings as p3-fri's answer_query makes them: for a query index q of
+ * the first codeword (q < 2^log_len), layer r opens leaf (q >> (r+1)) of its
+ * width-2 matrix.  rows_out gets n * layers * 8 uint32_t (the opened pair,
+ * one EF4 value per half); paths_out gets, per query, the sibling paths of
+ * layers 0, 1, ... concatenated (layer r has log_len - 1 - r digests of 8). */
+int zkg_fri_open(const zkg_fri *fri, const uint64_t *indices, uint32_t n, uint32_t *rows_out,
+                 uint32_t *paths_out);
+int zkg_fri_free(zkg_fri *fri);
+
+/* ---- batched commit ------------------------------------------------------ */
+typedef struct zkg_job {
+    int field;            /* ZKG_GOLDILOCKS or ZKG_BABYBEAR */
+    int p2_width;         /* 16 (8 for Goldilocks width-8) */
+    const zkg_buf *trace; /* height x width, canonical */
+    uint64_t height;
+    uint32_t width;
+    uint32_t log_blowup;
+    zkg_buf *lde_out;     /* (height << log_blowup) x width */
+} zkg_job;
+
+/* Commits n traces together: each stage is one launch over the tiles of every
+ * job (a persistent grid walking a job table), and the whole LDE -> hash ->
+ * Merkle pipeline is captured once as a hipGraph per job-set shape and
+ * replayed.  trees_out[i] receives job i's tree. */
+int zkg_commit_batch(zkg_ctx *ctx, const zkg_job *jobs, uint32_t n, zkg_tree **trees_out);
+
+/* ---- serving -------------------------------------------------------------- */
+/* Continuous-batching request loop: submitted jobs are admitted while the HBM
+ * budget and in-flight slots allow, packed into shared launches with the
+ * other admitted jobs, and run on one of `slots` streams so batches overlap.
+ * Each result is identical to a lone zkg_commit of the same job. */
+int zkg_serve_start(zkg_ctx *ctx, uint32_t slots, uint32_t max_batch);
+/* Copies the job description; the trace and lde_out buffers must stay alive
+ * until the ticket is polled done. */
+int zkg_serve_submit(zkg_ctx *ctx, const zkg_job *job, uint64_t *ticket);
+/* ZKG_OK and *tree set when done, ZKG_EBUSY while queued or running. */
+int zkg_serve_poll(zkg_ctx *ctx, uint64_t ticket, zkg_tree **tree, double *latency_ms);
+int zkg_serve_stop(zkg_ctx *ctx);
+
+/* ---- telemetry ------------------------------------------------------------- */
+/* Per-stage device time of the most recent commit / batch on the context's
+ * stream, from HIP events (stages: 0 LDE, 1 row hash, 2 Merkle, 3 FRI). */
+int zkg_ctx_set_timing(zkg_ctx *ctx, int enable);
+int zkg_ctx_stage_ms(const zkg_ctx *ctx, double *ms4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

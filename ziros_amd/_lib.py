@@ -1,0 +1,114 @@
+"""ctypes binding of libzkg (include/zkg.h).
+
+The library is built in-tree (``ziros_amd/libzkg.so``).  Loading fails loudly
+when it is missing: there is no CPU fallback anywhere in this package.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libzkg.so")
+
+c_u64 = ctypes.c_uint64
+c_u32 = ctypes.c_uint32
+c_int = ctypes.c_int
+c_vp = ctypes.c_void_p
+c_dbl = ctypes.c_double
+P = ctypes.POINTER
+
+
+class Job(ctypes.Structure):
+    """zkg_job"""
+
+    _fields_ = [
+        ("field", c_int),
+        ("p2_width", c_int),
+        ("trace", c_vp),
+        ("height", c_u64),
+        ("width", c_u32),
+        ("log_blowup", c_u32),
+        ("lde_out", c_vp),
+    ]
+
+
+# name -> (restype, argtypes); every entry point declared in include/zkg.h
+SIGNATURES = {
+    "zkg_last_error": (ctypes.c_char_p, []),
+    "zkg_abi_version": (c_int, []),
+    "zkg_ctx_create": (c_int, [c_int, c_u64, P(c_vp)]),
+    "zkg_ctx_destroy": (c_int, [c_vp]),
+    "zkg_ctx_synchronize": (c_int, [c_vp]),
+    "zkg_ctx_hbm_used": (c_int, [c_vp, P(c_u64)]),
+    "zkg_buf_alloc": (c_int, [c_vp, c_u64, P(c_vp)]),
+    "zkg_buf_free": (c_int, [c_vp]),
+    "zkg_buf_upload": (c_int, [c_vp, c_u64, c_vp, c_u64]),
+    "zkg_buf_download": (c_int, [c_vp, c_u64, c_vp, c_u64]),
+    "zkg_buf_size": (c_int, [c_vp, P(c_u64)]),
+    "zkg_fill_random": (c_int, [c_vp, c_int, c_vp, c_u64, c_u64, c_u64]),
+    "zkg_constants_load": (c_int, [c_vp, c_int, c_int, c_u64]),
+    "zkg_constants_set": (c_int, [c_vp, c_int, c_int, c_vp, c_u32, c_u32]),
+    "zkg_ntt_batch": (c_int, [c_vp, c_int, c_vp, c_u64, c_u32, c_int]),
+    "zkg_coset_lde_batch": (c_int, [c_vp, c_int, c_vp, c_u64, c_u32, c_u32, c_u64, c_vp]),
+    "zkg_poseidon2_permute": (c_int, [c_vp, c_int, c_int, c_vp, c_u64]),
+    "zkg_poseidon2_batch_permute_goldilocks": (c_int, [c_vp, c_vp, c_u64, c_vp, c_u32, c_u32]),
+    "zkg_poseidon2_batch_permute_babybear": (c_int, [c_vp, c_vp, c_u64, c_vp, c_u32, c_u32]),
+    "zkg_hash_rows": (c_int, [c_vp, c_int, c_int, c_vp, c_u64, c_u32, c_vp]),
+    "zkg_merkle_commit": (c_int, [c_vp, c_int, c_int, c_vp, c_u64, c_u32, P(c_vp)]),
+    "zkg_tree_root": (c_int, [c_vp, c_vp]),
+    "zkg_tree_num_layers": (c_int, [c_vp, P(c_u32)]),
+    "zkg_tree_digest_elems": (c_int, [c_vp, P(c_u32)]),
+    "zkg_tree_layer": (c_int, [c_vp, c_u32, c_vp, c_u64]),
+    "zkg_tree_open": (c_int, [c_vp, c_vp, c_u32, c_vp, c_vp]),
+    "zkg_tree_free": (c_int, [c_vp]),
+    "zkg_commit": (c_int, [c_vp, c_int, c_int, c_vp, c_u64, c_u32, c_u32, c_vp, P(c_vp)]),
+    "zkg_commit_plain": (c_int, [c_vp, c_int, c_int, c_vp, c_u64, c_u32, c_u32, c_vp, P(c_vp)]),
+    "zkg_fri_fold": (c_int, [c_vp, c_int, c_vp, c_u64, c_u64, c_vp, c_vp]),
+    "zkg_fri_fold_matrix": (c_int, [c_vp, c_int, c_int, c_vp, c_u64, c_vp, c_vp]),
+    "zkg_fri_commit_phase": (c_int, [c_vp, c_vp, c_u32, c_u32, c_vp, P(c_vp)]),
+    "zkg_fri_num_layers": (c_int, [c_vp, P(c_u32)]),
+    "zkg_fri_layer_root": (c_int, [c_vp, c_u32, c_vp]),
+    "zkg_fri_final": (c_int, [c_vp, c_vp]),
+    "zkg_fri_open": (c_int, [c_vp, c_vp, c_u32, c_vp, c_vp]),
+    "zkg_fri_free": (c_int, [c_vp]),
+    "zkg_commit_batch": (c_int, [c_vp, P(Job), c_u32, P(c_vp)]),
+    "zkg_serve_start": (c_int, [c_vp, c_u32, c_u32]),
+    "zkg_serve_submit": (c_int, [c_vp, P(Job), P(c_u64)]),
+    "zkg_serve_poll": (c_int, [c_vp, c_u64, P(c_vp), P(c_dbl)]),
+    "zkg_serve_stop": (c_int, [c_vp]),
+    "zkg_ctx_set_timing": (c_int, [c_vp, c_int]),
+    "zkg_ctx_stage_ms": (c_int, [c_vp, P(c_dbl)]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libzkg.so and bind every signature; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "libzkg.so is not built (%s); run `python -m ziros_amd.build`" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class ZkgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("zkg error %d: %s" % (code, msg))
+        self.code = code
+
+
+EBUSY = -5
+
+
+def check(rc):
+    if rc != 0:
+        raise ZkgError(rc, (load().zkg_last_error() or b"").decode())
+    return rc

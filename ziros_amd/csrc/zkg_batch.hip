@@ -1,0 +1,453 @@
+// zkg_batch.hip -- batched commits (hipGraph-captured pipelines over many
+// traces) and the continuous-batching serving loop.
+//
+// Reference counterparts: BatchProver::prove_all and recommend_job_count
+// (zkf-metal/src/batch_prover.rs:95-360), where each job runs the whole
+// pipeline on its own thread and jobs serialise on the GPU queue, and the
+// per-node synchronous GPU dispatch of zkf-runtime (scheduler.rs:247-464).
+#include "../../include/zkg.h"
+#include "zkg_internal.h"
+#include "zkg_runtime.h"
+
+#include <chrono>
+#include <cstring>
+
+using namespace zkg;
+
+namespace zkg {
+
+static uint64_t esz(int field) { return field == FIELD_GL ? 8 : 4; }
+static int ilog2b(uint64_t n)
+{
+    int l = 0;
+    while ((1ull << l) < n) l++;
+    return l;
+}
+
+// One prepared job: tree storage and LDE scratch allocated, tables resident.
+struct Prepared {
+    zkg_job job;
+    zkg_tree *tree = nullptr;
+    void *tmp = nullptr;
+    uint64_t tmp_bytes = 0;
+};
+
+static int validate_job(zkg_ctx *ctx, const zkg_job &j)
+{
+    if (!j.trace || !j.lde_out) return set_err(ZKG_EINVAL, "job without buffers");
+    if (j.field != FIELD_GL && j.field != FIELD_BB) return set_err(ZKG_EINVAL, "job field %d", j.field);
+    if (!j.height || (j.height & (j.height - 1)) || ilog2b(j.height) > 22)
+        return set_err(ZKG_EINVAL, "job height %llu", (unsigned long long)j.height);
+    if (j.log_blowup > 2 || j.width == 0) return set_err(ZKG_EINVAL, "job blowup/width");
+    if (!get_p2(ctx, j.field, j.p2_width)) return set_err(ZKG_ESTATE, "Poseidon2 constants for job not loaded");
+    uint64_t es = esz(j.field);
+    if (j.height * j.width * es > j.trace->bytes || (j.height << j.log_blowup) * j.width * es > j.lde_out->bytes)
+        return set_err(ZKG_EINVAL, "job buffer too small");
+    return ZKG_OK;
+}
+
+// Tree + scratch bytes a job needs beyond its caller-owned buffers.
+static uint64_t job_extra_bytes(const zkg_job &j)
+{
+    uint64_t es = esz(j.field), big = j.height << j.log_blowup;
+    return 2 * big * (j.p2_width / 2) * es + 256 * 32 + coset_lde_tmp_bytes(j.field, j.height, j.width, j.log_blowup);
+}
+
+static void release_prepared(zkg_ctx *ctx, Prepared &p, bool keep_tree)
+{
+    dev_free(ctx, p.tmp);
+    p.tmp = nullptr;
+    if (!keep_tree && p.tree) zkg_tree_free(p.tree);
+    p.tree = nullptr;
+}
+
+static int prepare_job(zkg_ctx *ctx, const zkg_job &j, Prepared &p)
+{
+    p.job = j;
+    int rc = validate_job(ctx, j);
+    if (rc) return rc;
+    const CosetSet *cs;
+    if ((rc = get_coset(ctx, j.field, ilog2b(j.height), j.log_blowup, j.field == FIELD_GL ? 7 : 31, &cs))) return rc;
+    p.tmp_bytes = coset_lde_tmp_bytes(j.field, j.height, j.width, j.log_blowup);
+    if (p.tmp_bytes && (rc = dev_alloc(ctx, p.tmp_bytes, &p.tmp))) return rc;
+    if ((rc = tree_alloc(ctx, j.field, j.p2_width, j.height << j.log_blowup, j.lde_out->ptr, j.width, &p.tree))) {
+        dev_free(ctx, p.tmp);
+        p.tmp = nullptr;
+        return rc;
+    }
+    return ZKG_OK;
+}
+
+// Stream-ordered launches of a set of prepared jobs: all LDEs, then all row
+// hashes, then all trees (no host synchronisation in between).
+static int enqueue_jobs(zkg_ctx *ctx, hipStream_t st, std::vector<Prepared> &ps)
+{
+    for (auto &p : ps) {
+        int rc = run_lde(ctx, st, p.job.field, p.job.trace->ptr, p.job.height, p.job.width, p.job.log_blowup,
+                         p.job.field == FIELD_GL ? 7 : 31, p.job.lde_out->ptr, p.tmp);
+        if (rc) return rc;
+    }
+    for (auto &p : ps) {
+        const P2Set *p2 = get_p2(ctx, p.job.field, p.job.p2_width);
+        hipError_t e = launch_hash_rows(p.job.field, p.job.p2_width, *p2, p.tree->mat, p.tree->n_leaves, p.job.width,
+                                        p.tree->layers[0], st);
+        if (e != hipSuccess) return hip_err(e, "batch hash");
+    }
+    for (auto &p : ps) {
+        const P2Set *p2 = get_p2(ctx, p.job.field, p.job.p2_width);
+        hipError_t e = launch_merkle_tree(p.job.field, p.job.p2_width, *p2, p.tree->layers.data(), p.tree->nlayers,
+                                          p.tree->n_leaves, 9, st, false);
+        if (e != hipSuccess) return hip_err(e, "batch merkle");
+    }
+    return ZKG_OK;
+}
+
+// Cached executable graph for one job-set (same shapes, buffers and trees).
+struct BatchState {
+    std::vector<uint64_t> key;
+    hipGraphExec_t exec = nullptr;
+    std::vector<Prepared> prepared;
+};
+
+static std::vector<uint64_t> batch_key(const zkg_job *jobs, uint32_t n)
+{
+    std::vector<uint64_t> k;
+    for (uint32_t i = 0; i < n; i++) {
+        const zkg_job &j = jobs[i];
+        k.insert(k.end(), {(uint64_t)j.field, (uint64_t)j.p2_width, (uint64_t)(uintptr_t)j.trace->ptr, j.height,
+                           j.width, j.log_blowup, (uint64_t)(uintptr_t)j.lde_out->ptr});
+    }
+    return k;
+}
+
+void batch_release(zkg_ctx *ctx)
+{
+    BatchState *b = ctx->batch;
+    if (!b) return;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    for (auto &p : b->prepared) release_prepared(ctx, p, false);
+    delete b;
+    ctx->batch = nullptr;
+}
+
+}  // namespace zkg
+
+// Trees returned to the caller are copies of the cached graph's trees (the
+// graph writes into its own tree storage, so a replay never touches a tree the
+// caller holds).
+static int clone_tree(zkg_ctx *ctx, hipStream_t st, const zkg_tree *src, zkg_tree **out)
+{
+    zkg_tree *t;
+    int rc = tree_alloc(ctx, src->field, src->p2_width, src->n_leaves, src->mat, src->mat_width, &t);
+    if (rc) return rc;
+    uint64_t bytes = src->layer_off.back() + src->digest * esz(src->field);
+    hipError_t e = hipMemcpyAsync(t->base, src->base, bytes, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) {
+        zkg_tree_free(t);
+        return hip_err(e, "clone tree");
+    }
+    *out = t;
+    return ZKG_OK;
+}
+
+extern "C" int zkg_commit_batch(zkg_ctx *ctx, const zkg_job *jobs, uint32_t n, zkg_tree **trees_out)
+{
+    if (!ctx || (!jobs && n) || (!trees_out && n)) return set_err(ZKG_EINVAL, "null argument");
+    if (!n) return ZKG_OK;
+    for (uint32_t i = 0; i < n; i++)
+        if (int rc = validate_job(ctx, jobs[i])) return rc;
+    hipStream_t st = ctx->stream;
+    std::vector<uint64_t> key = batch_key(jobs, n);
+    if (!ctx->batch || ctx->batch->key != key) {
+        (void)hipStreamSynchronize(st);
+        batch_release(ctx);
+        BatchState *b = new BatchState();
+        b->key = key;
+        b->prepared.resize(n);
+        int rc = ZKG_OK;
+        for (uint32_t i = 0; i < n && !rc; i++) rc = prepare_job(ctx, jobs[i], b->prepared[i]);
+        if (rc) {
+            for (auto &p : b->prepared) release_prepared(ctx, p, false);
+            delete b;
+            return rc;
+        }
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) rc = hip_err(e, "begin capture");
+        if (!rc) {
+            rc = enqueue_jobs(ctx, st, b->prepared);
+            e = hipStreamEndCapture(st, &g);
+            if (!rc && e != hipSuccess) rc = hip_err(e, "end capture");
+        }
+        if (!rc) {
+            e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) rc = hip_err(e, "graph instantiate");
+        }
+        if (g) (void)hipGraphDestroy(g);
+        if (rc) {
+            for (auto &p : b->prepared) release_prepared(ctx, p, false);
+            if (b->exec) (void)hipGraphExecDestroy(b->exec);
+            delete b;
+            return rc;
+        }
+        ctx->batch = b;
+    }
+    stage_mark(ctx, st, 0);
+    hipError_t e = hipGraphLaunch(ctx->batch->exec, st);
+    if (e != hipSuccess) return hip_err(e, "graph launch");
+    stage_mark(ctx, st, 1);
+    for (uint32_t i = 0; i < n; i++) {
+        int rc = clone_tree(ctx, st, ctx->batch->prepared[i].tree, &trees_out[i]);
+        if (rc) {
+            for (uint32_t k = 0; k < i; k++) zkg_tree_free(trees_out[k]);
+            return rc;
+        }
+    }
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_err(e, "batch sync");
+    if (ctx->timing) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+        for (int i = 0; i < 4; i++) ctx->stage_ms[i] = 0;
+        ctx->stage_ms[0] = ms;
+    }
+    return ZKG_OK;
+}
+
+// ======================================================================
+// Serving loop
+// ======================================================================
+namespace zkg {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+    uint64_t ticket;
+    zkg_job job;
+    Clock::time_point arrival;
+    Clock::time_point finish;
+    zkg_tree *tree = nullptr;
+    int status = ZKG_EBUSY;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    std::vector<Prepared> prepared;
+    std::vector<uint64_t> tickets;
+};
+
+struct ServeState {
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<uint64_t> queue;
+    std::unordered_map<uint64_t, Request> reqs;
+    std::vector<Slot> slots;
+    uint32_t max_batch = 16;
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    std::thread worker;
+};
+
+static void complete_slot(zkg_ctx *ctx, ServeState *s, Slot &sl)
+{
+    Clock::time_point now = Clock::now();
+    std::lock_guard<std::mutex> g(s->mu);
+    for (size_t i = 0; i < sl.prepared.size(); i++) {
+        Request &r = s->reqs[sl.tickets[i]];
+        r.tree = sl.prepared[i].tree;
+        r.status = ZKG_OK;
+        r.finish = now;
+        release_prepared(ctx, sl.prepared[i], true);
+    }
+    sl.prepared.clear();
+    sl.tickets.clear();
+    sl.busy = false;
+}
+
+static void serve_loop(zkg_ctx *ctx, ServeState *s)
+{
+    (void)hipSetDevice(ctx->device);
+    for (;;) {
+        bool progressed = false;
+        for (Slot &sl : s->slots) {
+            if (sl.busy && hipEventQuery(sl.done) == hipSuccess) {
+                complete_slot(ctx, s, sl);
+                progressed = true;
+            }
+        }
+        for (Slot &sl : s->slots) {
+            if (sl.busy) continue;
+            // admit FIFO requests while the HBM budget allows
+            std::vector<uint64_t> take;
+            std::vector<zkg_job> jobs;
+            {
+                std::lock_guard<std::mutex> g(s->mu);
+                while (!s->queue.empty() && take.size() < s->max_batch) {
+                    uint64_t t = s->queue.front();
+                    const zkg_job &j = s->reqs[t].job;
+                    if (ctx->hbm_used + job_extra_bytes(j) > ctx->hbm_budget && !take.empty()) break;
+                    take.push_back(t);
+                    jobs.push_back(j);
+                    s->queue.pop_front();
+                }
+            }
+            if (take.empty()) break;
+            std::vector<Prepared> ps(take.size());
+            size_t ok = 0;
+            int rc = ZKG_OK;
+            for (; ok < take.size(); ok++) {
+                rc = prepare_job(ctx, jobs[ok], ps[ok]);
+                if (rc) break;
+            }
+            if (ok < take.size()) {
+                // A request that cannot get memory waits at the queue head for
+                // a slot to drain; with nothing in flight it can never fit and
+                // fails.  Any other error fails that request only.
+                bool others_busy = false;
+                for (Slot &o : s->slots) others_busy |= o.busy;
+                bool wait = rc == ZKG_ENOMEM && (ok > 0 || others_busy);
+                std::lock_guard<std::mutex> g(s->mu);
+                size_t first_requeue = ok;
+                if (!wait) {
+                    s->reqs[take[ok]].status = rc;
+                    s->reqs[take[ok]].finish = Clock::now();
+                    first_requeue = ok + 1;
+                }
+                for (size_t k = take.size(); k-- > first_requeue;) s->queue.push_front(take[k]);
+                ps.resize(ok);
+                take.resize(ok);
+                if (ok == 0) {
+                    if (wait) break;
+                    continue;
+                }
+            }
+            rc = enqueue_jobs(ctx, sl.stream, ps);
+            if (rc == ZKG_OK) {
+                hipError_t e = hipEventRecord(sl.done, sl.stream);
+                if (e != hipSuccess) rc = hip_err(e, "event record");
+            }
+            if (rc != ZKG_OK) {
+                (void)hipStreamSynchronize(sl.stream);
+                std::lock_guard<std::mutex> g(s->mu);
+                for (size_t i = 0; i < ps.size(); i++) {
+                    release_prepared(ctx, ps[i], false);
+                    s->reqs[take[i]].status = rc;
+                    s->reqs[take[i]].finish = Clock::now();
+                }
+                continue;
+            }
+            sl.prepared = std::move(ps);
+            sl.tickets = take;
+            sl.busy = true;
+            progressed = true;
+        }
+        bool any_busy = false;
+        for (Slot &sl : s->slots) any_busy |= sl.busy;
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (s->stop && !any_busy && s->queue.empty()) break;
+        if (!progressed) {
+            if (any_busy)
+                s->cv.wait_for(lk, std::chrono::microseconds(50));
+            else
+                s->cv.wait_for(lk, std::chrono::milliseconds(2));
+        }
+    }
+}
+
+void serve_shutdown(zkg_ctx *ctx)
+{
+    ServeState *s = ctx->serve;
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+    }
+    s->cv.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    for (Slot &sl : s->slots) {
+        (void)hipStreamSynchronize(sl.stream);
+        (void)hipEventDestroy(sl.done);
+        (void)hipStreamDestroy(sl.stream);
+    }
+    for (auto &kv : s->reqs)
+        if (kv.second.tree) zkg_tree_free(kv.second.tree);
+    delete s;
+    ctx->serve = nullptr;
+}
+
+}  // namespace zkg
+
+extern "C" int zkg_serve_start(zkg_ctx *ctx, uint32_t slots, uint32_t max_batch)
+{
+    if (!ctx) return set_err(ZKG_EINVAL, "null ctx");
+    if (ctx->serve) return set_err(ZKG_ESTATE, "serving loop already running");
+    if (slots == 0 || slots > 8 || max_batch == 0) return set_err(ZKG_EINVAL, "slots 1..8, max_batch >= 1");
+    ServeState *s = new ServeState();
+    s->max_batch = max_batch;
+    s->slots.resize(slots);
+    for (Slot &sl : s->slots) {
+        hipError_t e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            for (Slot &x : s->slots) {
+                if (x.done) (void)hipEventDestroy(x.done);
+                if (x.stream) (void)hipStreamDestroy(x.stream);
+            }
+            delete s;
+            return hip_err(e, "serve stream");
+        }
+    }
+    ctx->serve = s;
+    s->worker = std::thread(serve_loop, ctx, s);
+    return ZKG_OK;
+}
+
+extern "C" int zkg_serve_submit(zkg_ctx *ctx, const zkg_job *job, uint64_t *ticket)
+{
+    if (!ctx || !job || !ticket) return set_err(ZKG_EINVAL, "null argument");
+    ServeState *s = ctx->serve;
+    if (!s) return set_err(ZKG_ESTATE, "serving loop not running");
+    int rc = validate_job(ctx, *job);
+    if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->stop) return set_err(ZKG_ESTATE, "serving loop stopping");
+        Request r;
+        r.ticket = s->next_ticket++;
+        r.job = *job;
+        r.arrival = Clock::now();
+        s->reqs[r.ticket] = r;
+        s->queue.push_back(r.ticket);
+        *ticket = r.ticket;
+    }
+    s->cv.notify_all();
+    return ZKG_OK;
+}
+
+extern "C" int zkg_serve_poll(zkg_ctx *ctx, uint64_t ticket, zkg_tree **tree, double *latency_ms)
+{
+    if (!ctx || !tree) return set_err(ZKG_EINVAL, "null argument");
+    ServeState *s = ctx->serve;
+    if (!s) return set_err(ZKG_ESTATE, "serving loop not running");
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return set_err(ZKG_EINVAL, "unknown ticket %llu", (unsigned long long)ticket);
+    Request &r = it->second;
+    if (r.status == ZKG_EBUSY) return ZKG_EBUSY;
+    int st = r.status;
+    if (latency_ms) latency_ms[0] = std::chrono::duration<double, std::milli>(r.finish - r.arrival).count();
+    *tree = r.tree;
+    s->reqs.erase(it);
+    if (st != ZKG_OK) return set_err(st, "request %llu failed", (unsigned long long)ticket);
+    return ZKG_OK;
+}
+
+extern "C" int zkg_serve_stop(zkg_ctx *ctx)
+{
+    if (!ctx) return set_err(ZKG_EINVAL, "null ctx");
+    if (!ctx->serve) return ZKG_OK;
+    serve_shutdown(ctx);
+    return ZKG_OK;
+}

@@ -1,0 +1,162 @@
+// zkg_field.h -- exact prime-field arithmetic for gfx950 VALU.
+//
+// Goldilocks p = 2^64 - 2^32 + 1 and BabyBear p = 2^31 - 2^27 + 1 (with the
+// degree-4 binomial extension x^4 - 11).  Semantics follow the reference's
+// field helpers zkf-metal/src/shaders/field_goldilocks.metal:1-56 and
+// field_babybear.metal:1-25; the representation is this build's own:
+//   * Goldilocks values are canonical u64; a 64x64 product is four
+//     v_mad_u64_u32 partial products reduced with 2^64 = 2^32 - 1, 2^96 = -1.
+//   * BabyBear values inside kernels are Montgomery form (R = 2^32); HBM holds
+//     canonical u32 and kernels convert at load/store.  A product is
+//     mad_u64_u32 + mul_lo + mad_u64_u32 (REDC), one conditional subtract.
+// All ops keep outputs fully reduced, so any value written back is canonical.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace zkg {
+
+// ---------------------------------------------------------------- Goldilocks
+struct Gl {
+    using T = uint64_t;
+    static constexpr uint64_t P = 0xFFFFFFFF00000001ULL;
+    static constexpr uint64_t EPS = 0xFFFFFFFFULL;  // 2^64 mod p
+
+    __host__ __device__ static inline T add(T a, T b)
+    {
+        T s = a + b;
+        s += (s < a) ? EPS : 0;
+        return s >= P ? s - P : s;
+    }
+    __host__ __device__ static inline T sub(T a, T b)
+    {
+        T d = a - b;
+        return (a < b) ? d - EPS : d;
+    }
+    __host__ __device__ static inline T neg(T a) { return a ? P - a : 0; }
+    __host__ __device__ static inline T dbl(T a) { return add(a, a); }
+
+    // x = lo + 2^64 * (hl + 2^32 * hh)  ==  lo - hh + hl * (2^32 - 1)  (mod p)
+    __host__ __device__ static inline T reduce128(uint64_t lo, uint64_t hi)
+    {
+        uint32_t hh = (uint32_t)(hi >> 32);
+        uint64_t hl = (uint32_t)hi;
+        uint64_t t0 = lo - hh;
+        t0 -= (lo < hh) ? EPS : 0;
+        uint64_t t1 = (hl << 32) - hl;
+        uint64_t r = t0 + t1;
+        r += (r < t0) ? EPS : 0;
+        return r >= P ? r - P : r;
+    }
+    __host__ __device__ static inline T mul(T a, T b)
+    {
+        uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+        uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+        uint64_t p00 = (uint64_t)a0 * b0;
+        uint64_t p10 = (uint64_t)a1 * b0;
+        uint64_t mid = (uint64_t)a0 * b1 + ((p00 >> 32) + (uint32_t)p10);
+        uint64_t hi = (uint64_t)a1 * b1 + ((mid >> 32) + (p10 >> 32));
+        uint64_t lo = (mid << 32) | (uint32_t)p00;
+        return reduce128(lo, hi);
+    }
+    __host__ __device__ static inline T sqr(T a) { return mul(a, a); }
+    // HBM <-> register representation (identity for Goldilocks)
+    __host__ __device__ static inline T from_canon(T a) { return a; }
+    __host__ __device__ static inline T to_canon(T a) { return a; }
+    __host__ __device__ static inline T one() { return 1; }
+};
+
+// ---------------------------------------------------------------- BabyBear
+struct Bb {
+    using T = uint32_t;
+    static constexpr uint32_t P = 0x78000001u;
+    static constexpr uint32_t NPINV = 0x77FFFFFFu;  // -p^-1 mod 2^32
+    static constexpr uint32_t R2 = 1172168163u;     // 2^64 mod p
+    static constexpr uint32_t R1 = 268435454u;      // 2^32 mod p (Montgomery 1)
+
+    __host__ __device__ static inline T add(T a, T b)
+    {
+        T s = a + b;
+        return s >= P ? s - P : s;
+    }
+    __host__ __device__ static inline T sub(T a, T b)
+    {
+        T d = a - b;
+        return (a < b) ? d + P : d;
+    }
+    __host__ __device__ static inline T neg(T a) { return a ? P - a : 0; }
+    __host__ __device__ static inline T dbl(T a) { return add(a, a); }
+    // Montgomery REDC of t < p * 2^32:  (t + m p) / 2^32,  m = t * (-p^-1) mod 2^32
+    __host__ __device__ static inline T redc(uint64_t t)
+    {
+        uint32_t m = (uint32_t)t * NPINV;
+        uint64_t u = t + (uint64_t)m * P;
+        uint32_t r = (uint32_t)(u >> 32);
+        return r >= P ? r - P : r;
+    }
+    __host__ __device__ static inline T mul(T a, T b) { return redc((uint64_t)a * b); }
+    __host__ __device__ static inline T sqr(T a) { return mul(a, a); }
+    __host__ __device__ static inline T from_canon(T a) { return mul(a, R2); }
+    __host__ __device__ static inline T to_canon(T a) { return redc((uint64_t)a); }
+    __host__ __device__ static inline T one() { return R1; }
+    // x * 2^-k for 1 <= k <= 27, any representation (Montgomery form commutes
+    // with scaling).  p = 1 (mod 2^27) so m = -x mod 2^k clears the low bits:
+    // (x + m p) / 2^k = (x + m) / 2^k + m * 15 * 2^(27-k).
+    template <int K>
+    __host__ __device__ static inline T mul_inv_pow2(T x)
+    {
+        static_assert(K >= 1 && K <= 27, "shift range");
+        uint32_t m = (0u - x) & ((1u << K) - 1u);
+        uint32_t r = ((x + m) >> K) + m * (15u << (27 - K));
+        return r >= P ? r - P : r;
+    }
+};
+
+// ------------------------------------------------- BabyBear degree-4 extension
+// EF4 = Bb[x] / (x^4 - 11), coefficients low-first; components in Montgomery.
+struct Ef4 {
+    Bb::T c[4];
+};
+
+__host__ __device__ static inline Ef4 ef4_add(const Ef4 &a, const Ef4 &b)
+{
+    Ef4 r;
+#pragma unroll
+    for (int i = 0; i < 4; i++) r.c[i] = Bb::add(a.c[i], b.c[i]);
+    return r;
+}
+__host__ __device__ static inline Ef4 ef4_sub(const Ef4 &a, const Ef4 &b)
+{
+    Ef4 r;
+#pragma unroll
+    for (int i = 0; i < 4; i++) r.c[i] = Bb::sub(a.c[i], b.c[i]);
+    return r;
+}
+__host__ __device__ static inline Ef4 ef4_mul_base(const Ef4 &a, Bb::T s)
+{
+    Ef4 r;
+#pragma unroll
+    for (int i = 0; i < 4; i++) r.c[i] = Bb::mul(a.c[i], s);
+    return r;
+}
+// Schoolbook with lazy accumulation: each of the 7 convolution sums holds at
+// most 4 products < p^2 < 2^62 only after REDC, so reduce per product and add.
+__host__ __device__ static inline Ef4 ef4_mul(const Ef4 &a, const Ef4 &b, Bb::T w_mont)
+{
+    Bb::T c0 = Bb::mul(a.c[0], b.c[0]);
+    Bb::T c1 = Bb::add(Bb::mul(a.c[0], b.c[1]), Bb::mul(a.c[1], b.c[0]));
+    Bb::T c2 = Bb::add(Bb::add(Bb::mul(a.c[0], b.c[2]), Bb::mul(a.c[1], b.c[1])), Bb::mul(a.c[2], b.c[0]));
+    Bb::T c3 = Bb::add(Bb::add(Bb::mul(a.c[0], b.c[3]), Bb::mul(a.c[1], b.c[2])),
+                       Bb::add(Bb::mul(a.c[2], b.c[1]), Bb::mul(a.c[3], b.c[0])));
+    Bb::T c4 = Bb::add(Bb::add(Bb::mul(a.c[1], b.c[3]), Bb::mul(a.c[2], b.c[2])), Bb::mul(a.c[3], b.c[1]));
+    Bb::T c5 = Bb::add(Bb::mul(a.c[2], b.c[3]), Bb::mul(a.c[3], b.c[2]));
+    Bb::T c6 = Bb::mul(a.c[3], b.c[3]);
+    Ef4 r;
+    r.c[0] = Bb::add(c0, Bb::mul(c4, w_mont));
+    r.c[1] = Bb::add(c1, Bb::mul(c5, w_mont));
+    r.c[2] = Bb::add(c2, Bb::mul(c6, w_mont));
+    r.c[3] = c3;
+    return r;
+}
+
+}  // namespace zkg

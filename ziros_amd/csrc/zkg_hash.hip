@@ -1,0 +1,499 @@
+// zkg_hash.hip -- Poseidon2 row hashing, Merkle trees and FRI folding for gfx950.
+//
+// Semantics (reference file:line):
+//   * permutation: zkf-metal/src/shaders/poseidon2.metal:17-264;
+//   * row hash = PaddingFreeSponge<P, W, W/2, W/2> and node = TruncatedPermutation
+//     <P, 2, W/2, W> (zkf-backends/src/plonky3.rs:56-57; restated at
+//     zkf-backends/src/wrapping/poseidon2_goldilocks.rs:143-225,372-398);
+//   * tree layers: vendor/p3-merkle-tree-gpu/src/merkle_tree.rs:84-403
+//     (power-of-two heights: layer i+1 = compress of pairs of layer i);
+//   * openings: vendor/p3-merkle-tree-gpu/src/mmcs.rs:179-219;
+//   * FRI fold: p3-fri 0.4.2 TwoAdicFriFolding::fold_matrix (arity 2) and the
+//     reference FriAccelerator formula zkf-metal/src/shaders/fri.metal:14-53.
+//
+// Design: one Poseidon2 state per lane, held in VGPRs for the whole
+// permutation (constants are wave-uniform scalar loads).  Merkle levels are
+// fused: a block compresses 512 digests down to one, passing each level
+// through LDS, so a 2^21-leaf tree takes three launches.  The FRI fold of a
+// pair of outputs is fused with the Poseidon2 hash of the next layer's leaf
+// (one leaf = two folded EF4 values = 8 base elements = one absorption).
+#include "zkg_field.h"
+#include "zkg_internal.h"
+#include "zkg_poseidon2.h"
+
+namespace zkg {
+
+namespace {
+
+constexpr int HASH_THREADS = 256;
+constexpr int MERKLE_THREADS = 256;
+constexpr int MAX_FUSE = 9;  // 2 * MERKLE_THREADS leaves -> 1 root
+
+__device__ __forceinline__ uint32_t brev(uint32_t x, int bits)
+{
+    return bits ? (__brev(x) >> (32 - bits)) : 0u;
+}
+
+// ---------------------------------------------------------------- permute
+template <class F, int W>
+__global__ __launch_bounds__(HASH_THREADS) void k_p2_permute(typename F::T *__restrict__ states, uint64_t n,
+                                                             const typename F::T *__restrict__ rc,
+                                                             const typename F::T *__restrict__ diag)
+{
+    using T = typename F::T;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    T s[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) s[k] = F::from_canon(states[i * W + k]);
+    p2_permute<F, W>(s, rc, diag);
+#pragma unroll
+    for (int k = 0; k < W; k++) states[i * W + k] = F::to_canon(s[k]);
+}
+
+// ---------------------------------------------------------------- row hash
+// Sponge over one row per lane: overwrite the rate with the next <= W/2
+// elements, permute; a partial last chunk leaves the other rate lanes as they
+// were.  A zero-width row hashes to the zero digest.
+template <class F, int W>
+__global__ __launch_bounds__(HASH_THREADS) void k_hash_rows(const typename F::T *__restrict__ mat, uint64_t height,
+                                                            uint32_t width, typename F::T *__restrict__ digests,
+                                                            const typename F::T *__restrict__ rc,
+                                                            const typename F::T *__restrict__ diag)
+{
+    using T = typename F::T;
+    constexpr int R = W / 2;
+    uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= height) return;
+    T s[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) s[k] = 0;
+    const T *src = mat + row * width;
+    uint32_t pos = 0;
+    for (; pos + R <= width; pos += R) {
+#pragma unroll
+        for (int k = 0; k < R; k++) s[k] = F::from_canon(src[pos + k]);
+        p2_permute<F, W>(s, rc, diag);
+    }
+    if (pos < width) {
+        uint32_t take = width - pos;
+#pragma unroll
+        for (int k = 0; k < R; k++)
+            if ((uint32_t)k < take) s[k] = F::from_canon(src[pos + k]);
+        p2_permute<F, W>(s, rc, diag);
+    }
+#pragma unroll
+    for (int k = 0; k < R; k++) digests[row * R + k] = F::to_canon(s[k]);
+}
+
+// ---------------------------------------------------------------- Merkle
+template <class T>
+struct LayerPtrs {
+    T *p[MAX_FUSE];
+};
+
+// Block b compresses input digests [b 2^k, (b+1) 2^k) of `in` into levels
+// 1..k; level l node i goes to out.p[l-1][i].
+template <class F, int W>
+__global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_fused(const typename F::T *__restrict__ in,
+                                                                 LayerPtrs<typename F::T> out, int k,
+                                                                 const typename F::T *__restrict__ rc,
+                                                                 const typename F::T *__restrict__ diag)
+{
+    using T = typename F::T;
+    constexpr int D = W / 2;
+    __shared__ T lds[D * MERKLE_THREADS];  // element-major: lds[e * 256 + node]
+    const int t = threadIdx.x;
+    int nodes = 1 << (k - 1);
+    T s[W];
+    if (t < nodes) {
+        uint64_t i = (uint64_t)blockIdx.x * nodes + t;
+#pragma unroll
+        for (int e = 0; e < W; e++) s[e] = F::from_canon(in[2 * i * D + e]);
+        p2_permute<F, W>(s, rc, diag);
+#pragma unroll
+        for (int e = 0; e < D; e++) {
+            out.p[0][i * D + e] = F::to_canon(s[e]);
+            lds[e * MERKLE_THREADS + t] = s[e];
+        }
+    }
+    for (int lvl = 1; lvl < k; lvl++) {
+        __syncthreads();
+        nodes >>= 1;
+        if (t < nodes) {
+#pragma unroll
+            for (int e = 0; e < D; e++) {
+                s[e] = lds[e * MERKLE_THREADS + 2 * t];
+                s[D + e] = lds[e * MERKLE_THREADS + 2 * t + 1];
+            }
+        }
+        __syncthreads();
+        if (t < nodes) {
+            p2_permute<F, W>(s, rc, diag);
+            uint64_t i = (uint64_t)blockIdx.x * nodes + t;
+#pragma unroll
+            for (int e = 0; e < D; e++) {
+                out.p[lvl][i * D + e] = F::to_canon(s[e]);
+                lds[e * MERKLE_THREADS + t] = s[e];
+            }
+        }
+    }
+}
+
+// One level, one compression per lane (the unfused per-level path).
+template <class F, int W>
+__global__ __launch_bounds__(HASH_THREADS) void k_merkle_level(const typename F::T *__restrict__ in,
+                                                               typename F::T *__restrict__ out, uint64_t nodes,
+                                                               const typename F::T *__restrict__ rc,
+                                                               const typename F::T *__restrict__ diag)
+{
+    using T = typename F::T;
+    constexpr int D = W / 2;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nodes) return;
+    T s[W];
+#pragma unroll
+    for (int e = 0; e < W; e++) s[e] = F::from_canon(in[2 * i * D + e]);
+    p2_permute<F, W>(s, rc, diag);
+#pragma unroll
+    for (int e = 0; e < D; e++) out[i * D + e] = F::to_canon(s[e]);
+}
+
+// ---------------------------------------------------------------- generator
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+// Element i of stream `seed`: first mix(seed + (i+1) phi + a psi), a = 0,1,..,
+// shifted to the field's bit width, that is below p.
+template <class F>
+__global__ void k_gen(typename F::T *__restrict__ out, uint64_t seed, uint64_t start, uint64_t n)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t z0 = seed + (start + i + 1) * 0x9e3779b97f4a7c15ULL;
+    for (uint64_t a = 0;; a++) {
+        uint64_t v = mix64(z0 + a * 0x632be59bd9b4e019ULL);
+        if (sizeof(typename F::T) == 4) v >>= 33;
+        if (v < F::P) {
+            out[i] = (typename F::T)v;
+            return;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- FRI
+template <class F>
+__device__ __forceinline__ typename F::T inv_root_pow(const typename F::T *__restrict__ lo,
+                                                      const typename F::T *__restrict__ hi, uint32_t e, int logN)
+{
+    uint32_t ep = (e << (LOG_TW_TOP - logN)) & ((1u << LOG_TW_TOP) - 1u);
+    return F::mul(lo[ep & ((1u << TW_SPLIT) - 1u)], hi[ep >> TW_SPLIT]);
+}
+
+__device__ __forceinline__ Ef4 ef4_load(const uint32_t *__restrict__ p)
+{
+    uint4 v = *reinterpret_cast<const uint4 *>(p);
+    Ef4 r;
+    r.c[0] = Bb::from_canon(v.x);
+    r.c[1] = Bb::from_canon(v.y);
+    r.c[2] = Bb::from_canon(v.z);
+    r.c[3] = Bb::from_canon(v.w);
+    return r;
+}
+
+__device__ __forceinline__ void ef4_store(uint32_t *__restrict__ p, const Ef4 &a)
+{
+    uint4 v;
+    v.x = Bb::to_canon(a.c[0]);
+    v.y = Bb::to_canon(a.c[1]);
+    v.z = Bb::to_canon(a.c[2]);
+    v.w = Bb::to_canon(a.c[3]);
+    *reinterpret_cast<uint4 *>(p) = v;
+}
+
+struct FoldArgs {
+    uint32_t beta[4];  // Montgomery
+    uint32_t half;     // 1/2, Montgomery
+    uint32_t w11;      // 11, Montgomery
+};
+
+// out[i] = (lo + hi)/2 + beta (lo - hi) t_i,  t_i = (1/2) g^-rev(i),
+// g = two_adic_generator(log n + 1): the arity-2 fold of p3-fri's
+// fold_matrix, (1/2 + beta t_i) lo + (1/2 - beta t_i) hi, regrouped.
+__device__ __forceinline__ Ef4 fold_one(const Ef4 &lo, const Ef4 &hi, uint32_t t, const FoldArgs &a)
+{
+    Ef4 beta;
+#pragma unroll
+    for (int e = 0; e < 4; e++) beta.c[e] = a.beta[e];
+    Ef4 sum = ef4_mul_base(ef4_add(lo, hi), a.half);
+    Ef4 diff = ef4_mul_base(ef4_sub(lo, hi), t);
+    return ef4_add(sum, ef4_mul(beta, diff, a.w11));
+}
+
+// Each lane folds output pair (2j, 2j+1) and, if `digests` is set, hashes the
+// pair as one leaf (8 base elements) of the next layer's tree.
+__global__ __launch_bounds__(HASH_THREADS) void k_fri_fold_ef4(const uint32_t *__restrict__ in, uint64_t n_out,
+                                                               int log_n, uint32_t *__restrict__ out,
+                                                               uint32_t *__restrict__ digests, FoldArgs a,
+                                                               const uint32_t *__restrict__ lo_inv,
+                                                               const uint32_t *__restrict__ hi_inv,
+                                                               const uint32_t *__restrict__ rc)
+{
+    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t npairs = (n_out + 1) / 2;
+    if (j >= npairs) return;
+    Ef4 r[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        uint64_t i = 2 * j + h;
+        if (i < n_out) {
+            Ef4 lo = ef4_load(in + 8 * i), hi = ef4_load(in + 8 * i + 4);
+            uint32_t g = inv_root_pow<Bb>(lo_inv, hi_inv, brev((uint32_t)i, log_n), log_n + 1);
+            r[h] = fold_one(lo, hi, Bb::mul(g, a.half), a);
+            ef4_store(out + 4 * i, r[h]);
+        }
+    }
+    if (digests && 2 * j + 1 < n_out) {
+        uint32_t s[16];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            s[e] = r[0].c[e];
+            s[4 + e] = r[1].c[e];
+        }
+#pragma unroll
+        for (int e = 8; e < 16; e++) s[e] = 0;
+        p2_permute<Bb, 16>(s, rc, nullptr);
+#pragma unroll
+        for (int e = 0; e < 8; e++) digests[j * 8 + e] = Bb::to_canon(s[e]);
+    }
+}
+
+// Base-field version of the same fold (beta in the base field).
+template <class F>
+__global__ __launch_bounds__(HASH_THREADS) void k_fri_fold_base(const typename F::T *__restrict__ in, uint64_t n_out,
+                                                                int log_n, typename F::T *__restrict__ out,
+                                                                typename F::T beta, typename F::T half,
+                                                                const typename F::T *__restrict__ lo_inv,
+                                                                const typename F::T *__restrict__ hi_inv)
+{
+    using T = typename F::T;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_out) return;
+    T lo = F::from_canon(in[2 * i]), hi = F::from_canon(in[2 * i + 1]);
+    T g = inv_root_pow<F>(lo_inv, hi_inv, brev((uint32_t)i, log_n), log_n + 1);
+    T t = F::mul(g, half);
+    T sum = F::mul(F::add(lo, hi), half);
+    out[i] = F::to_canon(F::add(sum, F::mul(beta, F::mul(F::sub(lo, hi), t))));
+}
+
+// FriAccelerator::fold_*: g[i] = (f[2i] + f[2i+1]) / 2 + alpha (f[2i] - f[2i+1]) itw[i]
+template <class F>
+__global__ void k_fri_fold_explicit(const typename F::T *__restrict__ in, uint64_t n_out,
+                                    typename F::T *__restrict__ out, typename F::T alpha, typename F::T half,
+                                    const typename F::T *__restrict__ itw)
+{
+    using T = typename F::T;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_out) return;
+    T a = F::from_canon(in[2 * i]), b = F::from_canon(in[2 * i + 1]);
+    T s = F::mul(F::add(a, b), half);
+    T d = F::mul(F::mul(F::sub(a, b), F::from_canon(itw[i])), alpha);
+    out[i] = F::to_canon(F::add(s, d));
+}
+
+// ---------------------------------------------------------------- openings
+template <class T>
+struct OpenLayers {
+    T *p[32];
+};
+
+// Block q: opened row index[q] of the leaf matrix and its sibling path
+// [layer_l[(index >> l) ^ 1] for l < nlayers - 1].
+template <class T>
+__global__ void k_gather_open(const T *__restrict__ mat, uint32_t mat_width, OpenLayers<T> layers, int nlayers,
+                              int D, const uint64_t *__restrict__ idx, T *__restrict__ rows_out,
+                              T *__restrict__ paths_out)
+{
+    const uint32_t q = blockIdx.x;
+    const uint64_t index = idx[q];
+    for (uint32_t c = threadIdx.x; c < mat_width; c += blockDim.x)
+        rows_out[(uint64_t)q * mat_width + c] = mat[index * mat_width + c];
+    const int plen = nlayers - 1;
+    for (int t = threadIdx.x; t < plen * D; t += blockDim.x) {
+        int l = t / D, e = t - l * D;
+        uint64_t sib = (index >> l) ^ 1ull;
+        paths_out[((uint64_t)q * plen + l) * D + e] = layers.p[l][sib * D + e];
+    }
+}
+
+template <class F, int W>
+hipError_t merkle_impl(const P2Set &p2, void *const *layers, int nlayers, uint64_t n, int fuse, hipStream_t st,
+                       bool sync_each)
+{
+    using T = typename F::T;
+    int lvl = 0;
+    if (fuse == 1) {
+        for (; lvl < nlayers - 1; lvl++) {
+            uint64_t nodes = n >> (lvl + 1);
+            hipLaunchKernelGGL((k_merkle_level<F, W>), dim3((unsigned)((nodes + HASH_THREADS - 1) / HASH_THREADS)),
+                               dim3(HASH_THREADS), 0, st, (const T *)layers[lvl], (T *)layers[lvl + 1], nodes,
+                               (const T *)p2.rc, (const T *)p2.diag);
+            hipError_t e = hipGetLastError();
+            if (e == hipSuccess && sync_each) e = hipStreamSynchronize(st);
+            if (e) return e;
+        }
+        return hipSuccess;
+    }
+    while (lvl < nlayers - 1) {
+        int k = nlayers - 1 - lvl;
+        if (k > fuse) k = fuse;
+        if (k > MAX_FUSE) k = MAX_FUSE;
+        LayerPtrs<T> lp;
+        for (int i = 0; i < MAX_FUSE; i++) lp.p[i] = i < k ? (T *)layers[lvl + 1 + i] : nullptr;
+        uint64_t blocks = (n >> lvl) >> k;
+        hipLaunchKernelGGL((k_merkle_fused<F, W>), dim3((unsigned)blocks), dim3(MERKLE_THREADS), 0, st,
+                           (const T *)layers[lvl], lp, k, (const T *)p2.rc, (const T *)p2.diag);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess && sync_each) e = hipStreamSynchronize(st);
+        if (e) return e;
+        lvl += k;
+    }
+    return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t launch_p2_permute(int field, int width, const P2Set &p2, void *states, uint64_t n, hipStream_t st)
+{
+    if (!n) return hipSuccess;
+    dim3 g((unsigned)((n + HASH_THREADS - 1) / HASH_THREADS)), b(HASH_THREADS);
+    if (field == FIELD_BB && width == 16)
+        hipLaunchKernelGGL((k_p2_permute<Bb, 16>), g, b, 0, st, (uint32_t *)states, n, (const uint32_t *)p2.rc,
+                           (const uint32_t *)p2.diag);
+    else if (field == FIELD_GL && width == 16)
+        hipLaunchKernelGGL((k_p2_permute<Gl, 16>), g, b, 0, st, (uint64_t *)states, n, (const uint64_t *)p2.rc,
+                           (const uint64_t *)p2.diag);
+    else if (field == FIELD_GL && width == 8)
+        hipLaunchKernelGGL((k_p2_permute<Gl, 8>), g, b, 0, st, (uint64_t *)states, n, (const uint64_t *)p2.rc,
+                           (const uint64_t *)p2.diag);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_hash_rows(int field, int width, const P2Set &p2, const void *mat, uint64_t height,
+                            uint32_t mat_width, void *digests, hipStream_t st)
+{
+    if (!height) return hipSuccess;
+    dim3 g((unsigned)((height + HASH_THREADS - 1) / HASH_THREADS)), b(HASH_THREADS);
+    if (field == FIELD_BB && width == 16)
+        hipLaunchKernelGGL((k_hash_rows<Bb, 16>), g, b, 0, st, (const uint32_t *)mat, height, mat_width,
+                           (uint32_t *)digests, (const uint32_t *)p2.rc, (const uint32_t *)p2.diag);
+    else if (field == FIELD_GL && width == 16)
+        hipLaunchKernelGGL((k_hash_rows<Gl, 16>), g, b, 0, st, (const uint64_t *)mat, height, mat_width,
+                           (uint64_t *)digests, (const uint64_t *)p2.rc, (const uint64_t *)p2.diag);
+    else if (field == FIELD_GL && width == 8)
+        hipLaunchKernelGGL((k_hash_rows<Gl, 8>), g, b, 0, st, (const uint64_t *)mat, height, mat_width,
+                           (uint64_t *)digests, (const uint64_t *)p2.rc, (const uint64_t *)p2.diag);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_merkle_tree(int field, int width, const P2Set &p2, void *const *layers, int nlayers,
+                              uint64_t n_leaves, int fuse_levels, hipStream_t st, bool sync_each)
+{
+    if (fuse_levels < 1) fuse_levels = 1;
+    if (field == FIELD_BB && width == 16)
+        return merkle_impl<Bb, 16>(p2, layers, nlayers, n_leaves, fuse_levels, st, sync_each);
+    if (field == FIELD_GL && width == 16)
+        return merkle_impl<Gl, 16>(p2, layers, nlayers, n_leaves, fuse_levels, st, sync_each);
+    if (field == FIELD_GL && width == 8)
+        return merkle_impl<Gl, 8>(p2, layers, nlayers, n_leaves, fuse_levels, st, sync_each);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_gen(int field, void *out, uint64_t seed, uint64_t start, uint64_t n, hipStream_t st)
+{
+    if (!n) return hipSuccess;
+    dim3 g((unsigned)((n + 255) / 256)), b(256);
+    if (field == FIELD_GL)
+        hipLaunchKernelGGL(k_gen<Gl>, g, b, 0, st, (uint64_t *)out, seed, start, n);
+    else
+        hipLaunchKernelGGL(k_gen<Bb>, g, b, 0, st, (uint32_t *)out, seed, start, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fri_fold(int ext, int field, const TwiddleSet &tw, const P2Set &p2, const void *in,
+                           uint64_t n_out, const uint32_t *beta_reg, void *out, void *leaf_digests, hipStream_t st)
+{
+    if (!n_out) return hipSuccess;
+    int log_n = 0;
+    while ((1ull << log_n) < n_out) log_n++;
+    if (ext) {
+        FoldArgs a;
+        for (int e = 0; e < 4; e++) a.beta[e] = beta_reg[e];
+        a.half = bb_to_mont_host((Bb::P + 1) / 2);
+        a.w11 = bb_to_mont_host(11);
+        uint64_t npairs = (n_out + 1) / 2;
+        hipLaunchKernelGGL(k_fri_fold_ef4, dim3((unsigned)((npairs + HASH_THREADS - 1) / HASH_THREADS)),
+                           dim3(HASH_THREADS), 0, st, (const uint32_t *)in, n_out, log_n, (uint32_t *)out,
+                           (uint32_t *)leaf_digests, a, (const uint32_t *)tw.top_lo_inv,
+                           (const uint32_t *)tw.top_hi_inv, (const uint32_t *)p2.rc);
+        return hipGetLastError();
+    }
+    dim3 g((unsigned)((n_out + HASH_THREADS - 1) / HASH_THREADS)), b(HASH_THREADS);
+    if (field == FIELD_GL)
+        hipLaunchKernelGGL(k_fri_fold_base<Gl>, g, b, 0, st, (const uint64_t *)in, n_out, log_n, (uint64_t *)out,
+                           (uint64_t)beta_reg[0] | ((uint64_t)beta_reg[1] << 32), (uint64_t)(Gl::P + 1) / 2,
+                           (const uint64_t *)tw.top_lo_inv, (const uint64_t *)tw.top_hi_inv);
+    else
+        hipLaunchKernelGGL(k_fri_fold_base<Bb>, g, b, 0, st, (const uint32_t *)in, n_out, log_n, (uint32_t *)out,
+                           beta_reg[0], bb_to_mont_host((Bb::P + 1) / 2), (const uint32_t *)tw.top_lo_inv,
+                           (const uint32_t *)tw.top_hi_inv);
+    return hipGetLastError();
+}
+
+hipError_t launch_fri_fold_explicit(int field, const void *in, uint64_t n_out, uint64_t alpha, const void *inv_tw,
+                                    void *out, hipStream_t st)
+{
+    if (!n_out) return hipSuccess;
+    dim3 g((unsigned)((n_out + 255) / 256)), b(256);
+    if (field == FIELD_GL)
+        hipLaunchKernelGGL(k_fri_fold_explicit<Gl>, g, b, 0, st, (const uint64_t *)in, n_out, (uint64_t *)out, alpha,
+                           (uint64_t)(Gl::P + 1) / 2, (const uint64_t *)inv_tw);
+    else
+        hipLaunchKernelGGL(k_fri_fold_explicit<Bb>, g, b, 0, st, (const uint32_t *)in, n_out, (uint32_t *)out,
+                           bb_to_mont_host((uint32_t)alpha), bb_to_mont_host((Bb::P + 1) / 2),
+                           (const uint32_t *)inv_tw);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_open(int field, int digest, const void *mat, uint32_t mat_width, void *const *layers,
+                              int nlayers, const uint64_t *d_indices, uint32_t n, void *rows_out, void *paths_out,
+                              hipStream_t st)
+{
+    if (!n) return hipSuccess;
+    if (nlayers > 32) return hipErrorInvalidValue;
+    if (field == FIELD_GL) {
+        OpenLayers<uint64_t> ol;
+        for (int i = 0; i < 32; i++) ol.p[i] = i < nlayers ? (uint64_t *)layers[i] : nullptr;
+        hipLaunchKernelGGL(k_gather_open<uint64_t>, dim3(n), dim3(64), 0, st, (const uint64_t *)mat, mat_width, ol,
+                           nlayers, digest, d_indices, (uint64_t *)rows_out, (uint64_t *)paths_out);
+    } else {
+        OpenLayers<uint32_t> ol;
+        for (int i = 0; i < 32; i++) ol.p[i] = i < nlayers ? (uint32_t *)layers[i] : nullptr;
+        hipLaunchKernelGGL(k_gather_open<uint32_t>, dim3(n), dim3(64), 0, st, (const uint32_t *)mat, mat_width, ol,
+                           nlayers, digest, d_indices, (uint32_t *)rows_out, (uint32_t *)paths_out);
+    }
+    return hipGetLastError();
+}
+
+uint32_t bb_to_mont_host(uint32_t x) { return Bb::from_canon(x); }
+uint32_t bb_from_mont_host(uint32_t x) { return Bb::to_canon(x); }
+
+}  // namespace zkg

@@ -1,0 +1,89 @@
+// zkg_internal.h -- declarations shared by the kernel and runtime translation
+// units of libzkg.  Not part of the public C ABI (include/zkg.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace zkg {
+
+enum FieldId : int { FIELD_GL = 0, FIELD_BB = 1 };
+
+constexpr int LOG_TW_TILE = 11;   // largest in-LDS transform: 2^11 points
+constexpr int LOG_TW_TOP = 24;    // four-step / coset tables cover orders <= 2^24
+constexpr int TW_SPLIT = 12;      // omega^e = lo[e & 4095] * hi[e >> 12]
+
+// Resident twiddle tables of one field, register form (BabyBear Montgomery).
+struct TwiddleSet {
+    void *tile_fwd = nullptr;  // omega_2048^j, j < 1024
+    void *tile_inv = nullptr;  // omega_2048^-j
+    void *top_lo_fwd = nullptr, *top_hi_fwd = nullptr;  // omega_2^24^i, omega_2^24^(4096 i)
+    void *top_lo_inv = nullptr, *top_hi_inv = nullptr;
+};
+
+// Coset weights for one (N, blowup, shift): chunk b of the LDE evaluates the
+// coefficients times (s_b^k / N), s_b = shift * omega_{BN}^{rev_B(b)}, stored as
+// lo[b][k & 4095] (with 1/N folded in) and hi[b][k >> 12].
+struct CosetSet {
+    void *lo = nullptr;  // B * 4096
+    void *hi = nullptr;  // B * 1024
+    uint32_t log_n = 0, log_blowup = 0;
+};
+
+// Poseidon2 constants in register form, flat [initial][internal][terminal].
+struct P2Set {
+    void *rc = nullptr;
+    void *diag = nullptr;
+    uint64_t seed = 0;
+    bool loaded = false;
+};
+
+struct PassTimes;
+
+// --- NTT / LDE (zkg_ntt.hip) -------------------------------------------------
+// All matrices are row-major (height x width) in HBM, canonical values.
+hipError_t launch_coset_lde(int field, const TwiddleSet &tw, const CosetSet &cs, const void *in,
+                            void *out, void *tmp, uint64_t height, uint32_t width,
+                            uint32_t log_blowup, hipStream_t st);
+// Bytes of scratch that launch_coset_lde needs (0 for single-pass sizes).
+uint64_t coset_lde_tmp_bytes(int field, uint64_t height, uint32_t width, uint32_t log_blowup);
+hipError_t launch_ntt(int field, const TwiddleSet &tw, void *data, void *tmp, uint64_t height,
+                      uint32_t width, int inverse, hipStream_t st);
+uint64_t ntt_tmp_bytes(int field, uint64_t height, uint32_t width);
+// Unfused per-stage radix-2 path (one HBM pass per stage): the plain GPU
+// denominator used for comparison with the fused path.
+// tmp: height * width elements.
+hipError_t launch_lde_plain(int field, const TwiddleSet &tw, const void *in, void *out, void *tmp,
+                            uint64_t height, uint32_t width, uint32_t log_blowup, uint64_t shift,
+                            hipStream_t st, bool sync_each);
+
+// --- Poseidon2 / Merkle / FRI (zkg_hash.hip) -------------------------------
+hipError_t launch_p2_permute(int field, int width, const P2Set &p2, void *states, uint64_t n,
+                             hipStream_t st);
+hipError_t launch_hash_rows(int field, int width, const P2Set &p2, const void *mat,
+                            uint64_t height, uint32_t mat_width, void *digests, hipStream_t st);
+// Builds levels 1.. of a power-of-two tree whose level 0 (n digests) is
+// layers[0]; layers[i] receives n >> i digests.  fuse_levels = 1 gives one
+// launch per level (the plain path).
+hipError_t launch_merkle_tree(int field, int width, const P2Set &p2, void *const *layers,
+                              int nlayers, uint64_t n_leaves, int fuse_levels, hipStream_t st,
+                              bool sync_each);
+hipError_t launch_gen(int field, void *out, uint64_t seed, uint64_t start, uint64_t n,
+                      hipStream_t st);
+// FRI fold over EF4 (ext = 1) or a base field; twiddles from the resident
+// inverse top tables.  If leaf_digests is not null, also hashes rows of the
+// folded codeword viewed as a width-2 (ext) matrix into leaf digests (fused).
+hipError_t launch_fri_fold(int ext, int field, const TwiddleSet &tw, const P2Set &p2,
+                           const void *in, uint64_t n_out, const uint32_t *beta_mont,
+                           void *out, void *leaf_digests, hipStream_t st);
+// FriAccelerator semantics: explicit inverse twiddles.
+hipError_t launch_fri_fold_explicit(int field, const void *in, uint64_t n_out, uint64_t alpha,
+                                    const void *inv_tw, void *out, hipStream_t st);
+hipError_t launch_gather_open(int field, int digest, const void *mat, uint32_t mat_width,
+                              void *const *layers, int nlayers, const uint64_t *d_indices,
+                              uint32_t n, void *rows_out, void *paths_out, hipStream_t st);
+
+// Register-form conversion helpers on the host.
+uint32_t bb_to_mont_host(uint32_t x);
+uint32_t bb_from_mont_host(uint32_t x);
+
+}  // namespace zkg

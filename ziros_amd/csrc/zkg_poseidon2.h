@@ -1,0 +1,139 @@
+// zkg_poseidon2.h -- register-resident Poseidon2 permutation, one state per lane.
+//
+// Round schedule of zkf-metal/src/shaders/poseidon2.metal:17-264 (restated in
+// zkf-backends/src/wrapping/poseidon2_goldilocks.rs:117-339):
+//   MDS-light; RF/2 external rounds (add RC to every lane of the state, x^7,
+//   MDS-light); RP internal rounds (s0 += rc, s0 = s0^7, s_i = d_i s_i + sum);
+//   RF/2 external rounds.
+// MDS-light applies the circulant M4 = [[2,3,1,1],[1,2,3,1],[1,1,2,3],[3,1,1,2]]
+// to every 4-chunk and then adds the per-position sum over chunks.
+//
+// Constants live in device memory in register form (BabyBear: Montgomery) in
+// the reference's flat GPU layout [initial RF/2*W][internal RP][terminal RF/2*W]
+// (zkf-metal/src/poseidon2/goldilocks.rs:42-83, babybear.rs:76-111); every
+// index is wave-uniform, so the loads are scalar.
+#pragma once
+#include "zkg_field.h"
+
+namespace zkg {
+
+template <class F>
+__device__ __forceinline__ typename F::T sbox7(typename F::T x)
+{
+    typename F::T x2 = F::sqr(x);
+    typename F::T x3 = F::mul(x2, x);
+    typename F::T x4 = F::sqr(x2);
+    return F::mul(x3, x4);
+}
+
+template <class F, int W>
+__device__ __forceinline__ void mds_light(typename F::T (&s)[W])
+{
+    using T = typename F::T;
+#pragma unroll
+    for (int c = 0; c < W; c += 4) {
+        T a = s[c], b = s[c + 1], cc = s[c + 2], d = s[c + 3];
+        T ab = F::add(a, b), cd = F::add(cc, d);
+        T t = F::add(ab, cd);
+        T tb = F::add(t, b), td = F::add(t, d);
+        s[c] = F::add(tb, ab);               // 2a + 3b + c + d
+        s[c + 1] = F::add(tb, F::dbl(cc));   // a + 2b + 3c + d
+        s[c + 2] = F::add(td, cd);           // a + b + 2c + 3d
+        s[c + 3] = F::add(td, F::dbl(a));    // 3a + b + c + 2d
+    }
+    if (W > 4) {
+        T sum[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            sum[k] = s[k];
+#pragma unroll
+            for (int c = 4; c < W; c += 4) sum[k] = F::add(sum[k], s[c + k]);
+        }
+#pragma unroll
+        for (int i = 0; i < W; i++) s[i] = F::add(s[i], sum[i & 3]);
+    }
+}
+
+// Generic internal diffusion: s_i = d_i * s_i + sum (d in register form).
+template <class F, int W>
+__device__ __forceinline__ void internal_diag_generic(typename F::T (&s)[W], typename F::T sum,
+                                                      const typename F::T *__restrict__ diag)
+{
+#pragma unroll
+    for (int i = 0; i < W; i++) s[i] = F::add(F::mul(s[i], diag[i]), sum);
+}
+
+// BabyBear width 16: V = [-2, 1, 2, 1/2, 3, 4, -1/2, -3, -4, 2^-8, 1/4, 1/8,
+// 2^-27, -2^-8, -1/16, -2^-27] (zkf-metal/src/poseidon2/babybear.rs:14-58),
+// applied with doublings and exact power-of-two divisions instead of products.
+__device__ __forceinline__ void internal_diag_bb16(uint32_t (&s)[16], uint32_t sum)
+{
+    using F = Bb;
+    s[0] = F::sub(sum, F::dbl(s[0]));
+    s[1] = F::add(sum, s[1]);
+    s[2] = F::add(sum, F::dbl(s[2]));
+    s[3] = F::add(sum, F::mul_inv_pow2<1>(s[3]));
+    s[4] = F::add(sum, F::add(F::dbl(s[4]), s[4]));
+    s[5] = F::add(sum, F::dbl(F::dbl(s[5])));
+    s[6] = F::sub(sum, F::mul_inv_pow2<1>(s[6]));
+    s[7] = F::sub(sum, F::add(F::dbl(s[7]), s[7]));
+    s[8] = F::sub(sum, F::dbl(F::dbl(s[8])));
+    s[9] = F::add(sum, F::mul_inv_pow2<8>(s[9]));
+    s[10] = F::add(sum, F::mul_inv_pow2<2>(s[10]));
+    s[11] = F::add(sum, F::mul_inv_pow2<3>(s[11]));
+    s[12] = F::add(sum, F::mul_inv_pow2<27>(s[12]));
+    s[13] = F::sub(sum, F::mul_inv_pow2<8>(s[13]));
+    s[14] = F::sub(sum, F::mul_inv_pow2<4>(s[14]));
+    s[15] = F::sub(sum, F::mul_inv_pow2<27>(s[15]));
+}
+
+template <class F, int W>
+struct P2Spec;
+template <>
+struct P2Spec<Gl, 16> {
+    static constexpr int RF = 8, RP = 22;
+};
+template <>
+struct P2Spec<Gl, 8> {
+    static constexpr int RF = 8, RP = 22;
+};
+template <>
+struct P2Spec<Bb, 16> {
+    static constexpr int RF = 8, RP = 13;
+};
+
+template <class F, int W>
+__device__ __forceinline__ void external_round(typename F::T (&s)[W], const typename F::T *__restrict__ rc)
+{
+#pragma unroll
+    for (int i = 0; i < W; i++) s[i] = sbox7<F>(F::add(s[i], rc[i]));
+    mds_light<F, W>(s);
+}
+
+template <class F, int W>
+__device__ __forceinline__ void p2_permute(typename F::T (&s)[W], const typename F::T *__restrict__ rc,
+                                           const typename F::T *__restrict__ diag)
+{
+    using T = typename F::T;
+    constexpr int HALF = P2Spec<F, W>::RF / 2, RP = P2Spec<F, W>::RP;
+    mds_light<F, W>(s);
+#pragma unroll 1
+    for (int r = 0; r < HALF; r++) external_round<F, W>(s, rc + r * W);
+    const T *ir = rc + HALF * W;
+#pragma unroll 1
+    for (int r = 0; r < RP; r++) {
+        s[0] = sbox7<F>(F::add(s[0], ir[r]));
+        T sum = s[0];
+#pragma unroll
+        for (int i = 1; i < W; i++) sum = F::add(sum, s[i]);
+        if constexpr (std::is_same<F, Bb>::value && W == 16)
+            internal_diag_bb16(s, sum);
+        else
+            internal_diag_generic<F, W>(s, sum, diag);
+    }
+    const T *tr = ir + RP;
+#pragma unroll 1
+    for (int r = 0; r < HALF; r++) external_round<F, W>(s, tr + r * W);
+}
+
+}  // namespace zkg

@@ -1,0 +1,97 @@
+// zkg_runtime.h -- handle layouts and internal helpers of the libzkg runtime.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <atomic>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <tuple>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/zkg.h"
+#include "zkg_internal.h"
+
+namespace zkg {
+
+struct CosetKey {
+    int field, log_n, log_b;
+    uint64_t shift;
+    bool operator<(const CosetKey &o) const
+    {
+        return std::tie(field, log_n, log_b, shift) < std::tie(o.field, o.log_n, o.log_b, o.shift);
+    }
+};
+
+struct BatchState;  // zkg_batch.hip
+struct ServeState;  // zkg_batch.hip
+
+}  // namespace zkg
+
+struct zkg_ctx {
+    int device = 0;
+    uint64_t hbm_budget = 0;
+    uint64_t hbm_used = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mem_mu, table_mu;
+    std::unordered_map<void *, uint64_t> alloc_sizes;
+    zkg::TwiddleSet tw[2];
+    zkg::P2Set p2[3];  // GL-16, GL-8, BB-16
+    std::map<zkg::CosetKey, zkg::CosetSet> cosets;
+    bool timing = false;
+    hipEvent_t ev[8] = {};
+    double stage_ms[4] = {0, 0, 0, 0};
+    zkg::BatchState *batch = nullptr;
+    zkg::ServeState *serve = nullptr;
+};
+
+struct zkg_buf {
+    zkg_ctx *ctx;
+    void *ptr;
+    uint64_t bytes;
+};
+
+struct zkg_tree {
+    zkg_ctx *ctx = nullptr;
+    int field = 0, p2_width = 16, digest = 8;
+    uint64_t n_leaves = 0;
+    int nlayers = 0;
+    const void *mat = nullptr;  // leaf matrix (not owned)
+    uint32_t mat_width = 0;
+    void *base = nullptr;  // one allocation for every digest layer
+    std::vector<uint64_t> layer_off;
+    std::vector<void *> layers;
+};
+
+struct zkg_fri {
+    zkg_ctx *ctx = nullptr;
+    uint32_t log_len = 0, log_final = 0;
+    std::vector<void *> codewords;  // layer r codeword (2^(log_len - r) EF4), last = final
+    std::vector<zkg_tree *> trees;  // layer r tree over width-2 rows of codeword r
+};
+
+namespace zkg {
+
+int set_err(int code, const char *fmt, ...);
+int hip_err(hipError_t e, const char *what);
+int dev_alloc(zkg_ctx *ctx, uint64_t bytes, void **out);
+void dev_free(zkg_ctx *ctx, void *p);
+int get_coset(zkg_ctx *ctx, int field, int log_n, int log_b, uint64_t shift, const CosetSet **out);
+const P2Set *get_p2(zkg_ctx *ctx, int field, int width);
+int run_lde(zkg_ctx *ctx, hipStream_t st, int field, const void *in, uint64_t height, uint32_t width,
+            uint32_t log_blowup, uint64_t shift, void *out, void *tmp);
+int tree_alloc(zkg_ctx *ctx, int field, int p2_width, uint64_t n_leaves, const void *mat, uint32_t mat_width,
+               zkg_tree **out);
+int tree_build(zkg_ctx *ctx, hipStream_t st, zkg_tree *t, bool plain);
+int tree_open(const zkg_tree *t, const uint64_t *idx, uint32_t n, void *rows_out, void *paths_out);
+void stage_mark(zkg_ctx *ctx, hipStream_t st, int i);
+void stage_collect(zkg_ctx *ctx, int nmarks);
+void batch_release(zkg_ctx *ctx);
+void serve_shutdown(zkg_ctx *ctx);
+
+}  // namespace zkg

@@ -1,0 +1,1084 @@
+// zkg_runtime.hip -- host runtime behind include/zkg.h: contexts, HBM
+// accounting, resident constant tables, trees, the commit pipeline, FRI commit
+// phase, batched commits and the serving loop.
+//
+// Reference counterparts: MetalContext (zkf-metal/src/device.rs:250+,
+// 1469-1480), GpuBufferAllocator (zkf-runtime/src/buffer_bridge.rs:77-84),
+// MetalMerkleTreeMmcs (zkf-metal/src/merkle.rs:32-443), MetalFri
+// (zkf-metal/src/fri.rs:17-147), BatchProver (zkf-metal/src/batch_prover.rs:
+// 95-360).  Unlike the reference, twiddles and round constants stay resident
+// in HBM for the context's lifetime, and a commit is one stream-ordered
+// sequence of launches with no host round trip.
+#include "../../include/zkg.h"
+#include "zkg_field.h"
+#include "zkg_internal.h"
+#include "zkg_runtime.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using namespace zkg;
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+
+int zkg::set_err(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int zkg::hip_err(hipError_t e, const char *what)
+{
+    return set_err(e == hipErrorOutOfMemory ? ZKG_ENOMEM : ZKG_EDEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                       \
+    do {                                          \
+        hipError_t _e = (expr);                   \
+        if (_e != hipSuccess) return hip_err(_e, what); \
+    } while (0)
+
+extern "C" const char *zkg_last_error(void) { return g_err.c_str(); }
+extern "C" int zkg_abi_version(void) { return ZKG_ABI_VERSION; }
+
+// ------------------------------------------------------------------ host field helpers
+namespace {
+
+template <class F>
+typename F::T hpow(typename F::T a, uint64_t e)
+{
+    typename F::T r = F::one();
+    while (e) {
+        if (e & 1) r = F::mul(r, a);
+        a = F::mul(a, a);
+        e >>= 1;
+    }
+    return r;
+}
+
+// omega of order 2^log_n in register form (p3 two_adic_generator: the order
+// 2^adicity root squared down).
+template <class F>
+typename F::T root_of_unity(int log_n)
+{
+    if constexpr (sizeof(typename F::T) == 8) {
+        uint64_t w = 0x185629dcda58878cULL;
+        for (int i = log_n; i < 32; i++) w = Gl::mul(w, w);
+        return w;
+    } else {
+        uint32_t w = Bb::from_canon(0x1a427a41u);
+        for (int i = log_n; i < 27; i++) w = Bb::mul(w, w);
+        return w;
+    }
+}
+
+template <class F>
+typename F::T finv(typename F::T a)
+{
+    return hpow<F>(a, (uint64_t)F::P - 2);
+}
+
+int ilog2u(uint64_t n)
+{
+    int l = 0;
+    while ((1ull << l) < n) l++;
+    return l;
+}
+
+bool is_pow2(uint64_t n) { return n && !(n & (n - 1)); }
+
+uint32_t rev_bits(uint32_t x, int bits)
+{
+    uint32_t r = 0;
+    for (int i = 0; i < bits; i++) {
+        r = (r << 1) | (x & 1);
+        x >>= 1;
+    }
+    return r;
+}
+
+// rand 0.9.2 SmallRng (Xoshiro256++, SplitMix64 seeding) -- the generator the
+// reference feeds to Poseidon2::new_from_rng_128 (zkf-backends/src/plonky3.rs:
+// 606-620).
+struct SmallRng {
+    uint64_t s[4];
+    explicit SmallRng(uint64_t st)
+    {
+        for (int i = 0; i < 4; i++) {
+            st += 0x9e3779b97f4a7c15ULL;
+            uint64_t z = st;
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+            z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+            s[i] = z ^ (z >> 31);
+        }
+    }
+    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    uint64_t next_u64()
+    {
+        uint64_t out = rotl(s[0] + s[3], 23) + s[0];
+        uint64_t t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        return out;
+    }
+    uint32_t next_u32() { return (uint32_t)(next_u64() >> 32); }
+};
+
+// p3 StandardUniform sampling, returned in canonical form.
+//   Goldilocks: next_u64 rejected at >= p.
+//   BabyBear (MontyField31): next_u32 >> 1 rejected at >= p, taken as the
+//   Montgomery form, so the canonical value is v * 2^-32.
+uint64_t sample_canon(SmallRng &r, int field)
+{
+    if (field == FIELD_GL) {
+        for (;;) {
+            uint64_t v = r.next_u64();
+            if (v < Gl::P) return v;
+        }
+    }
+    for (;;) {
+        uint32_t v = r.next_u32() >> 1;
+        if (v < Bb::P) return Bb::to_canon(v);  // REDC(v) = v * 2^-32
+    }
+}
+
+const uint64_t GL_DIAG16[16] = {
+    0xde9b91a467d6afc0ULL, 0xc5f16b9c76a9be17ULL, 0x0ab0fef2d540ac55ULL, 0x3001d27009d05773ULL,
+    0xed23b1f906d3d9ebULL, 0x5ce73743cba97054ULL, 0x1c3bab944af4ba24ULL, 0x2faa105854dbafaeULL,
+    0x53ffb3ae6d421a10ULL, 0xbcda9df8884ba396ULL, 0xfc1273e4a31807bbULL, 0xc77952573d5142c0ULL,
+    0x56683339a819b85eULL, 0x328fcbd8f0ddc8ebULL, 0xb5101e303fce9cb7ULL, 0x774487b8c40089bbULL,
+};
+// p3-goldilocks MATRIX_DIAG_8_GOLDILOCKS (not in the reference tree).
+const uint64_t GL_DIAG8[8] = {
+    0xa98811a1fed4e3a5ULL, 0x1cc48b54f377e2a0ULL, 0xe40cd4f6c5609a26ULL, 0x11de79ebca97a4a3ULL,
+    0x9177c73d8b7e929cULL, 0x2a6fe8085797e791ULL, 0x3de6e93329f8d5adULL, 0x3f7af9125da962feULL,
+};
+
+int p2_index(int field, int width)
+{
+    if (field == FIELD_GL && width == 16) return 0;
+    if (field == FIELD_GL && width == 8) return 1;
+    if (field == FIELD_BB && width == 16) return 2;
+    return -1;
+}
+
+int rounds_p_of(int field) { return field == FIELD_GL ? 22 : 13; }
+
+}  // namespace
+
+// ------------------------------------------------------------------ context internals
+int zkg::dev_alloc(zkg_ctx *ctx, uint64_t bytes, void **out)
+{
+    std::lock_guard<std::mutex> g(ctx->mem_mu);
+    if (ctx->hbm_used + bytes > ctx->hbm_budget)
+        return set_err(ZKG_ENOMEM, "HBM budget exceeded: %llu + %llu > %llu", (unsigned long long)ctx->hbm_used,
+                       (unsigned long long)bytes, (unsigned long long)ctx->hbm_budget);
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 256);
+    if (e != hipSuccess) return hip_err(e, "hipMalloc");
+    ctx->hbm_used += bytes;
+    ctx->alloc_sizes[p] = bytes;
+    *out = p;
+    return ZKG_OK;
+}
+
+void zkg::dev_free(zkg_ctx *ctx, void *p)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> g(ctx->mem_mu);
+    auto it = ctx->alloc_sizes.find(p);
+    if (it != ctx->alloc_sizes.end()) {
+        ctx->hbm_used -= it->second;
+        ctx->alloc_sizes.erase(it);
+    }
+    (void)hipFree(p);
+}
+
+namespace {
+
+template <class F>
+int upload_vec(zkg_ctx *ctx, const std::vector<typename F::T> &v, void **out)
+{
+    int rc = dev_alloc(ctx, v.size() * sizeof(typename F::T), out);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(*out, v.data(), v.size() * sizeof(typename F::T), hipMemcpyHostToDevice), "upload table");
+    return ZKG_OK;
+}
+
+template <class F>
+int build_twiddles(zkg_ctx *ctx, TwiddleSet &ts)
+{
+    using T = typename F::T;
+    const int half_tile = 1 << (LOG_TW_TILE - 1);
+    T w = root_of_unity<F>(LOG_TW_TILE), wi = finv<F>(w);
+    std::vector<T> tf(half_tile), ti(half_tile);
+    T a = F::one(), b = F::one();
+    for (int j = 0; j < half_tile; j++) {
+        tf[j] = a;
+        ti[j] = b;
+        a = F::mul(a, w);
+        b = F::mul(b, wi);
+    }
+    T top = root_of_unity<F>(LOG_TW_TOP), topi = finv<F>(top);
+    const int nlo = 1 << TW_SPLIT, nhi = 1 << (LOG_TW_TOP - TW_SPLIT);
+    std::vector<T> lf(nlo), li(nlo), hf(nhi), hi(nhi);
+    T x = F::one(), y = F::one();
+    for (int i = 0; i < nlo; i++) {
+        lf[i] = x;
+        li[i] = y;
+        x = F::mul(x, top);
+        y = F::mul(y, topi);
+    }
+    // x = top^4096 now
+    T step = x, stepi = y;
+    x = F::one();
+    y = F::one();
+    for (int i = 0; i < nhi; i++) {
+        hf[i] = x;
+        hi[i] = y;
+        x = F::mul(x, step);
+        y = F::mul(y, stepi);
+    }
+    int rc;
+    if ((rc = upload_vec<F>(ctx, tf, &ts.tile_fwd))) return rc;
+    if ((rc = upload_vec<F>(ctx, ti, &ts.tile_inv))) return rc;
+    if ((rc = upload_vec<F>(ctx, lf, &ts.top_lo_fwd))) return rc;
+    if ((rc = upload_vec<F>(ctx, hf, &ts.top_hi_fwd))) return rc;
+    if ((rc = upload_vec<F>(ctx, li, &ts.top_lo_inv))) return rc;
+    if ((rc = upload_vec<F>(ctx, hi, &ts.top_hi_inv))) return rc;
+    return ZKG_OK;
+}
+
+template <class F>
+int build_coset(zkg_ctx *ctx, int log_n, int log_b, uint64_t shift_canon, CosetSet &cs)
+{
+    using T = typename F::T;
+    const int B = 1 << log_b, nlo = 1 << TW_SPLIT, nhi = 1 << (LOG_TW_TOP - TW_SPLIT - 2);
+    std::vector<T> lo((size_t)B * nlo), hi((size_t)B * nhi);
+    T shift = F::from_canon((T)shift_canon);
+    T wbig = root_of_unity<F>(log_n + log_b);
+    // 1/N
+    T ninv = finv<F>(F::from_canon((T)(1ull << log_n)));
+    for (int b = 0; b < B; b++) {
+        T sb = F::mul(shift, hpow<F>(wbig, rev_bits((uint32_t)b, log_b)));
+        T acc = ninv;
+        for (int i = 0; i < nlo; i++) {
+            lo[(size_t)b * nlo + i] = acc;
+            acc = F::mul(acc, sb);
+        }
+        T step = hpow<F>(sb, (uint64_t)nlo), h = F::one();
+        for (int i = 0; i < nhi; i++) {
+            hi[(size_t)b * nhi + i] = h;
+            h = F::mul(h, step);
+        }
+    }
+    int rc;
+    if ((rc = upload_vec<F>(ctx, lo, &cs.lo))) return rc;
+    if ((rc = upload_vec<F>(ctx, hi, &cs.hi))) return rc;
+    cs.log_n = log_n;
+    cs.log_blowup = log_b;
+    return ZKG_OK;
+}
+
+}  // namespace
+
+int zkg::get_coset(zkg_ctx *ctx, int field, int log_n, int log_b, uint64_t shift, const CosetSet **out)
+{
+    std::lock_guard<std::mutex> g(ctx->table_mu);
+    CosetKey key{field, log_n, log_b, shift};
+    auto it = ctx->cosets.find(key);
+    if (it == ctx->cosets.end()) {
+        CosetSet cs;
+        int rc = field == FIELD_GL ? build_coset<Gl>(ctx, log_n, log_b, shift, cs)
+                                   : build_coset<Bb>(ctx, log_n, log_b, shift, cs);
+        if (rc) return rc;
+        it = ctx->cosets.emplace(key, cs).first;
+    }
+    *out = &it->second;
+    return ZKG_OK;
+}
+
+const P2Set *zkg::get_p2(zkg_ctx *ctx, int field, int width)
+{
+    int i = p2_index(field, width);
+    if (i < 0 || !ctx->p2[i].loaded) return nullptr;
+    return &ctx->p2[i];
+}
+
+namespace {
+
+template <class F>
+int load_p2(zkg_ctx *ctx, int idx, int width, const std::vector<uint64_t> &flat_canon, const uint64_t *diag_canon)
+{
+    using T = typename F::T;
+    std::vector<T> rc(flat_canon.size()), dg(width);
+    for (size_t i = 0; i < flat_canon.size(); i++) rc[i] = F::from_canon((T)flat_canon[i]);
+    for (int i = 0; i < width; i++) dg[i] = F::from_canon((T)diag_canon[i]);
+    P2Set &p = ctx->p2[idx];
+    dev_free(ctx, p.rc);
+    dev_free(ctx, p.diag);
+    p.rc = p.diag = nullptr;
+    int r;
+    if ((r = upload_vec<F>(ctx, rc, &p.rc))) return r;
+    if ((r = upload_vec<F>(ctx, dg, &p.diag))) return r;
+    p.loaded = true;
+    return ZKG_OK;
+}
+
+void bb_diag16(uint64_t *d)
+{
+    const uint64_t p = Bb::P;
+    auto canon_pow_inv2 = [](int e) -> uint64_t {
+        uint32_t inv2 = Bb::from_canon((Bb::P + 1) / 2);
+        return Bb::to_canon(hpow<Bb>(inv2, e));
+    };
+    uint64_t i1 = canon_pow_inv2(1);
+    uint64_t v[16] = {p - 2, 1, 2, i1, 3, 4, p - i1, p - 3, p - 4, canon_pow_inv2(8), canon_pow_inv2(2),
+                      canon_pow_inv2(3), canon_pow_inv2(27), p - canon_pow_inv2(8), p - canon_pow_inv2(4),
+                      p - canon_pow_inv2(27)};
+    memcpy(d, v, sizeof v);
+}
+
+int load_p2_flat(zkg_ctx *ctx, int field, int width, const std::vector<uint64_t> &flat)
+{
+    int idx = p2_index(field, width);
+    uint64_t diag[16];
+    if (field == FIELD_GL)
+        memcpy(diag, width == 16 ? GL_DIAG16 : GL_DIAG8, width * sizeof(uint64_t));
+    else
+        bb_diag16(diag);
+    return field == FIELD_GL ? load_p2<Gl>(ctx, idx, width, flat, diag) : load_p2<Bb>(ctx, idx, width, flat, diag);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ context API
+extern "C" int zkg_ctx_create(int device, uint64_t budget, zkg_ctx **out)
+{
+    if (!out) return set_err(ZKG_EINVAL, "null out");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+    if (device < 0 || device >= ndev) return set_err(ZKG_EINVAL, "device %d not present (%d devices)", device, ndev);
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    zkg_ctx *c = new zkg_ctx();
+    c->device = device;
+    c->hbm_budget = budget ? budget : (288ull << 30);
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return hip_err(e, "hipStreamCreate");
+    }
+    for (int i = 0; i < 8; i++) (void)hipEventCreate(&c->ev[i]);
+    int rc = build_twiddles<Gl>(c, c->tw[FIELD_GL]);
+    if (!rc) rc = build_twiddles<Bb>(c, c->tw[FIELD_BB]);
+    if (rc) {
+        zkg_ctx_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return ZKG_OK;
+}
+
+extern "C" int zkg_ctx_destroy(zkg_ctx *ctx)
+{
+    if (!ctx) return ZKG_OK;
+    serve_shutdown(ctx);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    batch_release(ctx);
+    std::vector<void *> ptrs;
+    for (auto &kv : ctx->alloc_sizes) ptrs.push_back(kv.first);
+    for (void *p : ptrs) dev_free(ctx, p);
+    for (int i = 0; i < 8; i++) (void)hipEventDestroy(ctx->ev[i]);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return ZKG_OK;
+}
+
+extern "C" int zkg_ctx_synchronize(zkg_ctx *ctx)
+{
+    if (!ctx) return set_err(ZKG_EINVAL, "null ctx");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return ZKG_OK;
+}
+
+extern "C" int zkg_ctx_hbm_used(const zkg_ctx *ctx, uint64_t *bytes)
+{
+    if (!ctx || !bytes) return set_err(ZKG_EINVAL, "null argument");
+    *bytes = ctx->hbm_used;
+    return ZKG_OK;
+}
+
+extern "C" int zkg_ctx_set_timing(zkg_ctx *ctx, int enable)
+{
+    if (!ctx) return set_err(ZKG_EINVAL, "null ctx");
+    ctx->timing = enable != 0;
+    return ZKG_OK;
+}
+
+extern "C" int zkg_ctx_stage_ms(const zkg_ctx *ctx, double *ms4)
+{
+    if (!ctx || !ms4) return set_err(ZKG_EINVAL, "null argument");
+    for (int i = 0; i < 4; i++) ms4[i] = ctx->stage_ms[i];
+    return ZKG_OK;
+}
+
+// ------------------------------------------------------------------ buffers
+extern "C" int zkg_buf_alloc(zkg_ctx *ctx, uint64_t bytes, zkg_buf **out)
+{
+    if (!ctx || !out) return set_err(ZKG_EINVAL, "null argument");
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    void *p;
+    int rc = dev_alloc(ctx, bytes, &p);
+    if (rc) return rc;
+    zkg_buf *b = new zkg_buf{ctx, p, bytes};
+    *out = b;
+    return ZKG_OK;
+}
+
+extern "C" int zkg_buf_free(zkg_buf *b)
+{
+    if (!b) return ZKG_OK;
+    (void)hipStreamSynchronize(b->ctx->stream);
+    dev_free(b->ctx, b->ptr);
+    delete b;
+    return ZKG_OK;
+}
+
+extern "C" int zkg_buf_upload(zkg_buf *b, uint64_t off, const void *src, uint64_t bytes)
+{
+    if (!b || (!src && bytes)) return set_err(ZKG_EINVAL, "null argument");
+    if (off + bytes > b->bytes) return set_err(ZKG_EINVAL, "upload of %llu bytes at %llu exceeds buffer of %llu",
+                                               (unsigned long long)bytes, (unsigned long long)off,
+                                               (unsigned long long)b->bytes);
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream), "sync");
+    HIP_TRY(hipMemcpy((char *)b->ptr + off, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+    return ZKG_OK;
+}
+
+extern "C" int zkg_buf_download(const zkg_buf *b, uint64_t off, void *dst, uint64_t bytes)
+{
+    if (!b || (!dst && bytes)) return set_err(ZKG_EINVAL, "null argument");
+    if (off + bytes > b->bytes) return set_err(ZKG_EINVAL, "download exceeds buffer");
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream), "sync");
+    HIP_TRY(hipMemcpy(dst, (const char *)b->ptr + off, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    return ZKG_OK;
+}
+
+extern "C" int zkg_buf_size(const zkg_buf *b, uint64_t *bytes)
+{
+    if (!b || !bytes) return set_err(ZKG_EINVAL, "null argument");
+    *bytes = b->bytes;
+    return ZKG_OK;
+}
+
+static uint64_t esize(int field) { return field == FIELD_GL ? 8 : 4; }
+
+static int check_field(int field)
+{
+    if (field != FIELD_GL && field != FIELD_BB) return set_err(ZKG_EINVAL, "unknown field %d", field);
+    return ZKG_OK;
+}
+
+extern "C" int zkg_fill_random(zkg_ctx *ctx, int field, zkg_buf *buf, uint64_t seed, uint64_t start, uint64_t n)
+{
+    if (!ctx || !buf) return set_err(ZKG_EINVAL, "null argument");
+    if (int rc = check_field(field)) return rc;
+    if (n * esize(field) > buf->bytes) return set_err(ZKG_EINVAL, "buffer too small");
+    HIP_TRY(launch_gen(field, buf->ptr, seed, start, n, ctx->stream), "gen");
+    return ZKG_OK;
+}
+
+// ------------------------------------------------------------------ constants
+extern "C" int zkg_constants_load(zkg_ctx *ctx, int field, int width, uint64_t seed)
+{
+    if (!ctx) return set_err(ZKG_EINVAL, "null ctx");
+    if (p2_index(field, width) < 0) return set_err(ZKG_EINVAL, "unsupported Poseidon2 field %d width %d", field, width);
+    // RNG order of Poseidon2::new_from_rng_128 / flatten_round_constants:
+    // initial rf/2 x W, terminal rf/2 x W, internal rp; flat layout
+    // [initial][internal][terminal].
+    SmallRng r(seed);
+    const int half = 4, rp = rounds_p_of(field);
+    std::vector<uint64_t> init(half * width), term(half * width), internal(rp);
+    for (auto &v : init) v = sample_canon(r, field);
+    for (auto &v : term) v = sample_canon(r, field);
+    for (auto &v : internal) v = sample_canon(r, field);
+    std::vector<uint64_t> flat;
+    flat.insert(flat.end(), init.begin(), init.end());
+    flat.insert(flat.end(), internal.begin(), internal.end());
+    flat.insert(flat.end(), term.begin(), term.end());
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "sync");
+    int rc = load_p2_flat(ctx, field, width, flat);
+    if (!rc) ctx->p2[p2_index(field, width)].seed = seed;
+    return rc;
+}
+
+extern "C" int zkg_constants_set(zkg_ctx *ctx, int field, int width, const void *flat, uint32_t n_ext, uint32_t n_int)
+{
+    if (!ctx || !flat) return set_err(ZKG_EINVAL, "null argument");
+    if (p2_index(field, width) < 0) return set_err(ZKG_EINVAL, "unsupported Poseidon2 field %d width %d", field, width);
+    if (n_ext != 8 || (int)n_int != rounds_p_of(field))
+        return set_err(ZKG_EINVAL, "rounds (%u, %u) differ from the compiled schedule (8, %d)", n_ext, n_int,
+                       rounds_p_of(field));
+    size_t n = (size_t)n_ext * width + n_int;
+    std::vector<uint64_t> v(n);
+    for (size_t i = 0; i < n; i++) {
+        uint64_t x = field == FIELD_GL ? ((const uint64_t *)flat)[i] : ((const uint32_t *)flat)[i];
+        if (x >= (field == FIELD_GL ? Gl::P : (uint64_t)Bb::P)) return set_err(ZKG_EINVAL, "constant %zu not canonical", i);
+        v[i] = x;
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "sync");
+    return load_p2_flat(ctx, field, width, v);
+}
+
+// ------------------------------------------------------------------ transforms
+extern "C" int zkg_ntt_batch(zkg_ctx *ctx, int field, zkg_buf *buf, uint64_t height, uint32_t width, int inverse)
+{
+    if (!ctx || !buf) return set_err(ZKG_EINVAL, "null argument");
+    if (int rc = check_field(field)) return rc;
+    if (!is_pow2(height) || ilog2u(height) > 22) return set_err(ZKG_EINVAL, "height %llu not a power of two <= 2^22", (unsigned long long)height);
+    if (height * width * esize(field) > buf->bytes) return set_err(ZKG_EINVAL, "buffer too small");
+    if (height == 1 || width == 0) return ZKG_OK;
+    uint64_t tb = ntt_tmp_bytes(field, height, width);
+    void *tmp = nullptr;
+    if (tb) {
+        int rc = dev_alloc(ctx, tb, &tmp);
+        if (rc) return rc;
+    }
+    hipError_t e = launch_ntt(field, ctx->tw[field], buf->ptr, tmp, height, width, inverse, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dev_free(ctx, tmp);
+    if (e != hipSuccess) return hip_err(e, "ntt");
+    return ZKG_OK;
+}
+
+int zkg::run_lde(zkg_ctx *ctx, hipStream_t st, int field, const void *in, uint64_t height, uint32_t width,
+                 uint32_t log_blowup, uint64_t shift, void *out, void *tmp)
+{
+    const CosetSet *cs;
+    int rc = get_coset(ctx, field, ilog2u(height), log_blowup, shift, &cs);
+    if (rc) return rc;
+    HIP_TRY(launch_coset_lde(field, ctx->tw[field], *cs, in, out, tmp, height, width, log_blowup, st), "coset lde");
+    return ZKG_OK;
+}
+
+static int check_lde_shape(int field, uint64_t height, uint32_t width, uint32_t log_blowup)
+{
+    if (int rc = check_field(field)) return rc;
+    if (!is_pow2(height) || ilog2u(height) > 22)
+        return set_err(ZKG_EINVAL, "height %llu not a power of two <= 2^22", (unsigned long long)height);
+    if (log_blowup > 2 || ilog2u(height) + (int)log_blowup > LOG_TW_TOP)
+        return set_err(ZKG_EINVAL, "log_blowup %u unsupported (0..2)", log_blowup);
+    if (width == 0) return set_err(ZKG_EINVAL, "width 0");
+    return ZKG_OK;
+}
+
+extern "C" int zkg_coset_lde_batch(zkg_ctx *ctx, int field, const zkg_buf *in, uint64_t height, uint32_t width,
+                                   uint32_t log_blowup, uint64_t shift, zkg_buf *out)
+{
+    if (!ctx || !in || !out) return set_err(ZKG_EINVAL, "null argument");
+    if (int rc = check_lde_shape(field, height, width, log_blowup)) return rc;
+    uint64_t es = esize(field);
+    if (height * width * es > in->bytes || (height << log_blowup) * width * es > out->bytes)
+        return set_err(ZKG_EINVAL, "buffer too small");
+    if (shift >= (field == FIELD_GL ? Gl::P : (uint64_t)Bb::P) || shift == 0)
+        return set_err(ZKG_EINVAL, "shift must be a nonzero canonical element");
+    uint64_t tb = coset_lde_tmp_bytes(field, height, width, log_blowup);
+    void *tmp = nullptr;
+    if (tb) {
+        int rc = dev_alloc(ctx, tb, &tmp);
+        if (rc) return rc;
+    }
+    int rc = run_lde(ctx, ctx->stream, field, in->ptr, height, width, log_blowup, shift, out->ptr, tmp);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    dev_free(ctx, tmp);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_err(e, "lde sync");
+    return ZKG_OK;
+}
+
+// ------------------------------------------------------------------ Poseidon2
+extern "C" int zkg_poseidon2_permute(zkg_ctx *ctx, int field, int width, zkg_buf *states, uint64_t n)
+{
+    if (!ctx || !states) return set_err(ZKG_EINVAL, "null argument");
+    const P2Set *p2 = get_p2(ctx, field, width);
+    if (!p2) return set_err(ZKG_ESTATE, "Poseidon2 constants for field %d width %d not loaded", field, width);
+    if (n * width * esize(field) > states->bytes) return set_err(ZKG_EINVAL, "buffer too small");
+    HIP_TRY(launch_p2_permute(field, width, *p2, states->ptr, n, ctx->stream), "permute");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "sync");
+    return ZKG_OK;
+}
+
+template <class T>
+static int permute_host(zkg_ctx *ctx, int field, T *states, uint64_t n, const T *rc, uint32_t n_ext, uint32_t n_int)
+{
+    if (!ctx || (!states && n) || !rc) return set_err(ZKG_EINVAL, "null argument");
+    int r = zkg_constants_set(ctx, field, 16, rc, n_ext, n_int);
+    if (r) return r;
+    if (!n) return ZKG_OK;
+    void *d;
+    uint64_t bytes = n * 16 * sizeof(T);
+    if ((r = dev_alloc(ctx, bytes, &d))) return r;
+    hipError_t e = hipMemcpy(d, states, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_p2_permute(field, 16, *get_p2(ctx, field, 16), d, n, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(states, d, bytes, hipMemcpyDeviceToHost);
+    dev_free(ctx, d);
+    if (e != hipSuccess) return hip_err(e, "batch permute");
+    return ZKG_OK;
+}
+
+extern "C" int zkg_poseidon2_batch_permute_goldilocks(zkg_ctx *ctx, uint64_t *states, uint64_t n,
+                                                      const uint64_t *rc, uint32_t n_ext, uint32_t n_int)
+{
+    return permute_host<uint64_t>(ctx, FIELD_GL, states, n, rc, n_ext, n_int);
+}
+
+extern "C" int zkg_poseidon2_batch_permute_babybear(zkg_ctx *ctx, uint32_t *states, uint64_t n, const uint32_t *rc,
+                                                    uint32_t n_ext, uint32_t n_int)
+{
+    return permute_host<uint32_t>(ctx, FIELD_BB, states, n, rc, n_ext, n_int);
+}
+
+extern "C" int zkg_hash_rows(zkg_ctx *ctx, int field, int p2_width, const zkg_buf *mat, uint64_t height,
+                             uint32_t mat_width, zkg_buf *digests)
+{
+    if (!ctx || !mat || !digests) return set_err(ZKG_EINVAL, "null argument");
+    const P2Set *p2 = get_p2(ctx, field, p2_width);
+    if (!p2) return set_err(ZKG_ESTATE, "Poseidon2 constants for field %d width %d not loaded", field, p2_width);
+    uint64_t es = esize(field);
+    if (height * mat_width * es > mat->bytes || height * (p2_width / 2) * es > digests->bytes)
+        return set_err(ZKG_EINVAL, "buffer too small");
+    HIP_TRY(launch_hash_rows(field, p2_width, *p2, mat->ptr, height, mat_width, digests->ptr, ctx->stream), "hash rows");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "sync");
+    return ZKG_OK;
+}
+
+// ------------------------------------------------------------------ trees
+int zkg::tree_alloc(zkg_ctx *ctx, int field, int p2_width, uint64_t n_leaves, const void *mat, uint32_t mat_width,
+                    zkg_tree **out)
+{
+    zkg_tree *t = new zkg_tree();
+    t->ctx = ctx;
+    t->field = field;
+    t->p2_width = p2_width;
+    t->digest = p2_width / 2;
+    t->n_leaves = n_leaves;
+    t->nlayers = ilog2u(n_leaves) + 1;
+    t->mat = mat;
+    t->mat_width = mat_width;
+    uint64_t off = 0, es = esize(field);
+    for (int i = 0; i < t->nlayers; i++) {
+        t->layer_off.push_back(off);
+        off += (n_leaves >> i) * t->digest * es;
+        off = (off + 255) & ~255ull;
+    }
+    int rc = dev_alloc(ctx, off, &t->base);
+    if (rc) {
+        delete t;
+        return rc;
+    }
+    for (int i = 0; i < t->nlayers; i++) t->layers.push_back((char *)t->base + t->layer_off[i]);
+    *out = t;
+    return ZKG_OK;
+}
+
+int zkg::tree_build(zkg_ctx *ctx, hipStream_t st, zkg_tree *t, bool plain)
+{
+    const P2Set *p2 = get_p2(ctx, t->field, t->p2_width);
+    HIP_TRY(launch_hash_rows(t->field, t->p2_width, *p2, t->mat, t->n_leaves, t->mat_width, t->layers[0], st),
+            "hash rows");
+    if (plain) HIP_TRY(hipStreamSynchronize(st), "sync");
+    HIP_TRY(launch_merkle_tree(t->field, t->p2_width, *p2, t->layers.data(), t->nlayers, t->n_leaves, plain ? 1 : 9, st,
+                               plain),
+            "merkle");
+    return ZKG_OK;
+}
+
+extern "C" int zkg_merkle_commit(zkg_ctx *ctx, int field, int p2_width, const zkg_buf *mat, uint64_t height,
+                                 uint32_t width, zkg_tree **out)
+{
+    if (!ctx || !mat || !out) return set_err(ZKG_EINVAL, "null argument");
+    if (int rc = check_field(field)) return rc;
+    if (!get_p2(ctx, field, p2_width))
+        return set_err(ZKG_ESTATE, "Poseidon2 constants for field %d width %d not loaded", field, p2_width);
+    if (!is_pow2(height) || ilog2u(height) > 31) return set_err(ZKG_EINVAL, "height must be a power of two");
+    if (height * width * esize(field) > mat->bytes) return set_err(ZKG_EINVAL, "buffer too small");
+    zkg_tree *t;
+    int rc = tree_alloc(ctx, field, p2_width, height, mat->ptr, width, &t);
+    if (rc) return rc;
+    rc = tree_build(ctx, ctx->stream, t, false);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (!rc && e != hipSuccess) rc = hip_err(e, "merkle sync");
+    if (rc) {
+        zkg_tree_free(t);
+        return rc;
+    }
+    *out = t;
+    return ZKG_OK;
+}
+
+extern "C" int zkg_tree_root(const zkg_tree *t, void *root)
+{
+    if (!t || !root) return set_err(ZKG_EINVAL, "null argument");
+    HIP_TRY(hipStreamSynchronize(t->ctx->stream), "sync");
+    HIP_TRY(hipMemcpy(root, t->layers.back(), t->digest * esize(t->field), hipMemcpyDeviceToHost), "root");
+    return ZKG_OK;
+}
+
+extern "C" int zkg_tree_num_layers(const zkg_tree *t, uint32_t *n)
+{
+    if (!t || !n) return set_err(ZKG_EINVAL, "null argument");
+    *n = (uint32_t)t->nlayers;
+    return ZKG_OK;
+}
+
+extern "C" int zkg_tree_digest_elems(const zkg_tree *t, uint32_t *n)
+{
+    if (!t || !n) return set_err(ZKG_EINVAL, "null argument");
+    *n = (uint32_t)t->digest;
+    return ZKG_OK;
+}
+
+extern "C" int zkg_tree_layer(const zkg_tree *t, uint32_t layer, void *out, uint64_t cap)
+{
+    if (!t || !out) return set_err(ZKG_EINVAL, "null argument");
+    if ((int)layer >= t->nlayers) return set_err(ZKG_EINVAL, "layer %u of %d", layer, t->nlayers);
+    uint64_t bytes = (t->n_leaves >> layer) * t->digest * esize(t->field);
+    if (bytes > cap) return set_err(ZKG_EINVAL, "output too small");
+    HIP_TRY(hipStreamSynchronize(t->ctx->stream), "sync");
+    HIP_TRY(hipMemcpy(out, t->layers[layer], bytes, hipMemcpyDeviceToHost), "layer");
+    return ZKG_OK;
+}
+
+int zkg::tree_open(const zkg_tree *t, const uint64_t *idx, uint32_t n, void *rows_out, void *paths_out)
+{
+    if (!n) return ZKG_OK;
+    for (uint32_t i = 0; i < n; i++)
+        if (idx[i] >= t->n_leaves) return set_err(ZKG_EINVAL, "index %llu out of range %llu", (unsigned long long)idx[i],
+                                                  (unsigned long long)t->n_leaves);
+    zkg_ctx *ctx = t->ctx;
+    uint64_t es = esize(t->field);
+    uint64_t rb = (uint64_t)n * t->mat_width * es, pb = (uint64_t)n * (t->nlayers - 1) * t->digest * es;
+    void *d_idx, *d_rows, *d_paths;
+    int rc = dev_alloc(ctx, n * 8ull, &d_idx);
+    if (rc) return rc;
+    if ((rc = dev_alloc(ctx, rb, &d_rows))) {
+        dev_free(ctx, d_idx);
+        return rc;
+    }
+    if ((rc = dev_alloc(ctx, pb, &d_paths))) {
+        dev_free(ctx, d_idx);
+        dev_free(ctx, d_rows);
+        return rc;
+    }
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(d_idx, idx, n * 8ull, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = launch_gather_open(t->field, t->digest, t->mat, t->mat_width, t->layers.data(), t->nlayers,
+                               (const uint64_t *)d_idx, n, d_rows, d_paths, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess && rows_out && rb) e = hipMemcpy(rows_out, d_rows, rb, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && paths_out && pb) e = hipMemcpy(paths_out, d_paths, pb, hipMemcpyDeviceToHost);
+    dev_free(ctx, d_idx);
+    dev_free(ctx, d_rows);
+    dev_free(ctx, d_paths);
+    if (e != hipSuccess) return hip_err(e, "open");
+    return ZKG_OK;
+}
+
+extern "C" int zkg_tree_open(const zkg_tree *t, const uint64_t *idx, uint32_t n, void *rows_out, void *paths_out)
+{
+    if (!t || (!idx && n)) return set_err(ZKG_EINVAL, "null argument");
+    return tree_open(t, idx, n, rows_out, paths_out);
+}
+
+extern "C" int zkg_tree_free(zkg_tree *t)
+{
+    if (!t) return ZKG_OK;
+    (void)hipStreamSynchronize(t->ctx->stream);
+    dev_free(t->ctx, t->base);
+    delete t;
+    return ZKG_OK;
+}
+
+// ------------------------------------------------------------------ commit
+static uint64_t generator_of(int field) { return field == FIELD_GL ? 7 : 31; }
+
+void zkg::stage_mark(zkg_ctx *ctx, hipStream_t st, int i)
+{
+    if (ctx->timing) (void)hipEventRecord(ctx->ev[i], st);
+}
+
+void zkg::stage_collect(zkg_ctx *ctx, int nmarks)
+{
+    if (!ctx->timing) return;
+    (void)hipEventSynchronize(ctx->ev[nmarks - 1]);
+    for (int i = 0; i < 4; i++) ctx->stage_ms[i] = 0;
+    for (int i = 0; i + 1 < nmarks && i < 4; i++) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]);
+        ctx->stage_ms[i] = ms;
+    }
+}
+
+static int commit_common(zkg_ctx *ctx, int field, int p2_width, const zkg_buf *trace, uint64_t height,
+                         uint32_t width, uint32_t log_blowup, zkg_buf *lde_out, zkg_tree **out, bool plain)
+{
+    if (!ctx || !trace || !lde_out || !out) return set_err(ZKG_EINVAL, "null argument");
+    if (int rc = check_lde_shape(field, height, width, log_blowup)) return rc;
+    if (!get_p2(ctx, field, p2_width))
+        return set_err(ZKG_ESTATE, "Poseidon2 constants for field %d width %d not loaded", field, p2_width);
+    uint64_t es = esize(field), big = height << log_blowup;
+    if (height * width * es > trace->bytes || big * width * es > lde_out->bytes)
+        return set_err(ZKG_EINVAL, "buffer too small");
+    uint64_t tb = plain ? height * width * es : coset_lde_tmp_bytes(field, height, width, log_blowup);
+    void *tmp = nullptr;
+    int rc;
+    if (tb && (rc = dev_alloc(ctx, tb, &tmp))) return rc;
+    zkg_tree *t;
+    if ((rc = tree_alloc(ctx, field, p2_width, big, lde_out->ptr, width, &t))) {
+        dev_free(ctx, tmp);
+        return rc;
+    }
+    hipStream_t st = ctx->stream;
+    stage_mark(ctx, st, 0);
+    if (plain) {
+        hipError_t e = launch_lde_plain(field, ctx->tw[field], trace->ptr, lde_out->ptr, tmp, height, width, log_blowup,
+                                        generator_of(field), st, true);
+        if (e != hipSuccess) rc = hip_err(e, "plain lde");
+    } else {
+        rc = run_lde(ctx, st, field, trace->ptr, height, width, log_blowup, generator_of(field), lde_out->ptr, tmp);
+    }
+    stage_mark(ctx, st, 1);
+    if (!rc) {
+        const P2Set *p2 = get_p2(ctx, field, p2_width);
+        hipError_t e = launch_hash_rows(field, p2_width, *p2, t->mat, big, width, t->layers[0], st);
+        if (e == hipSuccess && plain) e = hipStreamSynchronize(st);
+        stage_mark(ctx, st, 2);
+        if (e == hipSuccess)
+            e = launch_merkle_tree(field, p2_width, *p2, t->layers.data(), t->nlayers, big, plain ? 1 : 9, st, plain);
+        stage_mark(ctx, st, 3);
+        if (e != hipSuccess) rc = hip_err(e, "tree");
+    }
+    hipError_t e = hipStreamSynchronize(st);
+    if (!rc && e != hipSuccess) rc = hip_err(e, "commit sync");
+    stage_collect(ctx, 4);
+    dev_free(ctx, tmp);
+    if (rc) {
+        zkg_tree_free(t);
+        return rc;
+    }
+    *out = t;
+    return ZKG_OK;
+}
+
+extern "C" int zkg_commit(zkg_ctx *ctx, int field, int p2_width, const zkg_buf *trace, uint64_t height, uint32_t width,
+                          uint32_t log_blowup, zkg_buf *lde_out, zkg_tree **out)
+{
+    return commit_common(ctx, field, p2_width, trace, height, width, log_blowup, lde_out, out, false);
+}
+
+extern "C" int zkg_commit_plain(zkg_ctx *ctx, int field, int p2_width, const zkg_buf *trace, uint64_t height,
+                                uint32_t width, uint32_t log_blowup, zkg_buf *lde_out, zkg_tree **out)
+{
+    return commit_common(ctx, field, p2_width, trace, height, width, log_blowup, lde_out, out, true);
+}
+
+// ------------------------------------------------------------------ FRI
+extern "C" int zkg_fri_fold(zkg_ctx *ctx, int field, const zkg_buf *evals, uint64_t n_out, uint64_t alpha,
+                            const zkg_buf *itw, zkg_buf *out)
+{
+    if (!ctx || !evals || !itw || !out) return set_err(ZKG_EINVAL, "null argument");
+    if (int rc = check_field(field)) return rc;
+    uint64_t es = esize(field);
+    if (2 * n_out * es > evals->bytes || n_out * es > itw->bytes || n_out * es > out->bytes)
+        return set_err(ZKG_EINVAL, "buffer too small");
+    if (alpha >= (field == FIELD_GL ? Gl::P : (uint64_t)Bb::P)) return set_err(ZKG_EINVAL, "alpha not canonical");
+    HIP_TRY(launch_fri_fold_explicit(field, evals->ptr, n_out, alpha, itw->ptr, out->ptr, ctx->stream), "fri fold");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "sync");
+    return ZKG_OK;
+}
+
+extern "C" int zkg_fri_fold_matrix(zkg_ctx *ctx, int ext, int field, const zkg_buf *evals, uint64_t n_out,
+                                   const void *beta, zkg_buf *out)
+{
+    if (!ctx || !evals || !beta || !out) return set_err(ZKG_EINVAL, "null argument");
+    if (ext) field = FIELD_BB;
+    if (int rc = check_field(field)) return rc;
+    if (!is_pow2(n_out) || ilog2u(n_out) + 1 > LOG_TW_TOP) return set_err(ZKG_EINVAL, "n_out must be a power of two <= 2^23");
+    uint64_t es = ext ? 16 : esize(field);
+    if (2 * n_out * es > evals->bytes || n_out * es > out->bytes) return set_err(ZKG_EINVAL, "buffer too small");
+    uint32_t br[4] = {0, 0, 0, 0};
+    if (ext) {
+        for (int i = 0; i < 4; i++) {
+            uint32_t v = ((const uint32_t *)beta)[i];
+            if (v >= Bb::P) return set_err(ZKG_EINVAL, "beta not canonical");
+            br[i] = Bb::from_canon(v);
+        }
+    } else if (field == FIELD_GL) {
+        uint64_t v = *(const uint64_t *)beta;
+        if (v >= Gl::P) return set_err(ZKG_EINVAL, "beta not canonical");
+        br[0] = (uint32_t)v;
+        br[1] = (uint32_t)(v >> 32);
+    } else {
+        uint64_t v = *(const uint64_t *)beta;
+        if (v >= Bb::P) return set_err(ZKG_EINVAL, "beta not canonical");
+        br[0] = Bb::from_canon((uint32_t)v);
+    }
+    P2Set none;
+    HIP_TRY(launch_fri_fold(ext, field, ctx->tw[field], none, evals->ptr, n_out, br, out->ptr, nullptr, ctx->stream),
+            "fold");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "sync");
+    return ZKG_OK;
+}
+
+extern "C" int zkg_fri_commit_phase(zkg_ctx *ctx, const zkg_buf *cw, uint32_t log_len, uint32_t log_final,
+                                    const uint32_t *betas, zkg_fri **out)
+{
+    if (!ctx || !cw || !betas || !out) return set_err(ZKG_EINVAL, "null argument");
+    if (log_final >= log_len || log_len > 23 || log_final < 1)
+        return set_err(ZKG_EINVAL, "need 1 <= log_final < log_len <= 23");
+    if ((16ull << log_len) > cw->bytes) return set_err(ZKG_EINVAL, "codeword buffer too small");
+    const P2Set *p2 = get_p2(ctx, FIELD_BB, 16);
+    if (!p2) return set_err(ZKG_ESTATE, "BabyBear Poseidon2-16 constants not loaded");
+    const int nl = (int)(log_len - log_final);
+    zkg_fri *f = new zkg_fri();
+    f->ctx = ctx;
+    f->log_len = log_len;
+    f->log_final = log_final;
+    int rc = ZKG_OK;
+    hipStream_t st = ctx->stream;
+    // codewords of layers 1..nl (layer 0 is the caller's buffer, copied so
+    // openings stay valid after the caller reuses it)
+    for (int r = 0; r <= nl && !rc; r++) {
+        void *p;
+        rc = dev_alloc(ctx, 16ull << (log_len - r), &p);
+        if (!rc) f->codewords.push_back(p);
+    }
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(f->codewords[0], cw->ptr, 16ull << log_len, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) rc = hip_err(e, "copy codeword");
+    }
+    for (int r = 0; r < nl && !rc; r++) {
+        zkg_tree *t;
+        uint64_t leaves = 1ull << (log_len - 1 - r);
+        rc = tree_alloc(ctx, FIELD_BB, 16, leaves, f->codewords[r], 8, &t);
+        if (rc) break;
+        f->trees.push_back(t);
+    }
+    stage_mark(ctx, st, 0);
+    if (!rc) {
+        hipError_t e = launch_hash_rows(FIELD_BB, 16, *p2, f->codewords[0], 1ull << (log_len - 1), 8,
+                                        f->trees[0]->layers[0], st);
+        for (int r = 0; r < nl && e == hipSuccess; r++) {
+            zkg_tree *t = f->trees[r];
+            e = launch_merkle_tree(FIELD_BB, 16, *p2, t->layers.data(), t->nlayers, t->n_leaves, 9, st, false);
+            if (e != hipSuccess) break;
+            uint32_t br[4];
+            for (int i = 0; i < 4; i++) {
+                uint32_t v = betas[4 * r + i];
+                br[i] = Bb::from_canon(v >= Bb::P ? v - Bb::P : v);
+            }
+            uint64_t n_out = 1ull << (log_len - 1 - r);
+            void *next_digests = (r + 1 < nl) ? f->trees[r + 1]->layers[0] : nullptr;
+            e = launch_fri_fold(1, FIELD_BB, ctx->tw[FIELD_BB], *p2, f->codewords[r], n_out, br, f->codewords[r + 1],
+                                next_digests, st);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rc = hip_err(e, "fri commit phase");
+    }
+    stage_mark(ctx, st, 1);
+    if (ctx->timing) {
+        (void)hipEventSynchronize(ctx->ev[1]);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+        for (int i = 0; i < 4; i++) ctx->stage_ms[i] = 0;
+        ctx->stage_ms[3] = ms;
+    }
+    if (rc) {
+        zkg_fri_free(f);
+        return rc;
+    }
+    *out = f;
+    return ZKG_OK;
+}
+
+extern "C" int zkg_fri_num_layers(const zkg_fri *f, uint32_t *n)
+{
+    if (!f || !n) return set_err(ZKG_EINVAL, "null argument");
+    *n = (uint32_t)f->trees.size();
+    return ZKG_OK;
+}
+
+extern "C" int zkg_fri_layer_root(const zkg_fri *f, uint32_t layer, uint32_t *root8)
+{
+    if (!f || !root8) return set_err(ZKG_EINVAL, "null argument");
+    if (layer >= f->trees.size()) return set_err(ZKG_EINVAL, "layer out of range");
+    return zkg_tree_root(f->trees[layer], root8);
+}
+
+extern "C" int zkg_fri_final(const zkg_fri *f, uint32_t *out)
+{
+    if (!f || !out) return set_err(ZKG_EINVAL, "null argument");
+    HIP_TRY(hipStreamSynchronize(f->ctx->stream), "sync");
+    HIP_TRY(hipMemcpy(out, f->codewords.back(), 16ull << f->log_final, hipMemcpyDeviceToHost), "final");
+    return ZKG_OK;
+}
+
+extern "C" int zkg_fri_open(const zkg_fri *f, const uint64_t *idx, uint32_t n, uint32_t *rows_out, uint32_t *paths_out)
+{
+    if (!f || (!idx && n)) return set_err(ZKG_EINVAL, "null argument");
+    const int nl = (int)f->trees.size();
+    for (uint32_t q = 0; q < n; q++)
+        if (idx[q] >= (1ull << f->log_len)) return set_err(ZKG_EINVAL, "query index out of range");
+    std::vector<uint64_t> li(n);
+    std::vector<uint32_t> rows(8), path;
+    size_t path_off = 0;
+    size_t per_query_path = 0;
+    for (int r = 0; r < nl; r++) per_query_path += (size_t)(f->trees[r]->nlayers - 1) * 8;
+    for (uint32_t q = 0; q < n; q++) {
+        path_off = 0;
+        for (int r = 0; r < nl; r++) {
+            const zkg_tree *t = f->trees[r];
+            uint64_t leaf = idx[q] >> (r + 1);
+            path.resize((size_t)(t->nlayers - 1) * 8);
+            int rc = tree_open(t, &leaf, 1, rows.data(), path.data());
+            if (rc) return rc;
+            if (rows_out) memcpy(rows_out + ((size_t)q * nl + r) * 8, rows.data(), 32);
+            if (paths_out) memcpy(paths_out + q * per_query_path + path_off, path.data(), path.size() * 4);
+            path_off += path.size();
+        }
+    }
+    return ZKG_OK;
+}
+
+extern "C" int zkg_fri_free(zkg_fri *f)
+{
+    if (!f) return ZKG_OK;
+    (void)hipStreamSynchronize(f->ctx->stream);
+    for (zkg_tree *t : f->trees) {
+        t->mat = nullptr;
+        zkg_tree_free(t);
+    }
+    for (void *p : f->codewords) dev_free(f->ctx, p);
+    delete f;
+    return ZKG_OK;
+}

@@ -1,0 +1,394 @@
+"""Host-side mirror of the reference's accelerator interfaces over libzkg.
+
+Names follow the reference so that call sites read the same:
+
+* ``HipDft`` -- p3 ``TwoAdicSubgroupDft`` as MetalDft implements it
+  (zkf-metal/src/ntt/p3_adapter.rs:35-459): ``dft_batch``, ``idft_batch``,
+  ``coset_lde_batch`` (bit-reversed rows, as TwoAdicFriPcs commits them).
+* ``HipPoseidon2`` -- ``Poseidon2Accelerator`` (zkf-core/src/acceleration.rs:
+  106-140): ``batch_permute_goldilocks`` / ``batch_permute_babybear``.
+* ``HipMerkleTreeMmcs`` -- ``Mmcs::{commit, open_batch}`` of MetalMerkleTreeMmcs
+  (zkf-metal/src/merkle.rs:55-138).
+* ``HipFri`` -- ``FriAccelerator::fold_*`` (zkf-core/src/acceleration.rs:196-217)
+  plus the FRI commit phase over the BabyBear degree-4 extension.
+
+Arrays are numpy (uint64 for Goldilocks, uint32 for BabyBear), row-major.
+Everything runs on the GPU through libzkg; errors raise ``ZkgError``.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import Job, ZkgError, check
+
+GOLDILOCKS = 0
+BABYBEAR = 1
+GENERATOR = {GOLDILOCKS: 7, BABYBEAR: 31}
+DTYPE = {GOLDILOCKS: np.uint64, BABYBEAR: np.uint32}
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """One device context (zkg_ctx): resident twiddles and round constants."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        check(self.lib.zkg_ctx_create(device, hbm_budget, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.zkg_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def synchronize(self):
+        check(self.lib.zkg_ctx_synchronize(self.h))
+
+    def hbm_used(self):
+        v = ctypes.c_uint64()
+        check(self.lib.zkg_ctx_hbm_used(self.h, ctypes.byref(v)))
+        return v.value
+
+    def load_constants(self, field, width=16, seed=42):
+        check(self.lib.zkg_constants_load(self.h, field, width, seed))
+
+    def set_timing(self, on=True):
+        check(self.lib.zkg_ctx_set_timing(self.h, 1 if on else 0))
+
+    def stage_ms(self):
+        a = (ctypes.c_double * 4)()
+        check(self.lib.zkg_ctx_stage_ms(self.h, a))
+        return list(a)
+
+    # buffers
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        b = DeviceBuffer(self, max(arr.nbytes, 1))
+        b.upload(arr)
+        return b
+
+    def random(self, field, n, seed, start=0):
+        b = DeviceBuffer(self, max(n * np.dtype(DTYPE[field]).itemsize, 1))
+        check(self.lib.zkg_fill_random(self.h, field, b.h, seed, start, n))
+        return b
+
+
+class DeviceBuffer:
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = nbytes
+        h = ctypes.c_void_p()
+        check(ctx.lib.zkg_buf_alloc(ctx.h, nbytes, ctypes.byref(h)))
+        self.h = h
+
+    def upload(self, arr, offset=0):
+        arr = np.ascontiguousarray(arr)
+        check(self.ctx.lib.zkg_buf_upload(self.h, offset, _ptr(arr), arr.nbytes))
+
+    def download(self, dtype, count, offset=0):
+        out = np.empty(count, dtype=dtype)
+        check(self.ctx.lib.zkg_buf_download(self.h, offset, _ptr(out), out.nbytes))
+        return out
+
+    def free(self):
+        if self.h:
+            self.ctx.lib.zkg_buf_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+class HipDft:
+    """p3 TwoAdicSubgroupDft over libzkg (MetalDft counterpart)."""
+
+    def __init__(self, ctx, field):
+        self.ctx, self.field = ctx, field
+        self.dtype = DTYPE[field]
+
+    def _run(self, mat, inverse):
+        mat = np.ascontiguousarray(mat, dtype=self.dtype)
+        h, w = mat.shape
+        b = self.ctx.upload(mat)
+        check(self.ctx.lib.zkg_ntt_batch(self.ctx.h, self.field, b.h, h, w, 1 if inverse else 0))
+        out = b.download(self.dtype, h * w).reshape(h, w)
+        b.free()
+        return out
+
+    def dft_batch(self, mat):
+        return self._run(mat, False)
+
+    def idft_batch(self, mat):
+        return self._run(mat, True)
+
+    def coset_lde_batch(self, mat, log_blowup, shift=None):
+        """bit_reverse_rows(coset_lde_batch(mat, log_blowup, shift))"""
+        mat = np.ascontiguousarray(mat, dtype=self.dtype)
+        h, w = mat.shape
+        shift = GENERATOR[self.field] if shift is None else shift
+        bi = self.ctx.upload(mat)
+        bo = self.ctx.alloc(max((h << log_blowup) * w * mat.itemsize, 1))
+        check(self.ctx.lib.zkg_coset_lde_batch(self.ctx.h, self.field, bi.h, h, w, log_blowup, shift, bo.h))
+        out = bo.download(self.dtype, (h << log_blowup) * w).reshape(h << log_blowup, w)
+        bi.free()
+        bo.free()
+        return out
+
+
+class HipPoseidon2:
+    """Poseidon2Accelerator counterpart (width 16, explicit constants)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def name(self):
+        return "hip-gfx950-poseidon2"
+
+    def is_available(self):
+        return True
+
+    def batch_permute_goldilocks(self, states, round_constants, n_external_rounds, n_internal_rounds):
+        s = np.ascontiguousarray(states, dtype=np.uint64)
+        rc = np.ascontiguousarray(round_constants, dtype=np.uint64)
+        check(self.ctx.lib.zkg_poseidon2_batch_permute_goldilocks(
+            self.ctx.h, _ptr(s), s.size // 16, _ptr(rc), n_external_rounds, n_internal_rounds))
+        return s
+
+    def batch_permute_babybear(self, states, round_constants, n_external_rounds, n_internal_rounds):
+        s = np.ascontiguousarray(states, dtype=np.uint32)
+        rc = np.ascontiguousarray(round_constants, dtype=np.uint32)
+        check(self.ctx.lib.zkg_poseidon2_batch_permute_babybear(
+            self.ctx.h, _ptr(s), s.size // 16, _ptr(rc), n_external_rounds, n_internal_rounds))
+        return s
+
+    def permute(self, field, width, states):
+        """Permute with the context's resident constants."""
+        s = np.ascontiguousarray(states, dtype=DTYPE[field])
+        b = self.ctx.upload(s)
+        check(self.ctx.lib.zkg_poseidon2_permute(self.ctx.h, field, width, b.h, s.size // width))
+        out = b.download(DTYPE[field], s.size).reshape(s.shape)
+        b.free()
+        return out
+
+
+class MerkleTree:
+    """Prover data of one commitment (zkg_tree)."""
+
+    def __init__(self, ctx, handle, field, keep=()):
+        self.ctx, self.h, self.field = ctx, handle, field
+        self._keep = keep  # buffers that back the opened rows
+        n = ctypes.c_uint32()
+        check(ctx.lib.zkg_tree_num_layers(handle, ctypes.byref(n)))
+        self.num_layers = n.value
+        check(ctx.lib.zkg_tree_digest_elems(handle, ctypes.byref(n)))
+        self.digest = n.value
+
+    def root(self):
+        out = np.empty(self.digest, dtype=DTYPE[self.field])
+        check(self.ctx.lib.zkg_tree_root(self.h, _ptr(out)))
+        return out
+
+    def layer(self, i, n_leaves):
+        n = n_leaves >> i
+        out = np.empty(n * self.digest, dtype=DTYPE[self.field])
+        check(self.ctx.lib.zkg_tree_layer(self.h, i, _ptr(out), out.nbytes))
+        return out.reshape(n, self.digest)
+
+    def open_batch(self, indices, width):
+        idx = np.ascontiguousarray(indices, dtype=np.uint64)
+        n = idx.size
+        rows = np.empty(n * width, dtype=DTYPE[self.field])
+        paths = np.empty(n * (self.num_layers - 1) * self.digest, dtype=DTYPE[self.field])
+        check(self.ctx.lib.zkg_tree_open(self.h, _ptr(idx), n, _ptr(rows), _ptr(paths)))
+        return rows.reshape(n, width), paths.reshape(n, self.num_layers - 1, self.digest)
+
+    def free(self):
+        if self.h:
+            self.ctx.lib.zkg_tree_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+class HipMerkleTreeMmcs:
+    """Mmcs over PaddingFreeSponge / TruncatedPermutation of Poseidon2."""
+
+    def __init__(self, ctx, field, p2_width=16):
+        self.ctx, self.field, self.p2_width = ctx, field, p2_width
+
+    def commit_matrix(self, mat):
+        mat = np.ascontiguousarray(mat, dtype=DTYPE[self.field])
+        h, w = mat.shape
+        b = self.ctx.upload(mat)
+        t = ctypes.c_void_p()
+        check(self.ctx.lib.zkg_merkle_commit(self.ctx.h, self.field, self.p2_width, b.h, h, w, ctypes.byref(t)))
+        tree = MerkleTree(self.ctx, t, self.field, keep=(b,))
+        return tree.root(), tree
+
+    def open_batch(self, index, tree, width):
+        rows, paths = tree.open_batch([index], width)
+        return rows[0], paths[0]
+
+
+def commit_trace(ctx, field, trace_buf, height, width, log_blowup, p2_width=16, plain=False):
+    """Fused (or plain reference-structured) commit of a device-resident trace."""
+    es = np.dtype(DTYPE[field]).itemsize
+    lde = ctx.alloc(max((height << log_blowup) * width * es, 1))
+    t = ctypes.c_void_p()
+    fn = ctx.lib.zkg_commit_plain if plain else ctx.lib.zkg_commit
+    check(fn(ctx.h, field, p2_width, trace_buf.h, height, width, log_blowup, lde.h, ctypes.byref(t)))
+    return MerkleTree(ctx, t, field, keep=(lde,)), lde
+
+
+def make_job(field, trace_buf, height, width, log_blowup, lde_buf, p2_width=16):
+    return Job(field, p2_width, trace_buf.h, height, width, log_blowup, lde_buf.h)
+
+
+def commit_batch(ctx, jobs):
+    arr = (Job * len(jobs))(*jobs)
+    outs = (ctypes.c_void_p * len(jobs))()
+    check(ctx.lib.zkg_commit_batch(ctx.h, arr, len(jobs), outs))
+    return [MerkleTree(ctx, ctypes.c_void_p(outs[i]), jobs[i].field) for i in range(len(jobs))]
+
+
+class HipFri:
+    """FriAccelerator counterpart plus the EF4 commit phase."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def name(self):
+        return "hip-gfx950-fri"
+
+    def is_available(self):
+        return True
+
+    def _fold(self, field, evals, alpha, inv_twiddles):
+        dt = DTYPE[field]
+        ev = np.ascontiguousarray(evals, dtype=dt)
+        tw = np.ascontiguousarray(inv_twiddles, dtype=dt)
+        n = ev.size // 2
+        be, bt, bo = self.ctx.upload(ev), self.ctx.upload(tw), self.ctx.alloc(max(n * ev.itemsize, 1))
+        check(self.ctx.lib.zkg_fri_fold(self.ctx.h, field, be.h, n, int(alpha), bt.h, bo.h))
+        return bo.download(dt, n)
+
+    def fold_goldilocks(self, evals, alpha, inv_twiddles):
+        return self._fold(GOLDILOCKS, evals, alpha, inv_twiddles)
+
+    def fold_babybear(self, evals, alpha, inv_twiddles):
+        return self._fold(BABYBEAR, evals, alpha, inv_twiddles)
+
+    def fold_matrix_ef4(self, evals, beta):
+        """p3 fold_matrix, arity 2, over EF4: evals (2n, 4) uint32 -> (n, 4)."""
+        ev = np.ascontiguousarray(evals, dtype=np.uint32)
+        n = ev.shape[0] // 2
+        be, bo = self.ctx.upload(ev), self.ctx.alloc(max(n * 16, 1))
+        b = np.ascontiguousarray(beta, dtype=np.uint32)
+        check(self.ctx.lib.zkg_fri_fold_matrix(self.ctx.h, 1, BABYBEAR, be.h, n, _ptr(b), bo.h))
+        return bo.download(np.uint32, n * 4).reshape(n, 4)
+
+    def fold_matrix_base(self, field, evals, beta):
+        dt = DTYPE[field]
+        ev = np.ascontiguousarray(evals, dtype=dt)
+        n = ev.size // 2
+        be, bo = self.ctx.upload(ev), self.ctx.alloc(max(n * ev.itemsize, 1))
+        b = np.array([beta], dtype=np.uint64)
+        check(self.ctx.lib.zkg_fri_fold_matrix(self.ctx.h, 0, field, be.h, n, _ptr(b), bo.h))
+        return bo.download(dt, n)
+
+    def commit_phase(self, codeword_buf, log_len, log_final, betas):
+        b = np.ascontiguousarray(betas, dtype=np.uint32)
+        h = ctypes.c_void_p()
+        check(self.ctx.lib.zkg_fri_commit_phase(self.ctx.h, codeword_buf.h, log_len, log_final, _ptr(b),
+                                                ctypes.byref(h)))
+        return FriCommitment(self.ctx, h, log_len, log_final)
+
+
+class FriCommitment:
+    def __init__(self, ctx, h, log_len, log_final):
+        self.ctx, self.h, self.log_len, self.log_final = ctx, h, log_len, log_final
+        n = ctypes.c_uint32()
+        check(ctx.lib.zkg_fri_num_layers(h, ctypes.byref(n)))
+        self.num_layers = n.value
+
+    def layer_roots(self):
+        out = np.empty((self.num_layers, 8), dtype=np.uint32)
+        for r in range(self.num_layers):
+            check(self.ctx.lib.zkg_fri_layer_root(self.h, r, _ptr(out[r])))
+        return out
+
+    def final(self):
+        out = np.empty((1 << self.log_final) * 4, dtype=np.uint32)
+        check(self.ctx.lib.zkg_fri_final(self.h, _ptr(out)))
+        return out.reshape(-1, 4)
+
+    def open(self, indices):
+        idx = np.ascontiguousarray(indices, dtype=np.uint64)
+        n = idx.size
+        nl = self.num_layers
+        plen = sum(self.log_len - 1 - r for r in range(nl))
+        rows = np.empty(n * nl * 8, dtype=np.uint32)
+        paths = np.empty(n * plen * 8, dtype=np.uint32)
+        check(self.ctx.lib.zkg_fri_open(self.h, _ptr(idx), n, _ptr(rows), _ptr(paths)))
+        return rows.reshape(n, nl, 8), paths.reshape(n, plen, 8)
+
+    def free(self):
+        if self.h:
+            self.ctx.lib.zkg_fri_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching commit server (zkg_serve_*)."""
+
+    def __init__(self, ctx, slots=2, max_batch=16):
+        self.ctx = ctx
+        check(ctx.lib.zkg_serve_start(ctx.h, slots, max_batch))
+
+    def submit(self, job):
+        t = ctypes.c_uint64()
+        check(self.ctx.lib.zkg_serve_submit(self.ctx.h, ctypes.byref(job), ctypes.byref(t)))
+        return t.value
+
+    def poll(self, ticket):
+        """(tree handle, latency_ms) when done, None while pending."""
+        tr = ctypes.c_void_p()
+        lat = ctypes.c_double()
+        rc = self.ctx.lib.zkg_serve_poll(self.ctx.h, ticket, ctypes.byref(tr), ctypes.byref(lat))
+        if rc == _lib.EBUSY:
+            return None
+        check(rc)
+        return tr, lat.value
+
+    def stop(self):
+        check(self.ctx.lib.zkg_serve_stop(self.ctx.h))
