@@ -1,0 +1,318 @@
+"""Benchmark of the STARK trace-commitment path on one MI355X.
+
+Headline workload: BabyBear trace of 2^20 rows x 64 columns, coset LDE at
+blowup 2 (2^21 rows, bit-reversed), Poseidon2-16 row hashes, Merkle tree to
+one root.  A step is one full commit of that trace with the inputs already in
+HBM.  value = trace cells (rows x columns before the LDE) committed per second.
+
+Also reported on the same line:
+  roofline      the dominant kernel (Poseidon2 row hashing, VALU-bound):
+                algorithmic field products (772 per BabyBear-16 permutation)
+                per second against the measured Montgomery-product ceiling;
+  roofline_lde  the LDE kernels (HBM-bound): N W 4 (1 + B) bytes per commit
+                against 8 TB/s;
+  plain         the same commit through the unfused, reference-structured
+                path (one global pass per NTT stage, per-level Merkle
+                launches, a synchronisation after every launch);
+  batched       48 BabyBear traces (heights 2^U{12..20}, widths U{8..256}),
+                one hipGraph-replayed batch vs one plain call per trace;
+  served        continuous-batching stream (60% BabyBear / 40% Goldilocks,
+                heights 2^U{14..22}, widths U{16..128}, blowup 2 or 4,
+                Poisson arrivals): p50/p99 latency and cells/s vs one plain
+                call per request;
+  cpu_baseline  the CPU oracle (C restatement of the reference CPU path),
+                multithreaded over the host cores, on a bounded sample.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import ziros_amd as Z  # noqa: E402
+from ziros_amd._lib import check  # noqa: E402
+
+HBM_PEAK = 8.0e12
+BB_PERM_MULS = 772
+GL16_PERM_MULS = 952
+SEED_C2 = 0x5EED0002
+SEED_C3 = 0x5EED0003
+SEED_C5 = 0x5EED0005
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def probe_peak(kind=0, iters=4096):
+    lib = ctypes.CDLL(os.path.join(ROOT, "ziros_amd", "libzkg_probe.so"))
+    lib.zkg_probe_rate.restype = ctypes.c_int
+    lib.zkg_probe_rate.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
+    v = ctypes.c_double()
+    rc = lib.zkg_probe_rate(kind, iters, ctypes.byref(v))
+    if rc:
+        raise RuntimeError("probe failed: %d" % rc)
+    return v.value
+
+
+def free_tree(ctx, h):
+    ctx.lib.zkg_tree_free(h)
+
+
+def commit_once(ctx, fn, field, p2w, trace, n, w, lb, lde):
+    t = ctypes.c_void_p()
+    check(fn(ctx.h, field, p2w, trace.h, n, w, lb, lde.h, ctypes.byref(t)))
+    return t
+
+
+def bench_c2(ctx, steps, warmup):
+    n, w, lb = 1 << 20, 64, 1
+    trace = ctx.random(Z.BABYBEAR, n * w, SEED_C2)
+    lde = ctx.alloc((n << lb) * w * 4)
+    ctx.set_timing(True)
+    for _ in range(warmup):
+        free_tree(ctx, commit_once(ctx, ctx.lib.zkg_commit, Z.BABYBEAR, 16, trace, n, w, lb, lde))
+    ctx.synchronize()
+    stages = np.zeros(4)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t = commit_once(ctx, ctx.lib.zkg_commit, Z.BABYBEAR, 16, trace, n, w, lb, lde)
+        stages += ctx.stage_ms()
+        free_tree(ctx, t)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    stages /= steps
+    # plain (reference-structured) path
+    t0 = time.perf_counter()
+    plain_steps = max(1, min(steps, 3))
+    for _ in range(plain_steps):
+        free_tree(ctx, commit_once(ctx, ctx.lib.zkg_commit_plain, Z.BABYBEAR, 16, trace, n, w, lb, lde))
+    ctx.synchronize()
+    dt_plain = (time.perf_counter() - t0) / plain_steps
+    ctx.set_timing(False)
+    trace.free()
+    lde.free()
+    return dict(n=n, w=w, lb=lb, sec_per_step=dt / steps, stage_ms=stages.tolist(), plain_sec=dt_plain)
+
+
+def c3_shapes():
+    rng = np.random.default_rng(SEED_C3)
+    logs = rng.integers(12, 21, 48)
+    widths = rng.integers(1, 33, 48) * 8
+    return [(int(l), int(wd)) for l, wd in zip(logs, widths)]
+
+
+def bench_c3(ctx, steps, warmup):
+    shapes = c3_shapes()
+    bufs, jobs, cells = [], [], 0
+    for k, (lg, w) in enumerate(shapes):
+        n = 1 << lg
+        tr = ctx.random(Z.BABYBEAR, n * w, SEED_C3 + k)
+        lde = ctx.alloc((n << 1) * w * 4)
+        bufs.append((tr, lde))
+        jobs.append(Z.make_job(Z.BABYBEAR, tr, n, w, 1, lde))
+        cells += n * w
+    arr = (Z._lib.Job * len(jobs))(*jobs)
+    outs = (ctypes.c_void_p * len(jobs))()
+
+    def run():
+        check(ctx.lib.zkg_commit_batch(ctx.h, arr, len(jobs), outs))
+        for i in range(len(jobs)):
+            free_tree(ctx, ctypes.c_void_p(outs[i]))
+
+    for _ in range(max(1, warmup)):
+        run()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    # plain: one synchronous reference-structured call per trace
+    t0 = time.perf_counter()
+    for (tr, lde), (lg, w) in zip(bufs, shapes):
+        free_tree(ctx, commit_once(ctx, ctx.lib.zkg_commit_plain, Z.BABYBEAR, 16, tr, 1 << lg, w, 1, lde))
+    ctx.synchronize()
+    dt_plain = time.perf_counter() - t0
+    # fused single calls, one per trace (isolates the batching gain)
+    t0 = time.perf_counter()
+    for (tr, lde), (lg, w) in zip(bufs, shapes):
+        free_tree(ctx, commit_once(ctx, ctx.lib.zkg_commit, Z.BABYBEAR, 16, tr, 1 << lg, w, 1, lde))
+    ctx.synchronize()
+    dt_single = time.perf_counter() - t0
+    for tr, lde in bufs:
+        tr.free()
+        lde.free()
+    return dict(traces=len(jobs), cells=cells, batched_cells_per_s=cells / dt, plain_cells_per_s=cells / dt_plain,
+                single_fused_cells_per_s=cells / dt_single, batched_vs_plain=dt_plain / dt,
+                batch_ms=dt * 1e3)
+
+
+def bench_c5(ctx, n_req, rate_hz, slots=2, max_batch=8):
+    rng = np.random.default_rng(SEED_C5)
+    reqs = []
+    for k in range(n_req):
+        field = Z.BABYBEAR if rng.random() < 0.6 else Z.GOLDILOCKS
+        lg = int(rng.integers(14, 23))
+        w = int(rng.integers(16, 129))
+        lb = int(rng.integers(1, 3))
+        reqs.append((field, lg, w, lb))
+    gaps = rng.exponential(1.0 / rate_hz, n_req)
+    bufs, jobs, cells = [], [], 0
+    for k, (field, lg, w, lb) in enumerate(reqs):
+        n = 1 << lg
+        es = 8 if field == Z.GOLDILOCKS else 4
+        tr = ctx.random(field, n * w, SEED_C5 + k)
+        lde = ctx.alloc((n << lb) * w * es)
+        bufs.append((tr, lde))
+        jobs.append(Z.make_job(field, tr, n, w, lb, lde))
+        cells += n * w
+    ctx.synchronize()
+    # plain per-request path, back to back
+    t0 = time.perf_counter()
+    plain_lat = []
+    for (tr, lde), (field, lg, w, lb) in zip(bufs, reqs):
+        a = time.perf_counter()
+        free_tree(ctx, commit_once(ctx, ctx.lib.zkg_commit_plain, field, 16, tr, 1 << lg, w, lb, lde))
+        plain_lat.append((time.perf_counter() - a) * 1e3)
+    dt_plain = time.perf_counter() - t0
+    # served: Poisson arrivals into the continuous-batching loop
+    srv = Z.Server(ctx, slots=slots, max_batch=max_batch)
+    lat, pending = [], {}
+    t0 = time.perf_counter()
+    next_t = t0
+    k = 0
+    while k < n_req or pending:
+        now = time.perf_counter()
+        while k < n_req and now >= next_t:
+            pending[srv.submit(jobs[k])] = k
+            next_t += gaps[k]
+            k += 1
+        for t in list(pending):
+            r = srv.poll(t)
+            if r is not None:
+                free_tree(ctx, r[0])
+                lat.append(r[1])
+                del pending[t]
+        time.sleep(0.0002)
+    dt = time.perf_counter() - t0
+    srv.stop()
+    for tr, lde in bufs:
+        tr.free()
+        lde.free()
+    lat = np.array(lat)
+    return dict(requests=n_req, rate_hz=rate_hz, cells=cells, served_cells_per_s=cells / dt,
+                plain_cells_per_s=cells / dt_plain, served_vs_plain=dt_plain / dt,
+                p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                plain_p50_ms=float(np.percentile(plain_lat, 50)), plain_p99_ms=float(np.percentile(plain_lat, 99)))
+
+
+def cpu_baseline(log_rows=17, width=64):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle as O
+
+    n = 1 << log_rows
+    tr = O.gen(O.BB, SEED_C2, n * width).reshape(n, width)
+    p2 = O.Poseidon2(O.BB, 16, 42)
+    threads = os.cpu_count() or 1
+    O.set_threads(threads)
+    t0 = time.perf_counter()
+    O.commit(p2, tr, 1)
+    dt = time.perf_counter() - t0
+    cpu = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(value=n * width / dt, unit="cells/s", cores=O.get_threads(), kind="port",
+                sample="BabyBear %d x %d commit (blowup 2, Poseidon2-16, Merkle) = 1/%d of the workload; %s; %.1f s"
+                % (n, width, (1 << 20) // n, cpu, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-rate", type=float, default=100.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (see --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    ctx = Z.Context(0)
+    for field, width in ((Z.BABYBEAR, 16), (Z.GOLDILOCKS, 16), (Z.GOLDILOCKS, 8)):
+        ctx.load_constants(field, width, 42)
+    log("probe: VALU ceilings")
+    peak_bb = probe_peak(0)
+    peak_mul32 = probe_peak(2)
+    log("probe: %.3e BabyBear products/s, %.3e mul_lo_u32/s" % (peak_bb, peak_mul32))
+    log("c2: %d warmup + %d timed steps" % (args.warmup, args.steps))
+    c2 = bench_c2(ctx, args.steps, args.warmup)
+    n, w, lb = c2["n"], c2["w"], c2["lb"]
+    cells = n * w
+    value = cells / c2["sec_per_step"]
+    lde_ms, hash_ms, merkle_ms = c2["stage_ms"][0], c2["stage_ms"][1], c2["stage_ms"][2]
+    leaves = n << lb
+    hash_muls = leaves * ((w + 7) // 8) * BB_PERM_MULS
+    lde_bytes = n * w * 4 * (1 + (1 << lb))
+    log("c2: %.3f ms/step, stages %s" % (c2["sec_per_step"] * 1e3, c2["stage_ms"]))
+    out = {
+        "metric": "trace cells committed/sec, batched vs plain per-trace GPU; p99 commit latency",
+        "value": value,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": c2["sec_per_step"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": "BabyBear commit: 2^20 x 64 trace, coset LDE blowup 2 (2^21 rows, shift 31), "
+                               "Poseidon2-16 row sponge (8+13 rounds, rate 8), Merkle to one root",
+                   "height": n, "width": w, "log_blowup": lb, "field": "babybear", "poseidon2_width": 16},
+        "roofline": {"kernel": "k_hash_rows<Bb,16>", "bound": "valu", "achieved": hash_muls / (hash_ms * 1e-3) / 1e9,
+                     "peak": peak_bb / 1e9, "unit": "Gmodmul/s", "frac": hash_muls / (hash_ms * 1e-3) / peak_bb,
+                     "traffic": None,
+                     "note": "772 BabyBear products per permutation x %d permutations per launch; peak = measured "
+                             "Montgomery-product rate of the same routine" % (leaves * ((w + 7) // 8))},
+        "roofline_lde": {"kernels": "k_fs_pass1 + k_lde_mid + k_lde_final", "bound": "hbm",
+                         "achieved": lde_bytes / (lde_ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                         "frac": lde_bytes / (lde_ms * 1e-3) / HBM_PEAK, "traffic": None},
+        "stage_ms": {"lde": lde_ms, "row_hash": hash_ms, "merkle": merkle_ms},
+        "plain": {"cells_per_s": cells / c2["plain_sec"], "ms_per_commit": c2["plain_sec"] * 1e3,
+                  "fused_vs_plain": c2["plain_sec"] / c2["sec_per_step"]},
+        "probe": {"bb_montgomery_products_per_s": peak_bb, "mul_lo_u32_per_s": peak_mul32},
+    }
+    if not args.skip_batch:
+        log("c3: batched 48 traces")
+        out["batched"] = bench_c3(ctx, max(1, args.steps // 4), 1)
+        log("c3: %s" % out["batched"])
+    if not args.skip_serve:
+        log("c5: served stream")
+        out["served"] = bench_c5(ctx, args.serve_requests, args.serve_rate)
+        log("c5: %s" % out["served"])
+    ctx.close()
+    if not args.skip_cpu:
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

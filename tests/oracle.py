@@ -303,3 +303,15 @@ def set_threads(n):
 
 def get_threads():
     return lib().or_get_threads()
+
+
+def verify(p2, root, heights, widths, index, rows, path):
+    """p3 verify_batch of one opening (matrices given by heights/widths)."""
+    n = len(heights)
+    hs = np.ascontiguousarray(heights, dtype=np.uint64)
+    ws = np.ascontiguousarray(widths, dtype=np.uint32)
+    rws = [u64(r) for r in rows]
+    rptr = (c_vp * n)(*[_p(r) for r in rws])
+    pa = u64(path).reshape(-1)
+    return bool(lib().or_merkle_verify(ctypes.byref(p2.p), _p(u64(root)), n, _p(hs), _p(ws), index, rptr,
+                                       _p(pa) if pa.size else None, len(path)))

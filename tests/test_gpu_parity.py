@@ -1,0 +1,263 @@
+"""GPU parity: libzkg (HIP, gfx950) against the CPU oracle, bit-exact.
+
+Every test here runs the product path through the C ABI (ziros_amd binds
+libzkg.so with ctypes) and compares with oracle/liboracle.so on the same
+seeded inputs.  Integer/field work: the bar is bit-exact equality.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import ziros_amd as Z
+from test_oracle_poseidon2 import GOLDEN_GL16
+
+pytestmark = pytest.mark.gpu
+
+GL, BB = Z.GOLDILOCKS, Z.BABYBEAR
+DT = {GL: np.uint64, BB: np.uint32}
+
+
+def rnd(field, n, seed):
+    return O.gen(field, seed, n).astype(DT[field])
+
+
+# ----------------------------------------------------------------- Poseidon2
+def test_permute_golden_vector(ctx):
+    s = np.zeros(16, dtype=np.uint64)
+    s[0], s[1] = 12345, 67890
+    out = Z.HipPoseidon2(ctx).permute(GL, 16, s.reshape(1, 16))
+    assert [int(x) for x in out[0]] == GOLDEN_GL16
+
+
+@pytest.mark.parametrize("field,width", [(GL, 16), (GL, 8), (BB, 16)])
+def test_permute_vs_oracle(ctx, field, width):
+    p2 = O.Poseidon2(field, width, seed=42)
+    n = 777
+    st = rnd(field, n * width, 3).reshape(n, width)
+    p = O.PRIME[field]
+    st[0] = 0
+    st[1] = p - 1
+    st[2] = np.arange(width)
+    got = Z.HipPoseidon2(ctx).permute(field, width, st)
+    for i in list(range(8)) + [n // 2, n - 1]:
+        assert (got[i].astype(np.uint64) == p2.permute(st[i])).all(), i
+    # the rest in bulk
+    exp = np.stack([p2.permute(st[i]) for i in range(0, n, 7)])
+    assert (got[::7].astype(np.uint64) == exp).all()
+
+
+def test_batch_permute_explicit_constants(ctx):
+    """Poseidon2Accelerator form: flat [initial][internal][terminal] table."""
+    acc = Z.HipPoseidon2(ctx)
+    for field, seed in ((GL, 7), (BB, 19)):
+        p2 = O.Poseidon2(field, 16, seed=seed)
+        rc = p2.flat_constants().astype(DT[field])
+        st = rnd(field, 64 * 16, 5).reshape(64, 16)
+        fn = acc.batch_permute_goldilocks if field == GL else acc.batch_permute_babybear
+        got = fn(st.copy(), rc, 8, p2.p.rounds_p)
+        exp = np.stack([p2.permute(st[i]) for i in range(64)])
+        assert (got.reshape(64, 16).astype(np.uint64) == exp).all()
+    # restore the session constants
+    ctx.load_constants(GL, 16, 42)
+    ctx.load_constants(BB, 16, 42)
+
+
+def test_generator_matches_oracle(ctx):
+    for field in (GL, BB):
+        b = ctx.random(field, 5000, seed=77, start=123)
+        got = b.download(DT[field], 5000)
+        assert (got.astype(np.uint64) == O.gen(field, 77, 5000, start=123)).all()
+
+
+# ----------------------------------------------------------------- NTT / LDE
+@pytest.mark.parametrize("field", [GL, BB])
+@pytest.mark.parametrize("logn,width", [(0, 3), (1, 5), (3, 1), (6, 16), (10, 7), (11, 64), (12, 3), (13, 33),
+                                        (16, 8)])
+def test_ntt_vs_oracle(ctx, field, logn, width):
+    n = 1 << logn
+    m = rnd(field, n * width, 11 + logn).reshape(n, width)
+    dft = Z.HipDft(ctx, field)
+    assert (dft.dft_batch(m).astype(np.uint64) == O.dft_batch(field, m)).all()
+    assert (dft.idft_batch(m).astype(np.uint64) == O.dft_batch(field, m, inverse=True)).all()
+
+
+@pytest.mark.parametrize("field", [GL, BB])
+@pytest.mark.parametrize("logn,lb,width", [(0, 1, 4), (1, 1, 3), (4, 2, 7), (8, 0, 5), (11, 1, 16), (11, 2, 9),
+                                           (12, 1, 8), (13, 2, 20), (14, 1, 64), (15, 1, 3)])
+def test_coset_lde_vs_oracle(ctx, field, logn, lb, width):
+    n = 1 << logn
+    m = rnd(field, n * width, 21 + logn).reshape(n, width)
+    shift = O.lib().or_f_generator(field)
+    got = Z.HipDft(ctx, field).coset_lde_batch(m, lb)
+    assert (got.astype(np.uint64) == O.coset_lde_batch(field, m, lb, shift)).all()
+
+
+def test_coset_lde_other_shift(ctx):
+    m = rnd(BB, 1024 * 4, 5).reshape(1024, 4)
+    got = Z.HipDft(ctx, BB).coset_lde_batch(m, 1, shift=5)
+    assert (got.astype(np.uint64) == O.coset_lde_batch(BB, m, 1, 5)).all()
+
+
+# ----------------------------------------------------------------- hashing / Merkle
+@pytest.mark.parametrize("field,p2w", [(GL, 16), (GL, 8), (BB, 16)])
+@pytest.mark.parametrize("width", [1, 4, 7, 8, 9, 16, 33, 64])
+def test_hash_rows_vs_oracle(ctx, field, p2w, width):
+    p2 = O.Poseidon2(field, p2w)
+    h = 512
+    m = rnd(field, h * width, width).reshape(h, width)
+    root, tree = Z.HipMerkleTreeMmcs(ctx, field, p2w).commit_matrix(m)
+    exp = O.MerkleTree(p2, [m])
+    assert (tree.layer(0, h).astype(np.uint64) == exp.layers[0]).all()
+    assert (root.astype(np.uint64) == exp.root()).all()
+
+
+@pytest.mark.parametrize("field,p2w,logh,width", [(BB, 16, 0, 5), (BB, 16, 1, 5), (BB, 16, 10, 24), (GL, 16, 12, 8),
+                                                  (GL, 8, 11, 8), (BB, 16, 13, 8), (BB, 16, 9, 1)])
+def test_merkle_layers_and_openings(ctx, field, p2w, logh, width):
+    p2 = O.Poseidon2(field, p2w)
+    h = 1 << logh
+    m = rnd(field, h * width, 31 + logh).reshape(h, width)
+    root, tree = Z.HipMerkleTreeMmcs(ctx, field, p2w).commit_matrix(m)
+    exp = O.MerkleTree(p2, [m])
+    assert tree.num_layers == len(exp.layers)
+    for i in range(tree.num_layers):
+        assert (tree.layer(i, h).astype(np.uint64) == exp.layers[i]).all(), i
+    idx = np.array(sorted({0, h - 1, h // 2, (h * 3) // 7}), dtype=np.uint64)
+    rows, paths = tree.open_batch(idx, width)
+    for k, i in enumerate(idx):
+        i = int(i)
+        assert (rows[k].astype(np.uint64) == m[i]).all()
+        assert (paths[k].astype(np.uint64) == exp.path(i).reshape(paths[k].shape)).all()
+        assert exp.verify(root.astype(np.uint64), i, [rows[k]], paths[k].astype(np.uint64))
+
+
+@pytest.mark.parametrize("plain", [False, True])
+@pytest.mark.parametrize("field,p2w,logn,width,lb", [(GL, 8, 14, 8, 1), (GL, 16, 14, 8, 1), (BB, 16, 12, 16, 1),
+                                                     (BB, 16, 10, 40, 2), (GL, 16, 9, 3, 2)])
+def test_commit_vs_oracle(ctx, plain, field, p2w, logn, width, lb):
+    """Config 1 shape (Goldilocks 2^14 x 8, blowup 2) and smaller mixes."""
+    n = 1 << logn
+    p2 = O.Poseidon2(field, p2w)
+    tr = rnd(field, n * width, 0x5EED0001).reshape(n, width)
+    buf = ctx.upload(tr)
+    tree, lde = Z.commit_trace(ctx, field, buf, n, width, lb, p2_width=p2w, plain=plain)
+    root_o, lde_o = O.commit(p2, tr, lb)
+    got_lde = lde.download(DT[field], (n << lb) * width).reshape(n << lb, width)
+    assert (got_lde.astype(np.uint64) == lde_o).all()
+    assert (tree.root().astype(np.uint64) == root_o).all()
+
+
+# ----------------------------------------------------------------- FRI
+def test_fri_fold_reference_inputs(ctx):
+    """zkf-metal/src/fri.rs:187-224 inputs, FriAccelerator semantics."""
+    n = 2048
+    fri = Z.HipFri(ctx)
+    for field, fn in ((GL, fri.fold_goldilocks), (BB, fri.fold_babybear)):
+        ev = (np.arange(2 * n, dtype=np.uint64) * 7 + 1) % O.PRIME[field]
+        tw = (np.arange(n, dtype=np.uint64) * 13 + 3) % O.PRIME[field]
+        got = fn(ev.astype(DT[field]), 42, tw.astype(DT[field]))
+        assert (got.astype(np.uint64) == O.fri_fold_base(field, ev, 42, tw)).all()
+
+
+@pytest.mark.parametrize("logn", [0, 1, 5, 12, 16])
+def test_fri_fold_matrix_vs_oracle(ctx, logn):
+    n = 1 << logn
+    fri = Z.HipFri(ctx)
+    ev = rnd(BB, 2 * n * 4, 40 + logn).reshape(2 * n, 4)
+    beta = O.fri_betas(9, 1)[0]
+    got = fri.fold_matrix_ef4(ev, beta.astype(np.uint32))
+    assert (got.astype(np.uint64) == O.fri_fold_matrix_ef4(ev, beta)).all()
+    for field in (GL, BB):
+        e = rnd(field, 2 * n, 50 + logn)
+        b = int(O.gen(field, 3, 1)[0])
+        assert (fri.fold_matrix_base(field, e, b).astype(np.uint64) == O.fri_fold_matrix_base(field, e, b)).all()
+
+
+@pytest.mark.parametrize("log_len,log_final", [(12, 4), (9, 8), (14, 8)])
+def test_fri_commit_phase_vs_oracle(ctx, log_len, log_final):
+    p2 = O.Poseidon2(BB, 16)
+    cw = rnd(BB, (1 << log_len) * 4, 60 + log_len).reshape(-1, 4)
+    nl = log_len - log_final
+    betas = O.fri_betas(0x5EED0004, nl)
+    buf = ctx.upload(cw)
+    fc = Z.HipFri(ctx).commit_phase(buf, log_len, log_final, betas.astype(np.uint32))
+    roots_o, final_o = O.fri_commit_phase_ef4(p2, cw, log_len, log_final, betas)
+    assert (fc.layer_roots().astype(np.uint64) == roots_o).all()
+    assert (fc.final().astype(np.uint64) == final_o).all()
+    # query openings verify against every layer root
+    layers = [cw.astype(np.uint64)]
+    for r in range(nl - 1):
+        layers.append(O.fri_fold_matrix_ef4(layers[-1], betas[r]))
+    trees = [O.MerkleTree(p2, [layers[r].reshape(-1, 8)]) for r in range(nl)]
+    q = np.array([0, 1, (1 << log_len) - 1, 12345 % (1 << log_len)], dtype=np.uint64)
+    rows, paths = fc.open(q)
+    for k, qi in enumerate(q):
+        off = 0
+        for r in range(nl):
+            leaf = int(qi) >> (r + 1)
+            mat = layers[r].reshape(-1, 8)
+            assert (rows[k, r].astype(np.uint64) == mat[leaf]).all()
+            plen = log_len - 1 - r
+            path = paths[k, off:off + plen].astype(np.uint64)
+            off += plen
+            assert (path == trees[r].path(leaf)).all()
+            assert trees[r].verify(roots_o[r], leaf, [mat[leaf]], path)
+
+
+# ----------------------------------------------------------------- batch and serve
+def _jobs(ctx, specs):
+    jobs, keep, expect = [], [], []
+    for k, (field, p2w, logn, width, lb) in enumerate(specs):
+        n = 1 << logn
+        tr = rnd(field, n * width, 100 + k).reshape(n, width)
+        b = ctx.upload(tr)
+        lde = ctx.alloc((n << lb) * width * np.dtype(DT[field]).itemsize)
+        jobs.append(Z.make_job(field, b, n, width, lb, lde, p2w))
+        keep += [b, lde]
+        expect.append(O.commit(O.Poseidon2(field, p2w), tr, lb)[0])
+    return jobs, keep, expect
+
+
+SPECS = [(BB, 16, 12, 8, 1), (GL, 16, 10, 16, 1), (BB, 16, 14, 24, 1), (BB, 16, 8, 3, 2), (GL, 8, 11, 8, 1),
+         (BB, 16, 13, 64, 1)]
+
+
+def test_commit_batch_matches_lone_commits(ctx):
+    jobs, keep, expect = _jobs(ctx, SPECS)
+    for rep in range(3):  # capture, then replays of the cached graph
+        trees = Z.commit_batch(ctx, jobs)
+        for t, e in zip(trees, expect):
+            assert (t.root().astype(np.uint64) == e).all(), rep
+
+
+def test_serve_results_identical_in_mix(ctx):
+    import time
+
+    jobs, keep, expect = _jobs(ctx, SPECS * 2)
+    srv = Z.Server(ctx, slots=2, max_batch=4)
+    try:
+        tickets = [srv.submit(j) for j in jobs]
+        done = {}
+        t0 = time.time()
+        while len(done) < len(tickets) and time.time() - t0 < 60:
+            for i, t in enumerate(tickets):
+                if i in done:
+                    continue
+                r = srv.poll(t)
+                if r is not None:
+                    done[i] = r
+            time.sleep(0.001)
+        assert len(done) == len(tickets)
+        for i, (h, lat) in done.items():
+            tree = Z.MerkleTree(ctx, h, jobs[i].field)
+            assert (tree.root().astype(np.uint64) == expect[i]).all(), i
+            assert lat > 0
+    finally:
+        srv.stop()
+
+
+def test_errors_are_reported(ctx):
+    with pytest.raises(Z.ZkgError):
+        Z.HipDft(ctx, BB).coset_lde_batch(np.zeros((3, 2), dtype=np.uint32), 1)  # not a power of two
+    with pytest.raises(Z.ZkgError):
+        Z.HipDft(ctx, BB).coset_lde_batch(np.zeros((4, 2), dtype=np.uint32), 3)  # blowup too large

@@ -13,7 +13,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libzkg.so")
+PROBE_LIB = os.path.join(HERE, "libzkg_probe.so")
 SOURCES = ["zkg_ntt.hip", "zkg_hash.hip", "zkg_runtime.hip", "zkg_batch.hip"]
+PROBE_SOURCES = ["zkg_probe.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
@@ -32,21 +34,27 @@ def _compile(src):
     return obj, [m for m in msgs if m.strip()]
 
 
-def build(verbose=False):
-    os.makedirs(os.path.join(CSRC, "_obj"), exist_ok=True)
-    with ThreadPoolExecutor(max_workers=4) as ex:
-        results = list(ex.map(_compile, SOURCES))
-    objs = [o for o, _ in results]
-    if verbose:
-        for _, msgs in results:
-            for m in msgs:
-                print(m)
-    tmp = LIB + ".tmp"
+def _link(objs, out):
+    tmp = out + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
+
+
+def build(verbose=False):
+    """Compile every translation unit and link libzkg.so and libzkg_probe.so."""
+    os.makedirs(os.path.join(CSRC, "_obj"), exist_ok=True)
+    with ThreadPoolExecutor(max_workers=5) as ex:
+        results = list(ex.map(_compile, SOURCES + PROBE_SOURCES))
+    if verbose:
+        for _, msgs in results:
+            for m in msgs:
+                print(m)
+    objs = [o for o, _ in results]
+    _link(objs[: len(SOURCES)], LIB)
+    _link(objs[len(SOURCES):], PROBE_LIB)
     return LIB
 
 
