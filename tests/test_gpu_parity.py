@@ -92,6 +92,27 @@ def test_coset_lde_vs_oracle(ctx, field, logn, lb, width):
     assert (got.astype(np.uint64) == O.coset_lde_batch(field, m, lb, shift)).all()
 
 
+@pytest.mark.parametrize("field", [GL, BB])
+@pytest.mark.parametrize("logn,lb,width", [(18, 1, 3), (19, 2, 2), (20, 1, 9), (21, 1, 2), (22, 1, 1)])
+def test_coset_lde_large_vs_oracle(ctx, field, logn, lb, width):
+    """Four-step tiles of 2^9 .. 2^11 points (both passes)."""
+    n = 1 << logn
+    m = rnd(field, n * width, 70 + logn).reshape(n, width)
+    shift = O.lib().or_f_generator(field)
+    got = Z.HipDft(ctx, field).coset_lde_batch(m, lb)
+    assert (got.astype(np.uint64) == O.coset_lde_batch(field, m, lb, shift)).all()
+
+
+@pytest.mark.parametrize("field", [GL, BB])
+@pytest.mark.parametrize("logn,width", [(18, 2), (21, 1), (22, 1)])
+def test_ntt_large_vs_oracle(ctx, field, logn, width):
+    n = 1 << logn
+    m = rnd(field, n * width, 90 + logn).reshape(n, width)
+    dft = Z.HipDft(ctx, field)
+    assert (dft.dft_batch(m).astype(np.uint64) == O.dft_batch(field, m)).all()
+    assert (dft.idft_batch(m).astype(np.uint64) == O.dft_batch(field, m, inverse=True)).all()
+
+
 def test_coset_lde_other_shift(ctx):
     m = rnd(BB, 1024 * 4, 5).reshape(1024, 4)
     got = Z.HipDft(ctx, BB).coset_lde_batch(m, 1, shift=5)

@@ -11,24 +11,26 @@
 // of zkf-metal/src/ntt/radix2.rs):
 //   * A column transform of size N = N1 * N2 (N1, N2 <= 2^11) is a four-step:
 //     size-N1 transforms over rows strided by N2, a twiddle omega_N^(a b), and
-//     size-N2 transforms over contiguous rows.  Every sub-transform is staged
-//     in LDS as a tile of M points x C columns (C adjacent columns of a row are
-//     one coalesced segment).
+//     size-N2 transforms over contiguous rows.
+//   * Each sub-transform of M = 2^L points runs on a block tile of M points x C
+//     columns: the tile is staged through LDS with row-segment loads, then
+//     transformed in 1-3 radix-2^r rounds.  In a round every lane holds one
+//     2^r-point sub-DFT in VGPRs (radix-2 DIF fully unrolled; its twiddles are
+//     compile-time indices into the resident table, i.e. wave-uniform scalar
+//     loads), applies the inter-round twiddle and writes back in place, so a
+//     round needs one barrier instead of one per radix-2 stage.  The LDS tile
+//     is padded by one row every 2^r_last rows so the last round's strided
+//     accesses spread over the banks.
 //   * The LDE output rows are in bit-reversed order.  Row b*N + q holds
 //     p(s_b omega_N^rev(q)) with s_b = shift * omega_{BN}^{rev_B(b)}, so the
 //     LDE is one inverse transform followed by B forward size-N transforms of
-//     the coefficients weighted by s_b^k, and the last forward pass (a DIF,
-//     bit-reversed out) writes whole contiguous blocks of output rows.
-//   * Passes: A (inverse step 1), B (inverse step 2 fused with the B coset
-//     weightings and forward step 1, coefficients kept in registers between
-//     cosets), C (forward step 2, canonical store).  Three HBM round trips for
+//     the coefficients weighted by s_b^k.  Passes: A (inverse step 1), B
+//     (inverse step 2 fused with the B coset weightings and forward step 1,
+//     coefficients kept in VGPRs between cosets), C (forward step 2, canonical
+//     store of contiguous output row blocks).  Three HBM round trips for
 //     N > 2^11, one for N <= 2^11.
-//   * Twiddles are resident in HBM: the 2^11-point tile table is copied to LDS
-//     per block; omega_N^e for the four-step comes from two 4096-entry tables of
-//     omega_{2^24} powers (one product per lookup).
 //   * Block -> tile mapping is XCD-aware: tiles that share rows (adjacent
-//     column chunks) are given to blocks that land on the same XCD, so their
-//     row segments share an L2.
+//     column chunks) are given to blocks that land on the same XCD.
 #include "zkg_field.h"
 #include "zkg_internal.h"
 
@@ -37,13 +39,6 @@ namespace zkg {
 namespace {
 
 constexpr int NTT_THREADS = 256;
-constexpr int LDS_DATA_BYTES = 65536;
-
-template <class F>
-struct Lds {
-    static constexpr int ELEMS = LDS_DATA_BYTES / (int)sizeof(typename F::T);
-    static constexpr int KEEP = ELEMS / NTT_THREADS;  // register slots per thread
-};
 
 __device__ __forceinline__ uint32_t brev(uint32_t x, int bits)
 {
@@ -61,168 +56,271 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nblocks)
 
 template <class F>
 __device__ __forceinline__ typename F::T top_pow(const typename F::T *__restrict__ lo,
-                                                 const typename F::T *__restrict__ hi, uint64_t e,
-                                                 int logN)
+                                                 const typename F::T *__restrict__ hi, uint64_t e, int logN)
 {
     uint32_t ep = (uint32_t)(e << (LOG_TW_TOP - logN)) & ((1u << LOG_TW_TOP) - 1u);
     return F::mul(lo[ep & ((1u << TW_SPLIT) - 1u)], hi[ep >> TW_SPLIT]);
 }
 
-template <class F>
-__device__ __forceinline__ void load_tile_twiddles(typename F::T *tw, const typename F::T *__restrict__ g)
-{
-    for (int i = threadIdx.x; i < (1 << (LOG_TW_TILE - 1)); i += blockDim.x) tw[i] = g[i];
-}
+// ---------------------------------------------------------------------------
+// Tile plan: M = 2^L points split into NR radix rounds (most significant
+// digit first), at most 2^6 (BabyBear) / 2^5 (Goldilocks) points per lane.
+template <class F, int L>
+struct Plan {
+    static constexpr int RMAXLOG = sizeof(typename F::T) == 4 ? 6 : 5;
+    static constexpr int NR0 = (L + RMAXLOG - 1) / RMAXLOG;
+    static constexpr int NR = (L >= 4 && NR0 < 2) ? 2 : (NR0 < 1 ? 1 : NR0);
+    static constexpr int logr(int i) { return L / NR + (i < L % NR ? 1 : 0); }
+    static constexpr int LOGRMAX = logr(0);
+    static constexpr int LOGRLAST = logr(NR - 1);
+    static constexpr int LOG_TC = L - LOGRMAX;  // lanes per column
+    static constexpr int LOG_C = 8 - LOG_TC;    // columns per block tile
+    static constexpr int C = 1 << LOG_C;
+    static constexpr int M = 1 << L;
+    static constexpr int PADROWS = M + (M >> LOGRLAST);
+    static constexpr int LDS_ELEMS = PADROWS * C;
+    static_assert(LOG_TC <= 8, "plan needs <= 256 lanes per column");
+    // LDS slot of tile row `pos`, column `c`
+    static __device__ __forceinline__ int slot(int pos, int c) { return (pos + (pos >> LOGRLAST)) * C + c; }
+    // natural transform index held at row `pos` after the rounds: the digit
+    // groups of pos (q1 most significant) read back least significant first
+    static __device__ __forceinline__ uint32_t index_of(uint32_t pos)
+    {
+        uint32_t idx = 0;
+        int below = L, shift = 0;
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            below -= logr(i);
+            idx |= ((pos >> below) & ((1u << logr(i)) - 1u)) << shift;
+            shift += logr(i);
+        }
+        return idx;
+    }
+    // tile row that holds transform index `idx` (inverse of index_of)
+    static __device__ __forceinline__ uint32_t pos_of(uint32_t idx)
+    {
+        uint32_t pos = 0;
+        int below = L, shift = 0;
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            below -= logr(i);
+            pos |= ((idx >> shift) & ((1u << logr(i)) - 1u)) << below;
+            shift += logr(i);
+        }
+        return pos;
+    }
+};
 
-// DIF on an LDS tile [M][C]: natural order in, bit-reversed order out.
-template <class F>
-__device__ void lds_dif(typename F::T *buf, int logM, int logC, const typename F::T *tw)
+// R-point DFT in registers: radix-2 DIF, natural order in, x[p] = X[rev(p)]
+// out.  omega_{2h}^t = omega_2048^(t*1024/h) = tw[t * (1024/h)].
+template <class F, int R>
+__device__ __forceinline__ void reg_dft(typename F::T (&x)[R], const typename F::T *__restrict__ tw)
 {
     using T = typename F::T;
-    const int nbf = (1 << (logM - 1)) << logC;
-    for (int s = logM - 1; s >= 0; s--) {
-        for (int idx = threadIdx.x; idx < nbf; idx += blockDim.x) {
-            int c = idx & ((1 << logC) - 1), bf = idx >> logC;
-            int j = bf & ((1 << s) - 1), g = bf >> s;
-            int i0 = (g << (s + 1)) + j, i1 = i0 + (1 << s);
-            T a = buf[(i0 << logC) + c], b = buf[(i1 << logC) + c];
-            buf[(i0 << logC) + c] = F::add(a, b);
-            buf[(i1 << logC) + c] = F::mul(F::sub(a, b), tw[j << (LOG_TW_TILE - 1 - s)]);
+#pragma unroll
+    for (int h = R / 2; h >= 1; h >>= 1) {
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+            if ((i & h) == 0) {
+                T a = x[i], b = x[i + h];
+                x[i] = F::add(a, b);
+                const int t = i & (h - 1);
+                T d = F::sub(a, b);
+                x[i + h] = t == 0 ? d : F::mul(d, tw[t * (1024 / h)]);
+            }
         }
-        __syncthreads();
     }
 }
 
-// DIT on an LDS tile: bit-reversed order in, natural order out.
-template <class F>
-__device__ void lds_dit(typename F::T *buf, int logM, int logC, const typename F::T *tw)
+template <int R>
+__device__ constexpr int crev(int q)
+{
+    int r = 0;
+    for (int b = 1; b < R; b <<= 1) {
+        r = (r << 1) | (q & 1);
+        q >>= 1;
+    }
+    return r;
+}
+
+// One radix round of the tile transform (round I of the plan).  Sub-DFT s of
+// a column covers rows b*Mi + a + Mi1*j (j < R) and writes output q, times
+// omega_Mi^(a q) unless this is the last round, to row b*Mi + q*Mi1 + a: the
+// same rows, so rounds work in place and need no barrier inside.
+template <class F, int L, int I, int SHIFT>
+__device__ __forceinline__ void tile_round(typename F::T *lds, const typename F::T *__restrict__ tw)
 {
     using T = typename F::T;
-    const int nbf = (1 << (logM - 1)) << logC;
-    for (int s = 0; s < logM; s++) {
-        for (int idx = threadIdx.x; idx < nbf; idx += blockDim.x) {
-            int c = idx & ((1 << logC) - 1), bf = idx >> logC;
-            int j = bf & ((1 << s) - 1), g = bf >> s;
-            int i0 = (g << (s + 1)) + j, i1 = i0 + (1 << s);
-            T a = buf[(i0 << logC) + c];
-            T b = F::mul(buf[(i1 << logC) + c], tw[j << (LOG_TW_TILE - 1 - s)]);
-            buf[(i0 << logC) + c] = F::add(a, b);
-            buf[(i1 << logC) + c] = F::sub(a, b);
+    using P = Plan<F, L>;
+    constexpr int LOGR = P::logr(I);
+    constexpr int R = 1 << LOGR;
+    constexpr int LOG_MI = L - SHIFT;          // current sub-problem size
+    constexpr int LOG_MI1 = LOG_MI - LOGR;     // stride between its points
+    constexpr int NSUB = (P::M >> LOGR) >> P::LOG_TC;
+    const int c = threadIdx.x & (P::C - 1);
+    const int t = threadIdx.x >> P::LOG_C;
+#pragma unroll
+    for (int u = 0; u < NSUB; u++) {
+        const int s = t + (u << P::LOG_TC);
+        const int a = s & ((1 << LOG_MI1) - 1);
+        const int base = (s >> LOG_MI1) << LOG_MI;
+        T x[R];
+#pragma unroll
+        for (int j = 0; j < R; j++) x[j] = lds[P::slot(base + a + (j << LOG_MI1), c)];
+        reg_dft<F, R>(x, tw);
+        if (I + 1 < P::NR) {
+            // omega_Mi^(a q) = w^q, w = omega_2048^(a * 2048 / Mi)
+            T w = tw[a << (LOG_TW_TILE - LOG_MI)];
+            T p = w;
+#pragma unroll
+            for (int q = 1; q < R; q++) {
+                x[crev<R>(q)] = F::mul(x[crev<R>(q)], p);
+                if (q + 1 < R) p = F::mul(p, w);
+            }
         }
+#pragma unroll
+        for (int q = 0; q < R; q++) lds[P::slot(base + (q << LOG_MI1) + a, c)] = x[crev<R>(q)];
+    }
+}
+
+// Full tile transform: rows of the LDS tile in natural order in, row `pos`
+// holds transform index Plan::index_of(pos) out.  Ends with a barrier.
+template <class F, int L>
+__device__ __forceinline__ void tile_dft(typename F::T *lds, const typename F::T *__restrict__ tw)
+{
+    using P = Plan<F, L>;
+    tile_round<F, L, 0, 0>(lds, tw);
+    __syncthreads();
+    if constexpr (P::NR > 1) {
+        tile_round<F, L, 1, P::logr(0)>(lds, tw);
+        __syncthreads();
+    }
+    if constexpr (P::NR > 2) {
+        tile_round<F, L, 2, P::logr(0) + P::logr(1)>(lds, tw);
         __syncthreads();
     }
 }
 
 struct Geo {
     uint32_t width;
-    int logN, logN1, logN2, logC, logB;
+    int logN, logN1, logN2, logB;
     uint32_t nchunks;
 };
 
+// element (row, col) of a tile pass: idx runs over M * C in row-major order
+#define TILE_LOOP(P, idx) \
+    _Pragma("unroll 4") for (int idx = threadIdx.x; idx < (P::M << P::LOG_C); idx += NTT_THREADS)
+
 // ---------------------------------------------------------------------------
-// Pass 1 of a four-step transform (also LDE pass A): group n2 reads rows
-// n2 + N2 n1, transforms over n1 (DIF), multiplies by omega_N^(+-n2 k1) and
-// stores element k1 at row n2 + N2 k1.  Output stays in register form.
-template <class F>
+// Four-step pass 1 (also LDE pass A): group n2 reads rows n2 + N2 n1,
+// transforms over n1, multiplies by omega_N^(+-n2 k1) and stores element k1 at
+// row n2 + N2 k1 (register form).
+template <class F, int L>
 __global__ __launch_bounds__(NTT_THREADS) void k_fs_pass1(const typename F::T *__restrict__ src,
-                                                          typename F::T *__restrict__ dst, Geo g,
-                                                          int in_canon,
+                                                          typename F::T *__restrict__ dst, Geo g, int in_canon,
                                                           const typename F::T *__restrict__ tile_tw,
                                                           const typename F::T *__restrict__ top_lo,
                                                           const typename F::T *__restrict__ top_hi)
 {
     using T = typename F::T;
-    __shared__ T buf[Lds<F>::ELEMS];
-    __shared__ T tw[1 << (LOG_TW_TILE - 1)];
+    using P = Plan<F, L>;
+    __shared__ T lds[P::LDS_ELEMS];
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t n2 = tile / g.nchunks;
-    const uint32_t c0 = (tile % g.nchunks) << g.logC;
-    const int logM = g.logN1, C = 1 << g.logC;
-    load_tile_twiddles<F>(tw, tile_tw);
-    for (int idx = threadIdx.x; idx < (1 << (logM + g.logC)); idx += blockDim.x) {
-        uint32_t m = idx >> g.logC, col = c0 + (idx & (C - 1));
+    const uint32_t c0 = (tile % g.nchunks) << P::LOG_C;
+    TILE_LOOP(P, idx)
+    {
+        const int m = idx >> P::LOG_C, c = idx & (P::C - 1);
+        const uint32_t col = c0 + c;
         T v = 0;
         if (col < g.width) v = src[((uint64_t)n2 + ((uint64_t)m << g.logN2)) * g.width + col];
-        buf[idx] = in_canon ? F::from_canon(v) : v;
+        lds[P::slot(m, c)] = in_canon ? F::from_canon(v) : v;
     }
     __syncthreads();
-    lds_dif<F>(buf, logM, g.logC, tw);
-    for (int idx = threadIdx.x; idx < (1 << (logM + g.logC)); idx += blockDim.x) {
-        uint32_t p = idx >> g.logC, col = c0 + (idx & (C - 1));
+    tile_dft<F, L>(lds, tile_tw);
+    TILE_LOOP(P, idx)
+    {
+        const int pos = idx >> P::LOG_C, c = idx & (P::C - 1);
+        const uint32_t col = c0 + c;
         if (col >= g.width) continue;
-        uint32_t k1 = brev(p, logM);
+        const uint32_t k1 = P::index_of(pos);
         T w = top_pow<F>(top_lo, top_hi, (uint64_t)n2 * k1, g.logN);
-        dst[((uint64_t)n2 + ((uint64_t)k1 << g.logN2)) * g.width + col] = F::mul(buf[idx], w);
+        dst[((uint64_t)n2 + ((uint64_t)k1 << g.logN2)) * g.width + col] = F::mul(lds[P::slot(pos, c)], w);
     }
 }
 
-// Pass 2 of a plain four-step transform: group k1 reads contiguous rows
-// N2 k1 + n2, transforms over n2 (DIF) and stores element k2 at row k1 + N1 k2
-// in canonical form, times `scale` (1/N for the inverse, register form).
-template <class F>
+// Plain four-step pass 2: group k1 reads contiguous rows N2 k1 + n2,
+// transforms over n2 and stores element k2 at row k1 + N1 k2, canonical,
+// times `scale` (1/N for the inverse).
+template <class F, int L>
 __global__ __launch_bounds__(NTT_THREADS) void k_fs_pass2(const typename F::T *__restrict__ src,
                                                           typename F::T *__restrict__ dst, Geo g,
                                                           typename F::T scale,
                                                           const typename F::T *__restrict__ tile_tw)
 {
     using T = typename F::T;
-    __shared__ T buf[Lds<F>::ELEMS];
-    __shared__ T tw[1 << (LOG_TW_TILE - 1)];
+    using P = Plan<F, L>;
+    __shared__ T lds[P::LDS_ELEMS];
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t k1 = tile / g.nchunks;
-    const uint32_t c0 = (tile % g.nchunks) << g.logC;
-    const int logM = g.logN2, C = 1 << g.logC;
-    load_tile_twiddles<F>(tw, tile_tw);
-    for (int idx = threadIdx.x; idx < (1 << (logM + g.logC)); idx += blockDim.x) {
-        uint32_t m = idx >> g.logC, col = c0 + (idx & (C - 1));
+    const uint32_t c0 = (tile % g.nchunks) << P::LOG_C;
+    TILE_LOOP(P, idx)
+    {
+        const int m = idx >> P::LOG_C, c = idx & (P::C - 1);
+        const uint32_t col = c0 + c;
         T v = 0;
         if (col < g.width) v = src[(((uint64_t)k1 << g.logN2) + m) * g.width + col];
-        buf[idx] = v;
+        lds[P::slot(m, c)] = v;
     }
     __syncthreads();
-    lds_dif<F>(buf, logM, g.logC, tw);
-    for (int idx = threadIdx.x; idx < (1 << (logM + g.logC)); idx += blockDim.x) {
-        uint32_t p = idx >> g.logC, col = c0 + (idx & (C - 1));
+    tile_dft<F, L>(lds, tile_tw);
+    TILE_LOOP(P, idx)
+    {
+        const int pos = idx >> P::LOG_C, c = idx & (P::C - 1);
+        const uint32_t col = c0 + c;
         if (col >= g.width) continue;
-        uint32_t k2 = brev(p, logM);
-        dst[((uint64_t)k1 + ((uint64_t)k2 << g.logN1)) * g.width + col] = F::to_canon(F::mul(buf[idx], scale));
+        const uint32_t k2 = P::index_of(pos);
+        dst[((uint64_t)k1 + ((uint64_t)k2 << g.logN1)) * g.width + col] =
+            F::to_canon(F::mul(lds[P::slot(pos, c)], scale));
     }
 }
 
 // Single-pass transform (N <= 2^11): natural in/out, canonical in/out.
-template <class F>
+template <class F, int L>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_small(typename F::T *__restrict__ data, Geo g,
                                                            typename F::T scale,
                                                            const typename F::T *__restrict__ tile_tw)
 {
     using T = typename F::T;
-    __shared__ T buf[Lds<F>::ELEMS];
-    __shared__ T tw[1 << (LOG_TW_TILE - 1)];
-    const uint32_t c0 = xcd_tile(blockIdx.x, gridDim.x) << g.logC;
-    const int logM = g.logN, C = 1 << g.logC;
-    load_tile_twiddles<F>(tw, tile_tw);
-    for (int idx = threadIdx.x; idx < (1 << (logM + g.logC)); idx += blockDim.x) {
-        uint32_t m = idx >> g.logC, col = c0 + (idx & (C - 1));
+    using P = Plan<F, L>;
+    __shared__ T lds[P::LDS_ELEMS];
+    const uint32_t c0 = xcd_tile(blockIdx.x, gridDim.x) << P::LOG_C;
+    TILE_LOOP(P, idx)
+    {
+        const int m = idx >> P::LOG_C, c = idx & (P::C - 1);
+        const uint32_t col = c0 + c;
         T v = 0;
         if (col < g.width) v = F::from_canon(data[(uint64_t)m * g.width + col]);
-        buf[idx] = v;
+        lds[P::slot(m, c)] = v;
     }
     __syncthreads();
-    lds_dif<F>(buf, logM, g.logC, tw);
-    for (int idx = threadIdx.x; idx < (1 << (logM + g.logC)); idx += blockDim.x) {
-        uint32_t p = idx >> g.logC, col = c0 + (idx & (C - 1));
+    tile_dft<F, L>(lds, tile_tw);
+    TILE_LOOP(P, idx)
+    {
+        const int pos = idx >> P::LOG_C, c = idx & (P::C - 1);
+        const uint32_t col = c0 + c;
         if (col >= g.width) continue;
-        data[(uint64_t)brev(p, logM) * g.width + col] = F::to_canon(F::mul(buf[idx], scale));
+        data[(uint64_t)P::index_of(pos) * g.width + col] = F::to_canon(F::mul(lds[P::slot(pos, c)], scale));
     }
 }
 
 // ---------------------------------------------------------------------------
 // LDE pass B: group k1 reads contiguous rows N2 k1 + n2 of pass A's output,
-// finishes the inverse (DIF over n2 -> c[k1 + N1 k2] at slot rev(k2)), keeps
-// the coefficients in registers, and for each coset b applies s_b^k / N, the
-// forward step 1 (DIT over k2 -> j2 natural) and omega_N^(j2 k1); the result
-// goes to row b N + N2 k1 + j2 of the scratch matrix.
-template <class F>
+// finishes the inverse over n2 (c[k1 + N1 k2]), keeps the coefficients in
+// VGPRs, and for each coset b applies s_b^k / N, the forward transform over
+// k2 and omega_N^(j2 k1); the result goes to row b N + N2 k1 + j2 of the
+// scratch matrix.
+template <class F, int L>
 __global__ __launch_bounds__(NTT_THREADS) void k_lde_mid(const typename F::T *__restrict__ src,
                                                          typename F::T *__restrict__ dst, Geo g,
                                                          const typename F::T *__restrict__ tile_inv,
@@ -233,93 +331,95 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_mid(const typename F::T *__
                                                          const typename F::T *__restrict__ cos_hi)
 {
     using T = typename F::T;
-    __shared__ T buf[Lds<F>::ELEMS];
-    __shared__ T tw[1 << (LOG_TW_TILE - 1)];
+    using P = Plan<F, L>;
+    __shared__ T lds[P::LDS_ELEMS];
+    __shared__ T coef[P::M << P::LOG_C];  // c[k1 + N1 k2] at [k2][c], kept across cosets
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t k1 = tile / g.nchunks;
-    const uint32_t c0 = (tile % g.nchunks) << g.logC;
-    const int logM = g.logN2, C = 1 << g.logC;
-    const int total = 1 << (logM + g.logC);
-    load_tile_twiddles<F>(tw, tile_inv);
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        uint32_t m = idx >> g.logC, col = c0 + (idx & (C - 1));
+    const uint32_t c0 = (tile % g.nchunks) << P::LOG_C;
+    TILE_LOOP(P, idx)
+    {
+        const int m = idx >> P::LOG_C, c = idx & (P::C - 1);
+        const uint32_t col = c0 + c;
         T v = 0;
         if (col < g.width) v = src[(((uint64_t)k1 << g.logN2) + m) * g.width + col];
-        buf[idx] = v;
+        lds[P::slot(m, c)] = v;
     }
     __syncthreads();
-    lds_dif<F>(buf, logM, g.logC, tw);
-    T keep[Lds<F>::KEEP];
-#pragma unroll
-    for (int i = 0; i < Lds<F>::KEEP; i++) {
-        int idx = threadIdx.x + i * NTT_THREADS;
-        keep[i] = idx < total ? buf[idx] : T(0);
+    tile_dft<F, L>(lds, tile_inv);
+    TILE_LOOP(P, idx)
+    {
+        const int pos = idx >> P::LOG_C, c = idx & (P::C - 1);
+        coef[(P::index_of(pos) << P::LOG_C) + c] = lds[P::slot(pos, c)];
     }
     __syncthreads();
-    load_tile_twiddles<F>(tw, tile_fwd);
     for (int b = 0; b < (1 << g.logB); b++) {
         const T *clo = cos_lo + (size_t)b * (1u << TW_SPLIT);
         const T *chi = cos_hi + (size_t)b * (1u << (LOG_TW_TOP - TW_SPLIT - 2));
-#pragma unroll
-        for (int i = 0; i < Lds<F>::KEEP; i++) {
-            int idx = threadIdx.x + i * NTT_THREADS;
-            if (idx < total) {
-                uint32_t p = idx >> g.logC;
-                uint32_t k = k1 + (brev(p, logM) << g.logN1);
-                T w = F::mul(clo[k & ((1u << TW_SPLIT) - 1u)], chi[k >> TW_SPLIT]);
-                buf[idx] = F::mul(keep[i], w);
-            }
+        TILE_LOOP(P, idx)
+        {
+            const int k2 = idx >> P::LOG_C, c = idx & (P::C - 1);
+            const uint32_t k = k1 + ((uint32_t)k2 << g.logN1);
+            T w = F::mul(clo[k & ((1u << TW_SPLIT) - 1u)], chi[k >> TW_SPLIT]);
+            lds[P::slot(k2, c)] = F::mul(coef[idx], w);  // natural order for the forward rounds
         }
         __syncthreads();
-        lds_dit<F>(buf, logM, g.logC, tw);
-        for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-            uint32_t j2 = idx >> g.logC, col = c0 + (idx & (C - 1));
+        tile_dft<F, L>(lds, tile_fwd);
+        TILE_LOOP(P, idx)
+        {
+            const int pos = idx >> P::LOG_C, c = idx & (P::C - 1);
+            const uint32_t col = c0 + c;
             if (col >= g.width) continue;
+            const uint32_t j2 = P::index_of(pos);
             T w = top_pow<F>(top_lo, top_hi, (uint64_t)j2 * k1, g.logN);
-            dst[(((uint64_t)b << g.logN) + ((uint64_t)k1 << g.logN2) + j2) * g.width + col] = F::mul(buf[idx], w);
+            dst[(((uint64_t)b << g.logN) + ((uint64_t)k1 << g.logN2) + j2) * g.width + col] =
+                F::mul(lds[P::slot(pos, c)], w);
         }
         __syncthreads();
     }
 }
 
-// LDE pass C: coset b, group j2 reads rows b N + N2 k1 + j2, DIF over k1
-// (slot p holds j1 = rev(p)) and writes the contiguous block of output rows
-// b N + rev(j2) N1 + p in canonical form.
-template <class F>
+// LDE pass C: coset b, group j2 reads rows b N + N2 k1 + j2, transforms over
+// k1 and writes output row b N + rev(j2) N1 + rev(j1) (one contiguous block
+// of N1 rows per group) in canonical form.
+template <class F, int L>
 __global__ __launch_bounds__(NTT_THREADS) void k_lde_final(const typename F::T *__restrict__ src,
                                                            typename F::T *__restrict__ out, Geo g,
                                                            const typename F::T *__restrict__ tile_fwd)
 {
     using T = typename F::T;
-    __shared__ T buf[Lds<F>::ELEMS];
-    __shared__ T tw[1 << (LOG_TW_TILE - 1)];
+    using P = Plan<F, L>;
+    __shared__ T lds[P::LDS_ELEMS];
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t chunk = tile % g.nchunks;
     const uint32_t grp = tile / g.nchunks;  // b * N2 + j2
     const uint32_t j2 = grp & ((1u << g.logN2) - 1u), b = grp >> g.logN2;
-    const uint32_t c0 = chunk << g.logC;
-    const int logM = g.logN1, C = 1 << g.logC;
-    const int total = 1 << (logM + g.logC);
-    load_tile_twiddles<F>(tw, tile_fwd);
+    const uint32_t c0 = chunk << P::LOG_C;
     const uint64_t base = (uint64_t)b << g.logN;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        uint32_t k1 = idx >> g.logC, col = c0 + (idx & (C - 1));
+    TILE_LOOP(P, idx)
+    {
+        const int k1 = idx >> P::LOG_C, c = idx & (P::C - 1);
+        const uint32_t col = c0 + c;
         T v = 0;
         if (col < g.width) v = src[(base + ((uint64_t)k1 << g.logN2) + j2) * g.width + col];
-        buf[idx] = v;
+        lds[P::slot(k1, c)] = v;
     }
     __syncthreads();
-    lds_dif<F>(buf, logM, g.logC, tw);
-    const uint64_t obase = base + ((uint64_t)brev(j2, g.logN2) << g.logN1);
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        uint32_t p = idx >> g.logC, col = c0 + (idx & (C - 1));
+    tile_dft<F, L>(lds, tile_fwd);
+    const uint64_t obase = base + ((uint64_t)brev(j2, g.logN2) << L);
+    // output row obase + r holds j1 = rev_L(r): read it from tile row pos_of(j1)
+    TILE_LOOP(P, idx)
+    {
+        const int r = idx >> P::LOG_C, c = idx & (P::C - 1);
+        const uint32_t col = c0 + c;
         if (col >= g.width) continue;
-        out[(obase + p) * g.width + col] = F::to_canon(buf[idx]);
+        const uint32_t pos = P::pos_of(brev((uint32_t)r, L));
+        out[(obase + r) * g.width + col] = F::to_canon(lds[P::slot(pos, c)]);
     }
 }
 
 // Whole LDE in one pass for N <= 2^11: one block per column chunk.
-template <class F>
+template <class F, int L>
 __global__ __launch_bounds__(NTT_THREADS) void k_lde_small(const typename F::T *__restrict__ in,
                                                            typename F::T *__restrict__ out, Geo g,
                                                            const typename F::T *__restrict__ tile_inv,
@@ -327,44 +427,54 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_small(const typename F::T *
                                                            const typename F::T *__restrict__ cos_lo)
 {
     using T = typename F::T;
-    __shared__ T buf[Lds<F>::ELEMS];
-    __shared__ T tw[1 << (LOG_TW_TILE - 1)];
-    const uint32_t c0 = xcd_tile(blockIdx.x, gridDim.x) << g.logC;
-    const int logM = g.logN, C = 1 << g.logC;
-    const int total = 1 << (logM + g.logC);
-    load_tile_twiddles<F>(tw, tile_inv);
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        uint32_t m = idx >> g.logC, col = c0 + (idx & (C - 1));
+    using P = Plan<F, L>;
+    __shared__ T lds[P::LDS_ELEMS];
+    __shared__ T coef[P::M << P::LOG_C];  // coefficient k at [k][c]
+    const uint32_t c0 = xcd_tile(blockIdx.x, gridDim.x) << P::LOG_C;
+    TILE_LOOP(P, idx)
+    {
+        const int m = idx >> P::LOG_C, c = idx & (P::C - 1);
+        const uint32_t col = c0 + c;
         T v = 0;
         if (col < g.width) v = F::from_canon(in[(uint64_t)m * g.width + col]);
-        buf[idx] = v;
+        lds[P::slot(m, c)] = v;
     }
     __syncthreads();
-    if (logM > 0) lds_dif<F>(buf, logM, g.logC, tw);
-    T keep[Lds<F>::KEEP];
-#pragma unroll
-    for (int i = 0; i < Lds<F>::KEEP; i++) {
-        int idx = threadIdx.x + i * NTT_THREADS;
-        keep[i] = idx < total ? buf[idx] : T(0);
+    tile_dft<F, L>(lds, tile_inv);
+    TILE_LOOP(P, idx)
+    {
+        const int pos = idx >> P::LOG_C, c = idx & (P::C - 1);
+        coef[(P::index_of(pos) << P::LOG_C) + c] = lds[P::slot(pos, c)];
     }
     __syncthreads();
-    load_tile_twiddles<F>(tw, tile_fwd);
     for (int b = 0; b < (1 << g.logB); b++) {
         const T *clo = cos_lo + (size_t)b * (1u << TW_SPLIT);
-#pragma unroll
-        for (int i = 0; i < Lds<F>::KEEP; i++) {
-            int idx = threadIdx.x + i * NTT_THREADS;
-            if (idx < total) buf[idx] = F::mul(keep[i], clo[brev(idx >> g.logC, logM)]);
+        TILE_LOOP(P, idx)
+        {
+            const int k = idx >> P::LOG_C, c = idx & (P::C - 1);
+            lds[P::slot(k, c)] = F::mul(coef[idx], clo[k]);
         }
         __syncthreads();
-        if (logM > 0) lds_dit<F>(buf, logM, g.logC, tw);
-        for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-            uint32_t q = idx >> g.logC, col = c0 + (idx & (C - 1));
+        tile_dft<F, L>(lds, tile_fwd);
+        TILE_LOOP(P, idx)
+        {
+            const int r = idx >> P::LOG_C, c = idx & (P::C - 1);
+            const uint32_t col = c0 + c;
             if (col >= g.width) continue;
-            out[(((uint64_t)b << g.logN) + brev(q, logM)) * g.width + col] = F::to_canon(buf[idx]);
+            const uint32_t pos = P::pos_of(brev((uint32_t)r, L));
+            out[(((uint64_t)b << L) + r) * g.width + col] = F::to_canon(lds[P::slot(pos, c)]);
         }
         __syncthreads();
     }
+}
+
+// Height-1 LDE: the constant polynomial, B copies of the row.
+template <class F>
+__global__ void k_lde_one(const typename F::T *__restrict__ in, typename F::T *__restrict__ out, uint32_t width,
+                          int logB)
+{
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < (width << logB)) out[t] = in[t % width];
 }
 
 // ---------------------------------------------------------------------------
@@ -396,10 +506,10 @@ __global__ void k_dit_stage(typename F::T *__restrict__ data, uint64_t n, int lo
     data[i1 * width + c] = F::sub(a, b);
 }
 
-// coefficients (register form, n rows) -> c_k * shift^k / n, rows >= n are zero
+// coefficients (register form, n rows) -> c_k * shift^k / n
 template <class F>
-__global__ void k_scale_shift(typename F::T *__restrict__ data, uint64_t n, uint32_t width,
-                              typename F::T shift, typename F::T ninv)
+__global__ void k_scale_shift(typename F::T *__restrict__ data, uint64_t n, uint32_t width, typename F::T shift,
+                              typename F::T ninv)
 {
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n * width) return;
@@ -413,10 +523,16 @@ __global__ void k_scale_shift(typename F::T *__restrict__ data, uint64_t n, uint
 }
 
 template <class F>
-__global__ void k_to_canon(typename F::T *__restrict__ data, uint64_t n)
+__global__ void k_bitrev_swap_canon(typename F::T *__restrict__ data, uint64_t n, int logn, uint32_t width)
 {
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) data[t] = F::to_canon(data[t]);
+    if (t >= n * width) return;
+    uint64_t r = t / width, c = t - r * width;
+    uint64_t j = logn ? (uint64_t)(__brevll(r) >> (64 - logn)) : 0;
+    if (r > j) return;
+    typename F::T a = data[r * width + c], b = data[j * width + c];
+    data[r * width + c] = F::to_canon(b);
+    if (r != j) data[j * width + c] = F::to_canon(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -438,19 +554,7 @@ int ilog2(uint64_t n)
     return l;
 }
 
-// Column chunk for a tile of 2^logM points: as many columns as fit the LDS
-// tile, capped at the (power-of-two rounded) width.
-template <class F>
-int pick_logc(int logM, uint32_t width)
-{
-    int cap = ilog2(Lds<F>::ELEMS) - logM;
-    int lw = ilog2(width);
-    int lc = cap < lw ? cap : lw;
-    return lc < 0 ? 0 : lc;
-}
-
-template <class F>
-Geo make_geo(uint64_t height, uint32_t width, int logB, int logM)
+Geo make_geo(uint64_t height, uint32_t width, int logB)
 {
     Geo g;
     g.width = width;
@@ -458,15 +562,19 @@ Geo make_geo(uint64_t height, uint32_t width, int logB, int logM)
     g.logN1 = g.logN / 2;
     g.logN2 = g.logN - g.logN1;
     g.logB = logB;
-    g.logC = pick_logc<F>(logM, width);
-    g.nchunks = (width + (1u << g.logC) - 1) >> g.logC;
+    g.nchunks = 0;
     return g;
+}
+
+template <class F, int L>
+uint32_t chunks(uint32_t width)
+{
+    return (width + Plan<F, L>::C - 1) >> Plan<F, L>::LOG_C;
 }
 
 template <class F>
 typename F::T host_ninv(uint64_t n)
 {
-    // 1/n in register form
     using T = typename F::T;
     T inv = F::from_canon(1);
     T half;
@@ -478,28 +586,121 @@ typename F::T host_ninv(uint64_t n)
     return inv;
 }
 
+// Runtime L -> template L dispatch for L = 1 .. 11.
+template <template <class, int> class K, class F, class... A>
+hipError_t dispatch_l(int L, A &&...a)
+{
+    switch (L) {
+    case 1: return K<F, 1>::run(a...);
+    case 2: return K<F, 2>::run(a...);
+    case 3: return K<F, 3>::run(a...);
+    case 4: return K<F, 4>::run(a...);
+    case 5: return K<F, 5>::run(a...);
+    case 6: return K<F, 6>::run(a...);
+    case 7: return K<F, 7>::run(a...);
+    case 8: return K<F, 8>::run(a...);
+    case 9: return K<F, 9>::run(a...);
+    case 10: return K<F, 10>::run(a...);
+    case 11: return K<F, 11>::run(a...);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <class F, int L>
+struct RunPass1 {
+    static hipError_t run(const void *src, void *dst, Geo g, int in_canon, const void *tile, const void *lo,
+                          const void *hi, hipStream_t st)
+    {
+        using T = typename F::T;
+        g.nchunks = chunks<F, L>(g.width);
+        hipLaunchKernelGGL((k_fs_pass1<F, L>), dim3(g.nchunks << g.logN2), dim3(NTT_THREADS), 0, st, (const T *)src,
+                           (T *)dst, g, in_canon, (const T *)tile, (const T *)lo, (const T *)hi);
+        return hipGetLastError();
+    }
+};
+
+template <class F, int L>
+struct RunPass2 {
+    static hipError_t run(const void *src, void *dst, Geo g, typename F::T scale, const void *tile, hipStream_t st)
+    {
+        using T = typename F::T;
+        g.nchunks = chunks<F, L>(g.width);
+        hipLaunchKernelGGL((k_fs_pass2<F, L>), dim3(g.nchunks << g.logN1), dim3(NTT_THREADS), 0, st, (const T *)src,
+                           (T *)dst, g, scale, (const T *)tile);
+        return hipGetLastError();
+    }
+};
+
+template <class F, int L>
+struct RunSmall {
+    static hipError_t run(void *data, Geo g, typename F::T scale, const void *tile, hipStream_t st)
+    {
+        using T = typename F::T;
+        g.nchunks = chunks<F, L>(g.width);
+        hipLaunchKernelGGL((k_ntt_small<F, L>), dim3(g.nchunks), dim3(NTT_THREADS), 0, st, (T *)data, g, scale,
+                           (const T *)tile);
+        return hipGetLastError();
+    }
+};
+
+template <class F, int L>
+struct RunMid {
+    static hipError_t run(const void *src, void *dst, Geo g, const Tabs<F> &t, const CosetSet &cs, hipStream_t st)
+    {
+        using T = typename F::T;
+        g.nchunks = chunks<F, L>(g.width);
+        hipLaunchKernelGGL((k_lde_mid<F, L>), dim3(g.nchunks << g.logN1), dim3(NTT_THREADS), 0, st, (const T *)src,
+                           (T *)dst, g, t.tile_inv, t.tile_fwd, t.lo_fwd, t.hi_fwd, (const T *)cs.lo,
+                           (const T *)cs.hi);
+        return hipGetLastError();
+    }
+};
+
+template <class F, int L>
+struct RunFinal {
+    static hipError_t run(const void *src, void *out, Geo g, const void *tile, hipStream_t st)
+    {
+        using T = typename F::T;
+        g.nchunks = chunks<F, L>(g.width);
+        hipLaunchKernelGGL((k_lde_final<F, L>), dim3(g.nchunks << (g.logN2 + g.logB)), dim3(NTT_THREADS), 0, st,
+                           (const T *)src, (T *)out, g, (const T *)tile);
+        return hipGetLastError();
+    }
+};
+
+template <class F, int L>
+struct RunLdeSmall {
+    static hipError_t run(const void *in, void *out, Geo g, const Tabs<F> &t, const CosetSet &cs, hipStream_t st)
+    {
+        using T = typename F::T;
+        g.nchunks = chunks<F, L>(g.width);
+        hipLaunchKernelGGL((k_lde_small<F, L>), dim3(g.nchunks), dim3(NTT_THREADS), 0, st, (const T *)in, (T *)out, g,
+                           t.tile_inv, t.tile_fwd, (const T *)cs.lo);
+        return hipGetLastError();
+    }
+};
+
 template <class F>
 hipError_t coset_lde_impl(const TwiddleSet &tws, const CosetSet &cs, const void *in, void *out, void *tmp,
                           uint64_t height, uint32_t width, uint32_t log_blowup, hipStream_t st)
 {
     using T = typename F::T;
     Tabs<F> t(tws);
-    const int logN = ilog2(height);
-    if (logN <= LOG_TW_TILE) {
-        Geo g = make_geo<F>(height, width, log_blowup, logN);
-        hipLaunchKernelGGL(k_lde_small<F>, dim3(g.nchunks), dim3(NTT_THREADS), 0, st, (const T *)in, (T *)out, g,
-                           t.tile_inv, t.tile_fwd, (const T *)cs.lo);
+    Geo g = make_geo(height, width, log_blowup);
+    if (g.logN == 0) {
+        uint32_t n = width << log_blowup;
+        hipLaunchKernelGGL(k_lde_one<F>, dim3((n + 255) / 256), dim3(256), 0, st, (const T *)in, (T *)out, width,
+                           (int)log_blowup);
         return hipGetLastError();
     }
-    Geo g = make_geo<F>(height, width, log_blowup, (logN + 1) / 2);
+    if (g.logN <= LOG_TW_TILE) return dispatch_l<RunLdeSmall, F>(g.logN, in, out, g, t, cs, st);
     // pass A: in -> out[0 .. N*W) (scratch), pass B: -> tmp (B*N*W), pass C: -> out
-    hipLaunchKernelGGL(k_fs_pass1<F>, dim3(g.nchunks << g.logN2), dim3(NTT_THREADS), 0, st, (const T *)in, (T *)out,
-                       g, 1, t.tile_inv, t.lo_inv, t.hi_inv);
-    hipLaunchKernelGGL(k_lde_mid<F>, dim3(g.nchunks << g.logN1), dim3(NTT_THREADS), 0, st, (const T *)out, (T *)tmp,
-                       g, t.tile_inv, t.tile_fwd, t.lo_fwd, t.hi_fwd, (const T *)cs.lo, (const T *)cs.hi);
-    hipLaunchKernelGGL(k_lde_final<F>, dim3(g.nchunks << (g.logN2 + g.logB)), dim3(NTT_THREADS), 0, st,
-                       (const T *)tmp, (T *)out, g, t.tile_fwd);
-    return hipGetLastError();
+    hipError_t e = dispatch_l<RunPass1, F>(g.logN1, in, out, g, 1, (const void *)t.tile_inv, (const void *)t.lo_inv,
+                                           (const void *)t.hi_inv, st);
+    if (e) return e;
+    e = dispatch_l<RunMid, F>(g.logN2, (const void *)out, tmp, g, t, cs, st);
+    if (e) return e;
+    return dispatch_l<RunFinal, F>(g.logN1, (const void *)tmp, out, g, (const void *)t.tile_fwd, st);
 }
 
 template <class F>
@@ -508,33 +709,16 @@ hipError_t ntt_impl(const TwiddleSet &tws, void *data, void *tmp, uint64_t heigh
 {
     using T = typename F::T;
     Tabs<F> t(tws);
-    const int logN = ilog2(height);
+    Geo g = make_geo(height, width, 0);
+    if (g.logN == 0) return hipSuccess;
     T scale = inverse ? host_ninv<F>(height) : F::from_canon(1);
-    const T *tile = inverse ? t.tile_inv : t.tile_fwd;
-    if (logN <= LOG_TW_TILE) {
-        Geo g = make_geo<F>(height, width, 0, logN);
-        hipLaunchKernelGGL(k_ntt_small<F>, dim3(g.nchunks), dim3(NTT_THREADS), 0, st, (T *)data, g, scale, tile);
-        return hipGetLastError();
-    }
-    Geo g = make_geo<F>(height, width, 0, (logN + 1) / 2);
-    hipLaunchKernelGGL(k_fs_pass1<F>, dim3(g.nchunks << g.logN2), dim3(NTT_THREADS), 0, st, (const T *)data,
-                       (T *)tmp, g, 1, tile, inverse ? t.lo_inv : t.lo_fwd, inverse ? t.hi_inv : t.hi_fwd);
-    hipLaunchKernelGGL(k_fs_pass2<F>, dim3(g.nchunks << g.logN1), dim3(NTT_THREADS), 0, st, (const T *)tmp,
-                       (T *)data, g, scale, tile);
-    return hipGetLastError();
-}
-
-template <class F>
-__global__ void k_bitrev_swap_canon(typename F::T *__restrict__ data, uint64_t n, int logn, uint32_t width)
-{
-    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n * width) return;
-    uint64_t r = t / width, c = t - r * width;
-    uint64_t j = logn ? (uint64_t)(__brevll(r) >> (64 - logn)) : 0;
-    if (r > j) return;
-    typename F::T a = data[r * width + c], b = data[j * width + c];
-    data[r * width + c] = F::to_canon(b);
-    if (r != j) data[j * width + c] = F::to_canon(a);
+    const void *tile = inverse ? t.tile_inv : t.tile_fwd;
+    if (g.logN <= LOG_TW_TILE) return dispatch_l<RunSmall, F>(g.logN, data, g, scale, tile, st);
+    hipError_t e = dispatch_l<RunPass1, F>(g.logN1, (const void *)data, tmp, g, 1, tile,
+                                           (const void *)(inverse ? t.lo_inv : t.lo_fwd),
+                                           (const void *)(inverse ? t.hi_inv : t.hi_fwd), st);
+    if (e) return e;
+    return dispatch_l<RunPass2, F>(g.logN2, (const void *)tmp, data, g, scale, tile, st);
 }
 
 template <class F>
@@ -596,8 +780,8 @@ uint64_t ntt_tmp_bytes(int field, uint64_t height, uint32_t width)
     return height * width * (field == FIELD_GL ? 8 : 4);
 }
 
-hipError_t launch_coset_lde(int field, const TwiddleSet &tw, const CosetSet &cs, const void *in, void *out,
-                            void *tmp, uint64_t height, uint32_t width, uint32_t log_blowup, hipStream_t st)
+hipError_t launch_coset_lde(int field, const TwiddleSet &tw, const CosetSet &cs, const void *in, void *out, void *tmp,
+                            uint64_t height, uint32_t width, uint32_t log_blowup, hipStream_t st)
 {
     if (field == FIELD_GL) return coset_lde_impl<Gl>(tw, cs, in, out, tmp, height, width, log_blowup, st);
     return coset_lde_impl<Bb>(tw, cs, in, out, tmp, height, width, log_blowup, st);
