@@ -20,6 +20,7 @@
 #include "zkg_field.h"
 #include "zkg_internal.h"
 #include "zkg_poseidon2.h"
+#include "zkg_poseidon2_lanes.h"
 
 namespace zkg {
 
@@ -92,11 +93,12 @@ struct LayerPtrs {
     T *p[MAX_FUSE];
 };
 
-// Block b compresses input digests [b 2^k, (b+1) 2^k) of `in` into levels
-// 1..k; level l node i goes to out.p[l-1][i].
+// Block b compresses input digests [2 b nb1, 2 (b+1) nb1) of `in` into k
+// levels (nb1 >= 2^(k-1) nodes at the first, halving per level); level l
+// node i goes to out.p[l-1][i].
 template <class F, int W>
 __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_fused(const typename F::T *__restrict__ in,
-                                                                 LayerPtrs<typename F::T> out, int k,
+                                                                 LayerPtrs<typename F::T> out, int k, int nb1,
                                                                  const typename F::T *__restrict__ rc,
                                                                  const typename F::T *__restrict__ diag)
 {
@@ -104,7 +106,7 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_fused(const typename 
     constexpr int D = W / 2;
     __shared__ T lds[D * MERKLE_THREADS];  // element-major: lds[e * 256 + node]
     const int t = threadIdx.x;
-    int nodes = 1 << (k - 1);
+    int nodes = nb1;
     T s[W];
     if (t < nodes) {
         uint64_t i = (uint64_t)blockIdx.x * nodes + t;
@@ -135,6 +137,48 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_fused(const typename 
             for (int e = 0; e < D; e++) {
                 out.p[lvl][i * D + e] = F::to_canon(s[e]);
                 lds[e * MERKLE_THREADS + t] = s[e];
+            }
+        }
+    }
+}
+
+// Tree top: the same level fusion with each compression spread over a
+// 16-lane row (width-16 permutations only).  A block (16 rows) compresses 2^k
+// input digests, k <= 5, into levels 1..k.
+constexpr int LANES_FUSE = 5;
+
+template <class F>
+__global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_lanes(const typename F::T *__restrict__ in,
+                                                                 LayerPtrs<typename F::T> out, int k,
+                                                                 const typename F::T *__restrict__ rc,
+                                                                 const typename F::T *__restrict__ diag)
+{
+    using T = typename F::T;
+    __shared__ T lds[16 * 8];
+    const int g = threadIdx.x >> 4, lane = threadIdx.x & 15;
+    LaneConsts<F> kc;
+    load_lane_consts<F>(kc, lane, rc, diag);
+    int nodes = 1 << (k - 1);
+    T x = 0;
+    if (g < nodes) {
+        const uint64_t i = (uint64_t)blockIdx.x * nodes + g;
+        x = p2_permute_lanes<F>(F::from_canon(in[2 * i * 8 + lane]), lane, kc, rc);
+        if (lane < 8) {
+            out.p[0][i * 8 + lane] = F::to_canon(x);
+            lds[g * 8 + lane] = x;
+        }
+    }
+    for (int lvl = 1; lvl < k; lvl++) {
+        __syncthreads();
+        nodes >>= 1;
+        if (g < nodes) x = lds[2 * g * 8 + lane];  // lanes 8..15 read the right child
+        __syncthreads();
+        if (g < nodes) {
+            const uint64_t i = (uint64_t)blockIdx.x * nodes + g;
+            x = p2_permute_lanes<F>(x, lane, kc, rc);
+            if (lane < 8) {
+                out.p[lvl][i * 8 + lane] = F::to_canon(x);
+                lds[g * 8 + lane] = x;
             }
         }
     }
@@ -348,15 +392,36 @@ hipError_t merkle_impl(const P2Set &p2, void *const *layers, int nlayers, uint64
         }
         return hipSuccess;
     }
+    // Large levels: one compression per lane, 3 levels per launch (every
+    // level keeps full waves).  Levels of <= 2^17 digests are latency-bound:
+    // width-16 trees finish with 16-lane compressions, 5 levels per launch;
+    // width-8 trees fuse up to 9 levels per launch.
+    constexpr uint64_t LANE_SPLIT_MAX = 1ull << 17;
     while (lvl < nlayers - 1) {
-        int k = nlayers - 1 - lvl;
+        const int left = nlayers - 1 - lvl;
+        const uint64_t level_n = n >> lvl;
+        const bool lanes = W == 16 && level_n <= LANE_SPLIT_MAX;
+        int k = lanes ? LANES_FUSE : (level_n <= 512 ? MAX_FUSE : 3);
+        if (k > left) k = left;
         if (k > fuse) k = fuse;
-        if (k > MAX_FUSE) k = MAX_FUSE;
         LayerPtrs<T> lp;
         for (int i = 0; i < MAX_FUSE; i++) lp.p[i] = i < k ? (T *)layers[lvl + 1 + i] : nullptr;
-        uint64_t blocks = (n >> lvl) >> k;
-        hipLaunchKernelGGL((k_merkle_fused<F, W>), dim3((unsigned)blocks), dim3(MERKLE_THREADS), 0, st,
-                           (const T *)layers[lvl], lp, k, (const T *)p2.rc, (const T *)p2.diag);
+        if constexpr (W == 16) {
+            if (lanes) {
+                // a block = 16 rows of 16 lanes = 2^k inputs
+                hipLaunchKernelGGL((k_merkle_lanes<F>), dim3((unsigned)(level_n >> k)), dim3(MERKLE_THREADS), 0, st,
+                                   (const T *)layers[lvl], lp, k, (const T *)p2.rc, (const T *)p2.diag);
+                hipError_t e = hipGetLastError();
+                if (e == hipSuccess && sync_each) e = hipStreamSynchronize(st);
+                if (e) return e;
+                lvl += k;
+                continue;
+            }
+        }
+        // a block = nb1 first-level nodes (up to one per lane)
+        const uint64_t nb1 = level_n / 2 < MERKLE_THREADS ? level_n / 2 : MERKLE_THREADS;
+        hipLaunchKernelGGL((k_merkle_fused<F, W>), dim3((unsigned)(level_n / 2 / nb1)), dim3(MERKLE_THREADS), 0, st,
+                           (const T *)layers[lvl], lp, k, (int)nb1, (const T *)p2.rc, (const T *)p2.diag);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess && sync_each) e = hipStreamSynchronize(st);
         if (e) return e;

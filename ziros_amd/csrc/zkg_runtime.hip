@@ -184,17 +184,60 @@ int rounds_p_of(int field) { return field == FIELD_GL ? 22 : 13; }
 }  // namespace
 
 // ------------------------------------------------------------------ context internals
+// Device memory: a per-context cache of freed blocks (sizes rounded to 4 KiB,
+// 2 MiB above 1 MiB) so steady-state commits allocate nothing from HIP.  A
+// block is only cached after the work that used it has completed (every free
+// site follows a stream synchronisation or a completed event).  Live plus
+// cached bytes stay within the HBM budget: cached blocks are released first
+// when a new allocation would exceed it.
+static uint64_t round_alloc(uint64_t b)
+{
+    if (b == 0) b = 256;
+    uint64_t g = b >= (1ull << 20) ? (2ull << 20) : 4096;
+    return (b + g - 1) / g * g;
+}
+
+static void cache_evict_locked(zkg_ctx *ctx, uint64_t need)
+{
+    while (!ctx->cache.empty() && ctx->hbm_used + ctx->cached_bytes + need > ctx->hbm_budget) {
+        auto it = std::prev(ctx->cache.end());  // largest first
+        ctx->cached_bytes -= it->first;
+        (void)hipFree(it->second);
+        ctx->cache.erase(it);
+    }
+}
+
 int zkg::dev_alloc(zkg_ctx *ctx, uint64_t bytes, void **out)
 {
     std::lock_guard<std::mutex> g(ctx->mem_mu);
-    if (ctx->hbm_used + bytes > ctx->hbm_budget)
+    const uint64_t sz = round_alloc(bytes);
+    auto it = ctx->cache.lower_bound(sz);
+    if (it != ctx->cache.end() && it->first <= sz + sz / 4) {
+        void *p = it->second;
+        uint64_t have = it->first;
+        ctx->cached_bytes -= have;
+        ctx->cache.erase(it);
+        ctx->hbm_used += have;
+        ctx->alloc_sizes[p] = have;
+        *out = p;
+        return ZKG_OK;
+    }
+    if (ctx->hbm_used + sz > ctx->hbm_budget)
         return set_err(ZKG_ENOMEM, "HBM budget exceeded: %llu + %llu > %llu", (unsigned long long)ctx->hbm_used,
-                       (unsigned long long)bytes, (unsigned long long)ctx->hbm_budget);
+                       (unsigned long long)sz, (unsigned long long)ctx->hbm_budget);
+    cache_evict_locked(ctx, sz);
     void *p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes ? bytes : 256);
+    hipError_t e = hipMalloc(&p, sz);
+    if (e == hipErrorOutOfMemory && !ctx->cache.empty()) {
+        (void)hipGetLastError();
+        for (auto &kv : ctx->cache) (void)hipFree(kv.second);
+        ctx->cache.clear();
+        ctx->cached_bytes = 0;
+        e = hipMalloc(&p, sz);
+    }
     if (e != hipSuccess) return hip_err(e, "hipMalloc");
-    ctx->hbm_used += bytes;
-    ctx->alloc_sizes[p] = bytes;
+    ctx->hbm_used += sz;
+    ctx->alloc_sizes[p] = sz;
     *out = p;
     return ZKG_OK;
 }
@@ -204,11 +247,23 @@ void zkg::dev_free(zkg_ctx *ctx, void *p)
     if (!p) return;
     std::lock_guard<std::mutex> g(ctx->mem_mu);
     auto it = ctx->alloc_sizes.find(p);
-    if (it != ctx->alloc_sizes.end()) {
-        ctx->hbm_used -= it->second;
-        ctx->alloc_sizes.erase(it);
+    if (it == ctx->alloc_sizes.end()) {
+        (void)hipFree(p);
+        return;
     }
-    (void)hipFree(p);
+    uint64_t sz = it->second;
+    ctx->hbm_used -= sz;
+    ctx->alloc_sizes.erase(it);
+    ctx->cache.emplace(sz, p);
+    ctx->cached_bytes += sz;
+}
+
+void zkg::cache_release(zkg_ctx *ctx)
+{
+    std::lock_guard<std::mutex> g(ctx->mem_mu);
+    for (auto &kv : ctx->cache) (void)hipFree(kv.second);
+    ctx->cache.clear();
+    ctx->cached_bytes = 0;
 }
 
 namespace {
@@ -406,6 +461,7 @@ extern "C" int zkg_ctx_destroy(zkg_ctx *ctx)
     std::vector<void *> ptrs;
     for (auto &kv : ctx->alloc_sizes) ptrs.push_back(kv.first);
     for (void *p : ptrs) dev_free(ctx, p);
+    cache_release(ctx);
     for (int i = 0; i < 8; i++) (void)hipEventDestroy(ctx->ev[i]);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
