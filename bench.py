@@ -61,6 +61,17 @@ def probe_peak(kind=0, iters=4096):
     return v.value
 
 
+def probe_copy(nbytes=1 << 30):
+    lib = ctypes.CDLL(os.path.join(ROOT, "ziros_amd", "libzkg_probe.so"))
+    lib.zkg_probe_copy.restype = ctypes.c_int
+    lib.zkg_probe_copy.argtypes = [ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
+    v = ctypes.c_double()
+    rc = lib.zkg_probe_copy(nbytes, ctypes.byref(v))
+    if rc:
+        raise RuntimeError("copy probe failed: %d" % rc)
+    return v.value
+
+
 def free_tree(ctx, h):
     ctx.lib.zkg_tree_free(h)
 
@@ -279,7 +290,11 @@ def main():
     log("probe: VALU ceilings")
     peak_bb = probe_peak(0)
     peak_mul32 = probe_peak(2)
-    log("probe: %.3e BabyBear products/s, %.3e mul_lo_u32/s" % (peak_bb, peak_mul32))
+    peak_bbf = probe_peak(4)
+    copy_gbps = probe_copy()
+    log("probe: %.3e BabyBear Montgomery products/s, %.3e BabyBear FP64 products/s, %.3e mul_lo_u32/s, "
+        "copy %.0f GB/s" % (peak_bb, peak_bbf, peak_mul32, copy_gbps))
+    peak_bb = max(peak_bb, peak_bbf)
     log("c2: %d warmup + %d timed steps" % (args.warmup, args.steps))
     c2 = bench_c2(ctx, args.steps, args.warmup)
     n, w, lb = c2["n"], c2["w"], c2["lb"]
@@ -309,15 +324,17 @@ def main():
         "roofline": {"kernel": "k_hash_rows<Bb,16>", "bound": "valu", "achieved": hash_muls / (hash_ms * 1e-3) / 1e9,
                      "peak": peak_bb / 1e9, "unit": "Gmodmul/s", "frac": hash_muls / (hash_ms * 1e-3) / peak_bb,
                      "traffic": None,
-                     "note": "772 BabyBear products per permutation x %d permutations per launch; peak = measured "
-                             "Montgomery-product rate of the same routine" % (leaves * ((w + 7) // 8))},
+                     "note": "772 BabyBear products per permutation x %d permutations per launch; peak = the "
+                             "faster measured BabyBear product rate (exact FP64 or Montgomery), 8 independent "
+                             "chains per lane over the whole chip" % (leaves * ((w + 7) // 8))},
         "roofline_lde": {"kernels": "k_fs_pass1 + k_lde_mid + k_lde_final", "bound": "hbm",
                          "achieved": lde_bytes / (lde_ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                          "frac": lde_bytes / (lde_ms * 1e-3) / HBM_PEAK, "traffic": None},
         "stage_ms": {"lde": lde_ms, "row_hash": hash_ms, "merkle": merkle_ms},
         "plain": {"cells_per_s": cells / c2["plain_sec"], "ms_per_commit": c2["plain_sec"] * 1e3,
                   "fused_vs_plain": c2["plain_sec"] / c2["sec_per_step"]},
-        "probe": {"bb_montgomery_products_per_s": peak_bb, "mul_lo_u32_per_s": peak_mul32},
+        "probe": {"bb_product_ceiling_per_s": peak_bb, "bb_fp64_products_per_s": peak_bbf,
+                  "mul_lo_u32_per_s": peak_mul32, "copy_1GiB_GBps": copy_gbps},
     }
     if not args.skip_batch:
         log("c3: batched 48 traces")

@@ -24,9 +24,14 @@ FLAGS = [
 ]
 
 
+# FP64 code that relies on exact products and explicit fma must not be
+# contracted or reassociated by the compiler.
+EXTRA = {"zkg_probe.hip": ["-ffp-contract=off"], "zkg_hash.hip": ["-ffp-contract=off"]}
+
+
 def _compile(src):
     obj = os.path.join(CSRC, "_obj", src.replace(".hip", ".o"))
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, *FLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     msgs = [l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l]
     if r.returncode != 0:

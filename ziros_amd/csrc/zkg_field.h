@@ -112,6 +112,44 @@ struct Bb {
     }
 };
 
+// ------------------------------------------------- BabyBear in exact FP64
+// BabyBear elements as exact integers held in doubles, for the permutation
+// kernels: MI355X issues FP64 FMA at full rate, sums of field elements stay
+// exact below 2^53 without any reduction, and a product needs 6 FP64
+// operations.  Values are representatives (any sign, |x| < 2^41 at every
+// product input); only digests leaving the kernel are canonicalised.
+// Compile with -ffp-contract=off: every fused step below is an explicit fma.
+struct Bf {
+    using T = double;
+    static constexpr double P = 2013265921.0;
+    static constexpr double PINV = 1.0 / 2013265921.0;
+    static constexpr double MAGIC = 6755399441055744.0;  // 1.5 * 2^52: add-round to integer
+
+    // round(x / p) for |x| < 2^51 * p: x * PINV is exact inside the fma and
+    // the sum with MAGIC lies in [2^52, 2^53), whose spacing is 1.
+    __host__ __device__ static inline double quot(double x) { return __builtin_fma(x, PINV, MAGIC) - MAGIC; }
+    // a * b mod p with |a|, |b| < 2^41: h + l = a b exactly (l from fma), q is
+    // round(ab/p) up to +-1, and h - q p is an integer of magnitude < 2^32, so
+    // the fma computing it and the final add are exact.  |result| < 1.6 p.
+    __host__ __device__ static inline double mul(double a, double b)
+    {
+        double h = a * b;
+        double l = __builtin_fma(a, b, -h);
+        double q = quot(h);
+        return __builtin_fma(-q, P, h) + l;
+    }
+    // x mod p into [-p/2 - 1, p/2 + 1] for |x| < 2^52 (exact).
+    __host__ __device__ static inline double red(double x) { return __builtin_fma(-quot(x), P, x); }
+    // canonical u32 in [0, p)
+    __host__ __device__ static inline uint32_t canon(double x)
+    {
+        double r = red(x);
+        r = r < 0.0 ? r + P : r;
+        return (uint32_t)r;
+    }
+    __host__ __device__ static inline double from_u32(uint32_t v) { return (double)v; }
+};
+
 // ------------------------------------------------- BabyBear degree-4 extension
 // EF4 = Bb[x] / (x^4 - 11), coefficients low-first; components in Montgomery.
 struct Ef4 {

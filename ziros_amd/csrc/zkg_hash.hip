@@ -30,6 +30,11 @@ constexpr int HASH_THREADS = 256;
 constexpr int MERKLE_THREADS = 256;
 constexpr int MAX_FUSE = 9;  // 2 * MERKLE_THREADS leaves -> 1 root
 
+// BabyBear width-16 permutations run in exact FP64 (Bf); the Montgomery form
+// serves the 16-lane tree-top kernel and the FRI fold.
+template <class F, int W>
+constexpr bool IS_BB16 = std::is_same<F, Bb>::value && W == 16;
+
 __device__ __forceinline__ uint32_t brev(uint32_t x, int bits)
 {
     return bits ? (__brev(x) >> (32 - bits)) : 0u;
@@ -39,11 +44,21 @@ __device__ __forceinline__ uint32_t brev(uint32_t x, int bits)
 template <class F, int W>
 __global__ __launch_bounds__(HASH_THREADS) void k_p2_permute(typename F::T *__restrict__ states, uint64_t n,
                                                              const typename F::T *__restrict__ rc,
-                                                             const typename F::T *__restrict__ diag)
+                                                             const typename F::T *__restrict__ diag,
+                                                             const double *__restrict__ rcf)
 {
     using T = typename F::T;
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if constexpr (IS_BB16<F, W>) {
+        double s[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) s[k] = Bf::from_u32(states[i * 16 + k]);
+        p2_permute_bf(s, rcf);
+#pragma unroll
+        for (int k = 0; k < 16; k++) states[i * 16 + k] = Bf::canon(s[k]);
+        return;
+    }
     T s[W];
 #pragma unroll
     for (int k = 0; k < W; k++) s[k] = F::from_canon(states[i * W + k]);
@@ -60,12 +75,35 @@ template <class F, int W>
 __global__ __launch_bounds__(HASH_THREADS) void k_hash_rows(const typename F::T *__restrict__ mat, uint64_t height,
                                                             uint32_t width, typename F::T *__restrict__ digests,
                                                             const typename F::T *__restrict__ rc,
-                                                            const typename F::T *__restrict__ diag)
+                                                            const typename F::T *__restrict__ diag,
+                                                            const double *__restrict__ rcf)
 {
     using T = typename F::T;
     constexpr int R = W / 2;
     uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= height) return;
+    if constexpr (IS_BB16<F, W>) {
+        double s[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) s[k] = 0.0;
+        const uint32_t *src = mat + row * width;
+        uint32_t pos = 0;
+        for (; pos + 8 <= width; pos += 8) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) s[k] = Bf::from_u32(src[pos + k]);
+            p2_permute_bf(s, rcf);
+        }
+        if (pos < width) {
+            uint32_t take = width - pos;
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if ((uint32_t)k < take) s[k] = Bf::from_u32(src[pos + k]);
+            p2_permute_bf(s, rcf);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) digests[row * 8 + k] = Bf::canon(s[k]);
+        return;
+    }
     T s[W];
 #pragma unroll
     for (int k = 0; k < W; k++) s[k] = 0;
@@ -100,13 +138,53 @@ template <class F, int W>
 __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_fused(const typename F::T *__restrict__ in,
                                                                  LayerPtrs<typename F::T> out, int k, int nb1,
                                                                  const typename F::T *__restrict__ rc,
-                                                                 const typename F::T *__restrict__ diag)
+                                                                 const typename F::T *__restrict__ diag,
+                                                                 const double *__restrict__ rcf)
 {
     using T = typename F::T;
     constexpr int D = W / 2;
     __shared__ T lds[D * MERKLE_THREADS];  // element-major: lds[e * 256 + node]
     const int t = threadIdx.x;
     int nodes = nb1;
+    if constexpr (IS_BB16<F, W>) {
+        // canonical digests in HBM and LDS, FP64 state in registers
+        double s[16];
+        if (t < nodes) {
+            uint64_t i = (uint64_t)blockIdx.x * nodes + t;
+#pragma unroll
+            for (int e = 0; e < 16; e++) s[e] = Bf::from_u32(in[2 * i * 8 + e]);
+            p2_permute_bf(s, rcf);
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                uint32_t v = Bf::canon(s[e]);
+                out.p[0][i * 8 + e] = v;
+                lds[e * MERKLE_THREADS + t] = v;
+            }
+        }
+        for (int lvl = 1; lvl < k; lvl++) {
+            __syncthreads();
+            nodes >>= 1;
+            if (t < nodes) {
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    s[e] = Bf::from_u32(lds[e * MERKLE_THREADS + 2 * t]);
+                    s[8 + e] = Bf::from_u32(lds[e * MERKLE_THREADS + 2 * t + 1]);
+                }
+            }
+            __syncthreads();
+            if (t < nodes) {
+                p2_permute_bf(s, rcf);
+                uint64_t i = (uint64_t)blockIdx.x * nodes + t;
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    uint32_t v = Bf::canon(s[e]);
+                    out.p[lvl][i * 8 + e] = v;
+                    lds[e * MERKLE_THREADS + t] = v;
+                }
+            }
+        }
+        return;
+    }
     T s[W];
     if (t < nodes) {
         uint64_t i = (uint64_t)blockIdx.x * nodes + t;
@@ -421,7 +499,8 @@ hipError_t merkle_impl(const P2Set &p2, void *const *layers, int nlayers, uint64
         // a block = nb1 first-level nodes (up to one per lane)
         const uint64_t nb1 = level_n / 2 < MERKLE_THREADS ? level_n / 2 : MERKLE_THREADS;
         hipLaunchKernelGGL((k_merkle_fused<F, W>), dim3((unsigned)(level_n / 2 / nb1)), dim3(MERKLE_THREADS), 0, st,
-                           (const T *)layers[lvl], lp, k, (int)nb1, (const T *)p2.rc, (const T *)p2.diag);
+                           (const T *)layers[lvl], lp, k, (int)nb1, (const T *)p2.rc, (const T *)p2.diag,
+                           (const double *)p2.rc_f64);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess && sync_each) e = hipStreamSynchronize(st);
         if (e) return e;
@@ -438,13 +517,13 @@ hipError_t launch_p2_permute(int field, int width, const P2Set &p2, void *states
     dim3 g((unsigned)((n + HASH_THREADS - 1) / HASH_THREADS)), b(HASH_THREADS);
     if (field == FIELD_BB && width == 16)
         hipLaunchKernelGGL((k_p2_permute<Bb, 16>), g, b, 0, st, (uint32_t *)states, n, (const uint32_t *)p2.rc,
-                           (const uint32_t *)p2.diag);
+                           (const uint32_t *)p2.diag, (const double *)p2.rc_f64);
     else if (field == FIELD_GL && width == 16)
         hipLaunchKernelGGL((k_p2_permute<Gl, 16>), g, b, 0, st, (uint64_t *)states, n, (const uint64_t *)p2.rc,
-                           (const uint64_t *)p2.diag);
+                           (const uint64_t *)p2.diag, nullptr);
     else if (field == FIELD_GL && width == 8)
         hipLaunchKernelGGL((k_p2_permute<Gl, 8>), g, b, 0, st, (uint64_t *)states, n, (const uint64_t *)p2.rc,
-                           (const uint64_t *)p2.diag);
+                           (const uint64_t *)p2.diag, nullptr);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -457,13 +536,14 @@ hipError_t launch_hash_rows(int field, int width, const P2Set &p2, const void *m
     dim3 g((unsigned)((height + HASH_THREADS - 1) / HASH_THREADS)), b(HASH_THREADS);
     if (field == FIELD_BB && width == 16)
         hipLaunchKernelGGL((k_hash_rows<Bb, 16>), g, b, 0, st, (const uint32_t *)mat, height, mat_width,
-                           (uint32_t *)digests, (const uint32_t *)p2.rc, (const uint32_t *)p2.diag);
+                           (uint32_t *)digests, (const uint32_t *)p2.rc, (const uint32_t *)p2.diag,
+                           (const double *)p2.rc_f64);
     else if (field == FIELD_GL && width == 16)
         hipLaunchKernelGGL((k_hash_rows<Gl, 16>), g, b, 0, st, (const uint64_t *)mat, height, mat_width,
-                           (uint64_t *)digests, (const uint64_t *)p2.rc, (const uint64_t *)p2.diag);
+                           (uint64_t *)digests, (const uint64_t *)p2.rc, (const uint64_t *)p2.diag, nullptr);
     else if (field == FIELD_GL && width == 8)
         hipLaunchKernelGGL((k_hash_rows<Gl, 8>), g, b, 0, st, (const uint64_t *)mat, height, mat_width,
-                           (uint64_t *)digests, (const uint64_t *)p2.rc, (const uint64_t *)p2.diag);
+                           (uint64_t *)digests, (const uint64_t *)p2.rc, (const uint64_t *)p2.diag, nullptr);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

@@ -33,6 +33,7 @@ struct CosetSet {
 struct P2Set {
     void *rc = nullptr;
     void *diag = nullptr;
+    void *rc_f64 = nullptr;  // BabyBear only: canonical constants as doubles
     uint64_t seed = 0;
     bool loaded = false;
 };

@@ -136,4 +136,95 @@ __device__ __forceinline__ void p2_permute(typename F::T (&s)[W], const typename
     for (int r = 0; r < HALF; r++) external_round<F, W>(s, tr + r * W);
 }
 
+// ---------------------------------------------------------------------------
+// BabyBear width-16 permutation in exact FP64 (see Bf in zkg_field.h).
+// Bounds (p < 2^31): the input state is first reduced to |s| <= p/2 + 1
+// (capacity lanes carry unreduced values between sponge absorptions); MDS-light
+// multiplies a bound by at most 35, S-box outputs are < 1.6 p, so external
+// rounds see |s| < 57 p < 2^37; internal rounds reduce the sum and the lanes
+// with |V_i| >= 2 before use and the V = 1 lane grows by < p/2 per round, so
+// every product input stays below 2^37 (the product routine needs < 2^41) and
+// every sum below 2^41 (exactness needs < 2^53).
+__host__ __device__ __forceinline__ void mds_light_bf(double (&s)[16])
+{
+#pragma unroll
+    for (int c = 0; c < 16; c += 4) {
+        double a = s[c], b = s[c + 1], cc = s[c + 2], d = s[c + 3];
+        double quad = (a + b) + (cc + d);
+        s[c] = __builtin_fma(2.0, b, quad + a);        // 2a + 3b + c + d
+        s[c + 1] = __builtin_fma(2.0, cc, quad + b);   // a + 2b + 3c + d
+        s[c + 2] = __builtin_fma(2.0, d, quad + cc);   // a + b + 2c + 3d
+        s[c + 3] = __builtin_fma(2.0, a, quad + d);    // 3a + b + c + 2d
+    }
+    double sum[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) sum[k] = (s[k] + s[4 + k]) + (s[8 + k] + s[12 + k]);
+#pragma unroll
+    for (int i = 0; i < 16; i++) s[i] = s[i] + sum[i & 3];
+}
+
+__host__ __device__ __forceinline__ double sbox7_bf(double x)
+{
+    double x2 = Bf::mul(x, x);
+    double x3 = Bf::mul(x2, x);
+    double x4 = Bf::mul(x2, x2);
+    return Bf::mul(x3, x4);
+}
+
+// canonical representatives of 2^-k (k <= 27) mod p: ((p + 1) / 2)^k
+__host__ __device__ constexpr double bf_inv_pow2(int k)
+{
+    uint64_t r = 1, h = (2013265921ull + 1) / 2;
+    for (int i = 0; i < k; i++) r = r * h % 2013265921ull;
+    return (double)r;
+}
+
+// rc: flat [initial 4x16][internal 13][terminal 4x16], canonical, as doubles.
+__host__ __device__ __forceinline__ void p2_permute_bf(double (&s)[16], const double *__restrict__ rc)
+{
+#pragma unroll
+    for (int i = 0; i < 16; i++) s[i] = Bf::red(s[i]);
+    mds_light_bf(s);
+#pragma unroll 1
+    for (int r = 0; r < 4; r++) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) s[i] = sbox7_bf(s[i] + rc[r * 16 + i]);
+        mds_light_bf(s);
+    }
+    const double *ir = rc + 64;
+#pragma unroll 1
+    for (int r = 0; r < 13; r++) {
+        s[0] = sbox7_bf(s[0] + ir[r]);
+        double sum = s[0];
+#pragma unroll
+        for (int i = 1; i < 16; i++) sum += s[i];
+        sum = Bf::red(sum);
+        // s_i = V_i s_i + sum, V = [-2, 1, 2, 1/2, 3, 4, -1/2, -3, -4, 2^-8,
+        // 1/4, 1/8, 2^-27, -2^-8, -1/16, -2^-27]
+        s[0] = __builtin_fma(-2.0, s[0], sum);
+        s[1] = s[1] + sum;
+        s[2] = __builtin_fma(2.0, Bf::red(s[2]), sum);
+        s[3] = Bf::mul(s[3], bf_inv_pow2(1)) + sum;
+        s[4] = __builtin_fma(3.0, Bf::red(s[4]), sum);
+        s[5] = __builtin_fma(4.0, Bf::red(s[5]), sum);
+        s[6] = Bf::mul(s[6], -bf_inv_pow2(1)) + sum;
+        s[7] = __builtin_fma(-3.0, Bf::red(s[7]), sum);
+        s[8] = __builtin_fma(-4.0, Bf::red(s[8]), sum);
+        s[9] = Bf::mul(s[9], bf_inv_pow2(8)) + sum;
+        s[10] = Bf::mul(s[10], bf_inv_pow2(2)) + sum;
+        s[11] = Bf::mul(s[11], bf_inv_pow2(3)) + sum;
+        s[12] = Bf::mul(s[12], bf_inv_pow2(27)) + sum;
+        s[13] = Bf::mul(s[13], -bf_inv_pow2(8)) + sum;
+        s[14] = Bf::mul(s[14], -bf_inv_pow2(4)) + sum;
+        s[15] = Bf::mul(s[15], -bf_inv_pow2(27)) + sum;
+    }
+    const double *tr = ir + 13;
+#pragma unroll 1
+    for (int r = 0; r < 4; r++) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) s[i] = sbox7_bf(s[i] + tr[r * 16 + i]);
+        mds_light_bf(s);
+    }
+}
+
 }  // namespace zkg

@@ -5,7 +5,8 @@
 // Each lane runs 8 independent multiply chains (enough ILP to keep the VALU
 // issuing) for `iters` steps; the grid holds 8 waves per SIMD on every CU.
 // Kinds: 0 = BabyBear Montgomery product (the exact routine the kernels use),
-// 1 = Goldilocks product, 2 = raw 32-bit v_mul_lo_u32.
+// 1 = Goldilocks product, 2 = raw 32-bit v_mul_lo_u32, 3 = FP64 FMA,
+// 4 = BabyBear product in exact FP64 arithmetic (6 FP64 operations).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -41,6 +42,34 @@ __global__ __launch_bounds__(256) void k_chain(uint64_t *out, uint32_t iters, ui
 #pragma unroll
         for (int i = 0; i < 8; i++) acc ^= x[i];
         out[t] = acc;
+    } else if constexpr (KIND == 3) {
+        double x[8], y = 1.0000001, z = 1e-9;
+#pragma unroll
+        for (int i = 0; i < 8; i++) x[i] = (double)(t + i);
+        for (uint32_t k = 0; k < iters; k++)
+#pragma unroll
+            for (int i = 0; i < 8; i++) x[i] = __builtin_fma(x[i], y, z);
+        double acc = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) acc += x[i];
+        out[t] = (uint64_t)acc;
+    } else if constexpr (KIND == 4) {
+        const double P = 2013265921.0, PINV = 1.0 / 2013265921.0, MAGIC = 6755399441055744.0;
+        double x[8], y = (double)(seed % 2013265921ull);
+#pragma unroll
+        for (int i = 0; i < 8; i++) x[i] = (double)((t * 2654435761u + i) % 2013265921u);
+        for (uint32_t k = 0; k < iters; k++)
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                double h = x[i] * y;
+                double l = __builtin_fma(x[i], y, -h);
+                double q = __builtin_fma(h, PINV, MAGIC) - MAGIC;
+                x[i] = __builtin_fma(-q, P, h) + l;
+            }
+        double acc = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) acc += x[i];
+        out[t] = (uint64_t)(acc + 4e9);
     } else {
         uint32_t x[8], y = (uint32_t)seed | 1u;
 #pragma unroll
@@ -55,7 +84,43 @@ __global__ __launch_bounds__(256) void k_chain(uint64_t *out, uint32_t iters, ui
     }
 }
 
+__global__ void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace
+
+// Plain copy of `bytes` (a multiple of 16) between two fresh buffers: the
+// known-size reference that scales HBM traffic counters.  Returns GB/s.
+extern "C" int zkg_probe_copy(uint64_t bytes, double *gbps)
+{
+    uint4 *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess) return -3;
+    if (hipMalloc(&b, bytes) != hipSuccess) {
+        (void)hipFree(a);
+        return -3;
+    }
+    (void)hipMemset(a, 1, bytes);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const uint64_t n = bytes / 16;
+    (void)hipEventRecord(e0, 0);
+    hipLaunchKernelGGL(k_copy, dim3(256 * 16), dim3(256), 0, 0, a, b, n);
+    (void)hipEventRecord(e1, 0);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess || ms <= 0) return -2;
+    *gbps = 2.0 * bytes / (ms * 1e-3) / 1e9;
+    return 0;
+}
 
 // Operations per second of `kind` over the whole device (8 products per lane
 // per iteration).  Returns 0 on success.
@@ -76,6 +141,10 @@ extern "C" int zkg_probe_rate(int kind, uint32_t iters, double *ops_per_s)
             hipLaunchKernelGGL(k_chain<0>, dim3(blocks), dim3(256), 0, 0, out, it, 12345);
         else if (kind == 1)
             hipLaunchKernelGGL(k_chain<1>, dim3(blocks), dim3(256), 0, 0, out, it, 12345);
+        else if (kind == 3)
+            hipLaunchKernelGGL(k_chain<3>, dim3(blocks), dim3(256), 0, 0, out, it, 12345);
+        else if (kind == 4)
+            hipLaunchKernelGGL(k_chain<4>, dim3(blocks), dim3(256), 0, 0, out, it, 12345);
         else
             hipLaunchKernelGGL(k_chain<2>, dim3(blocks), dim3(256), 0, 0, out, it, 12345);
     };

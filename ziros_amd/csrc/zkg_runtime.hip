@@ -389,10 +389,17 @@ int load_p2(zkg_ctx *ctx, int idx, int width, const std::vector<uint64_t> &flat_
     P2Set &p = ctx->p2[idx];
     dev_free(ctx, p.rc);
     dev_free(ctx, p.diag);
-    p.rc = p.diag = nullptr;
+    dev_free(ctx, p.rc_f64);
+    p.rc = p.diag = p.rc_f64 = nullptr;
     int r;
     if ((r = upload_vec<F>(ctx, rc, &p.rc))) return r;
     if ((r = upload_vec<F>(ctx, dg, &p.diag))) return r;
+    if constexpr (sizeof(typename F::T) == 4) {
+        std::vector<double> rf(flat_canon.size());
+        for (size_t i = 0; i < flat_canon.size(); i++) rf[i] = (double)flat_canon[i];
+        if ((r = dev_alloc(ctx, rf.size() * sizeof(double), &p.rc_f64))) return r;
+        HIP_TRY(hipMemcpy(p.rc_f64, rf.data(), rf.size() * sizeof(double), hipMemcpyHostToDevice), "upload rc");
+    }
     p.loaded = true;
     return ZKG_OK;
 }
