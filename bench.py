@@ -72,6 +72,20 @@ def probe_copy(nbytes=1 << 30):
     return v.value
 
 
+def profiled_traffic():
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (calibrated against a 1 GiB copy), read from the committed summary."""
+    out = {}
+    try:
+        for line in open(os.path.join(ROOT, "profiles", "commit_bb_2p20x64_traffic.md")):
+            k, _, v = line.partition(":")
+            if k.startswith("traffic_") and v.strip().isdigit():
+                out[k] = int(v)
+    except OSError:
+        pass
+    return out
+
+
 def free_tree(ctx, h):
     ctx.lib.zkg_tree_free(h)
 
@@ -305,6 +319,7 @@ def main():
     hash_muls = leaves * ((w + 7) // 8) * BB_PERM_MULS
     lde_bytes = n * w * 4 * (1 + (1 << lb))
     log("c2: %.3f ms/step, stages %s" % (c2["sec_per_step"] * 1e3, c2["stage_ms"]))
+    traffic = profiled_traffic()
     out = {
         "metric": "trace cells committed/sec, batched vs plain per-trace GPU; p99 commit latency",
         "value": value,
@@ -323,13 +338,14 @@ def main():
                    "height": n, "width": w, "log_blowup": lb, "field": "babybear", "poseidon2_width": 16},
         "roofline": {"kernel": "k_hash_rows<Bb,16>", "bound": "valu", "achieved": hash_muls / (hash_ms * 1e-3) / 1e9,
                      "peak": peak_bb / 1e9, "unit": "Gmodmul/s", "frac": hash_muls / (hash_ms * 1e-3) / peak_bb,
-                     "traffic": None,
+                     "traffic": traffic.get("traffic_hash_rows_bytes"),
                      "note": "772 BabyBear products per permutation x %d permutations per launch; peak = the "
                              "faster measured BabyBear product rate (exact FP64 or Montgomery), 8 independent "
                              "chains per lane over the whole chip" % (leaves * ((w + 7) // 8))},
         "roofline_lde": {"kernels": "k_fs_pass1 + k_lde_mid + k_lde_final", "bound": "hbm",
                          "achieved": lde_bytes / (lde_ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                         "frac": lde_bytes / (lde_ms * 1e-3) / HBM_PEAK, "traffic": None},
+                         "frac": lde_bytes / (lde_ms * 1e-3) / HBM_PEAK,
+                         "traffic": traffic.get("traffic_lde_bytes"), "algorithmic_bytes": lde_bytes},
         "stage_ms": {"lde": lde_ms, "row_hash": hash_ms, "merkle": merkle_ms},
         "plain": {"cells_per_s": cells / c2["plain_sec"], "ms_per_commit": c2["plain_sec"] * 1e3,
                   "fused_vs_plain": c2["plain_sec"] / c2["sec_per_step"]},
