@@ -126,6 +126,60 @@ def bench_c2(ctx, steps, warmup):
     return dict(n=n, w=w, lb=lb, sec_per_step=dt / steps, stage_ms=stages.tolist(), plain_sec=dt_plain)
 
 
+def bench_c1(ctx, steps):
+    """Config 1: Goldilocks 2^14 x 8, blowup 2, Poseidon2 width 8 (rate 4,
+    digest 4) as configured, and width 16 (rate 8, digest 8) as the reference
+    STARK uses.  Small: latency of one synchronous commit."""
+    n, w, lb = 1 << 14, 8, 1
+    trace = ctx.random(Z.GOLDILOCKS, n * w, 0x5EED0001)
+    lde = ctx.alloc((n << lb) * w * 8)
+    out = {}
+    for p2w in (8, 16):
+        for _ in range(3):
+            free_tree(ctx, commit_once(ctx, ctx.lib.zkg_commit, Z.GOLDILOCKS, p2w, trace, n, w, lb, lde))
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            free_tree(ctx, commit_once(ctx, ctx.lib.zkg_commit, Z.GOLDILOCKS, p2w, trace, n, w, lb, lde))
+        dt = (time.perf_counter() - t0) / steps
+        t0 = time.perf_counter()
+        for _ in range(3):
+            free_tree(ctx, commit_once(ctx, ctx.lib.zkg_commit_plain, Z.GOLDILOCKS, p2w, trace, n, w, lb, lde))
+        dtp = (time.perf_counter() - t0) / 3
+        out["poseidon2_w%d" % p2w] = dict(ms_per_commit=dt * 1e3, cells_per_s=n * w / dt, plain_ms=dtp * 1e3)
+    trace.free()
+    lde.free()
+    return out
+
+
+def bench_c4(ctx, steps):
+    """Config 4: FRI commit phase over EF4, codeword 2^22 (a 2^20-coefficient
+    polynomial at blowup 4) folded to 2^8, one Poseidon2-16 tree per layer."""
+    import numpy as np  # noqa: F811
+
+    log_len, log_final = 22, 8
+    coeff = ctx.random(Z.BABYBEAR, (1 << 20) * 4, 0x5EED0004)
+    cw = ctx.alloc((1 << log_len) * 16)
+    check(ctx.lib.zkg_coset_lde_batch(ctx.h, Z.BABYBEAR, coeff.h, 1 << 20, 4, 2, 31, cw.h))
+    rng = np.random.default_rng(0x5EED0004)
+    betas = rng.integers(0, 0x78000001, (log_len - log_final) * 4).astype(np.uint32)
+    fri = Z.HipFri(ctx)
+    ctx.set_timing(True)
+    for _ in range(2):
+        fri.commit_phase(cw, log_len, log_final, betas).free()
+    t0 = time.perf_counter()
+    dev = 0.0
+    for _ in range(steps):
+        f = fri.commit_phase(cw, log_len, log_final, betas)
+        dev += ctx.stage_ms()[3]
+        f.free()
+    dt = (time.perf_counter() - t0) / steps
+    ctx.set_timing(False)
+    coeff.free()
+    cw.free()
+    return dict(codeword=1 << log_len, layers=log_len - log_final, ms_per_commit_phase=dt * 1e3,
+                device_ms=dev / steps, codeword_elems_per_s=(1 << log_len) / dt)
+
+
 def c3_shapes():
     rng = np.random.default_rng(SEED_C3)
     logs = rng.integers(12, 21, 48)
@@ -352,6 +406,12 @@ def main():
         "probe": {"bb_product_ceiling_per_s": peak_bb, "bb_fp64_products_per_s": peak_bbf,
                   "mul_lo_u32_per_s": peak_mul32, "copy_1GiB_GBps": copy_gbps},
     }
+    log("c1: goldilocks 2^14 x 8")
+    out["c1"] = bench_c1(ctx, max(5, args.steps))
+    log("c1: %s" % out["c1"])
+    log("c4: fri commit phase 2^22 -> 2^8")
+    out["c4_fri"] = bench_c4(ctx, max(3, args.steps // 2))
+    log("c4: %s" % out["c4_fri"])
     if not args.skip_batch:
         log("c3: batched 48 traces")
         out["batched"] = bench_c3(ctx, max(1, args.steps // 4), 1)

@@ -9,6 +9,7 @@
 #include "zkg_internal.h"
 #include "zkg_runtime.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 
@@ -78,28 +79,83 @@ static int prepare_job(zkg_ctx *ctx, const zkg_job &j, Prepared &p)
     return ZKG_OK;
 }
 
-// Stream-ordered launches of a set of prepared jobs: all LDEs, then all row
-// hashes, then all trees (no host synchronisation in between).
-static int enqueue_jobs(zkg_ctx *ctx, hipStream_t st, std::vector<Prepared> &ps)
+// One job's pipeline on one stream: LDE, row hash, tree.
+static int enqueue_one(zkg_ctx *ctx, hipStream_t st, Prepared &p)
 {
-    for (auto &p : ps) {
-        int rc = run_lde(ctx, st, p.job.field, p.job.trace->ptr, p.job.height, p.job.width, p.job.log_blowup,
-                         p.job.field == FIELD_GL ? 7 : 31, p.job.lde_out->ptr, p.tmp);
-        if (rc) return rc;
-    }
-    for (auto &p : ps) {
-        const P2Set *p2 = get_p2(ctx, p.job.field, p.job.p2_width);
-        hipError_t e = launch_hash_rows(p.job.field, p.job.p2_width, *p2, p.tree->mat, p.tree->n_leaves, p.job.width,
-                                        p.tree->layers[0], st);
-        if (e != hipSuccess) return hip_err(e, "batch hash");
-    }
-    for (auto &p : ps) {
-        const P2Set *p2 = get_p2(ctx, p.job.field, p.job.p2_width);
-        hipError_t e = launch_merkle_tree(p.job.field, p.job.p2_width, *p2, p.tree->layers.data(), p.tree->nlayers,
-                                          p.tree->n_leaves, 9, st, false);
-        if (e != hipSuccess) return hip_err(e, "batch merkle");
-    }
+    int rc = run_lde(ctx, st, p.job.field, p.job.trace->ptr, p.job.height, p.job.width, p.job.log_blowup,
+                     p.job.field == FIELD_GL ? 7 : 31, p.job.lde_out->ptr, p.tmp);
+    if (rc) return rc;
+    const P2Set *p2 = get_p2(ctx, p.job.field, p.job.p2_width);
+    hipError_t e = launch_hash_rows(p.job.field, p.job.p2_width, *p2, p.tree->mat, p.tree->n_leaves, p.job.width,
+                                    p.tree->layers[0], st);
+    if (e != hipSuccess) return hip_err(e, "batch hash");
+    e = launch_merkle_tree(p.job.field, p.job.p2_width, *p2, p.tree->layers.data(), p.tree->nlayers,
+                           p.tree->n_leaves, 9, st, false);
+    if (e != hipSuccess) return hip_err(e, "batch merkle");
     return ZKG_OK;
+}
+
+// Fork/join over K side streams: jobs, largest first, are dealt round-robin
+// to the branches so their pipelines overlap on the device (small jobs'
+// latency-bound phases fill in around large jobs).  Under stream capture the
+// branches become parallel paths of the graph.
+struct Fork {
+    int k = 0;
+    hipStream_t s[8] = {};
+    hipEvent_t start = nullptr;
+    hipEvent_t join[8] = {};
+};
+
+static int fork_init(Fork &f, int k)
+{
+    f.k = k;
+    hipError_t e = hipEventCreateWithFlags(&f.start, hipEventDisableTiming);
+    for (int i = 0; i < k && e == hipSuccess; i++) {
+        e = hipStreamCreateWithFlags(&f.s[i], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&f.join[i], hipEventDisableTiming);
+    }
+    return e == hipSuccess ? ZKG_OK : hip_err(e, "fork streams");
+}
+
+static void fork_destroy(Fork &f)
+{
+    for (int i = 0; i < f.k; i++) {
+        if (f.s[i]) {
+            (void)hipStreamSynchronize(f.s[i]);
+            (void)hipStreamDestroy(f.s[i]);
+        }
+        if (f.join[i]) (void)hipEventDestroy(f.join[i]);
+    }
+    if (f.start) (void)hipEventDestroy(f.start);
+    f = Fork();
+}
+
+static uint64_t job_cells(const zkg_job &j) { return (j.height << j.log_blowup) * j.width * (j.field == FIELD_GL ? 4 : 1); }
+
+static int enqueue_jobs(zkg_ctx *ctx, hipStream_t st, std::vector<Prepared> &ps, Fork *f)
+{
+    if (!f || f->k <= 1 || ps.size() <= 1) {
+        for (auto &p : ps)
+            if (int rc = enqueue_one(ctx, st, p)) return rc;
+        return ZKG_OK;
+    }
+    std::vector<size_t> order(ps.size());
+    for (size_t i = 0; i < ps.size(); i++) order[i] = i;
+    std::sort(order.begin(), order.end(),
+              [&](size_t a, size_t b) { return job_cells(ps[a].job) > job_cells(ps[b].job); });
+    const int k = (int)std::min<size_t>((size_t)f->k, ps.size());
+    hipError_t e = hipEventRecord(f->start, st);
+    for (int i = 0; i < k && e == hipSuccess; i++) e = hipStreamWaitEvent(f->s[i], f->start, 0);
+    if (e != hipSuccess) return hip_err(e, "fork");
+    int rc = ZKG_OK;
+    for (size_t j = 0; j < order.size() && !rc; j++) rc = enqueue_one(ctx, f->s[j % k], ps[order[j]]);
+    // always join, so the origin stream (or capture) is left consistent
+    for (int i = 0; i < k; i++) {
+        hipError_t e2 = hipEventRecord(f->join[i], f->s[i]);
+        if (e2 == hipSuccess) e2 = hipStreamWaitEvent(st, f->join[i], 0);
+        if (e2 != hipSuccess && !rc) rc = hip_err(e2, "join");
+    }
+    return rc;
 }
 
 // Cached executable graph for one job-set (same shapes, buffers and trees).
@@ -107,6 +163,7 @@ struct BatchState {
     std::vector<uint64_t> key;
     hipGraphExec_t exec = nullptr;
     std::vector<Prepared> prepared;
+    Fork fork;
 };
 
 static std::vector<uint64_t> batch_key(const zkg_job *jobs, uint32_t n)
@@ -126,6 +183,7 @@ void batch_release(zkg_ctx *ctx)
     if (!b) return;
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     for (auto &p : b->prepared) release_prepared(ctx, p, false);
+    fork_destroy(b->fork);
     delete b;
     ctx->batch = nullptr;
 }
@@ -171,11 +229,12 @@ extern "C" int zkg_commit_batch(zkg_ctx *ctx, const zkg_job *jobs, uint32_t n, z
             delete b;
             return rc;
         }
+        if (!rc) rc = fork_init(b->fork, 4);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-        if (e != hipSuccess) rc = hip_err(e, "begin capture");
+        if (!rc && e != hipSuccess) rc = hip_err(e, "begin capture");
         if (!rc) {
-            rc = enqueue_jobs(ctx, st, b->prepared);
+            rc = enqueue_jobs(ctx, st, b->prepared, &b->fork);
             e = hipStreamEndCapture(st, &g);
             if (!rc && e != hipSuccess) rc = hip_err(e, "end capture");
         }
@@ -187,6 +246,7 @@ extern "C" int zkg_commit_batch(zkg_ctx *ctx, const zkg_job *jobs, uint32_t n, z
         if (rc) {
             for (auto &p : b->prepared) release_prepared(ctx, p, false);
             if (b->exec) (void)hipGraphExecDestroy(b->exec);
+            fork_destroy(b->fork);
             delete b;
             return rc;
         }
@@ -233,6 +293,7 @@ struct Request {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
+    Fork fork;
     bool busy = false;
     std::vector<Prepared> prepared;
     std::vector<uint64_t> tickets;
@@ -323,7 +384,7 @@ static void serve_loop(zkg_ctx *ctx, ServeState *s)
                     continue;
                 }
             }
-            rc = enqueue_jobs(ctx, sl.stream, ps);
+            rc = enqueue_jobs(ctx, sl.stream, ps, &sl.fork);
             if (rc == ZKG_OK) {
                 hipError_t e = hipEventRecord(sl.done, sl.stream);
                 if (e != hipSuccess) rc = hip_err(e, "event record");
@@ -368,6 +429,7 @@ void serve_shutdown(zkg_ctx *ctx)
     if (s->worker.joinable()) s->worker.join();
     for (Slot &sl : s->slots) {
         (void)hipStreamSynchronize(sl.stream);
+        fork_destroy(sl.fork);
         (void)hipEventDestroy(sl.done);
         (void)hipStreamDestroy(sl.stream);
     }
@@ -390,13 +452,15 @@ extern "C" int zkg_serve_start(zkg_ctx *ctx, uint32_t slots, uint32_t max_batch)
     for (Slot &sl : s->slots) {
         hipError_t e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
-        if (e != hipSuccess) {
+        int rc = e == hipSuccess ? fork_init(sl.fork, 2) : hip_err(e, "serve stream");
+        if (rc != ZKG_OK) {
             for (Slot &x : s->slots) {
+                fork_destroy(x.fork);
                 if (x.done) (void)hipEventDestroy(x.done);
                 if (x.stream) (void)hipStreamDestroy(x.stream);
             }
             delete s;
-            return hip_err(e, "serve stream");
+            return rc;
         }
     }
     ctx->serve = s;
