@@ -47,6 +47,9 @@ struct zkg_ctx {
     std::map<zkg::CosetKey, zkg::CosetSet> cosets;
     bool timing = false;
     hipEvent_t ev[8] = {};
+    hipStream_t side[2] = {};      // FRI layer trees overlap the fold chain
+    hipEvent_t side_join[2] = {};
+    hipEvent_t fri_ev[32] = {};    // leaf digests of FRI layer r ready
     double stage_ms[4] = {0, 0, 0, 0};
     zkg::BatchState *batch = nullptr;
     zkg::ServeState *serve = nullptr;

@@ -448,6 +448,11 @@ extern "C" int zkg_ctx_create(int device, uint64_t budget, zkg_ctx **out)
         return hip_err(e, "hipStreamCreate");
     }
     for (int i = 0; i < 8; i++) (void)hipEventCreate(&c->ev[i]);
+    for (int i = 0; i < 2; i++) {
+        (void)hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking);
+        (void)hipEventCreateWithFlags(&c->side_join[i], hipEventDisableTiming);
+    }
+    for (int i = 0; i < 32; i++) (void)hipEventCreateWithFlags(&c->fri_ev[i], hipEventDisableTiming);
     int rc = build_twiddles<Gl>(c, c->tw[FIELD_GL]);
     if (!rc) rc = build_twiddles<Bb>(c, c->tw[FIELD_BB]);
     if (rc) {
@@ -470,6 +475,15 @@ extern "C" int zkg_ctx_destroy(zkg_ctx *ctx)
     for (void *p : ptrs) dev_free(ctx, p);
     cache_release(ctx);
     for (int i = 0; i < 8; i++) (void)hipEventDestroy(ctx->ev[i]);
+    for (int i = 0; i < 2; i++) {
+        if (ctx->side[i]) {
+            (void)hipStreamSynchronize(ctx->side[i]);
+            (void)hipStreamDestroy(ctx->side[i]);
+        }
+        if (ctx->side_join[i]) (void)hipEventDestroy(ctx->side_join[i]);
+    }
+    for (int i = 0; i < 32; i++)
+        if (ctx->fri_ev[i]) (void)hipEventDestroy(ctx->fri_ev[i]);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return ZKG_OK;
@@ -1049,11 +1063,18 @@ extern "C" int zkg_fri_commit_phase(zkg_ctx *ctx, const zkg_buf *cw, uint32_t lo
     }
     stage_mark(ctx, st, 0);
     if (!rc) {
+        // The fold chain runs on the context stream; layer r's tree (which
+        // needs only that layer's leaf digests) runs on a side stream, so
+        // the latency-bound tree tops overlap the later folds.
         hipError_t e = launch_hash_rows(FIELD_BB, 16, *p2, f->codewords[0], 1ull << (log_len - 1), 8,
                                         f->trees[0]->layers[0], st);
+        if (e == hipSuccess) e = hipEventRecord(ctx->fri_ev[0], st);
         for (int r = 0; r < nl && e == hipSuccess; r++) {
             zkg_tree *t = f->trees[r];
-            e = launch_merkle_tree(FIELD_BB, 16, *p2, t->layers.data(), t->nlayers, t->n_leaves, 9, st, false);
+            hipStream_t side = ctx->side[r & 1];
+            e = hipStreamWaitEvent(side, ctx->fri_ev[r], 0);
+            if (e == hipSuccess)
+                e = launch_merkle_tree(FIELD_BB, 16, *p2, t->layers.data(), t->nlayers, t->n_leaves, 9, side, false);
             if (e != hipSuccess) break;
             uint32_t br[4];
             for (int i = 0; i < 4; i++) {
@@ -1064,6 +1085,12 @@ extern "C" int zkg_fri_commit_phase(zkg_ctx *ctx, const zkg_buf *cw, uint32_t lo
             void *next_digests = (r + 1 < nl) ? f->trees[r + 1]->layers[0] : nullptr;
             e = launch_fri_fold(1, FIELD_BB, ctx->tw[FIELD_BB], *p2, f->codewords[r], n_out, br, f->codewords[r + 1],
                                 next_digests, st);
+            if (e == hipSuccess) e = hipEventRecord(ctx->fri_ev[r + 1], st);
+        }
+        for (int i = 0; i < 2; i++) {  // join the side streams
+            hipError_t e2 = hipEventRecord(ctx->side_join[i], ctx->side[i]);
+            if (e2 == hipSuccess) e2 = hipStreamWaitEvent(st, ctx->side_join[i], 0);
+            if (e == hipSuccess) e = e2;
         }
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) rc = hip_err(e, "fri commit phase");
