@@ -17,46 +17,80 @@
 namespace zkg {
 
 // ---------------------------------------------------------------- Goldilocks
+// 64-bit values are handled as 32-bit limbs with explicit carry chains
+// (v_add_co/v_addc, v_sub_co/v_subb): a carry selects a correction instead of
+// a 64-bit compare.  Inputs and outputs are canonical (< p).
+__host__ __device__ inline uint64_t gl_join(uint32_t lo, uint32_t hi) { return ((uint64_t)hi << 32) | lo; }
+
+// a + b mod 2^64 and the carry out
+__host__ __device__ inline uint64_t add_c(uint64_t a, uint64_t b, uint32_t &c)
+{
+    uint32_t c0, c1;
+    uint32_t lo = __builtin_addc((uint32_t)a, (uint32_t)b, 0u, &c0);
+    uint32_t hi = __builtin_addc((uint32_t)(a >> 32), (uint32_t)(b >> 32), c0, &c1);
+    c = c1;
+    return gl_join(lo, hi);
+}
+// a - b mod 2^64 and the borrow out
+__host__ __device__ inline uint64_t sub_b(uint64_t a, uint64_t b, uint32_t &bw)
+{
+    uint32_t b0, b1;
+    uint32_t lo = __builtin_subc((uint32_t)a, (uint32_t)b, 0u, &b0);
+    uint32_t hi = __builtin_subc((uint32_t)(a >> 32), (uint32_t)(b >> 32), b0, &b1);
+    bw = b1;
+    return gl_join(lo, hi);
+}
+
 struct Gl {
     using T = uint64_t;
     static constexpr uint64_t P = 0xFFFFFFFF00000001ULL;
     static constexpr uint64_t EPS = 0xFFFFFFFFULL;  // 2^64 mod p
 
+    // s = a + b; t = s + EPS = s - p (mod 2^64).  The sum reaches p exactly
+    // when it wrapped (then t is the reduced value and does not wrap again)
+    // or when t wraps.
     __host__ __device__ static inline T add(T a, T b)
     {
-        T s = a + b;
-        s += (s < a) ? EPS : 0;
-        return s >= P ? s - P : s;
+        uint32_t c, c2;
+        T s = add_c(a, b, c);
+        T t = add_c(s, EPS, c2);
+        return (c | c2) ? t : s;
     }
+    // d = a - b; on borrow the true value d - 2^64 equals d - EPS (mod p), and
+    // d > EPS then, so the correction cannot borrow again.
     __host__ __device__ static inline T sub(T a, T b)
     {
-        T d = a - b;
-        return (a < b) ? d - EPS : d;
+        uint32_t bw, bw2;
+        T d = sub_b(a, b, bw);
+        return sub_b(d, bw ? EPS : 0, bw2);
     }
     __host__ __device__ static inline T neg(T a) { return a ? P - a : 0; }
     __host__ __device__ static inline T dbl(T a) { return add(a, a); }
 
-    // x = lo + 2^64 * (hl + 2^32 * hh)  ==  lo - hh + hl * (2^32 - 1)  (mod p)
+    // x = lo + 2^64 (hl + 2^32 hh) == lo - hh + hl (2^32 - 1)  (mod p)
     __host__ __device__ static inline T reduce128(uint64_t lo, uint64_t hi)
     {
-        uint32_t hh = (uint32_t)(hi >> 32);
-        uint64_t hl = (uint32_t)hi;
-        uint64_t t0 = lo - hh;
-        t0 -= (lo < hh) ? EPS : 0;
-        uint64_t t1 = (hl << 32) - hl;
-        uint64_t r = t0 + t1;
-        r += (r < t0) ? EPS : 0;
-        return r >= P ? r - P : r;
+        uint32_t hh = (uint32_t)(hi >> 32), hl = (uint32_t)hi;
+        uint32_t b1, b2, c, c2, c3;
+        T t0 = sub_b(lo, hh, b1);
+        t0 = sub_b(t0, b1 ? EPS : 0, b2);  // lo < hh: subtract 2^64 mod p
+        T t1 = ((uint64_t)hl << 32) - hl;  // hl (2^32 - 1) < 2^64
+        T r = add_c(t0, t1, c);
+        r = add_c(r, c ? EPS : 0, c2);    // wrapped: add 2^64 mod p (cannot wrap again)
+        T t = add_c(r, EPS, c3);          // canonical: r >= p iff r + EPS wraps
+        (void)b2;
+        (void)c2;
+        return c3 ? t : r;
     }
     __host__ __device__ static inline T mul(T a, T b)
     {
         uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
         uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
         uint64_t p00 = (uint64_t)a0 * b0;
-        uint64_t p10 = (uint64_t)a1 * b0;
-        uint64_t mid = (uint64_t)a0 * b1 + ((p00 >> 32) + (uint32_t)p10);
-        uint64_t hi = (uint64_t)a1 * b1 + ((mid >> 32) + (p10 >> 32));
-        uint64_t lo = (mid << 32) | (uint32_t)p00;
+        uint64_t t = (uint64_t)a0 * b1 + (p00 >> 32);            // < 2^64
+        uint64_t u = (uint64_t)a1 * b0 + (uint32_t)t;            // < 2^64
+        uint64_t hi = (uint64_t)a1 * b1 + ((t >> 32) + (u >> 32));
+        uint64_t lo = ((uint64_t)(uint32_t)u << 32) | (uint32_t)p00;
         return reduce128(lo, hi);
     }
     __host__ __device__ static inline T sqr(T a) { return mul(a, a); }

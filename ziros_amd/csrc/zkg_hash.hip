@@ -363,7 +363,7 @@ __global__ __launch_bounds__(HASH_THREADS) void k_fri_fold_ef4(const uint32_t *_
                                                                uint32_t *__restrict__ digests, FoldArgs a,
                                                                const uint32_t *__restrict__ lo_inv,
                                                                const uint32_t *__restrict__ hi_inv,
-                                                               const uint32_t *__restrict__ rc)
+                                                               const double *__restrict__ rcf)
 {
     uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t npairs = (n_out + 1) / 2;
@@ -380,17 +380,17 @@ __global__ __launch_bounds__(HASH_THREADS) void k_fri_fold_ef4(const uint32_t *_
         }
     }
     if (digests && 2 * j + 1 < n_out) {
-        uint32_t s[16];
+        double s[16];
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-            s[e] = r[0].c[e];
-            s[4 + e] = r[1].c[e];
+            s[e] = Bf::from_u32(Bb::to_canon(r[0].c[e]));
+            s[4 + e] = Bf::from_u32(Bb::to_canon(r[1].c[e]));
         }
 #pragma unroll
-        for (int e = 8; e < 16; e++) s[e] = 0;
-        p2_permute<Bb, 16>(s, rc, nullptr);
+        for (int e = 8; e < 16; e++) s[e] = 0.0;
+        p2_permute_bf(s, rcf);
 #pragma unroll
-        for (int e = 0; e < 8; e++) digests[j * 8 + e] = Bb::to_canon(s[e]);
+        for (int e = 0; e < 8; e++) digests[j * 8 + e] = Bf::canon(s[e]);
     }
 }
 
@@ -588,7 +588,7 @@ hipError_t launch_fri_fold(int ext, int field, const TwiddleSet &tw, const P2Set
         hipLaunchKernelGGL(k_fri_fold_ef4, dim3((unsigned)((npairs + HASH_THREADS - 1) / HASH_THREADS)),
                            dim3(HASH_THREADS), 0, st, (const uint32_t *)in, n_out, log_n, (uint32_t *)out,
                            (uint32_t *)leaf_digests, a, (const uint32_t *)tw.top_lo_inv,
-                           (const uint32_t *)tw.top_hi_inv, (const uint32_t *)p2.rc);
+                           (const uint32_t *)tw.top_hi_inv, (const double *)p2.rc_f64);
         return hipGetLastError();
     }
     dim3 g((unsigned)((n_out + HASH_THREADS - 1) / HASH_THREADS)), b(HASH_THREADS);
