@@ -7,7 +7,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libzkg.so")
+# ZKG_LIBRARY selects another build of the same library (e.g. a tuning variant).
+LIB_PATH = os.environ.get("ZKG_LIBRARY") or os.path.join(_HERE, "libzkg.so")
 
 c_u64 = ctypes.c_uint64
 c_u32 = ctypes.c_uint32

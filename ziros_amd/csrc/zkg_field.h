@@ -66,6 +66,7 @@ struct Gl {
     }
     __host__ __device__ static inline T neg(T a) { return a ? P - a : 0; }
     __host__ __device__ static inline T dbl(T a) { return add(a, a); }
+    __host__ __device__ static inline T sub_for_mul(T a, T b) { return sub(a, b); }
 
     // x = lo + 2^64 (hl + 2^32 hh) == lo - hh + hl (2^32 - 1)  (mod p)
     __host__ __device__ static inline T reduce128(uint64_t lo, uint64_t hi)
@@ -108,25 +109,33 @@ struct Bb {
     static constexpr uint32_t R2 = 1172168163u;     // 2^64 mod p
     static constexpr uint32_t R1 = 268435454u;      // 2^32 mod p (Montgomery 1)
 
+    // Reductions are unsigned minima: for x < 2p, x - p wraps above x exactly
+    // when x < p, so min(x, x - p) is x mod p with no compare or carry mask.
+    __host__ __device__ static inline T umin(T a, T b) { return __builtin_elementwise_min(a, b); }
     __host__ __device__ static inline T add(T a, T b)
     {
         T s = a + b;
-        return s >= P ? s - P : s;
+        return umin(s, s - P);
     }
+    // d = a - b wraps to d > 2^32 - p > p when a < b, and then d + p wraps
+    // back below p; otherwise d < p <= d + p.
     __host__ __device__ static inline T sub(T a, T b)
     {
         T d = a - b;
-        return (a < b) ? d + P : d;
+        return umin(d, d + P);
     }
     __host__ __device__ static inline T neg(T a) { return a ? P - a : 0; }
     __host__ __device__ static inline T dbl(T a) { return add(a, a); }
+    // a - b + p in [1, 2p): a valid left operand of mul (REDC needs a b < p 2^32
+    // and a canonical right operand), one reduction cheaper than sub.
+    __host__ __device__ static inline T sub_for_mul(T a, T b) { return a + (P - b); }
     // Montgomery REDC of t < p * 2^32:  (t + m p) / 2^32,  m = t * (-p^-1) mod 2^32
     __host__ __device__ static inline T redc(uint64_t t)
     {
         uint32_t m = (uint32_t)t * NPINV;
         uint64_t u = t + (uint64_t)m * P;
         uint32_t r = (uint32_t)(u >> 32);
-        return r >= P ? r - P : r;
+        return umin(r, r - P);
     }
     __host__ __device__ static inline T mul(T a, T b) { return redc((uint64_t)a * b); }
     __host__ __device__ static inline T sqr(T a) { return mul(a, a); }
@@ -142,7 +151,7 @@ struct Bb {
         static_assert(K >= 1 && K <= 27, "shift range");
         uint32_t m = (0u - x) & ((1u << K) - 1u);
         uint32_t r = ((x + m) >> K) + m * (15u << (27 - K));
-        return r >= P ? r - P : r;
+        return umin(r, r - P);
     }
 };
 

@@ -12,10 +12,31 @@ constexpr int LOG_TW_TILE = 11;   // largest in-LDS transform: 2^11 points
 constexpr int LOG_TW_TOP = 24;    // four-step / coset tables cover orders <= 2^24
 constexpr int TW_SPLIT = 12;      // omega^e = lo[e & 4095] * hi[e >> 12]
 
+// Radix split of a 2^L-point in-LDS transform: NR rounds of at most 2^rmax
+// points per lane (rmax 6 for 32-bit, 5 for 64-bit elements), larger first.
+#ifndef ZKG_PLAN_RMAX64
+#define ZKG_PLAN_RMAX64 5
+#endif
+__host__ __device__ constexpr int plan_rmax(int elem_bytes) { return elem_bytes == 4 ? 6 : ZKG_PLAN_RMAX64; }
+__host__ __device__ constexpr int plan_nr(int rmax, int L)
+{
+    return ((L + rmax - 1) / rmax < 2 && L >= 4) ? 2 : ((L + rmax - 1) / rmax < 1 ? 1 : (L + rmax - 1) / rmax);
+}
+__host__ __device__ constexpr int plan_logr(int rmax, int L, int i)
+{
+    return L / plan_nr(rmax, L) + (i < L % plan_nr(rmax, L) ? 1 : 0);
+}
+// Inter-round twiddles of the 2^L plan, round I (not the last; at most two
+// such rounds): omega_Mi^(a q) at rtw_offset(L, I) + q (Mi / R) + a, where Mi
+// is the round's sub-problem size and R its radix.
+__host__ __device__ constexpr int rtw_offset(int L, int I) { return (1 << (L + 1)) + (I ? (1 << L) : 0); }
+constexpr int RTW_ENTRIES = 1 << (LOG_TW_TILE + 2);
+
 // Resident twiddle tables of one field, register form (BabyBear Montgomery).
 struct TwiddleSet {
     void *tile_fwd = nullptr;  // omega_2048^j, j < 1024
     void *tile_inv = nullptr;  // omega_2048^-j
+    void *round_fwd = nullptr, *round_inv = nullptr;  // RTW_ENTRIES inter-round twiddles
     void *top_lo_fwd = nullptr, *top_hi_fwd = nullptr;  // omega_2^24^i, omega_2^24^(4096 i)
     void *top_lo_inv = nullptr, *top_hi_inv = nullptr;
 };
@@ -50,6 +71,9 @@ uint64_t coset_lde_tmp_bytes(int field, uint64_t height, uint32_t width, uint32_
 hipError_t launch_ntt(int field, const TwiddleSet &tw, void *data, void *tmp, uint64_t height,
                       uint32_t width, int inverse, hipStream_t st);
 uint64_t ntt_tmp_bytes(int field, uint64_t height, uint32_t width);
+// Fills the RTW_ENTRIES inter-round table (register form) of one direction
+// from omega_2048 of that direction (register form).
+void fill_round_twiddles(int field, uint64_t w2048, void *table);
 // Unfused per-stage radix-2 path (one HBM pass per stage): the plain GPU
 // denominator used for comparison with the fused path.
 // tmp: height * width elements.

@@ -31,6 +31,8 @@
 //     N > 2^11, one for N <= 2^11.
 //   * Block -> tile mapping is XCD-aware: tiles that share rows (adjacent
 //     column chunks) are given to blocks that land on the same XCD.
+#include <vector>
+
 #include "zkg_field.h"
 #include "zkg_internal.h"
 
@@ -67,19 +69,21 @@ __device__ __forceinline__ typename F::T top_pow(const typename F::T *__restrict
 // digit first), at most 2^6 (BabyBear) / 2^5 (Goldilocks) points per lane.
 template <class F, int L>
 struct Plan {
-    static constexpr int RMAXLOG = sizeof(typename F::T) == 4 ? 6 : 5;
-    static constexpr int NR0 = (L + RMAXLOG - 1) / RMAXLOG;
-    static constexpr int NR = (L >= 4 && NR0 < 2) ? 2 : (NR0 < 1 ? 1 : NR0);
-    static constexpr int logr(int i) { return L / NR + (i < L % NR ? 1 : 0); }
+    static constexpr int RMAXLOG = plan_rmax(sizeof(typename F::T));
+    static constexpr int NR = plan_nr(RMAXLOG, L);
+    static constexpr int logr(int i) { return plan_logr(RMAXLOG, L, i); }
     static constexpr int LOGRMAX = logr(0);
     static constexpr int LOGRLAST = logr(NR - 1);
     static constexpr int LOG_TC = L - LOGRMAX;  // lanes per column
     static constexpr int LOG_C = 8 - LOG_TC;    // columns per block tile
     static constexpr int C = 1 << LOG_C;
     static constexpr int M = 1 << L;
+    static constexpr int RSTEP = NTT_THREADS >> LOG_C;               // tile rows per sweep of the block
+    static constexpr int ITERS = (M << LOG_C) / NTT_THREADS;         // tile elements per lane
     static constexpr int PADROWS = M + (M >> LOGRLAST);
     static constexpr int LDS_ELEMS = PADROWS * C;
     static_assert(LOG_TC <= 8, "plan needs <= 256 lanes per column");
+    static_assert(NR <= 3, "inter-round twiddle table holds two rounds per plan");
     // LDS slot of tile row `pos`, column `c`
     static __device__ __forceinline__ int slot(int pos, int c) { return (pos + (pos >> LOGRLAST)) * C + c; }
     // natural transform index held at row `pos` after the rounds: the digit
@@ -111,6 +115,15 @@ struct Plan {
     }
 };
 
+// Per-block twiddle vectors in LDS are indexed k + k / 32: the rows a wave
+// reads at once differ by multiples of 32 and land on distinct banks.
+__device__ __forceinline__ int vpad(int k) { return k + (k >> 5); }
+template <int M>
+constexpr int vpad_len()
+{
+    return M + (M >> 5) + 1;
+}
+
 // R-point DFT in registers: radix-2 DIF, natural order in, x[p] = X[rev(p)]
 // out.  omega_{2h}^t = omega_2048^(t*1024/h) = tw[t * (1024/h)].
 template <class F, int R>
@@ -125,8 +138,7 @@ __device__ __forceinline__ void reg_dft(typename F::T (&x)[R], const typename F:
                 T a = x[i], b = x[i + h];
                 x[i] = F::add(a, b);
                 const int t = i & (h - 1);
-                T d = F::sub(a, b);
-                x[i + h] = t == 0 ? d : F::mul(d, tw[t * (1024 / h)]);
+                x[i + h] = t == 0 ? F::sub(a, b) : F::mul(F::sub_for_mul(a, b), tw[t * (1024 / h)]);
             }
         }
     }
@@ -146,9 +158,14 @@ __device__ constexpr int crev(int q)
 // One radix round of the tile transform (round I of the plan).  Sub-DFT s of
 // a column covers rows b*Mi + a + Mi1*j (j < R) and writes output q, times
 // omega_Mi^(a q) unless this is the last round, to row b*Mi + q*Mi1 + a: the
-// same rows, so rounds work in place and need no barrier inside.
-template <class F, int L, int I, int SHIFT>
-__device__ __forceinline__ void tile_round(typename F::T *lds, const typename F::T *__restrict__ tw)
+// same rows, so rounds work in place and need no barrier inside.  The
+// inter-round twiddles come from the resident table rtw ([q][a] per round).
+// With WEIGHTED, the round reads its inputs from `src` ([row][c], unpadded)
+// times the per-row weight wt[vpad(row)] instead of from the tile.
+template <class F, int L, int I, int SHIFT, bool WEIGHTED>
+__device__ __forceinline__ void tile_round(typename F::T *lds, const typename F::T *__restrict__ tw,
+                                           const typename F::T *__restrict__ rtw, const typename F::T *src,
+                                           const typename F::T *wt)
 {
     using T = typename F::T;
     using P = Plan<F, L>;
@@ -166,37 +183,41 @@ __device__ __forceinline__ void tile_round(typename F::T *lds, const typename F:
         const int base = (s >> LOG_MI1) << LOG_MI;
         T x[R];
 #pragma unroll
-        for (int j = 0; j < R; j++) x[j] = lds[P::slot(base + a + (j << LOG_MI1), c)];
+        for (int j = 0; j < R; j++) {
+            const int row = base + a + (j << LOG_MI1);
+            if constexpr (WEIGHTED)
+                x[j] = F::mul(src[(row << P::LOG_C) + c], wt[vpad(row)]);
+            else
+                x[j] = lds[P::slot(row, c)];
+        }
         reg_dft<F, R>(x, tw);
-        if (I + 1 < P::NR) {
-            // omega_Mi^(a q) = w^q, w = omega_2048^(a * 2048 / Mi)
-            T w = tw[a << (LOG_TW_TILE - LOG_MI)];
-            T p = w;
+        if constexpr (I + 1 < P::NR) {
+            const T *r = rtw + rtw_offset(L, I) + a;
 #pragma unroll
-            for (int q = 1; q < R; q++) {
-                x[crev<R>(q)] = F::mul(x[crev<R>(q)], p);
-                if (q + 1 < R) p = F::mul(p, w);
-            }
+            for (int q = 1; q < R; q++) x[crev<R>(q)] = F::mul(x[crev<R>(q)], r[q << LOG_MI1]);
         }
 #pragma unroll
         for (int q = 0; q < R; q++) lds[P::slot(base + (q << LOG_MI1) + a, c)] = x[crev<R>(q)];
     }
 }
 
-// Full tile transform: rows of the LDS tile in natural order in, row `pos`
-// holds transform index Plan::index_of(pos) out.  Ends with a barrier.
-template <class F, int L>
-__device__ __forceinline__ void tile_dft(typename F::T *lds, const typename F::T *__restrict__ tw)
+// Full tile transform: rows of the LDS tile in natural order in (or, with
+// WEIGHTED, src[row][c] * wt[vpad(row)]), row `pos` holds transform index
+// Plan::index_of(pos) out.  Ends with a barrier.
+template <class F, int L, bool WEIGHTED = false>
+__device__ __forceinline__ void tile_dft(typename F::T *lds, const typename F::T *__restrict__ tw,
+                                         const typename F::T *__restrict__ rtw, const typename F::T *src = nullptr,
+                                         const typename F::T *wt = nullptr)
 {
     using P = Plan<F, L>;
-    tile_round<F, L, 0, 0>(lds, tw);
+    tile_round<F, L, 0, 0, WEIGHTED>(lds, tw, rtw, src, wt);
     __syncthreads();
     if constexpr (P::NR > 1) {
-        tile_round<F, L, 1, P::logr(0)>(lds, tw);
+        tile_round<F, L, 1, P::logr(0), false>(lds, tw, rtw, nullptr, nullptr);
         __syncthreads();
     }
     if constexpr (P::NR > 2) {
-        tile_round<F, L, 2, P::logr(0) + P::logr(1)>(lds, tw);
+        tile_round<F, L, 2, P::logr(0) + P::logr(1), false>(lds, tw, rtw, nullptr, nullptr);
         __syncthreads();
     }
 }
@@ -207,56 +228,93 @@ struct Geo {
     uint32_t nchunks;
 };
 
-// element (row, col) of a tile pass: idx runs over M * C in row-major order
-#define TILE_LOOP(P, idx) \
-    _Pragma("unroll 4") for (int idx = threadIdx.x; idx < (P::M << P::LOG_C); idx += NTT_THREADS)
+// A lane of a tile pass always handles column c = tid % C of the tile and
+// rows r0 + i * RSTEP (i < ITERS), r0 = tid / C.  Whether that column exists
+// (the last chunk of a matrix may be partial) is therefore known once per lane.
+#ifndef ZKG_TILE_STORE_UNROLL
+#define ZKG_TILE_STORE_UNROLL 8
+#endif
+#define ZKG_PRAGMA_(x) _Pragma(#x)
+#define ZKG_UNROLL(n) ZKG_PRAGMA_(unroll n)
+#define TILE_STORE_LOOP(P, row) \
+    ZKG_UNROLL(ZKG_TILE_STORE_UNROLL) \
+    for (int row = (int)(threadIdx.x >> P::LOG_C); row < P::M; row += P::RSTEP)
+
+// HBM -> LDS: tile row m is matrix row row0 + m * row_stride, columns c0 ..
+// c0 + C - 1 (zero beyond the width).  Loads are unconditional (a missing
+// column reads column 0 and is replaced by zero) so each lane issues a batch
+// of them before the first wait.
+template <class F, int L>
+__device__ __forceinline__ void tile_load(typename F::T *lds, const typename F::T *__restrict__ src, uint64_t row0,
+                                          uint64_t row_stride, uint32_t width, uint32_t c0)
+{
+    using T = typename F::T;
+    using P = Plan<F, L>;
+    constexpr int BATCH = P::ITERS < 32 ? P::ITERS : 32;
+    const int c = threadIdx.x & (P::C - 1), r0 = threadIdx.x >> P::LOG_C;
+    const uint32_t col = c0 + c;
+    const bool ok = col < width;
+    const T *p = src + (row0 + (uint64_t)r0 * row_stride) * width + (ok ? col : 0u);
+    const uint64_t step = (uint64_t)P::RSTEP * row_stride * width;
+#pragma unroll
+    for (int i0 = 0; i0 < P::ITERS; i0 += BATCH) {
+        T v[BATCH];
+#pragma unroll
+        for (int i = 0; i < BATCH; i++) v[i] = p[(uint64_t)(i0 + i) * step];
+#pragma unroll
+        for (int i = 0; i < BATCH; i++) lds[P::slot(r0 + (i0 + i) * P::RSTEP, c)] = ok ? v[i] : T(0);
+    }
+}
+
+// Products inside the transforms are always data x constant, and the
+// constants (twiddles, coset weights, 1/N) are Montgomery form, so a product
+// is the true product whatever scale the data carries: the kernels below
+// transform canonical values directly and their outputs are canonical.
 
 // ---------------------------------------------------------------------------
 // Four-step pass 1 (also LDE pass A): group n2 reads rows n2 + N2 n1,
 // transforms over n1, multiplies by omega_N^(+-n2 k1) and stores element k1 at
-// row n2 + N2 k1 (register form).
+// row n2 + N2 k1.
 template <class F, int L>
 __global__ __launch_bounds__(NTT_THREADS) void k_fs_pass1(const typename F::T *__restrict__ src,
-                                                          typename F::T *__restrict__ dst, Geo g, int in_canon,
+                                                          typename F::T *__restrict__ dst, Geo g,
                                                           const typename F::T *__restrict__ tile_tw,
+                                                          const typename F::T *__restrict__ round_tw,
                                                           const typename F::T *__restrict__ top_lo,
                                                           const typename F::T *__restrict__ top_hi)
 {
     using T = typename F::T;
     using P = Plan<F, L>;
     __shared__ T lds[P::LDS_ELEMS];
+    __shared__ T wk[vpad_len<P::M>()];  // omega_N^(n2 k1) by k1
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t n2 = tile / g.nchunks;
     const uint32_t c0 = (tile % g.nchunks) << P::LOG_C;
-    TILE_LOOP(P, idx)
-    {
-        const int m = idx >> P::LOG_C, c = idx & (P::C - 1);
-        const uint32_t col = c0 + c;
-        T v = 0;
-        if (col < g.width) v = src[((uint64_t)n2 + ((uint64_t)m << g.logN2)) * g.width + col];
-        lds[P::slot(m, c)] = in_canon ? F::from_canon(v) : v;
-    }
+    for (int k = threadIdx.x; k < P::M; k += NTT_THREADS)
+        wk[vpad(k)] = top_pow<F>(top_lo, top_hi, (uint64_t)n2 * k, g.logN);
+    tile_load<F, L>(lds, src, n2, 1ull << g.logN2, g.width, c0);
     __syncthreads();
-    tile_dft<F, L>(lds, tile_tw);
-    TILE_LOOP(P, idx)
+    tile_dft<F, L>(lds, tile_tw, round_tw);
+    const int c = threadIdx.x & (P::C - 1);
+    if (c0 + c >= g.width) return;
+    T *d = dst + (uint64_t)n2 * g.width + c0 + c;
+    const uint64_t rs = (uint64_t)g.width << g.logN2;
+    TILE_STORE_LOOP(P, pos)
     {
-        const int pos = idx >> P::LOG_C, c = idx & (P::C - 1);
-        const uint32_t col = c0 + c;
-        if (col >= g.width) continue;
         const uint32_t k1 = P::index_of(pos);
-        T w = top_pow<F>(top_lo, top_hi, (uint64_t)n2 * k1, g.logN);
-        dst[((uint64_t)n2 + ((uint64_t)k1 << g.logN2)) * g.width + col] = F::mul(lds[P::slot(pos, c)], w);
+        d[k1 * rs] = F::mul(lds[P::slot(pos, c)], wk[vpad(k1)]);
     }
 }
 
 // Plain four-step pass 2: group k1 reads contiguous rows N2 k1 + n2,
-// transforms over n2 and stores element k2 at row k1 + N1 k2, canonical,
-// times `scale` (1/N for the inverse).
+// transforms over n2 and stores element k2 at row k1 + N1 k2, times `scale`
+// (1/N for the inverse).
 template <class F, int L>
 __global__ __launch_bounds__(NTT_THREADS) void k_fs_pass2(const typename F::T *__restrict__ src,
                                                           typename F::T *__restrict__ dst, Geo g,
                                                           typename F::T scale,
-                                                          const typename F::T *__restrict__ tile_tw)
+                                                          const typename F::T *__restrict__ tile_tw,
+                                                          const typename F::T *__restrict__ round_tw)
 {
     using T = typename F::T;
     using P = Plan<F, L>;
@@ -264,67 +322,57 @@ __global__ __launch_bounds__(NTT_THREADS) void k_fs_pass2(const typename F::T *_
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t k1 = tile / g.nchunks;
     const uint32_t c0 = (tile % g.nchunks) << P::LOG_C;
-    TILE_LOOP(P, idx)
-    {
-        const int m = idx >> P::LOG_C, c = idx & (P::C - 1);
-        const uint32_t col = c0 + c;
-        T v = 0;
-        if (col < g.width) v = src[(((uint64_t)k1 << g.logN2) + m) * g.width + col];
-        lds[P::slot(m, c)] = v;
-    }
+    tile_load<F, L>(lds, src, (uint64_t)k1 << g.logN2, 1, g.width, c0);
     __syncthreads();
-    tile_dft<F, L>(lds, tile_tw);
-    TILE_LOOP(P, idx)
+    tile_dft<F, L>(lds, tile_tw, round_tw);
+    const int c = threadIdx.x & (P::C - 1);
+    if (c0 + c >= g.width) return;
+    T *d = dst + (uint64_t)k1 * g.width + c0 + c;
+    const uint64_t rs = (uint64_t)g.width << g.logN1;
+    TILE_STORE_LOOP(P, pos)
     {
-        const int pos = idx >> P::LOG_C, c = idx & (P::C - 1);
-        const uint32_t col = c0 + c;
-        if (col >= g.width) continue;
-        const uint32_t k2 = P::index_of(pos);
-        dst[((uint64_t)k1 + ((uint64_t)k2 << g.logN1)) * g.width + col] =
-            F::to_canon(F::mul(lds[P::slot(pos, c)], scale));
+        d[P::index_of(pos) * rs] = F::mul(lds[P::slot(pos, c)], scale);
     }
 }
 
-// Single-pass transform (N <= 2^11): natural in/out, canonical in/out.
+// Single-pass transform (N <= 2^11): natural in/out.
 template <class F, int L>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_small(typename F::T *__restrict__ data, Geo g,
                                                            typename F::T scale,
-                                                           const typename F::T *__restrict__ tile_tw)
+                                                           const typename F::T *__restrict__ tile_tw,
+                                                           const typename F::T *__restrict__ round_tw)
 {
     using T = typename F::T;
     using P = Plan<F, L>;
     __shared__ T lds[P::LDS_ELEMS];
     const uint32_t c0 = xcd_tile(blockIdx.x, gridDim.x) << P::LOG_C;
-    TILE_LOOP(P, idx)
-    {
-        const int m = idx >> P::LOG_C, c = idx & (P::C - 1);
-        const uint32_t col = c0 + c;
-        T v = 0;
-        if (col < g.width) v = F::from_canon(data[(uint64_t)m * g.width + col]);
-        lds[P::slot(m, c)] = v;
-    }
+    tile_load<F, L>(lds, data, 0, 1, g.width, c0);
     __syncthreads();
-    tile_dft<F, L>(lds, tile_tw);
-    TILE_LOOP(P, idx)
+    tile_dft<F, L>(lds, tile_tw, round_tw);
+    const int c = threadIdx.x & (P::C - 1);
+    if (c0 + c >= g.width) return;
+    T *d = data + c0 + c;
+    TILE_STORE_LOOP(P, pos)
     {
-        const int pos = idx >> P::LOG_C, c = idx & (P::C - 1);
-        const uint32_t col = c0 + c;
-        if (col >= g.width) continue;
-        data[(uint64_t)P::index_of(pos) * g.width + col] = F::to_canon(F::mul(lds[P::slot(pos, c)], scale));
+        d[(uint64_t)P::index_of(pos) * g.width] = F::mul(lds[P::slot(pos, c)], scale);
     }
 }
 
 // ---------------------------------------------------------------------------
 // LDE pass B: group k1 reads contiguous rows N2 k1 + n2 of pass A's output,
-// finishes the inverse over n2 (c[k1 + N1 k2]), keeps the coefficients in
-// VGPRs, and for each coset b applies s_b^k / N, the forward transform over
-// k2 and omega_N^(j2 k1); the result goes to row b N + N2 k1 + j2 of the
-// scratch matrix.
+// finishes the inverse over n2 (c[k1 + N1 k2]), keeps the coefficients in LDS,
+// and for each coset b applies s_b^k / N, the forward transform over k2 and
+// omega_N^(j2 k1); the result goes to row b N + N2 k1 + j2 of the scratch
+// matrix.  The coset weights (by k2) and omega_N^(j2 k1) (by j2) are vectors
+// in LDS shared by the tile's columns; the weighting is fused into the first
+// forward round.
 template <class F, int L>
 __global__ __launch_bounds__(NTT_THREADS) void k_lde_mid(const typename F::T *__restrict__ src,
                                                          typename F::T *__restrict__ dst, Geo g,
                                                          const typename F::T *__restrict__ tile_inv,
                                                          const typename F::T *__restrict__ tile_fwd,
+                                                         const typename F::T *__restrict__ round_inv,
+                                                         const typename F::T *__restrict__ round_fwd,
                                                          const typename F::T *__restrict__ top_lo,
                                                          const typename F::T *__restrict__ top_hi,
                                                          const typename F::T *__restrict__ cos_lo,
@@ -334,58 +382,51 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_mid(const typename F::T *__
     using P = Plan<F, L>;
     __shared__ T lds[P::LDS_ELEMS];
     __shared__ T coef[P::M << P::LOG_C];  // c[k1 + N1 k2] at [k2][c], kept across cosets
+    __shared__ T wj[vpad_len<P::M>()];    // omega_N^(j2 k1) by j2
+    __shared__ T wb[vpad_len<P::M>()];    // s_b^(k1 + N1 k2) / N by k2, per coset
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t k1 = tile / g.nchunks;
     const uint32_t c0 = (tile % g.nchunks) << P::LOG_C;
-    TILE_LOOP(P, idx)
-    {
-        const int m = idx >> P::LOG_C, c = idx & (P::C - 1);
-        const uint32_t col = c0 + c;
-        T v = 0;
-        if (col < g.width) v = src[(((uint64_t)k1 << g.logN2) + m) * g.width + col];
-        lds[P::slot(m, c)] = v;
-    }
+    for (int j = threadIdx.x; j < P::M; j += NTT_THREADS)
+        wj[vpad(j)] = top_pow<F>(top_lo, top_hi, (uint64_t)j * k1, g.logN);
+    tile_load<F, L>(lds, src, (uint64_t)k1 << g.logN2, 1, g.width, c0);
     __syncthreads();
-    tile_dft<F, L>(lds, tile_inv);
-    TILE_LOOP(P, idx)
+    tile_dft<F, L>(lds, tile_inv, round_inv);
+    const int c = threadIdx.x & (P::C - 1);
+    const bool col_ok = c0 + c < g.width;
+    TILE_STORE_LOOP(P, pos)
     {
-        const int pos = idx >> P::LOG_C, c = idx & (P::C - 1);
         coef[(P::index_of(pos) << P::LOG_C) + c] = lds[P::slot(pos, c)];
     }
-    __syncthreads();
     for (int b = 0; b < (1 << g.logB); b++) {
         const T *clo = cos_lo + (size_t)b * (1u << TW_SPLIT);
         const T *chi = cos_hi + (size_t)b * (1u << (LOG_TW_TOP - TW_SPLIT - 2));
-        TILE_LOOP(P, idx)
-        {
-            const int k2 = idx >> P::LOG_C, c = idx & (P::C - 1);
+        for (int k2 = threadIdx.x; k2 < P::M; k2 += NTT_THREADS) {
             const uint32_t k = k1 + ((uint32_t)k2 << g.logN1);
-            T w = F::mul(clo[k & ((1u << TW_SPLIT) - 1u)], chi[k >> TW_SPLIT]);
-            lds[P::slot(k2, c)] = F::mul(coef[idx], w);  // natural order for the forward rounds
+            wb[vpad(k2)] = F::mul(clo[k & ((1u << TW_SPLIT) - 1u)], chi[k >> TW_SPLIT]);
         }
-        __syncthreads();
-        tile_dft<F, L>(lds, tile_fwd);
-        TILE_LOOP(P, idx)
-        {
-            const int pos = idx >> P::LOG_C, c = idx & (P::C - 1);
-            const uint32_t col = c0 + c;
-            if (col >= g.width) continue;
-            const uint32_t j2 = P::index_of(pos);
-            T w = top_pow<F>(top_lo, top_hi, (uint64_t)j2 * k1, g.logN);
-            dst[(((uint64_t)b << g.logN) + ((uint64_t)k1 << g.logN2) + j2) * g.width + col] =
-                F::mul(lds[P::slot(pos, c)], w);
+        __syncthreads();  // coef (first coset) and wb complete; previous coset's stores done
+        tile_dft<F, L, true>(lds, tile_fwd, round_fwd, coef, wb);
+        if (col_ok) {
+            T *d = dst + (((uint64_t)b << g.logN) + ((uint64_t)k1 << g.logN2)) * g.width + c0 + c;
+            TILE_STORE_LOOP(P, pos)
+            {
+                const uint32_t j2 = P::index_of(pos);
+                d[(uint64_t)j2 * g.width] = F::mul(lds[P::slot(pos, c)], wj[vpad(j2)]);
+            }
         }
-        __syncthreads();
+        __syncthreads();  // wb and the tile are rewritten by the next coset
     }
 }
 
 // LDE pass C: coset b, group j2 reads rows b N + N2 k1 + j2, transforms over
 // k1 and writes output row b N + rev(j2) N1 + rev(j1) (one contiguous block
-// of N1 rows per group) in canonical form.
+// of N1 rows per group).
 template <class F, int L>
 __global__ __launch_bounds__(NTT_THREADS) void k_lde_final(const typename F::T *__restrict__ src,
                                                            typename F::T *__restrict__ out, Geo g,
-                                                           const typename F::T *__restrict__ tile_fwd)
+                                                           const typename F::T *__restrict__ tile_fwd,
+                                                           const typename F::T *__restrict__ round_fwd)
 {
     using T = typename F::T;
     using P = Plan<F, L>;
@@ -396,25 +437,17 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_final(const typename F::T *
     const uint32_t j2 = grp & ((1u << g.logN2) - 1u), b = grp >> g.logN2;
     const uint32_t c0 = chunk << P::LOG_C;
     const uint64_t base = (uint64_t)b << g.logN;
-    TILE_LOOP(P, idx)
-    {
-        const int k1 = idx >> P::LOG_C, c = idx & (P::C - 1);
-        const uint32_t col = c0 + c;
-        T v = 0;
-        if (col < g.width) v = src[(base + ((uint64_t)k1 << g.logN2) + j2) * g.width + col];
-        lds[P::slot(k1, c)] = v;
-    }
+    tile_load<F, L>(lds, src, base + j2, 1ull << g.logN2, g.width, c0);
     __syncthreads();
-    tile_dft<F, L>(lds, tile_fwd);
+    tile_dft<F, L>(lds, tile_fwd, round_fwd);
     const uint64_t obase = base + ((uint64_t)brev(j2, g.logN2) << L);
     // output row obase + r holds j1 = rev_L(r): read it from tile row pos_of(j1)
-    TILE_LOOP(P, idx)
+    const int c = threadIdx.x & (P::C - 1);
+    if (c0 + c >= g.width) return;
+    T *d = out + obase * g.width + c0 + c;
+    TILE_STORE_LOOP(P, r)
     {
-        const int r = idx >> P::LOG_C, c = idx & (P::C - 1);
-        const uint32_t col = c0 + c;
-        if (col >= g.width) continue;
-        const uint32_t pos = P::pos_of(brev((uint32_t)r, L));
-        out[(obase + r) * g.width + col] = F::to_canon(lds[P::slot(pos, c)]);
+        d[(uint64_t)r * g.width] = lds[P::slot(P::pos_of(brev((uint32_t)r, L)), c)];
     }
 }
 
@@ -424,45 +457,36 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_small(const typename F::T *
                                                            typename F::T *__restrict__ out, Geo g,
                                                            const typename F::T *__restrict__ tile_inv,
                                                            const typename F::T *__restrict__ tile_fwd,
+                                                           const typename F::T *__restrict__ round_inv,
+                                                           const typename F::T *__restrict__ round_fwd,
                                                            const typename F::T *__restrict__ cos_lo)
 {
     using T = typename F::T;
     using P = Plan<F, L>;
     __shared__ T lds[P::LDS_ELEMS];
     __shared__ T coef[P::M << P::LOG_C];  // coefficient k at [k][c]
+    __shared__ T wb[vpad_len<P::M>()];    // s_b^k / N by k, per coset
     const uint32_t c0 = xcd_tile(blockIdx.x, gridDim.x) << P::LOG_C;
-    TILE_LOOP(P, idx)
-    {
-        const int m = idx >> P::LOG_C, c = idx & (P::C - 1);
-        const uint32_t col = c0 + c;
-        T v = 0;
-        if (col < g.width) v = F::from_canon(in[(uint64_t)m * g.width + col]);
-        lds[P::slot(m, c)] = v;
-    }
+    tile_load<F, L>(lds, in, 0, 1, g.width, c0);
     __syncthreads();
-    tile_dft<F, L>(lds, tile_inv);
-    TILE_LOOP(P, idx)
+    tile_dft<F, L>(lds, tile_inv, round_inv);
+    const int c = threadIdx.x & (P::C - 1);
+    const bool col_ok = c0 + c < g.width;
+    TILE_STORE_LOOP(P, pos)
     {
-        const int pos = idx >> P::LOG_C, c = idx & (P::C - 1);
         coef[(P::index_of(pos) << P::LOG_C) + c] = lds[P::slot(pos, c)];
     }
-    __syncthreads();
     for (int b = 0; b < (1 << g.logB); b++) {
         const T *clo = cos_lo + (size_t)b * (1u << TW_SPLIT);
-        TILE_LOOP(P, idx)
-        {
-            const int k = idx >> P::LOG_C, c = idx & (P::C - 1);
-            lds[P::slot(k, c)] = F::mul(coef[idx], clo[k]);
-        }
+        for (int k = threadIdx.x; k < P::M; k += NTT_THREADS) wb[vpad(k)] = clo[k];
         __syncthreads();
-        tile_dft<F, L>(lds, tile_fwd);
-        TILE_LOOP(P, idx)
-        {
-            const int r = idx >> P::LOG_C, c = idx & (P::C - 1);
-            const uint32_t col = c0 + c;
-            if (col >= g.width) continue;
-            const uint32_t pos = P::pos_of(brev((uint32_t)r, L));
-            out[(((uint64_t)b << L) + r) * g.width + col] = F::to_canon(lds[P::slot(pos, c)]);
+        tile_dft<F, L, true>(lds, tile_fwd, round_fwd, coef, wb);
+        if (col_ok) {
+            T *d = out + ((uint64_t)b << L) * g.width + c0 + c;
+            TILE_STORE_LOOP(P, r)
+            {
+                d[(uint64_t)r * g.width] = lds[P::slot(P::pos_of(brev((uint32_t)r, L)), c)];
+            }
         }
         __syncthreads();
     }
@@ -538,14 +562,42 @@ __global__ void k_bitrev_swap_canon(typename F::T *__restrict__ data, uint64_t n
 // ---------------------------------------------------------------------------
 template <class F>
 struct Tabs {
-    const typename F::T *tile_fwd, *tile_inv, *lo_fwd, *hi_fwd, *lo_inv, *hi_inv;
+    const typename F::T *tile_fwd, *tile_inv, *round_fwd, *round_inv, *lo_fwd, *hi_fwd, *lo_inv, *hi_inv;
     explicit Tabs(const TwiddleSet &t)
         : tile_fwd((const typename F::T *)t.tile_fwd), tile_inv((const typename F::T *)t.tile_inv),
+          round_fwd((const typename F::T *)t.round_fwd), round_inv((const typename F::T *)t.round_inv),
           lo_fwd((const typename F::T *)t.top_lo_fwd), hi_fwd((const typename F::T *)t.top_hi_fwd),
           lo_inv((const typename F::T *)t.top_lo_inv), hi_inv((const typename F::T *)t.top_hi_inv)
     {
     }
 };
+
+template <class F>
+void fill_round_twiddles_t(typename F::T w2048, typename F::T *tab)
+{
+    using T = typename F::T;
+    constexpr int RM = plan_rmax(sizeof(T));
+    for (int i = 0; i < RTW_ENTRIES; i++) tab[i] = 0;
+    for (int L = 1; L <= LOG_TW_TILE; L++) {
+        const int nr = plan_nr(RM, L);
+        int shift = 0;
+        for (int I = 0; I + 1 < nr; I++) {
+            const int logr = plan_logr(RM, L, I), log_mi = L - shift, log_mi1 = log_mi - logr;
+            T wm = F::one();  // omega_Mi = omega_2048^(2048 / Mi)
+            for (int k = 0; k < (1 << (LOG_TW_TILE - log_mi)); k++) wm = F::mul(wm, w2048);
+            std::vector<T> pw((size_t)1 << log_mi);
+            T acc = F::one();
+            for (auto &v : pw) {
+                v = acc;
+                acc = F::mul(acc, wm);
+            }
+            for (int q = 0; q < (1 << logr); q++)
+                for (int a = 0; a < (1 << log_mi1); a++)
+                    tab[rtw_offset(L, I) + (q << log_mi1) + a] = pw[(size_t)(a * q) & ((1u << log_mi) - 1u)];
+            shift += logr;
+        }
+    }
+}
 
 int ilog2(uint64_t n)
 {
@@ -608,37 +660,38 @@ hipError_t dispatch_l(int L, A &&...a)
 
 template <class F, int L>
 struct RunPass1 {
-    static hipError_t run(const void *src, void *dst, Geo g, int in_canon, const void *tile, const void *lo,
+    static hipError_t run(const void *src, void *dst, Geo g, const void *tile, const void *round, const void *lo,
                           const void *hi, hipStream_t st)
     {
         using T = typename F::T;
         g.nchunks = chunks<F, L>(g.width);
         hipLaunchKernelGGL((k_fs_pass1<F, L>), dim3(g.nchunks << g.logN2), dim3(NTT_THREADS), 0, st, (const T *)src,
-                           (T *)dst, g, in_canon, (const T *)tile, (const T *)lo, (const T *)hi);
+                           (T *)dst, g, (const T *)tile, (const T *)round, (const T *)lo, (const T *)hi);
         return hipGetLastError();
     }
 };
 
 template <class F, int L>
 struct RunPass2 {
-    static hipError_t run(const void *src, void *dst, Geo g, typename F::T scale, const void *tile, hipStream_t st)
+    static hipError_t run(const void *src, void *dst, Geo g, typename F::T scale, const void *tile, const void *round,
+                          hipStream_t st)
     {
         using T = typename F::T;
         g.nchunks = chunks<F, L>(g.width);
         hipLaunchKernelGGL((k_fs_pass2<F, L>), dim3(g.nchunks << g.logN1), dim3(NTT_THREADS), 0, st, (const T *)src,
-                           (T *)dst, g, scale, (const T *)tile);
+                           (T *)dst, g, scale, (const T *)tile, (const T *)round);
         return hipGetLastError();
     }
 };
 
 template <class F, int L>
 struct RunSmall {
-    static hipError_t run(void *data, Geo g, typename F::T scale, const void *tile, hipStream_t st)
+    static hipError_t run(void *data, Geo g, typename F::T scale, const void *tile, const void *round, hipStream_t st)
     {
         using T = typename F::T;
         g.nchunks = chunks<F, L>(g.width);
         hipLaunchKernelGGL((k_ntt_small<F, L>), dim3(g.nchunks), dim3(NTT_THREADS), 0, st, (T *)data, g, scale,
-                           (const T *)tile);
+                           (const T *)tile, (const T *)round);
         return hipGetLastError();
     }
 };
@@ -650,20 +703,20 @@ struct RunMid {
         using T = typename F::T;
         g.nchunks = chunks<F, L>(g.width);
         hipLaunchKernelGGL((k_lde_mid<F, L>), dim3(g.nchunks << g.logN1), dim3(NTT_THREADS), 0, st, (const T *)src,
-                           (T *)dst, g, t.tile_inv, t.tile_fwd, t.lo_fwd, t.hi_fwd, (const T *)cs.lo,
-                           (const T *)cs.hi);
+                           (T *)dst, g, t.tile_inv, t.tile_fwd, t.round_inv, t.round_fwd, t.lo_fwd, t.hi_fwd,
+                           (const T *)cs.lo, (const T *)cs.hi);
         return hipGetLastError();
     }
 };
 
 template <class F, int L>
 struct RunFinal {
-    static hipError_t run(const void *src, void *out, Geo g, const void *tile, hipStream_t st)
+    static hipError_t run(const void *src, void *out, Geo g, const Tabs<F> &t, hipStream_t st)
     {
         using T = typename F::T;
         g.nchunks = chunks<F, L>(g.width);
         hipLaunchKernelGGL((k_lde_final<F, L>), dim3(g.nchunks << (g.logN2 + g.logB)), dim3(NTT_THREADS), 0, st,
-                           (const T *)src, (T *)out, g, (const T *)tile);
+                           (const T *)src, (T *)out, g, t.tile_fwd, t.round_fwd);
         return hipGetLastError();
     }
 };
@@ -675,7 +728,7 @@ struct RunLdeSmall {
         using T = typename F::T;
         g.nchunks = chunks<F, L>(g.width);
         hipLaunchKernelGGL((k_lde_small<F, L>), dim3(g.nchunks), dim3(NTT_THREADS), 0, st, (const T *)in, (T *)out, g,
-                           t.tile_inv, t.tile_fwd, (const T *)cs.lo);
+                           t.tile_inv, t.tile_fwd, t.round_inv, t.round_fwd, (const T *)cs.lo);
         return hipGetLastError();
     }
 };
@@ -695,12 +748,12 @@ hipError_t coset_lde_impl(const TwiddleSet &tws, const CosetSet &cs, const void 
     }
     if (g.logN <= LOG_TW_TILE) return dispatch_l<RunLdeSmall, F>(g.logN, in, out, g, t, cs, st);
     // pass A: in -> out[0 .. N*W) (scratch), pass B: -> tmp (B*N*W), pass C: -> out
-    hipError_t e = dispatch_l<RunPass1, F>(g.logN1, in, out, g, 1, (const void *)t.tile_inv, (const void *)t.lo_inv,
-                                           (const void *)t.hi_inv, st);
+    hipError_t e = dispatch_l<RunPass1, F>(g.logN1, in, out, g, (const void *)t.tile_inv, (const void *)t.round_inv,
+                                           (const void *)t.lo_inv, (const void *)t.hi_inv, st);
     if (e) return e;
     e = dispatch_l<RunMid, F>(g.logN2, (const void *)out, tmp, g, t, cs, st);
     if (e) return e;
-    return dispatch_l<RunFinal, F>(g.logN1, (const void *)tmp, out, g, (const void *)t.tile_fwd, st);
+    return dispatch_l<RunFinal, F>(g.logN1, (const void *)tmp, out, g, t, st);
 }
 
 template <class F>
@@ -713,12 +766,13 @@ hipError_t ntt_impl(const TwiddleSet &tws, void *data, void *tmp, uint64_t heigh
     if (g.logN == 0) return hipSuccess;
     T scale = inverse ? host_ninv<F>(height) : F::from_canon(1);
     const void *tile = inverse ? t.tile_inv : t.tile_fwd;
-    if (g.logN <= LOG_TW_TILE) return dispatch_l<RunSmall, F>(g.logN, data, g, scale, tile, st);
-    hipError_t e = dispatch_l<RunPass1, F>(g.logN1, (const void *)data, tmp, g, 1, tile,
+    const void *round = inverse ? t.round_inv : t.round_fwd;
+    if (g.logN <= LOG_TW_TILE) return dispatch_l<RunSmall, F>(g.logN, data, g, scale, tile, round, st);
+    hipError_t e = dispatch_l<RunPass1, F>(g.logN1, (const void *)data, tmp, g, tile, round,
                                            (const void *)(inverse ? t.lo_inv : t.lo_fwd),
                                            (const void *)(inverse ? t.hi_inv : t.hi_fwd), st);
     if (e) return e;
-    return dispatch_l<RunPass2, F>(g.logN2, (const void *)tmp, data, g, scale, tile, st);
+    return dispatch_l<RunPass2, F>(g.logN2, (const void *)tmp, data, g, scale, tile, round, st);
 }
 
 template <class F>
@@ -778,6 +832,14 @@ uint64_t ntt_tmp_bytes(int field, uint64_t height, uint32_t width)
 {
     if (ilog2(height) <= LOG_TW_TILE) return 0;
     return height * width * (field == FIELD_GL ? 8 : 4);
+}
+
+void fill_round_twiddles(int field, uint64_t w2048, void *table)
+{
+    if (field == FIELD_GL)
+        fill_round_twiddles_t<Gl>(w2048, (uint64_t *)table);
+    else
+        fill_round_twiddles_t<Bb>((uint32_t)w2048, (uint32_t *)table);
 }
 
 hipError_t launch_coset_lde(int field, const TwiddleSet &tw, const CosetSet &cs, const void *in, void *out, void *tmp,

@@ -6,7 +6,8 @@
 // issuing) for `iters` steps; the grid holds 8 waves per SIMD on every CU.
 // Kinds: 0 = BabyBear Montgomery product (the exact routine the kernels use),
 // 1 = Goldilocks product, 2 = raw 32-bit v_mul_lo_u32, 3 = FP64 FMA,
-// 4 = BabyBear product in exact FP64 arithmetic (6 FP64 operations).
+// 4 = BabyBear product in exact FP64 arithmetic (6 FP64 operations),
+// 5 = raw v_mad_u64_u32 (multiply-with-carry step), 6 = v_mul_hi_u32 + v_add.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -70,6 +71,29 @@ __global__ __launch_bounds__(256) void k_chain(uint64_t *out, uint32_t iters, ui
 #pragma unroll
         for (int i = 0; i < 8; i++) acc += x[i];
         out[t] = (uint64_t)(acc + 4e9);
+    } else if constexpr (KIND == 5) {
+        uint64_t x[8];
+        const uint32_t y = (uint32_t)seed | 1u;
+#pragma unroll
+        for (int i = 0; i < 8; i++) x[i] = ((uint64_t)(t + i) << 32) | (t * 2654435761u + i);
+        for (uint32_t k = 0; k < iters; k++)
+#pragma unroll
+            for (int i = 0; i < 8; i++) x[i] = (uint64_t)(uint32_t)x[i] * y + (x[i] >> 32);
+        uint64_t acc = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) acc ^= x[i];
+        out[t] = acc;
+    } else if constexpr (KIND == 6) {
+        uint32_t x[8], y = (uint32_t)seed | 0x80000001u;
+#pragma unroll
+        for (int i = 0; i < 8; i++) x[i] = t * 2654435761u + i;
+        for (uint32_t k = 0; k < iters; k++)
+#pragma unroll
+            for (int i = 0; i < 8; i++) x[i] = __umulhi(x[i], y) + 0x9e3779b9u;
+        uint32_t acc = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) acc ^= x[i];
+        out[t] = acc;
     } else {
         uint32_t x[8], y = (uint32_t)seed | 1u;
 #pragma unroll
@@ -145,6 +169,10 @@ extern "C" int zkg_probe_rate(int kind, uint32_t iters, double *ops_per_s)
             hipLaunchKernelGGL(k_chain<3>, dim3(blocks), dim3(256), 0, 0, out, it, 12345);
         else if (kind == 4)
             hipLaunchKernelGGL(k_chain<4>, dim3(blocks), dim3(256), 0, 0, out, it, 12345);
+        else if (kind == 5)
+            hipLaunchKernelGGL(k_chain<5>, dim3(blocks), dim3(256), 0, 0, out, it, 12345);
+        else if (kind == 6)
+            hipLaunchKernelGGL(k_chain<6>, dim3(blocks), dim3(256), 0, 0, out, it, 12345);
         else
             hipLaunchKernelGGL(k_chain<2>, dim3(blocks), dim3(256), 0, 0, out, it, 12345);
     };
