@@ -278,36 +278,45 @@ def bench_c5(ctx, n_req, load, slots=2, max_batch=8):
         done[i] = prev
     plain_lat = (done - arrivals) * 1e3
     plain_span = done[-1] - arrivals[0]
-    # served: the same arrival times into the continuous-batching loop
-    srv = Z.Server(ctx, slots=slots, max_batch=max_batch)
-    lat, pending = [], {}
-    t0 = time.perf_counter()
-    last_done = t0
-    k = 0
-    while k < n_req or pending:
-        now = time.perf_counter() - t0
-        while k < n_req and now >= arrivals[k] - arrivals[0]:
-            pending[srv.submit(jobs[k])] = k
-            k += 1
-        for t in list(pending):
-            r = srv.poll(t)
-            if r is not None:
-                free_tree(ctx, r[0])
-                lat.append(r[1])
-                last_done = time.perf_counter()
-                del pending[t]
-        time.sleep(0.0001)
-    span = last_done - t0
-    srv.stop()
+
+    def serve(offsets):
+        """Submits job k at offsets[k] seconds; returns (latencies ms, span s)."""
+        srv = Z.Server(ctx, slots=slots, max_batch=max_batch)
+        lat, pending = [], {}
+        t0 = time.perf_counter()
+        last_done = t0
+        k = 0
+        while k < n_req or pending:
+            now = time.perf_counter() - t0
+            while k < n_req and now >= offsets[k]:
+                pending[srv.submit(jobs[k])] = k
+                k += 1
+            for t in list(pending):
+                r = srv.poll(t)
+                if r is not None:
+                    free_tree(ctx, r[0])
+                    lat.append(r[1])
+                    last_done = time.perf_counter()
+                    del pending[t]
+            time.sleep(0.0001)
+        srv.stop()
+        return np.array(lat), last_done - t0
+
+    # capacity: every request queued at t = 0 (plain: back-to-back service)
+    _, sat_span = serve(np.zeros(n_req))
+    # latency: the Poisson trace above, replayed into the serving loop
+    lat, span = serve(arrivals - arrivals[0])
     for tr, lde in bufs:
         tr.free()
         lde.free()
-    lat = np.array(lat)
-    return dict(requests=n_req, arrival_rate_hz=rate, load_vs_plain_capacity=load, cells=cells,
-                served_cells_per_s=cells / span, plain_cells_per_s=cells / plain_span,
-                served_vs_plain=plain_span / span,
-                p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
-                plain_p50_ms=float(np.percentile(plain_lat, 50)), plain_p99_ms=float(np.percentile(plain_lat, 99)),
+    return dict(requests=n_req, cells=cells,
+                capacity=dict(served_cells_per_s=cells / sat_span, plain_cells_per_s=cells / service.sum(),
+                              served_vs_plain=service.sum() / sat_span),
+                poisson=dict(arrival_rate_hz=rate, load_vs_plain_capacity=load,
+                             served_cells_per_s=cells / span, plain_cells_per_s=cells / plain_span,
+                             p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                             plain_p50_ms=float(np.percentile(plain_lat, 50)),
+                             plain_p99_ms=float(np.percentile(plain_lat, 99))),
                 slots=slots, max_batch=max_batch)
 
 

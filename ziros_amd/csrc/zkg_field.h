@@ -68,31 +68,68 @@ struct Gl {
     __host__ __device__ static inline T dbl(T a) { return add(a, a); }
     __host__ __device__ static inline T sub_for_mul(T a, T b) { return sub(a, b); }
 
-    // x = lo + 2^64 (hl + 2^32 hh) == lo - hh + hl (2^32 - 1)  (mod p)
-    __host__ __device__ static inline T reduce128(uint64_t lo, uint64_t hi)
+    // canonical representative of any u64: x >= p iff x + EPS wraps
+    __host__ __device__ static inline T canon(T x)
     {
-        uint32_t hh = (uint32_t)(hi >> 32), hl = (uint32_t)hi;
-        uint32_t b1, b2, c, c2, c3;
-        T t0 = sub_b(lo, hh, b1);
-        t0 = sub_b(t0, b1 ? EPS : 0, b2);  // lo < hh: subtract 2^64 mod p
-        T t1 = ((uint64_t)hl << 32) - hl;  // hl (2^32 - 1) < 2^64
-        T r = add_c(t0, t1, c);
-        r = add_c(r, c ? EPS : 0, c2);    // wrapped: add 2^64 mod p (cannot wrap again)
-        T t = add_c(r, EPS, c3);          // canonical: r >= p iff r + EPS wraps
-        (void)b2;
-        (void)c2;
-        return c3 ? t : r;
+        uint32_t c;
+        T t = add_c(x, EPS, c);
+        return c ? t : x;
     }
-    __host__ __device__ static inline T mul(T a, T b)
+    // x = lo + 2^64 x2 + 2^96 x3 == lo + x2 (2^32 - 1) - x3 (mod p), as some u64
+    // representative (not necessarily < p).  r = x2 EPS + lo is one
+    // v_mad_u64_u32; a wrap of it (+2^64) leaves r < (2^32 - 1)^2 and a borrow
+    // of r - x3 (-2^64) leaves r - x3 > 2^64 - 2^32, so the net correction
+    // (wrap - borrow) EPS never wraps again.
+    __host__ __device__ static inline T reduce128_lazy(uint64_t lo, uint64_t hi)
+    {
+        const uint32_t x2 = (uint32_t)hi, x3 = (uint32_t)(hi >> 32);
+        uint64_t r = (uint64_t)x2 * EPS + lo;
+        const bool c = r < lo;
+        const bool b = r < x3;
+        r -= x3;
+        if (c != b) r += c ? EPS : (0 - EPS);
+        return r;
+    }
+    __host__ __device__ static inline T reduce128(uint64_t lo, uint64_t hi) { return canon(reduce128_lazy(lo, hi)); }
+    // 128-bit product of any two u64
+    __host__ __device__ static inline void mul_wide(T a, T b, uint64_t &lo, uint64_t &hi)
     {
         uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
         uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
         uint64_t p00 = (uint64_t)a0 * b0;
         uint64_t t = (uint64_t)a0 * b1 + (p00 >> 32);            // < 2^64
         uint64_t u = (uint64_t)a1 * b0 + (uint32_t)t;            // < 2^64
-        uint64_t hi = (uint64_t)a1 * b1 + ((t >> 32) + (u >> 32));
-        uint64_t lo = ((uint64_t)(uint32_t)u << 32) | (uint32_t)p00;
+        hi = (uint64_t)a1 * b1 + ((t >> 32) + (u >> 32));
+        lo = ((uint64_t)(uint32_t)u << 32) | (uint32_t)p00;
+    }
+    __host__ __device__ static inline T mul(T a, T b)
+    {
+        uint64_t lo, hi;
+        mul_wide(a, b, lo, hi);
         return reduce128(lo, hi);
+    }
+    // a b for any u64 representatives, result a u64 representative
+    __host__ __device__ static inline T mul_lazy(T a, T b)
+    {
+        uint64_t lo, hi;
+        mul_wide(a, b, lo, hi);
+        return reduce128_lazy(lo, hi);
+    }
+    // a b + k (k any u64): the 128-bit product plus k cannot pass 2^128 - 1
+    __host__ __device__ static inline T mul_add_lazy(T a, T b, T k)
+    {
+        uint64_t lo, hi;
+        mul_wide(a, b, lo, hi);
+        uint32_t c;
+        lo = add_c(lo, k, c);
+        return reduce128_lazy(lo, hi + c);
+    }
+    // a + k for any u64 a and canonical k: a wrap leaves a + k - 2^64 < k, so
+    // adding 2^64 mod p = EPS once stays below 2^64.
+    __host__ __device__ static inline T add_const_lazy(T a, T k)
+    {
+        T s = a + k;
+        return s + (s < k ? EPS : 0);
     }
     __host__ __device__ static inline T sqr(T a) { return mul(a, a); }
     // HBM <-> register representation (identity for Goldilocks)
@@ -100,6 +137,47 @@ struct Gl {
     __host__ __device__ static inline T to_canon(T a) { return a; }
     __host__ __device__ static inline T one() { return 1; }
 };
+
+// Sums of a few Goldilocks representatives kept as lo + top 2^64 (top small),
+// folded back once with 2^64 = 2^32 - 1 (mod p).  Additions are one
+// v_add_co / v_addc / v_addc chain over the three 32-bit words.
+struct GlWide {
+    uint32_t w0, w1, top;
+};
+__host__ __device__ inline GlWide glw(uint64_t x) { return GlWide{(uint32_t)x, (uint32_t)(x >> 32), 0u}; }
+__host__ __device__ inline GlWide glw_add(GlWide a, uint64_t x)
+{
+    uint32_t c0, c1, c2;
+    GlWide r;
+    r.w0 = __builtin_addc(a.w0, (uint32_t)x, 0u, &c0);
+    r.w1 = __builtin_addc(a.w1, (uint32_t)(x >> 32), c0, &c1);
+    r.top = __builtin_addc(a.top, 0u, c1, &c2);
+    (void)c2;
+    return r;
+}
+__host__ __device__ inline GlWide glw_add(GlWide a, GlWide b)
+{
+    uint32_t c0, c1, c2;
+    GlWide r;
+    r.w0 = __builtin_addc(a.w0, b.w0, 0u, &c0);
+    r.w1 = __builtin_addc(a.w1, b.w1, c0, &c1);
+    r.top = __builtin_addc(a.top, b.top, c1, &c2);
+    (void)c2;
+    return r;
+}
+__host__ __device__ inline GlWide glw_dbl(uint64_t x)
+{
+    const uint64_t d = x << 1;
+    return GlWide{(uint32_t)d, (uint32_t)(d >> 32), (uint32_t)(x >> 63)};
+}
+// top EPS + lo < 2^65; if it wraps the remainder is below (2^32 - 1)^2 and
+// one more EPS fits.
+__host__ __device__ inline uint64_t glw_fold(GlWide a)
+{
+    const uint64_t lo = gl_join(a.w0, a.w1);
+    uint64_t r = (uint64_t)a.top * Gl::EPS + lo;
+    return r + (r < lo ? Gl::EPS : 0);
+}
 
 // ---------------------------------------------------------------- BabyBear
 struct Bb {

@@ -18,7 +18,7 @@
 namespace zkg {
 
 template <class F>
-__device__ __forceinline__ typename F::T sbox7(typename F::T x)
+__host__ __device__ __forceinline__ typename F::T sbox7(typename F::T x)
 {
     typename F::T x2 = F::sqr(x);
     typename F::T x3 = F::mul(x2, x);
@@ -27,7 +27,7 @@ __device__ __forceinline__ typename F::T sbox7(typename F::T x)
 }
 
 template <class F, int W>
-__device__ __forceinline__ void mds_light(typename F::T (&s)[W])
+__host__ __device__ __forceinline__ void mds_light(typename F::T (&s)[W])
 {
     using T = typename F::T;
 #pragma unroll
@@ -56,7 +56,7 @@ __device__ __forceinline__ void mds_light(typename F::T (&s)[W])
 
 // Generic internal diffusion: s_i = d_i * s_i + sum (d in register form).
 template <class F, int W>
-__device__ __forceinline__ void internal_diag_generic(typename F::T (&s)[W], typename F::T sum,
+__host__ __device__ __forceinline__ void internal_diag_generic(typename F::T (&s)[W], typename F::T sum,
                                                       const typename F::T *__restrict__ diag)
 {
 #pragma unroll
@@ -66,7 +66,7 @@ __device__ __forceinline__ void internal_diag_generic(typename F::T (&s)[W], typ
 // BabyBear width 16: V = [-2, 1, 2, 1/2, 3, 4, -1/2, -3, -4, 2^-8, 1/4, 1/8,
 // 2^-27, -2^-8, -1/16, -2^-27] (zkf-metal/src/poseidon2/babybear.rs:14-58),
 // applied with doublings and exact power-of-two divisions instead of products.
-__device__ __forceinline__ void internal_diag_bb16(uint32_t (&s)[16], uint32_t sum)
+__host__ __device__ __forceinline__ void internal_diag_bb16(uint32_t (&s)[16], uint32_t sum)
 {
     using F = Bb;
     s[0] = F::sub(sum, F::dbl(s[0]));
@@ -103,17 +103,96 @@ struct P2Spec<Bb, 16> {
 };
 
 template <class F, int W>
-__device__ __forceinline__ void external_round(typename F::T (&s)[W], const typename F::T *__restrict__ rc)
+__host__ __device__ __forceinline__ void external_round(typename F::T (&s)[W], const typename F::T *__restrict__ rc)
 {
 #pragma unroll
     for (int i = 0; i < W; i++) s[i] = sbox7<F>(F::add(s[i], rc[i]));
     mds_light<F, W>(s);
 }
 
+// ---------------------------------------------------------------------------
+// Goldilocks permutation with lazy reduction (width 8 or 16).  The state holds
+// u64 representatives (any value, congruent mod p); every product reduces its
+// 128-bit value once (Gl::reduce128_lazy), sums of several terms accumulate as
+// lo + top 2^64 and fold once, and the state is made canonical only at the end.
+// Bounds: MDS-light outputs are sums of at most 35 u64 terms (top <= 35) and
+// an internal-round sum has W terms, so every top stays far below 2^32.
+template <int W>
+__host__ __device__ __forceinline__ void mds_light_gl(uint64_t (&s)[W])
+{
+    GlWide y[W];
+#pragma unroll
+    for (int c = 0; c < W; c += 4) {
+        const uint64_t a = s[c], b = s[c + 1], cc = s[c + 2], d = s[c + 3];
+        const GlWide t = glw_add(glw_add(glw(a), b), glw_add(glw(cc), d));
+        y[c] = glw_add(glw_add(t, a), glw_dbl(b));        // 2a + 3b + c + d
+        y[c + 1] = glw_add(glw_add(t, b), glw_dbl(cc));   // a + 2b + 3c + d
+        y[c + 2] = glw_add(glw_add(t, cc), glw_dbl(d));   // a + b + 2c + 3d
+        y[c + 3] = glw_add(glw_add(t, d), glw_dbl(a));    // 3a + b + c + 2d
+    }
+    GlWide sum[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        sum[k] = y[k];
+#pragma unroll
+        for (int c = 4; c < W; c += 4) sum[k] = glw_add(sum[k], y[c + k]);
+    }
+#pragma unroll
+    for (int i = 0; i < W; i++) s[i] = glw_fold(glw_add(y[i], sum[i & 3]));
+}
+
+__host__ __device__ __forceinline__ uint64_t sbox7_gl(uint64_t x)
+{
+    const uint64_t x2 = Gl::mul_lazy(x, x);
+    const uint64_t x3 = Gl::mul_lazy(x2, x);
+    const uint64_t x4 = Gl::mul_lazy(x2, x2);
+    return Gl::mul_lazy(x3, x4);
+}
+
+// rc: flat [initial 4W][internal RP][terminal 4W] and diag[W], canonical.
+template <int W>
+__host__ __device__ __forceinline__ void p2_permute_gl(uint64_t (&s)[W], const uint64_t *__restrict__ rc,
+                                                       const uint64_t *__restrict__ diag)
+{
+    constexpr int RP = 22;
+    mds_light_gl<W>(s);
+#pragma unroll 1
+    for (int r = 0; r < 4; r++) {
+#pragma unroll
+        for (int i = 0; i < W; i++) s[i] = sbox7_gl(Gl::add_const_lazy(s[i], rc[r * W + i]));
+        mds_light_gl<W>(s);
+    }
+    const uint64_t *ir = rc + 4 * W;
+#pragma unroll 1
+    for (int r = 0; r < RP; r++) {
+        s[0] = sbox7_gl(Gl::add_const_lazy(s[0], ir[r]));
+        GlWide acc = glw(s[0]);
+#pragma unroll
+        for (int i = 1; i < W; i++) acc = glw_add(acc, s[i]);
+        const uint64_t sum = glw_fold(acc);
+#pragma unroll
+        for (int i = 0; i < W; i++) s[i] = Gl::mul_add_lazy(s[i], diag[i], sum);
+    }
+    const uint64_t *tr = ir + RP;
+#pragma unroll 1
+    for (int r = 0; r < 4; r++) {
+#pragma unroll
+        for (int i = 0; i < W; i++) s[i] = sbox7_gl(Gl::add_const_lazy(s[i], tr[r * W + i]));
+        mds_light_gl<W>(s);
+    }
+#pragma unroll
+    for (int i = 0; i < W; i++) s[i] = Gl::canon(s[i]);
+}
+
 template <class F, int W>
-__device__ __forceinline__ void p2_permute(typename F::T (&s)[W], const typename F::T *__restrict__ rc,
+__host__ __device__ __forceinline__ void p2_permute(typename F::T (&s)[W], const typename F::T *__restrict__ rc,
                                            const typename F::T *__restrict__ diag)
 {
+    if constexpr (std::is_same<F, Gl>::value) {
+        static_assert(P2Spec<Gl, W>::RF == 8 && P2Spec<Gl, W>::RP == 22, "Goldilocks schedule");
+        p2_permute_gl<W>(s, rc, diag);
+        return;
+    }
     using T = typename F::T;
     constexpr int HALF = P2Spec<F, W>::RF / 2, RP = P2Spec<F, W>::RP;
     mds_light<F, W>(s);
