@@ -13,9 +13,9 @@ constexpr int LOG_TW_TOP = 24;    // four-step / coset tables cover orders <= 2^
 constexpr int TW_SPLIT = 12;      // omega^e = lo[e & 4095] * hi[e >> 12]
 
 // Radix split of a 2^L-point in-LDS transform: NR rounds of at most 2^rmax
-// points per lane (rmax 6 for 32-bit, 5 for 64-bit elements), larger first.
+// points per lane (rmax 6 for 32-bit, 4 for 64-bit elements), larger first.
 #ifndef ZKG_PLAN_RMAX64
-#define ZKG_PLAN_RMAX64 5
+#define ZKG_PLAN_RMAX64 4
 #endif
 __host__ __device__ constexpr int plan_rmax(int elem_bytes) { return elem_bytes == 4 ? 6 : ZKG_PLAN_RMAX64; }
 __host__ __device__ constexpr int plan_nr(int rmax, int L)

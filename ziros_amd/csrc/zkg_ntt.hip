@@ -66,7 +66,7 @@ __device__ __forceinline__ typename F::T top_pow(const typename F::T *__restrict
 
 // ---------------------------------------------------------------------------
 // Tile plan: M = 2^L points split into NR radix rounds (most significant
-// digit first), at most 2^6 (BabyBear) / 2^5 (Goldilocks) points per lane.
+// digit first), at most 2^6 (BabyBear) / 2^4 (Goldilocks) points per lane.
 template <class F, int L>
 struct Plan {
     static constexpr int RMAXLOG = plan_rmax(sizeof(typename F::T));
@@ -160,9 +160,12 @@ __device__ constexpr int crev(int q)
 // omega_Mi^(a q) unless this is the last round, to row b*Mi + q*Mi1 + a: the
 // same rows, so rounds work in place and need no barrier inside.  The
 // inter-round twiddles come from the resident table rtw ([q][a] per round).
-// With WEIGHTED, the round reads its inputs from `src` ([row][c], unpadded)
-// times the per-row weight wt[vpad(row)] instead of from the tile.
-template <class F, int L, int I, int SHIFT, bool WEIGHTED>
+// Round inputs (WSRC): SRC_TILE reads the tile; the weighted forms (round 0
+// only, one sub-DFT per lane) read input j, row `row`, times the row weight
+// wt[vpad(row)], from the lane's registers src[j] (SRC_REGS) or from an LDS
+// coefficient array src[row][c] (SRC_COEF).
+enum { SRC_TILE = 0, SRC_REGS = 1, SRC_COEF = 2 };
+template <class F, int L, int I, int SHIFT, int WSRC>
 __device__ __forceinline__ void tile_round(typename F::T *lds, const typename F::T *__restrict__ tw,
                                            const typename F::T *__restrict__ rtw, const typename F::T *src,
                                            const typename F::T *wt)
@@ -174,6 +177,7 @@ __device__ __forceinline__ void tile_round(typename F::T *lds, const typename F:
     constexpr int LOG_MI = L - SHIFT;          // current sub-problem size
     constexpr int LOG_MI1 = LOG_MI - LOGR;     // stride between its points
     constexpr int NSUB = (P::M >> LOGR) >> P::LOG_TC;
+    static_assert(WSRC == SRC_TILE || (I == 0 && NSUB == 1), "weighted input is round 0, one sub-DFT per lane");
     const int c = threadIdx.x & (P::C - 1);
     const int t = threadIdx.x >> P::LOG_C;
 #pragma unroll
@@ -185,7 +189,9 @@ __device__ __forceinline__ void tile_round(typename F::T *lds, const typename F:
 #pragma unroll
         for (int j = 0; j < R; j++) {
             const int row = base + a + (j << LOG_MI1);
-            if constexpr (WEIGHTED)
+            if constexpr (WSRC == SRC_REGS)
+                x[j] = F::mul(src[j], wt[vpad(row)]);
+            else if constexpr (WSRC == SRC_COEF)
                 x[j] = F::mul(src[(row << P::LOG_C) + c], wt[vpad(row)]);
             else
                 x[j] = lds[P::slot(row, c)];
@@ -201,23 +207,24 @@ __device__ __forceinline__ void tile_round(typename F::T *lds, const typename F:
     }
 }
 
-// Full tile transform: rows of the LDS tile in natural order in (or, with
-// WEIGHTED, src[row][c] * wt[vpad(row)]), row `pos` holds transform index
+// Full tile transform: rows of the LDS tile in natural order in (or the
+// weighted inputs of round 0, see tile_round; with SRC_REGS lane input j is
+// row tid / C + j 2^LOG_TC), row `pos` holds transform index
 // Plan::index_of(pos) out.  Ends with a barrier.
-template <class F, int L, bool WEIGHTED = false>
+template <class F, int L, int WSRC = SRC_TILE>
 __device__ __forceinline__ void tile_dft(typename F::T *lds, const typename F::T *__restrict__ tw,
                                          const typename F::T *__restrict__ rtw, const typename F::T *src = nullptr,
                                          const typename F::T *wt = nullptr)
 {
     using P = Plan<F, L>;
-    tile_round<F, L, 0, 0, WEIGHTED>(lds, tw, rtw, src, wt);
+    tile_round<F, L, 0, 0, WSRC>(lds, tw, rtw, src, wt);
     __syncthreads();
     if constexpr (P::NR > 1) {
-        tile_round<F, L, 1, P::logr(0), false>(lds, tw, rtw, nullptr, nullptr);
+        tile_round<F, L, 1, P::logr(0), SRC_TILE>(lds, tw, rtw, nullptr, nullptr);
         __syncthreads();
     }
     if constexpr (P::NR > 2) {
-        tile_round<F, L, 2, P::logr(0) + P::logr(1), false>(lds, tw, rtw, nullptr, nullptr);
+        tile_round<F, L, 2, P::logr(0) + P::logr(1), SRC_TILE>(lds, tw, rtw, nullptr, nullptr);
         __syncthreads();
     }
 }
@@ -359,8 +366,40 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_small(typename F::T *__rest
 }
 
 // ---------------------------------------------------------------------------
+// The inverse transform's coefficients feed every coset's first forward round.
+// A lane's share of them (2^LOGRMAX values) stays in VGPRs when it is at most
+// 128 bytes; larger shares (BabyBear 2^11-point tiles) go to an LDS array,
+// since 64 more live registers would cost more occupancy than the LDS does.
+template <class F, int L>
+constexpr bool lde_coef_in_regs()
+{
+    return (1 << Plan<F, L>::LOGRMAX) * sizeof(typename F::T) <= 128;
+}
+
+// After the inverse tile transform (row pos holds coefficient index_of(pos)):
+// registers get the coefficients of rows tid / C + j 2^LOG_TC, or the LDS
+// array gets every coefficient of the tile in natural order.
+template <class F, int L, bool CREG>
+__device__ __forceinline__ void load_coefficients(const typename F::T *lds, typename F::T *creg,
+                                                  typename F::T *coef)
+{
+    using P = Plan<F, L>;
+    const int c = threadIdx.x & (P::C - 1);
+    if constexpr (CREG) {
+#pragma unroll
+        for (int j = 0; j < (1 << P::LOGRMAX); j++)
+            creg[j] = lds[P::slot(P::pos_of((threadIdx.x >> P::LOG_C) + (j << P::LOG_TC)), c)];
+    } else {
+        TILE_STORE_LOOP(P, pos)
+        {
+            coef[(P::index_of(pos) << P::LOG_C) + c] = lds[P::slot(pos, c)];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // LDE pass B: group k1 reads contiguous rows N2 k1 + n2 of pass A's output,
-// finishes the inverse over n2 (c[k1 + N1 k2]), keeps the coefficients in LDS,
+// finishes the inverse over n2 (c[k1 + N1 k2]), keeps the coefficients,
 // and for each coset b applies s_b^k / N, the forward transform over k2 and
 // omega_N^(j2 k1); the result goes to row b N + N2 k1 + j2 of the scratch
 // matrix.  The coset weights (by k2) and omega_N^(j2 k1) (by j2) are vectors
@@ -381,9 +420,8 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_mid(const typename F::T *__
     using T = typename F::T;
     using P = Plan<F, L>;
     __shared__ T lds[P::LDS_ELEMS];
-    __shared__ T coef[P::M << P::LOG_C];  // c[k1 + N1 k2] at [k2][c], kept across cosets
-    __shared__ T wj[vpad_len<P::M>()];    // omega_N^(j2 k1) by j2
-    __shared__ T wb[vpad_len<P::M>()];    // s_b^(k1 + N1 k2) / N by k2, per coset
+    __shared__ T wj[vpad_len<P::M>()];  // omega_N^(j2 k1) by j2
+    __shared__ T wb[vpad_len<P::M>()];  // s_b^(k1 + N1 k2) / N by k2, per coset
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t k1 = tile / g.nchunks;
     const uint32_t c0 = (tile % g.nchunks) << P::LOG_C;
@@ -394,10 +432,12 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_mid(const typename F::T *__
     tile_dft<F, L>(lds, tile_inv, round_inv);
     const int c = threadIdx.x & (P::C - 1);
     const bool col_ok = c0 + c < g.width;
-    TILE_STORE_LOOP(P, pos)
-    {
-        coef[(P::index_of(pos) << P::LOG_C) + c] = lds[P::slot(pos, c)];
-    }
+    // the coefficients c[k1 + N1 k2] this lane feeds into every coset's first
+    // forward round (k2 = tid / C + j 2^LOG_TC), kept in VGPRs across cosets
+    constexpr bool CREG = lde_coef_in_regs<F, L>();
+    __shared__ T coef[CREG ? 1 : (P::M << P::LOG_C)];  // c[k1 + N1 k2] at [k2][c] otherwise
+    T creg[CREG ? (1 << P::LOGRMAX) : 1];
+    load_coefficients<F, L, CREG>(lds, creg, coef);
     for (int b = 0; b < (1 << g.logB); b++) {
         const T *clo = cos_lo + (size_t)b * (1u << TW_SPLIT);
         const T *chi = cos_hi + (size_t)b * (1u << (LOG_TW_TOP - TW_SPLIT - 2));
@@ -405,8 +445,8 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_mid(const typename F::T *__
             const uint32_t k = k1 + ((uint32_t)k2 << g.logN1);
             wb[vpad(k2)] = F::mul(clo[k & ((1u << TW_SPLIT) - 1u)], chi[k >> TW_SPLIT]);
         }
-        __syncthreads();  // coef (first coset) and wb complete; previous coset's stores done
-        tile_dft<F, L, true>(lds, tile_fwd, round_fwd, coef, wb);
+        __syncthreads();  // creg loads (first coset) done, wb complete, previous coset's stores done
+        tile_dft<F, L, CREG ? SRC_REGS : SRC_COEF>(lds, tile_fwd, round_fwd, CREG ? creg : coef, wb);
         if (col_ok) {
             T *d = dst + (((uint64_t)b << g.logN) + ((uint64_t)k1 << g.logN2)) * g.width + c0 + c;
             TILE_STORE_LOOP(P, pos)
@@ -464,23 +504,22 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_small(const typename F::T *
     using T = typename F::T;
     using P = Plan<F, L>;
     __shared__ T lds[P::LDS_ELEMS];
-    __shared__ T coef[P::M << P::LOG_C];  // coefficient k at [k][c]
-    __shared__ T wb[vpad_len<P::M>()];    // s_b^k / N by k, per coset
+    __shared__ T wb[vpad_len<P::M>()];  // s_b^k / N by k, per coset
     const uint32_t c0 = xcd_tile(blockIdx.x, gridDim.x) << P::LOG_C;
     tile_load<F, L>(lds, in, 0, 1, g.width, c0);
     __syncthreads();
     tile_dft<F, L>(lds, tile_inv, round_inv);
     const int c = threadIdx.x & (P::C - 1);
     const bool col_ok = c0 + c < g.width;
-    TILE_STORE_LOOP(P, pos)
-    {
-        coef[(P::index_of(pos) << P::LOG_C) + c] = lds[P::slot(pos, c)];
-    }
+    constexpr bool CREG = lde_coef_in_regs<F, L>();
+    __shared__ T coef[CREG ? 1 : (P::M << P::LOG_C)];  // coefficient k at [k][c] otherwise
+    T creg[CREG ? (1 << P::LOGRMAX) : 1];
+    load_coefficients<F, L, CREG>(lds, creg, coef);
     for (int b = 0; b < (1 << g.logB); b++) {
         const T *clo = cos_lo + (size_t)b * (1u << TW_SPLIT);
         for (int k = threadIdx.x; k < P::M; k += NTT_THREADS) wb[vpad(k)] = clo[k];
         __syncthreads();
-        tile_dft<F, L, true>(lds, tile_fwd, round_fwd, coef, wb);
+        tile_dft<F, L, CREG ? SRC_REGS : SRC_COEF>(lds, tile_fwd, round_fwd, CREG ? creg : coef, wb);
         if (col_ok) {
             T *d = out + ((uint64_t)b << L) * g.width + c0 + c;
             TILE_STORE_LOOP(P, r)
