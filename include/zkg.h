@@ -124,12 +124,36 @@ int zkg_merkle_commit(zkg_ctx *ctx, int field, int p2_width, const zkg_buf *mat,
 int zkg_tree_root(const zkg_tree *tree, void *root_out);
 int zkg_tree_num_layers(const zkg_tree *tree, uint32_t *n);
 int zkg_tree_digest_elems(const zkg_tree *tree, uint32_t *n);
-/* Copies digest layer `layer` (height >> layer digests) to host memory. */
+/* Digests in layer `layer` (height >> layer for single-matrix trees; the
+ * padded length for zkg_mmcs_commit trees). */
+int zkg_tree_layer_len(const zkg_tree *tree, uint32_t layer, uint64_t *n);
+/* Copies digest layer `layer` (zkg_tree_layer_len digests) to host memory. */
 int zkg_tree_layer(const zkg_tree *tree, uint32_t layer, void *out, uint64_t out_bytes);
 /* Mmcs::open_batch (vendor/p3-merkle-tree-gpu/src/mmcs.rs:179-210) for n
  * indices: rows_out gets n * width elements, paths_out n * (layers-1) digests. */
 int zkg_tree_open(const zkg_tree *tree, const uint64_t *indices, uint32_t n, void *rows_out, void *paths_out);
 int zkg_tree_free(zkg_tree *tree);
+
+/* Mmcs::commit over several matrices of possibly different, not necessarily
+ * power-of-two heights (vendor/p3-merkle-tree-gpu/src/merkle_tree.rs:84-151,
+ * compress_and_inject :272-350).  Matrices are taken tallest first (stable);
+ * the rows of the tallest ones, concatenated in commit order, are hashed into
+ * layer 0, padded with zero digests to even length; each next layer
+ * compresses pairs (padded to even, a single root last) and, where a shorter
+ * group's heights round up to that layer's size, compresses the result with
+ * the hash of the group's rows.  Heights that round up to the same power of
+ * two must be equal (as p3 asserts).  The tree references the matrices for
+ * openings: the caller keeps them alive until zkg_tree_free.
+ * zkg_tree_root/num_layers/layer/free apply; zkg_tree_layer copies the
+ * layer's padded length. */
+int zkg_mmcs_commit(zkg_ctx *ctx, int field, int p2_width, uint32_t n_mats, const zkg_buf *const *mats,
+                    const uint64_t *heights, const uint32_t *widths, zkg_tree **out);
+/* Mmcs::open_batch (vendor/p3-merkle-tree-gpu/src/mmcs.rs:179-210) for n
+ * indices below the tallest height: per index, matrix j's row
+ * index >> (log2_ceil(max_h) - log2_ceil(h_j)) for every matrix in commit
+ * order (sum of widths elements), and log2_ceil(max_h) sibling digests.
+ * Also accepts single-matrix trees. */
+int zkg_mmcs_open(const zkg_tree *tree, const uint64_t *indices, uint32_t n, void *rows_out, void *paths_out);
 
 /* Fused commit of one trace: coset LDE at log_blowup with the field's
  * generator shift (p3 Val::GENERATOR: 7 or 31), row hash, tree.  lde_out

@@ -153,6 +153,66 @@ def test_merkle_layers_and_openings(ctx, field, p2w, logh, width):
         assert exp.verify(root.astype(np.uint64), i, [rows[k]], paths[k].astype(np.uint64))
 
 
+# Mixed-height MMCS (compress_and_inject): the oracle's layer lengths and
+# padding follow vendor/p3-merkle-tree-gpu/src/merkle_tree.rs:84-350.
+MMCS_SHAPES = [
+    [(100, 3), (13, 7), (100, 5), (50, 2), (1, 4)],  # injections at 64, 16 and the root; odd padding
+    [(4, 3), (1, 2)],  # injection at the root layer
+    [(7, 9)],  # one non-power-of-two matrix
+    [(1, 5)],  # a single row: the root is the row digest
+    [(33, 1), (17, 12), (9, 8), (5, 3), (3, 2), (2, 6)],
+    [(64, 17), (64, 0), (32, 1)],  # a zero-width matrix contributes nothing
+]
+
+
+@pytest.mark.parametrize("shape_id", range(len(MMCS_SHAPES)))
+@pytest.mark.parametrize("field,p2w", [(BB, 16), (GL, 16), (GL, 8)])
+def test_mmcs_mixed_heights(ctx, field, p2w, shape_id):
+    shapes = MMCS_SHAPES[shape_id]
+    p2 = O.Poseidon2(field, p2w)
+    mats = [rnd(field, h * w, 900 + 7 * k + shape_id).reshape(h, w) for k, (h, w) in enumerate(shapes)]
+    root, tree = Z.HipMerkleTreeMmcs(ctx, field, p2w).commit(mats)
+    exp = O.MerkleTree(p2, mats)
+    assert tree.num_layers == len(exp.layers)
+    for i in range(tree.num_layers):
+        assert tree.layer_len(i) == exp.layers[i].shape[0], i
+        assert (tree.layer(i).astype(np.uint64) == exp.layers[i]).all(), i
+    assert (root.astype(np.uint64) == exp.root()).all()
+    maxh = max(h for h, _ in shapes)
+    logmax = (maxh - 1).bit_length()
+    tw = sum(w for _, w in shapes)
+    idx = np.array(sorted({0, maxh - 1, maxh // 2, (maxh * 3) // 7}), dtype=np.uint64)
+    rows, paths = tree.open_mmcs(idx, tw)
+    for k, i in enumerate(idx):
+        i = int(i)
+        want = [m[i >> (logmax - (m.shape[0] - 1).bit_length())] for m in mats]
+        assert (rows[k].astype(np.uint64) == np.concatenate(want).astype(np.uint64)).all()
+        assert (paths[k].astype(np.uint64) == exp.path(i).reshape(paths[k].shape)).all()
+        split = np.cumsum([m.shape[1] for m in mats])[:-1]
+        assert exp.verify(root.astype(np.uint64), i, np.split(rows[k].astype(np.uint64), split),
+                          paths[k].astype(np.uint64))
+    tree.free()
+
+
+def test_mmcs_rejects_heights_sharing_a_power_of_two(ctx):
+    a, b = rnd(BB, 100 * 2, 1).reshape(100, 2), rnd(BB, 120 * 2, 2).reshape(120, 2)
+    with pytest.raises(Z.ZkgError):
+        Z.HipMerkleTreeMmcs(ctx, BB, 16).commit([a, b])
+
+
+def test_mmcs_single_matrix_matches_merkle_commit(ctx):
+    m = rnd(GL, 256 * 5, 77).reshape(256, 5)
+    mm = Z.HipMerkleTreeMmcs(ctx, GL, 16)
+    r1, t1 = mm.commit([m])
+    r2, t2 = mm.commit_matrix(m)
+    assert (r1 == r2).all()
+    rows, paths = t2.open_mmcs([3, 200], 5)
+    r_a, p_a = t2.open_batch([3, 200], 5)
+    assert (rows == r_a).all() and (paths == p_a).all()
+    t1.free()
+    t2.free()
+
+
 @pytest.mark.parametrize("plain", [False, True])
 @pytest.mark.parametrize("field,p2w,logn,width,lb", [(GL, 8, 14, 8, 1), (GL, 16, 14, 8, 1), (BB, 16, 12, 16, 1),
                                                      (BB, 16, 10, 40, 2), (GL, 16, 9, 3, 2)])

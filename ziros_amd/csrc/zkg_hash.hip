@@ -509,7 +509,135 @@ hipError_t merkle_impl(const P2Set &p2, void *const *layers, int nlayers, uint64
     return hipSuccess;
 }
 
+// ---------------------------------------------------------------- mixed-height MMCS
+// PaddingFreeSponge over row `row` of every matrix of a group, concatenated in
+// commit order (p3 first_digest_layer / compress_and_inject hash of
+// `matrices.flat_map(|m| m.row(i))`): stream element k overwrites rate slot
+// k % RATE, each full or final partial chunk permutes; an empty stream leaves
+// the zero state.  State in register form.
+template <class F, int W>
+__device__ void group_row_hash(const MatDesc *__restrict__ g, int n, uint64_t row, typename F::T (&s)[W],
+                               const typename F::T *__restrict__ rc, const typename F::T *__restrict__ diag)
+{
+    using T = typename F::T;
+    constexpr int RATE = W / 2;
+#pragma unroll
+    for (int e = 0; e < W; e++) s[e] = 0;
+    int m = 0;
+    uint64_t k = 0;
+    auto next_elem = [&](T &v) -> bool {
+        while (m < n && k >= g[m].width) {
+            m++;
+            k = 0;
+        }
+        if (m >= n) return false;
+        v = F::from_canon(((const T *)g[m].ptr)[row * g[m].width + k]);
+        k++;
+        return true;
+    };
+    bool more = true;
+    while (more) {
+        int filled = 0;
+#pragma unroll
+        for (int j = 0; j < RATE; j++) {
+            T v;
+            if (more && next_elem(v)) {
+                s[j] = v;
+                filled++;
+            } else {
+                more = false;
+            }
+        }
+        if (filled) p2_permute<F, W>(s, rc, diag);
+    }
+}
+
+template <class F, int W>
+__global__ __launch_bounds__(HASH_THREADS) void k_mmcs_leaves(const MatDesc *__restrict__ g, int n, uint64_t rows,
+                                                              typename F::T *__restrict__ digests,
+                                                              const typename F::T *__restrict__ rc,
+                                                              const typename F::T *__restrict__ diag)
+{
+    using T = typename F::T;
+    constexpr int D = W / 2;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    T s[W];
+    group_row_hash<F, W>(g, n, i, s, rc, diag);
+#pragma unroll
+    for (int e = 0; e < D; e++) digests[i * D + e] = F::to_canon(s[e]);
+}
+
+template <class F, int W>
+__global__ __launch_bounds__(HASH_THREADS) void k_mmcs_layer(const typename F::T *__restrict__ prev, uint64_t half,
+                                                             typename F::T *__restrict__ next,
+                                                             const MatDesc *__restrict__ inj, int n_inj,
+                                                             uint64_t inj_len, const typename F::T *__restrict__ rc,
+                                                             const typename F::T *__restrict__ diag)
+{
+    using T = typename F::T;
+    constexpr int D = W / 2;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= half) return;
+    T s[W];
+#pragma unroll
+    for (int e = 0; e < W; e++) s[e] = F::from_canon(prev[2 * i * D + e]);
+    p2_permute<F, W>(s, rc, diag);
+    if (n_inj) {
+        T r[W];
+        if (i < inj_len) {
+            group_row_hash<F, W>(inj, n_inj, i, r, rc, diag);
+        } else {
+#pragma unroll
+            for (int e = 0; e < W; e++) r[e] = 0;
+        }
+#pragma unroll
+        for (int e = 0; e < D; e++) s[D + e] = r[e];
+        p2_permute<F, W>(s, rc, diag);
+    }
+#pragma unroll
+    for (int e = 0; e < D; e++) next[i * D + e] = F::to_canon(s[e]);
+}
+
 }  // namespace
+
+hipError_t launch_mmcs_leaves(int field, int width, const P2Set &p2, const MatDesc *group, int n, uint64_t rows,
+                              void *digests, hipStream_t st)
+{
+    if (!rows) return hipSuccess;
+    dim3 g((unsigned)((rows + HASH_THREADS - 1) / HASH_THREADS)), b(HASH_THREADS);
+    if (field == FIELD_BB && width == 16)
+        hipLaunchKernelGGL((k_mmcs_leaves<Bb, 16>), g, b, 0, st, group, n, rows, (uint32_t *)digests,
+                           (const uint32_t *)p2.rc, (const uint32_t *)p2.diag);
+    else if (field == FIELD_GL && width == 16)
+        hipLaunchKernelGGL((k_mmcs_leaves<Gl, 16>), g, b, 0, st, group, n, rows, (uint64_t *)digests,
+                           (const uint64_t *)p2.rc, (const uint64_t *)p2.diag);
+    else if (field == FIELD_GL && width == 8)
+        hipLaunchKernelGGL((k_mmcs_leaves<Gl, 8>), g, b, 0, st, group, n, rows, (uint64_t *)digests,
+                           (const uint64_t *)p2.rc, (const uint64_t *)p2.diag);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_mmcs_layer(int field, int width, const P2Set &p2, const void *prev, uint64_t half, void *next,
+                             const MatDesc *inj, int n_inj, uint64_t inj_len, hipStream_t st)
+{
+    if (!half) return hipSuccess;
+    dim3 g((unsigned)((half + HASH_THREADS - 1) / HASH_THREADS)), b(HASH_THREADS);
+    if (field == FIELD_BB && width == 16)
+        hipLaunchKernelGGL((k_mmcs_layer<Bb, 16>), g, b, 0, st, (const uint32_t *)prev, half, (uint32_t *)next, inj,
+                           n_inj, inj_len, (const uint32_t *)p2.rc, (const uint32_t *)p2.diag);
+    else if (field == FIELD_GL && width == 16)
+        hipLaunchKernelGGL((k_mmcs_layer<Gl, 16>), g, b, 0, st, (const uint64_t *)prev, half, (uint64_t *)next, inj,
+                           n_inj, inj_len, (const uint64_t *)p2.rc, (const uint64_t *)p2.diag);
+    else if (field == FIELD_GL && width == 8)
+        hipLaunchKernelGGL((k_mmcs_layer<Gl, 8>), g, b, 0, st, (const uint64_t *)prev, half, (uint64_t *)next, inj,
+                           n_inj, inj_len, (const uint64_t *)p2.rc, (const uint64_t *)p2.diag);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
 
 hipError_t launch_p2_permute(int field, int width, const P2Set &p2, void *states, uint64_t n, hipStream_t st)
 {

@@ -59,6 +59,12 @@ struct P2Set {
     bool loaded = false;
 };
 
+// One matrix of a mixed-height MMCS commit (device pointer, row-major).
+struct MatDesc {
+    const void *ptr;
+    uint64_t width;
+};
+
 struct PassTimes;
 
 // --- NTT / LDE (zkg_ntt.hip) -------------------------------------------------
@@ -94,6 +100,14 @@ hipError_t launch_merkle_tree(int field, int width, const P2Set &p2, void *const
                               bool sync_each);
 hipError_t launch_gen(int field, void *out, uint64_t seed, uint64_t start, uint64_t n,
                       hipStream_t st);
+// Mixed-height MMCS.  Leaves: digest i = sponge over row i of every matrix of
+// the group (device array of n MatDesc), rows concatenated.  Layer: node i of
+// the next layer (i < half) = compress(prev[2i], prev[2i+1]), then, if n_inj,
+// compress(node, group hash of row i) for i < inj_len, compress(node, 0) above.
+hipError_t launch_mmcs_leaves(int field, int width, const P2Set &p2, const MatDesc *group, int n,
+                              uint64_t rows, void *digests, hipStream_t st);
+hipError_t launch_mmcs_layer(int field, int width, const P2Set &p2, const void *prev, uint64_t half,
+                             void *next, const MatDesc *inj, int n_inj, uint64_t inj_len, hipStream_t st);
 // FRI fold over EF4 (ext = 1) or a base field; twiddles from the resident
 // inverse top tables.  If leaf_digests is not null, also hashes rows of the
 // folded codeword viewed as a width-2 (ext) matrix into leaf digests (fused).

@@ -70,7 +70,15 @@ struct zkg_tree {
     uint32_t mat_width = 0;
     void *base = nullptr;  // one allocation for every digest layer
     std::vector<uint64_t> layer_off;
+    std::vector<uint64_t> layer_len;  // digests per layer
     std::vector<void *> layers;
+    // Mixed-height MMCS trees: the committed matrices in commit order (not
+    // owned) and the device descriptor array the layers were built from.
+    bool mmcs = false;
+    std::vector<zkg::MatDesc> mats;
+    std::vector<uint64_t> mat_heights;
+    uint64_t max_height = 0;
+    void *desc = nullptr;
 };
 
 struct zkg_fri {

@@ -769,6 +769,7 @@ int zkg::tree_alloc(zkg_ctx *ctx, int field, int p2_width, uint64_t n_leaves, co
     uint64_t off = 0, es = esize(field);
     for (int i = 0; i < t->nlayers; i++) {
         t->layer_off.push_back(off);
+        t->layer_len.push_back(n_leaves >> i);
         off += (n_leaves >> i) * t->digest * es;
         off = (off + 255) & ~255ull;
     }
@@ -839,11 +840,19 @@ extern "C" int zkg_tree_digest_elems(const zkg_tree *t, uint32_t *n)
     return ZKG_OK;
 }
 
+extern "C" int zkg_tree_layer_len(const zkg_tree *t, uint32_t layer, uint64_t *n)
+{
+    if (!t || !n) return set_err(ZKG_EINVAL, "null argument");
+    if ((int)layer >= t->nlayers) return set_err(ZKG_EINVAL, "layer %u of %d", layer, t->nlayers);
+    *n = t->layer_len[layer];
+    return ZKG_OK;
+}
+
 extern "C" int zkg_tree_layer(const zkg_tree *t, uint32_t layer, void *out, uint64_t cap)
 {
     if (!t || !out) return set_err(ZKG_EINVAL, "null argument");
     if ((int)layer >= t->nlayers) return set_err(ZKG_EINVAL, "layer %u of %d", layer, t->nlayers);
-    uint64_t bytes = (t->n_leaves >> layer) * t->digest * esize(t->field);
+    uint64_t bytes = t->layer_len[layer] * t->digest * esize(t->field);
     if (bytes > cap) return set_err(ZKG_EINVAL, "output too small");
     HIP_TRY(hipStreamSynchronize(t->ctx->stream), "sync");
     HIP_TRY(hipMemcpy(out, t->layers[layer], bytes, hipMemcpyDeviceToHost), "layer");
@@ -852,6 +861,7 @@ extern "C" int zkg_tree_layer(const zkg_tree *t, uint32_t layer, void *out, uint
 
 int zkg::tree_open(const zkg_tree *t, const uint64_t *idx, uint32_t n, void *rows_out, void *paths_out)
 {
+    if (t->mmcs) return set_err(ZKG_EINVAL, "tree has several matrices: use zkg_mmcs_open");
     if (!n) return ZKG_OK;
     for (uint32_t i = 0; i < n; i++)
         if (idx[i] >= t->n_leaves) return set_err(ZKG_EINVAL, "index %llu out of range %llu", (unsigned long long)idx[i],
@@ -897,7 +907,168 @@ extern "C" int zkg_tree_free(zkg_tree *t)
     if (!t) return ZKG_OK;
     (void)hipStreamSynchronize(t->ctx->stream);
     dev_free(t->ctx, t->base);
+    if (t->desc) dev_free(t->ctx, t->desc);
     delete t;
+    return ZKG_OK;
+}
+
+// ------------------------------------------------------------------ mixed-height MMCS
+static uint64_t next_pow2_u64(uint64_t x)
+{
+    uint64_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+// p3 MerkleTree::new over several matrices (vendor/p3-merkle-tree-gpu/src/
+// merkle_tree.rs:84-151, first_digest_layer :153-213, compress_and_inject
+// :272-350): matrices sorted tallest first (stable); the tallest group's
+// concatenated rows are hashed into layer 0, padded with zero digests to even
+// length unless it has one row; each next layer has plen/2 compressions,
+// padded to even (a single root when plen = 2); a group whose heights round up
+// to next_pow2(plen/2) is injected there.
+extern "C" int zkg_mmcs_commit(zkg_ctx *ctx, int field, int p2_width, uint32_t n_mats, const zkg_buf *const *mats,
+                               const uint64_t *heights, const uint32_t *widths, zkg_tree **out)
+{
+    if (!ctx || !mats || !heights || !widths || !out || !n_mats) return set_err(ZKG_EINVAL, "null argument");
+    if (int rc = check_field(field)) return rc;
+    const P2Set *p2 = get_p2(ctx, field, p2_width);
+    if (!p2) return set_err(ZKG_ESTATE, "Poseidon2 constants for field %d width %d not loaded", field, p2_width);
+    const uint64_t es = esize(field);
+    for (uint32_t i = 0; i < n_mats; i++) {
+        if (!mats[i]) return set_err(ZKG_EINVAL, "matrix %u is null", i);
+        if (!heights[i] || heights[i] > (1ull << 31)) return set_err(ZKG_EINVAL, "matrix %u: height %llu", i,
+                                                                    (unsigned long long)heights[i]);
+        if (heights[i] * widths[i] * es > mats[i]->bytes) return set_err(ZKG_EINVAL, "matrix %u: buffer too small", i);
+    }
+    std::vector<uint32_t> ord(n_mats);
+    for (uint32_t i = 0; i < n_mats; i++) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return heights[a] > heights[b]; });
+    for (uint32_t i = 1; i < n_mats; i++) {
+        uint64_t a = heights[ord[i - 1]], b = heights[ord[i]];
+        if (a != b && next_pow2_u64(a) == next_pow2_u64(b))
+            return set_err(ZKG_EINVAL, "heights %llu and %llu round up to the same power of two",
+                           (unsigned long long)a, (unsigned long long)b);
+    }
+    // layer lengths and the matrix group injected at each layer
+    const uint64_t maxh = heights[ord[0]];
+    std::vector<uint64_t> len{maxh == 1 ? 1 : maxh + (maxh & 1)};
+    std::vector<std::pair<uint32_t, uint32_t>> group{{0, 0}};  // [first, count) into ord, per layer
+    uint32_t cursor = 0;
+    while (cursor < n_mats && heights[ord[cursor]] == maxh) cursor++;
+    group[0] = {0, cursor};
+    while (len.back() != 1) {
+        const uint64_t plen = len.back(), np2 = next_pow2_u64(plen / 2);
+        uint32_t k = 0;
+        while (cursor + k < n_mats && next_pow2_u64(heights[ord[cursor + k]]) == np2) k++;
+        group.push_back({cursor, k});
+        cursor += k;
+        len.push_back(plen == 2 ? 1 : ((plen / 2 + 1) & ~1ull));
+    }
+    if (cursor != n_mats) return set_err(ZKG_EINVAL, "a matrix height matches no tree layer");
+
+    zkg_tree *t = new zkg_tree();
+    t->ctx = ctx;
+    t->field = field;
+    t->p2_width = p2_width;
+    t->digest = p2_width / 2;
+    t->n_leaves = maxh;
+    t->nlayers = (int)len.size();
+    t->mmcs = true;
+    t->max_height = maxh;
+    for (uint32_t i = 0; i < n_mats; i++) {
+        t->mats.push_back(MatDesc{mats[i]->ptr, widths[i]});
+        t->mat_heights.push_back(heights[i]);
+    }
+    uint64_t off = 0;
+    for (int i = 0; i < t->nlayers; i++) {
+        t->layer_off.push_back(off);
+        t->layer_len.push_back(len[i]);
+        off += len[i] * t->digest * es;
+        off = (off + 255) & ~255ull;
+    }
+    std::vector<MatDesc> desc;
+    for (uint32_t i = 0; i < n_mats; i++) desc.push_back(MatDesc{mats[ord[i]]->ptr, widths[ord[i]]});
+    int rc = dev_alloc(ctx, off, &t->base);
+    if (!rc) rc = dev_alloc(ctx, desc.size() * sizeof(MatDesc), &t->desc);
+    if (rc) {
+        zkg_tree_free(t);
+        return rc;
+    }
+    for (int i = 0; i < t->nlayers; i++) t->layers.push_back((char *)t->base + t->layer_off[i]);
+    const MatDesc *d = (const MatDesc *)t->desc;
+    hipStream_t st = ctx->stream;
+    hipError_t e = hipMemcpyAsync(t->desc, desc.data(), desc.size() * sizeof(MatDesc), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemsetAsync(t->base, 0, off, st);  // zero-digest padding
+    if (e == hipSuccess) e = launch_mmcs_leaves(field, p2_width, *p2, d + group[0].first, (int)group[0].second, maxh,
+                                                t->layers[0], st);
+    for (int i = 1; i < t->nlayers && e == hipSuccess; i++) {
+        const uint32_t g0 = group[i].first, gn = group[i].second;
+        e = launch_mmcs_layer(field, p2_width, *p2, t->layers[i - 1], len[i - 1] / 2, t->layers[i], d + g0, (int)gn,
+                              gn ? heights[ord[g0]] : 0, st);
+    }
+    // the host descriptor vector goes out of scope: finish the upload first
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        zkg_tree_free(t);
+        return hip_err(e, "mmcs commit");
+    }
+    *out = t;
+    return ZKG_OK;
+}
+
+static int log2_ceil_u64(uint64_t x)
+{
+    int l = 0;
+    while ((1ull << l) < x) l++;
+    return l;
+}
+
+// Mmcs::open_batch (vendor/p3-merkle-tree-gpu/src/mmcs.rs:179-210): matrix j
+// opens row index >> (log2_ceil(max_h) - log2_ceil(h_j)); the proof is the
+// sibling digest of every layer below the root.
+extern "C" int zkg_mmcs_open(const zkg_tree *t, const uint64_t *indices, uint32_t n, void *rows_out, void *paths_out)
+{
+    if (!t || (!indices && n)) return set_err(ZKG_EINVAL, "null argument");
+    std::vector<MatDesc> mats = t->mats;
+    std::vector<uint64_t> hs = t->mat_heights;
+    if (!t->mmcs) {
+        mats = {MatDesc{t->mat, t->mat_width}};
+        hs = {t->n_leaves};
+    }
+    const uint64_t maxh = t->mmcs ? t->max_height : t->n_leaves, es = esize(t->field);
+    const int logmax = log2_ceil_u64(maxh);
+    uint64_t row_elems = 0;
+    for (auto &m : mats) row_elems += m.width;
+    HIP_TRY(hipStreamSynchronize(t->ctx->stream), "sync");
+    for (uint32_t q = 0; q < n; q++) {
+        const uint64_t index = indices[q];
+        if (index >= maxh)
+            return set_err(ZKG_EINVAL, "index %llu out of range %llu", (unsigned long long)index,
+                           (unsigned long long)maxh);
+        if (rows_out) {
+            char *dst = (char *)rows_out + q * row_elems * es;
+            for (size_t j = 0; j < mats.size(); j++) {
+                const uint64_t r = index >> (logmax - log2_ceil_u64(hs[j]));
+                if (r >= hs[j])  // a padding position of a shorter matrix (p3 would panic on the row)
+                    return set_err(ZKG_EINVAL, "index %llu has no row in matrix %zu (height %llu)",
+                                   (unsigned long long)index, j, (unsigned long long)hs[j]);
+                const uint64_t rb = mats[j].width * es;
+                if (rb) HIP_TRY(hipMemcpy(dst, (const char *)mats[j].ptr + r * rb, rb, hipMemcpyDeviceToHost), "row");
+                dst += rb;
+            }
+        }
+        if (paths_out) {
+            char *dst = (char *)paths_out + (uint64_t)q * logmax * t->digest * es;
+            for (int i = 0; i < logmax; i++) {
+                const uint64_t sib = (index >> i) ^ 1;
+                HIP_TRY(hipMemcpy(dst + (uint64_t)i * t->digest * es,
+                                  (const char *)t->layers[i] + sib * t->digest * es, t->digest * es,
+                                  hipMemcpyDeviceToHost),
+                        "path");
+            }
+        }
+    }
     return ZKG_OK;
 }
 

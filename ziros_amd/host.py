@@ -206,8 +206,13 @@ class MerkleTree:
         check(self.ctx.lib.zkg_tree_root(self.h, _ptr(out)))
         return out
 
-    def layer(self, i, n_leaves):
-        n = n_leaves >> i
+    def layer_len(self, i):
+        n = ctypes.c_uint64()
+        check(self.ctx.lib.zkg_tree_layer_len(self.h, i, ctypes.byref(n)))
+        return n.value
+
+    def layer(self, i, n_leaves=None):
+        n = self.layer_len(i)
         out = np.empty(n * self.digest, dtype=DTYPE[self.field])
         check(self.ctx.lib.zkg_tree_layer(self.h, i, _ptr(out), out.nbytes))
         return out.reshape(n, self.digest)
@@ -219,6 +224,17 @@ class MerkleTree:
         paths = np.empty(n * (self.num_layers - 1) * self.digest, dtype=DTYPE[self.field])
         check(self.ctx.lib.zkg_tree_open(self.h, _ptr(idx), n, _ptr(rows), _ptr(paths)))
         return rows.reshape(n, width), paths.reshape(n, self.num_layers - 1, self.digest)
+
+    def open_mmcs(self, indices, total_width):
+        """Mmcs::open_batch for each index: the rows of every matrix (commit
+        order, concatenated) and log2_ceil(max height) sibling digests."""
+        idx = np.ascontiguousarray(indices, dtype=np.uint64)
+        n = idx.size
+        rows = np.empty(max(n * total_width, 1), dtype=DTYPE[self.field])
+        paths = np.empty(max(n * (self.num_layers - 1) * self.digest, 1), dtype=DTYPE[self.field])
+        check(self.ctx.lib.zkg_mmcs_open(self.h, _ptr(idx), n, _ptr(rows), _ptr(paths)))
+        return (rows[: n * total_width].reshape(n, total_width),
+                paths[: n * (self.num_layers - 1) * self.digest].reshape(n, self.num_layers - 1, self.digest))
 
     def free(self):
         if self.h:
@@ -246,6 +262,20 @@ class HipMerkleTreeMmcs:
         t = ctypes.c_void_p()
         check(self.ctx.lib.zkg_merkle_commit(self.ctx.h, self.field, self.p2_width, b.h, h, w, ctypes.byref(t)))
         tree = MerkleTree(self.ctx, t, self.field, keep=(b,))
+        return tree.root(), tree
+
+    def commit(self, mats):
+        """Mmcs::commit over matrices of different heights (compress_and_inject)."""
+        mats = [np.ascontiguousarray(m, dtype=DTYPE[self.field]) for m in mats]
+        bufs = [self.ctx.upload(m) for m in mats]
+        n = len(mats)
+        handles = (ctypes.c_void_p * n)(*[b.h for b in bufs])
+        heights = np.array([m.shape[0] for m in mats], dtype=np.uint64)
+        widths = np.array([m.shape[1] for m in mats], dtype=np.uint32)
+        t = ctypes.c_void_p()
+        check(self.ctx.lib.zkg_mmcs_commit(self.ctx.h, self.field, self.p2_width, n, handles, _ptr(heights),
+                                           _ptr(widths), ctypes.byref(t)))
+        tree = MerkleTree(self.ctx, t, self.field, keep=tuple(bufs))
         return tree.root(), tree
 
     def open_batch(self, index, tree, width):
