@@ -838,6 +838,102 @@ int or_fri_commit_phase_ef4(const or_p2 *p, uint64_t *cw, uint32_t log_len, uint
     return r;
 }
 
+/* ======================================================================
+ * PCS opening (DEEP quotient), restated from p3-fri 0.4.2
+ * TwoAdicFriPcs::open and p3-interpolation 0.4.2 interpolate_coset; neither
+ * is in the reference tree.  BabyBear base field, EF4 challenges.
+ * ====================================================================== */
+static void ef4_pow(const uint64_t *a, u128 e, uint64_t *out)
+{
+    uint64_t r[4] = {1, 0, 0, 0}, b[4] = {a[0], a[1], a[2], a[3]};
+    while (e) {
+        if (e & 1) or_ef4_mul(r, b, r);
+        or_ef4_mul(b, b, b);
+        e >>= 1;
+    }
+    memcpy(out, r, sizeof r);
+}
+
+/* a^(p^4 - 2): Fermat in the multiplicative group of EF4. */
+void or_ef4_inv(const uint64_t *a, uint64_t *out)
+{
+    const u128 p = OR_BB_P, q = p * p * p * p;
+    ef4_pow(a, q - 2, out);
+}
+
+/* x_i = g * w_H^rev(i) for i < height (bit-reversed coset order). */
+static uint64_t *deep_points(uint64_t height, uint64_t shift)
+{
+    uint32_t lg = log2_exact(height);
+    uint64_t w = or_f_two_adic_generator(OR_BABYBEAR, lg);
+    uint64_t *x = (uint64_t *)malloc(height * sizeof(uint64_t));
+    uint64_t acc = shift;
+    for (uint64_t j = 0; j < height; j++) {
+        x[rev_bits(j, lg)] = acc;
+        acc = or_bb_mul(acc, w);
+    }
+    return x;
+}
+
+void or_deep_open_values(const uint64_t *lde, uint64_t height, uint32_t width, uint32_t log_blowup,
+                         uint64_t shift, const uint64_t *z, uint64_t *ys)
+{
+    const uint64_t n = height >> log_blowup;
+    uint64_t *x = deep_points(height, shift);
+    memset(ys, 0, (size_t)width * 4 * sizeof(uint64_t));
+    /* rows 0..n-1 are p on g*H_n (their points x_i are the coset g*H_n) */
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t d[4] = {bb_sub(z[0], x[i]), z[1], z[2], z[3]}, w[4];
+        or_ef4_inv(d, w);
+        ef4_mul_base(w, x[i], w); /* x_i / (z - x_i) */
+        for (uint32_t k = 0; k < width; k++) {
+            uint64_t t[4];
+            ef4_mul_base(w, lde[i * width + k], t);
+            ef4_add(ys + 4 * k, t, ys + 4 * k);
+        }
+    }
+    /* scale (z^n - g^n) / (n g^n) */
+    uint64_t zn[4], gn = or_bb_pow(shift, n), s[4];
+    ef4_pow(z, n, zn);
+    zn[0] = bb_sub(zn[0], gn);
+    ef4_mul_base(zn, or_f_inv(OR_BABYBEAR, or_bb_mul(n % OR_BB_P, gn)), s);
+    for (uint32_t k = 0; k < width; k++) or_ef4_mul(ys + 4 * k, s, ys + 4 * k);
+    free(x);
+}
+
+void or_deep_reduce(const uint64_t *lde, uint64_t height, uint32_t width, uint64_t shift, const uint64_t *z,
+                    const uint64_t *alpha, uint64_t alpha_off, const uint64_t *ys, uint64_t *ro)
+{
+    uint64_t *x = deep_points(height, shift);
+    uint64_t *ap = (uint64_t *)malloc(((size_t)width + 1) * 4 * sizeof(uint64_t));
+    uint64_t ry[4] = {0, 0, 0, 0}, aoff[4];
+    ap[0] = 1, ap[1] = ap[2] = ap[3] = 0;
+    for (uint32_t k = 1; k <= width; k++) or_ef4_mul(ap + 4 * (k - 1), alpha, ap + 4 * k);
+    for (uint32_t k = 0; k < width; k++) {
+        uint64_t t[4];
+        or_ef4_mul(ap + 4 * k, ys + 4 * k, t);
+        ef4_add(ry, t, ry);
+    }
+    ef4_pow(alpha, alpha_off, aoff);
+#pragma omp parallel for schedule(static)
+    for (uint64_t i = 0; i < height; i++) {
+        uint64_t acc[4] = {0, 0, 0, 0}, t[4], d[4], inv[4];
+        for (uint32_t k = 0; k < width; k++) {
+            ef4_mul_base(ap + 4 * k, lde[i * width + k], t);
+            ef4_add(acc, t, acc);
+        }
+        ef4_sub(acc, ry, acc);
+        d[0] = bb_sub(x[i], z[0]);
+        d[1] = bb_sub(0, z[1]), d[2] = bb_sub(0, z[2]), d[3] = bb_sub(0, z[3]);
+        or_ef4_inv(d, inv);
+        or_ef4_mul(acc, inv, acc);
+        or_ef4_mul(acc, aoff, acc);
+        ef4_add(ro + 4 * i, acc, ro + 4 * i);
+    }
+    free(ap);
+    free(x);
+}
+
 void or_set_threads(int n)
 {
 #ifdef _OPENMP

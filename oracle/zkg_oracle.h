@@ -137,6 +137,21 @@ int or_fri_commit_phase_ef4(const or_p2 *p, uint64_t *codeword, uint32_t log_len
                             uint64_t *final_out, or_tree *trees_out);
 void or_gen_ef4_betas(uint64_t seed, int n, uint64_t *betas);
 
+/* ---- PCS opening: DEEP quotient over BabyBear with EF4 challenges ----
+ * Restates p3-fri 0.4.2 TwoAdicFriPcs::open (external to the reference; the
+ * reference's node is ProverOp::FriQueryOpen, zkf-runtime/src/graph.rs:105-108)
+ * and p3-interpolation 0.4.2 interpolate_coset.  An LDE of height H = B*N holds
+ * rows in bit-reversed order on the coset g*H_H: row i is p(x_i),
+ * x_i = g * w_H^rev(i).  EF4 values are 4 canonical coefficients. */
+void or_ef4_inv(const uint64_t *a, uint64_t *out);
+/* p_k(z) for each column k: barycentric interpolation over the first N rows
+ * (p on g*H_N): (z^N - g^N) / (N g^N) * sum_j e_j x_j / (z - x_j). */
+void or_deep_open_values(const uint64_t *lde, uint64_t height, uint32_t width, uint32_t log_blowup,
+                         uint64_t shift, const uint64_t *z, uint64_t *ys);
+/* ro[i] += alpha^off * sum_k alpha^k (p_k(x_i) - ys_k) / (x_i - z), for i < height. */
+void or_deep_reduce(const uint64_t *lde, uint64_t height, uint32_t width, uint64_t shift, const uint64_t *z,
+                    const uint64_t *alpha, uint64_t alpha_off, const uint64_t *ys, uint64_t *ro);
+
 /* ---- threading ---- */
 void or_set_threads(int n);
 int or_get_threads(void);

@@ -97,6 +97,9 @@ def lib():
         L.or_fri_commit_phase_ef4.restype = c_int
         L.or_fri_commit_phase_ef4.argtypes = [ctypes.POINTER(P2), c_vp, c_u32, c_u32, c_vp, c_vp, c_vp, c_vp]
         L.or_gen_ef4_betas.argtypes = [c_u64, c_int, c_vp]
+        L.or_ef4_inv.argtypes = [c_vp, c_vp]
+        L.or_deep_open_values.argtypes = [c_vp, c_u64, c_u32, c_u32, c_u64, c_vp, c_vp]
+        L.or_deep_reduce.argtypes = [c_vp, c_u64, c_u32, c_u64, c_vp, c_vp, c_u64, c_vp, c_vp]
         L.or_set_threads.argtypes = [c_int]
         L.or_get_threads.restype = c_int
         _lib = L
@@ -140,6 +143,32 @@ def ef4_mul(a, b):
     a, b = u64(a), u64(b)
     out = np.zeros(4, dtype=np.uint64)
     lib().or_ef4_mul(_p(a), _p(b), _p(out))
+    return out
+
+
+def ef4_inv(a):
+    a = u64(a)
+    out = np.zeros(4, dtype=np.uint64)
+    lib().or_ef4_inv(_p(a), _p(out))
+    return out
+
+
+# ---------------------------------------------------------------- DEEP quotient
+def deep_open_values(lde, log_blowup, shift, z):
+    """p_k(z) of every column of a bit-reversed coset LDE (EF4, shape (width, 4))."""
+    m = u64(lde)
+    h, w = m.shape
+    ys = np.zeros((w, 4), dtype=np.uint64)
+    lib().or_deep_open_values(_p(m), h, w, log_blowup, shift, _p(u64(z)), _p(ys))
+    return ys
+
+
+def deep_reduce(lde, shift, z, alpha, alpha_off, ys, ro=None):
+    """ro[i] += alpha^off * sum_k alpha^k (p_k(x_i) - ys_k) / (x_i - z); returns ro (height, 4)."""
+    m = u64(lde)
+    h, w = m.shape
+    out = np.zeros((h, 4), dtype=np.uint64) if ro is None else u64(ro).copy()
+    lib().or_deep_reduce(_p(m), h, w, shift, _p(u64(z)), _p(u64(alpha)), alpha_off, _p(u64(ys)), _p(out))
     return out
 
 

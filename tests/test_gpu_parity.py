@@ -343,3 +343,47 @@ def test_errors_are_reported(ctx):
         Z.HipDft(ctx, BB).coset_lde_batch(np.zeros((3, 2), dtype=np.uint32), 1)  # not a power of two
     with pytest.raises(Z.ZkgError):
         Z.HipDft(ctx, BB).coset_lde_batch(np.zeros((4, 2), dtype=np.uint32), 3)  # blowup too large
+
+
+# ----------------------------------------------------------------- DEEP quotient
+def _deep_case(ctx, logn, w, lb, seed):
+    n = 1 << logn
+    rng = np.random.default_rng(seed)
+    tr = rnd(BB, n * w, seed).reshape(n, w)
+    lde = O.coset_lde_batch(BB, tr, lb, 31)
+    z = [int(v) for v in rng.integers(0, O.BB_P, 4)]
+    alpha = [int(v) for v in rng.integers(0, O.BB_P, 4)]
+    buf = ctx.upload(lde.astype(np.uint32))
+    return lde, z, alpha, buf, Z.HipDeep(ctx, buf, n << lb, w, lb, 31)
+
+
+@pytest.mark.parametrize("logn,w,lb", [(0, 3, 1), (3, 5, 1), (7, 16, 1), (10, 64, 1), (9, 33, 2), (12, 8, 3),
+                                       (11, 300, 1)])
+def test_deep_open_values_vs_oracle(ctx, logn, w, lb):
+    lde, z, _, buf, deep = _deep_case(ctx, logn, w, lb, 40 + logn)
+    ys = deep.open_values(z)
+    assert (ys.astype(np.uint64) == O.deep_open_values(lde, lb, 31, z)).all()
+
+
+@pytest.mark.parametrize("logn,w,lb,off", [(0, 2, 1, 0), (4, 7, 1, 3), (8, 64, 1, 0), (10, 17, 2, 5), (11, 64, 1, 1)])
+def test_deep_reduce_vs_oracle(ctx, logn, w, lb, off):
+    lde, z, alpha, buf, deep = _deep_case(ctx, logn, w, lb, 70 + logn)
+    h = (1 << logn) << lb
+    base = rnd(BB, h * 4, 5).reshape(h, 4)
+    ro = ctx.upload(base.astype(np.uint32))
+    ys = deep.open(z, alpha, off, ro)
+    exp_ys = O.deep_open_values(lde, lb, 31, z)
+    assert (ys.astype(np.uint64) == exp_ys).all()
+    got = ro.download(np.uint32, h * 4).reshape(h, 4)
+    assert (got.astype(np.uint64) == O.deep_reduce(lde, 31, z, alpha, off, exp_ys, ro=base)).all()
+    # the separate entry point accumulates the same quotient again
+    deep.reduce(z, alpha, off, ys, ro)
+    twice = O.deep_reduce(lde, 31, z, alpha, off, exp_ys, ro=O.deep_reduce(lde, 31, z, alpha, off, exp_ys, ro=base))
+    assert (ro.download(np.uint32, h * 4).reshape(h, 4).astype(np.uint64) == twice).all()
+
+
+def test_deep_rejects_point_on_domain(ctx):
+    lde, _, alpha, buf, deep = _deep_case(ctx, 4, 3, 1, 3)
+    x = 31 * pow(int(O.two_adic_generator(BB, 5)), 7, O.BB_P) % O.BB_P  # a coset point
+    with pytest.raises(Z.ZkgError):
+        deep.open_values([x, 0, 0, 0])

@@ -1,5 +1,6 @@
 """ziros_amd -- MI355X-native STARK trace commitment (coset LDE, Poseidon2 row
-hashing, Merkle trees, FRI folding) behind the libzkg C ABI (include/zkg.h).
+hashing, Merkle trees, FRI folding, the DEEP quotient of the PCS opening)
+behind the libzkg C ABI (include/zkg.h).
 
 The compute path is hand-written HIP for gfx950 in ``csrc/``; this package
 only binds it.  There is no CPU fallback: if ``libzkg.so`` is missing, every
@@ -13,6 +14,7 @@ from .host import (  # noqa: F401
     Context,
     DeviceBuffer,
     FriCommitment,
+    HipDeep,
     HipDft,
     HipFri,
     HipMerkleTreeMmcs,

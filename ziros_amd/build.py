@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libzkg.so")
 PROBE_LIB = os.path.join(HERE, "libzkg_probe.so")
-SOURCES = ["zkg_ntt.hip", "zkg_hash.hip", "zkg_runtime.hip", "zkg_batch.hip"]
+SOURCES = ["zkg_ntt.hip", "zkg_hash.hip", "zkg_deep.hip", "zkg_runtime.hip", "zkg_batch.hip"]
 PROBE_SOURCES = ["zkg_probe.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -26,7 +26,8 @@ FLAGS = [
 
 # FP64 code that relies on exact products and explicit fma must not be
 # contracted or reassociated by the compiler.
-EXTRA = {"zkg_probe.hip": ["-ffp-contract=off"], "zkg_hash.hip": ["-ffp-contract=off"]}
+EXTRA = {"zkg_probe.hip": ["-ffp-contract=off"], "zkg_hash.hip": ["-ffp-contract=off"],
+         "zkg_deep.hip": ["-ffp-contract=off"]}
 
 
 def _compile(src):

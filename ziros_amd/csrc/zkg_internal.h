@@ -121,6 +121,24 @@ hipError_t launch_gather_open(int field, int digest, const void *mat, uint32_t m
                               void *const *layers, int nlayers, const uint64_t *d_indices,
                               uint32_t n, void *rows_out, void *paths_out, hipStream_t st);
 
+// --- DEEP quotient (zkg_deep.hip), BabyBear LDE, EF4 challenges -------------
+// EF4 arguments are 4 words; "mont" = Montgomery form, otherwise canonical.
+// inv: height EF4 = 1 / (z - x_i) (Montgomery), x_i = g w_H^rev(i).
+hipError_t launch_deep_inv(const TwiddleSet &tw, void *inv, uint64_t height, const uint32_t *z_mont,
+                           uint32_t g_mont, uint32_t w11, hipStream_t st);
+// Blocks of the column-sum pass over n rows (partial: blocks * width EF4).
+uint64_t deep_partial_blocks(uint64_t n);
+// ys (width EF4, canonical) = scale * sum_{i<n} e_ik x_i inv_i.
+hipError_t launch_deep_open_values(const TwiddleSet &tw, const void *lde, uint64_t height, uint32_t width,
+                                   uint64_t n, const void *inv, const uint32_t *z_mont, uint32_t g_mont,
+                                   uint32_t w11, const uint32_t *scale_mont, void *partial, void *ys,
+                                   hipStream_t st);
+// ro_i += (ry - sum_k c_k e_ik) * inv_i; coef: width x 8 doubles, the 16-bit
+// halves [lo0..lo3, hi0..hi3] of the canonical c_k = alpha^off alpha^k; ry and
+// ro canonical.
+hipError_t launch_deep_reduce(const void *lde, uint64_t height, uint32_t width, const void *inv,
+                              const void *coef, const uint32_t *ry, uint32_t w11, void *ro, hipStream_t st);
+
 // Register-form conversion helpers on the host.
 uint32_t bb_to_mont_host(uint32_t x);
 uint32_t bb_from_mont_host(uint32_t x);

@@ -283,6 +283,35 @@ class HipMerkleTreeMmcs:
         return rows[0], paths[0]
 
 
+class HipDeep:
+    """DEEP quotient of TwoAdicFriPcs::open over a device-resident BabyBear LDE
+    (bit-reversed rows on the coset shift * H); EF4 values as 4 canonical u32."""
+
+    def __init__(self, ctx, lde_buf, height, width, log_blowup, shift=None):
+        self.ctx, self.lde, self.height, self.width, self.log_blowup = ctx, lde_buf, height, width, log_blowup
+        self.shift = GENERATOR[BABYBEAR] if shift is None else shift
+
+    def open_values(self, z):
+        z = np.ascontiguousarray(z, dtype=np.uint32)
+        ys = np.empty((self.width, 4), dtype=np.uint32)
+        check(self.ctx.lib.zkg_deep_open_values(self.ctx.h, self.lde.h, self.height, self.width, self.log_blowup,
+                                                self.shift, _ptr(z), _ptr(ys)))
+        return ys
+
+    def reduce(self, z, alpha, alpha_offset, ys, ro_buf):
+        z, alpha = np.ascontiguousarray(z, dtype=np.uint32), np.ascontiguousarray(alpha, dtype=np.uint32)
+        ys = np.ascontiguousarray(ys, dtype=np.uint32)
+        check(self.ctx.lib.zkg_deep_reduce(self.ctx.h, self.lde.h, self.height, self.width, self.shift, _ptr(z),
+                                           _ptr(alpha), alpha_offset, _ptr(ys), ro_buf.h))
+
+    def open(self, z, alpha, alpha_offset, ro_buf):
+        z, alpha = np.ascontiguousarray(z, dtype=np.uint32), np.ascontiguousarray(alpha, dtype=np.uint32)
+        ys = np.empty((self.width, 4), dtype=np.uint32)
+        check(self.ctx.lib.zkg_deep_open(self.ctx.h, self.lde.h, self.height, self.width, self.log_blowup,
+                                         self.shift, _ptr(z), _ptr(alpha), alpha_offset, _ptr(ys), ro_buf.h))
+        return ys
+
+
 def commit_trace(ctx, field, trace_buf, height, width, log_blowup, p2_width=16, plain=False):
     """Fused (or plain reference-structured) commit of a device-resident trace."""
     es = np.dtype(DTYPE[field]).itemsize
