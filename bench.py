@@ -180,6 +180,36 @@ def bench_c4(ctx, steps):
                 device_ms=dev / steps, codeword_elems_per_s=(1 << log_len) / dt)
 
 
+def bench_deep(ctx, steps):
+    """PCS opening, DEEP quotient over the config-2 LDE (BabyBear 2^21 x 64 rows,
+    bit-reversed on the coset 31*H): opened values at one EF4 point, then the
+    reduced codeword.  Algorithmic bytes: the LDE once (reduce) plus its first
+    half (interpolation), and the EF4 codeword read and written."""
+    n, w, lb = 1 << 20, 64, 1
+    h = n << lb
+    tr = ctx.random(Z.BABYBEAR, n * w, SEED_C2)
+    lde = ctx.alloc(h * w * 4)
+    check(ctx.lib.zkg_coset_lde_batch(ctx.h, Z.BABYBEAR, tr.h, n, w, lb, 31, lde.h))
+    ro = ctx.upload(np.zeros(h * 4, dtype=np.uint32))
+    rng = np.random.default_rng(SEED_C2 + 1)
+    z = rng.integers(0, 0x78000001, 4).astype(np.uint32)
+    alpha = rng.integers(0, 0x78000001, 4).astype(np.uint32)
+    deep = Z.HipDeep(ctx, lde, h, w, lb, 31)
+    for _ in range(2):
+        deep.open(z, alpha, 0, ro)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        deep.open(z, alpha, 0, ro)
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    alg = h * w * 4 + n * w * 4 + h * 32
+    for b in (tr, lde, ro):
+        b.free()
+    return dict(lde_rows=h, width=w, ms_per_open=dt * 1e3, algorithmic_bytes=alg,
+                achieved_GBps=alg / dt / 1e9, hbm_frac=alg / dt / HBM_PEAK)
+
+
 def c3_shapes():
     rng = np.random.default_rng(SEED_C3)
     logs = rng.integers(12, 21, 48)
@@ -423,6 +453,9 @@ def main():
     log("c4: fri commit phase 2^22 -> 2^8")
     out["c4_fri"] = bench_c4(ctx, max(3, args.steps // 2))
     log("c4: %s" % out["c4_fri"])
+    log("deep: PCS opening DEEP quotient over the config-2 LDE")
+    out["pcs_open_deep"] = bench_deep(ctx, max(5, args.steps))
+    log("deep: %s" % out["pcs_open_deep"])
     if not args.skip_batch:
         log("c3: batched 48 traces")
         out["batched"] = bench_c3(ctx, max(1, args.steps // 4), 1)
