@@ -194,7 +194,7 @@ def bench_deep(ctx, steps):
     rng = np.random.default_rng(SEED_C2 + 1)
     z = rng.integers(0, 0x78000001, 4).astype(np.uint32)
     alpha = rng.integers(0, 0x78000001, 4).astype(np.uint32)
-    deep = Z.HipDeep(ctx, lde, h, w, lb, 31)
+    deep = Z.HipDeep(ctx, Z.BABYBEAR, lde, h, w, lb, 31)
     for _ in range(2):
         deep.open(z, alpha, 0, ro)
     ctx.synchronize()

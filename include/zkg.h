@@ -155,28 +155,32 @@ int zkg_mmcs_commit(zkg_ctx *ctx, int field, int p2_width, uint32_t n_mats, cons
  * Also accepts single-matrix trees. */
 int zkg_mmcs_open(const zkg_tree *tree, const uint64_t *indices, uint32_t n, void *rows_out, void *paths_out);
 
-/* ---- PCS opening: DEEP quotient (BabyBear LDE, EF4 challenges) --------------
+/* ---- PCS opening: DEEP quotient --------------------------------------------
  * Replaces the reduced-opening step of p3-fri 0.4.2 TwoAdicFriPcs::open and
  * p3-interpolation interpolate_coset (external; the reference's node is
  * ProverOp::FriQueryOpen, zkf-runtime/src/graph.rs:105-108).  `lde` is a
  * committed LDE as zkg_commit/zkg_coset_lde_batch leave it: height H = 2^k
  * rows in bit-reversed order on the coset shift * H_H, so row i holds
- * p(x_i), x_i = shift * w_H^rev(i).  EF4 values (x^4 - 11) are 4 canonical
- * u32 coefficients, low first.  z must not lie on the domain.
+ * p(x_i), x_i = shift * w_H^rev(i).  Challenge-field values (z, alpha, opened
+ * values, ro entries):
+ *   ZKG_BABYBEAR   EF4 = x^4 - 11, 4 canonical u32 coefficients, low
+ *                  first (base-field challenges, the reference's
+ *                  Challenge<F> = F of zkf-backends/src/plonky3.rs:55,
+ *                  are EF4 values with zero upper coefficients);
+ *   ZKG_GOLDILOCKS Goldilocks itself, one canonical u64 (Challenge<F> = F).
+ * z must not lie on the domain.
  *
- * Opened values: ys_out[k] = p_k(z) for every column k (width EF4), by
- * barycentric interpolation over the first H >> log_blowup rows. */
-int zkg_deep_open_values(zkg_ctx *ctx, const zkg_buf *lde, uint64_t height, uint32_t width, uint32_t log_blowup,
-                         uint64_t shift, const uint32_t *z, uint32_t *ys_out);
+ * Opened values: ys_out[k] = p_k(z) for every column k, by barycentric
+ * interpolation over the first H >> log_blowup rows. */
+int zkg_deep_open_values(zkg_ctx *ctx, int field, const zkg_buf *lde, uint64_t height, uint32_t width,
+                         uint32_t log_blowup, uint64_t shift, const void *z, void *ys_out);
 /* Reduced opening: ro[i] += alpha^alpha_offset * sum_k alpha^k (p_k(x_i) - ys[k]) / (x_i - z)
- * for every row (ro: H EF4, canonical, accumulated in place). */
-int zkg_deep_reduce(zkg_ctx *ctx, const zkg_buf *lde, uint64_t height, uint32_t width, uint64_t shift,
-                    const uint32_t *z, const uint32_t *alpha, uint64_t alpha_offset, const uint32_t *ys,
-                    zkg_buf *ro);
+ * for every row (ro: H challenge-field values, canonical, accumulated in place). */
+int zkg_deep_reduce(zkg_ctx *ctx, int field, const zkg_buf *lde, uint64_t height, uint32_t width, uint64_t shift,
+                    const void *z, const void *alpha, uint64_t alpha_offset, const void *ys, zkg_buf *ro);
 /* Both, sharing one pass of inverse denominators: ys_out as above, then ro. */
-int zkg_deep_open(zkg_ctx *ctx, const zkg_buf *lde, uint64_t height, uint32_t width, uint32_t log_blowup,
-                  uint64_t shift, const uint32_t *z, const uint32_t *alpha, uint64_t alpha_offset, uint32_t *ys_out,
-                  zkg_buf *ro);
+int zkg_deep_open(zkg_ctx *ctx, int field, const zkg_buf *lde, uint64_t height, uint32_t width, uint32_t log_blowup,
+                  uint64_t shift, const void *z, const void *alpha, uint64_t alpha_offset, void *ys_out, zkg_buf *ro);
 
 /* Fused commit of one trace: coset LDE at log_blowup with the field's
  * generator shift (p3 Val::GENERATOR: 7 or 31), row hash, tree.  lde_out

@@ -934,6 +934,57 @@ void or_deep_reduce(const uint64_t *lde, uint64_t height, uint32_t width, uint64
     free(x);
 }
 
+static uint64_t *deep_points_f(int f, uint64_t height, uint64_t shift)
+{
+    uint32_t lg = log2_exact(height);
+    uint64_t w = or_f_two_adic_generator(f, lg);
+    uint64_t *x = (uint64_t *)malloc(height * sizeof(uint64_t));
+    uint64_t acc = shift;
+    for (uint64_t j = 0; j < height; j++) {
+        x[rev_bits(j, lg)] = acc;
+        acc = or_f_mul(f, acc, w);
+    }
+    return x;
+}
+
+void or_deep_gl_open_values(const uint64_t *lde, uint64_t height, uint32_t width, uint32_t log_blowup,
+                            uint64_t shift, uint64_t z, uint64_t *ys)
+{
+    const int f = OR_GOLDILOCKS;
+    const uint64_t n = height >> log_blowup;
+    uint64_t *x = deep_points_f(f, height, shift);
+    memset(ys, 0, (size_t)width * sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t w = or_f_mul(f, x[i], or_f_inv(f, or_gl_sub(z, x[i]))); /* x_i / (z - x_i) */
+        for (uint32_t k = 0; k < width; k++) ys[k] = or_gl_add(ys[k], or_gl_mul(w, lde[i * width + k]));
+    }
+    uint64_t gn = or_gl_pow(shift, n);
+    uint64_t s = or_gl_mul(or_gl_sub(or_gl_pow(z, n), gn), or_f_inv(f, or_gl_mul(n % OR_GL_P, gn)));
+    for (uint32_t k = 0; k < width; k++) ys[k] = or_gl_mul(ys[k], s);
+    free(x);
+}
+
+void or_deep_gl_reduce(const uint64_t *lde, uint64_t height, uint32_t width, uint64_t shift, uint64_t z,
+                       uint64_t alpha, uint64_t alpha_off, const uint64_t *ys, uint64_t *ro)
+{
+    const int f = OR_GOLDILOCKS;
+    uint64_t *x = deep_points_f(f, height, shift);
+    uint64_t *ap = (uint64_t *)malloc(((size_t)width + 1) * sizeof(uint64_t));
+    uint64_t ry = 0, aoff = or_gl_pow(alpha, alpha_off);
+    ap[0] = 1;
+    for (uint32_t k = 1; k <= width; k++) ap[k] = or_gl_mul(ap[k - 1], alpha);
+    for (uint32_t k = 0; k < width; k++) ry = or_gl_add(ry, or_gl_mul(ap[k], ys[k]));
+#pragma omp parallel for schedule(static)
+    for (uint64_t i = 0; i < height; i++) {
+        uint64_t acc = 0;
+        for (uint32_t k = 0; k < width; k++) acc = or_gl_add(acc, or_gl_mul(ap[k], lde[i * width + k]));
+        acc = or_gl_mul(or_gl_sub(acc, ry), or_f_inv(f, or_gl_sub(x[i], z)));
+        ro[i] = or_gl_add(ro[i], or_gl_mul(acc, aoff));
+    }
+    free(ap);
+    free(x);
+}
+
 void or_set_threads(int n)
 {
 #ifdef _OPENMP

@@ -152,6 +152,14 @@ void or_deep_open_values(const uint64_t *lde, uint64_t height, uint32_t width, u
 void or_deep_reduce(const uint64_t *lde, uint64_t height, uint32_t width, uint64_t shift, const uint64_t *z,
                     const uint64_t *alpha, uint64_t alpha_off, const uint64_t *ys, uint64_t *ro);
 
+/* Goldilocks with base-field challenges (the reference's Challenge<F> = F,
+ * zkf-backends/src/plonky3.rs:55): the same two steps, z, alpha, ys and ro in
+ * Goldilocks. */
+void or_deep_gl_open_values(const uint64_t *lde, uint64_t height, uint32_t width, uint32_t log_blowup,
+                            uint64_t shift, uint64_t z, uint64_t *ys);
+void or_deep_gl_reduce(const uint64_t *lde, uint64_t height, uint32_t width, uint64_t shift, uint64_t z,
+                       uint64_t alpha, uint64_t alpha_off, const uint64_t *ys, uint64_t *ro);
+
 /* ---- threading ---- */
 void or_set_threads(int n);
 int or_get_threads(void);

@@ -100,6 +100,8 @@ def lib():
         L.or_ef4_inv.argtypes = [c_vp, c_vp]
         L.or_deep_open_values.argtypes = [c_vp, c_u64, c_u32, c_u32, c_u64, c_vp, c_vp]
         L.or_deep_reduce.argtypes = [c_vp, c_u64, c_u32, c_u64, c_vp, c_vp, c_u64, c_vp, c_vp]
+        L.or_deep_gl_open_values.argtypes = [c_vp, c_u64, c_u32, c_u32, c_u64, c_u64, c_vp]
+        L.or_deep_gl_reduce.argtypes = [c_vp, c_u64, c_u32, c_u64, c_u64, c_u64, c_u64, c_vp, c_vp]
         L.or_set_threads.argtypes = [c_int]
         L.or_get_threads.restype = c_int
         _lib = L
@@ -169,6 +171,23 @@ def deep_reduce(lde, shift, z, alpha, alpha_off, ys, ro=None):
     h, w = m.shape
     out = np.zeros((h, 4), dtype=np.uint64) if ro is None else u64(ro).copy()
     lib().or_deep_reduce(_p(m), h, w, shift, _p(u64(z)), _p(u64(alpha)), alpha_off, _p(u64(ys)), _p(out))
+    return out
+
+
+def deep_gl_open_values(lde, log_blowup, shift, z):
+    """Goldilocks, base-field z: p_k(z) for every column (shape (width,))."""
+    m = u64(lde)
+    h, w = m.shape
+    ys = np.zeros(w, dtype=np.uint64)
+    lib().or_deep_gl_open_values(_p(m), h, w, log_blowup, shift, z, _p(ys))
+    return ys
+
+
+def deep_gl_reduce(lde, shift, z, alpha, alpha_off, ys, ro=None):
+    m = u64(lde)
+    h, w = m.shape
+    out = np.zeros(h, dtype=np.uint64) if ro is None else u64(ro).copy()
+    lib().or_deep_gl_reduce(_p(m), h, w, shift, z, alpha, alpha_off, _p(u64(ys)), _p(out))
     return out
 
 

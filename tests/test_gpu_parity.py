@@ -354,7 +354,7 @@ def _deep_case(ctx, logn, w, lb, seed):
     z = [int(v) for v in rng.integers(0, O.BB_P, 4)]
     alpha = [int(v) for v in rng.integers(0, O.BB_P, 4)]
     buf = ctx.upload(lde.astype(np.uint32))
-    return lde, z, alpha, buf, Z.HipDeep(ctx, buf, n << lb, w, lb, 31)
+    return lde, z, alpha, buf, Z.HipDeep(ctx, BB, buf, n << lb, w, lb, 31)
 
 
 @pytest.mark.parametrize("logn,w,lb", [(0, 3, 1), (3, 5, 1), (7, 16, 1), (10, 64, 1), (9, 33, 2), (12, 8, 3),
@@ -387,3 +387,45 @@ def test_deep_rejects_point_on_domain(ctx):
     x = 31 * pow(int(O.two_adic_generator(BB, 5)), 7, O.BB_P) % O.BB_P  # a coset point
     with pytest.raises(Z.ZkgError):
         deep.open_values([x, 0, 0, 0])
+
+
+def test_deep_base_field_challenges_babybear(ctx):
+    """The reference's Challenge<F> = F: z, alpha in BabyBear are EF4 values with
+    zero upper coefficients, and everything stays in the base field."""
+    lde, _, _, buf, deep = _deep_case(ctx, 9, 12, 1, 123)
+    z, alpha = [987654321, 0, 0, 0], [13579, 0, 0, 0]
+    h = (1 << 9) << 1
+    ro = ctx.upload(np.zeros(h * 4, dtype=np.uint32))
+    ys = deep.open(z, alpha, 2, ro)
+    exp_ys = O.deep_open_values(lde, 1, 31, z)
+    assert (ys.astype(np.uint64) == exp_ys).all() and (exp_ys[:, 1:] == 0).all()
+    got = ro.download(np.uint32, h * 4).reshape(h, 4).astype(np.uint64)
+    assert (got == O.deep_reduce(lde, 31, z, alpha, 2, exp_ys)).all() and (got[:, 1:] == 0).all()
+
+
+@pytest.mark.parametrize("logn,w,lb,off", [(0, 2, 1, 0), (4, 7, 1, 3), (8, 64, 1, 0), (10, 17, 2, 5), (11, 64, 1, 1),
+                                           (9, 100, 3, 0)])
+def test_deep_goldilocks_vs_oracle(ctx, logn, w, lb, off):
+    n = 1 << logn
+    h = n << lb
+    rng = np.random.default_rng(300 + logn)
+    tr = rnd(GL, n * w, 300 + logn).reshape(n, w)
+    lde = O.coset_lde_batch(GL, tr, lb, 7)
+    z, alpha = int(rng.integers(0, 2**62)), int(rng.integers(0, 2**62))
+    buf = ctx.upload(lde)
+    deep = Z.HipDeep(ctx, GL, buf, h, w, lb, 7)
+    base = rnd(GL, h, 9)
+    ro = ctx.upload(base)
+    ys = deep.open(z, alpha, off, ro)
+    exp_ys = O.deep_gl_open_values(lde, lb, 7, z)
+    assert (ys == exp_ys).all()
+    assert (ro.download(np.uint64, h) == O.deep_gl_reduce(lde, 7, z, alpha, off, exp_ys, ro=base)).all()
+    assert (deep.open_values(z) == exp_ys).all()
+
+
+def test_deep_goldilocks_rejects_point_on_domain(ctx):
+    lde = O.coset_lde_batch(GL, rnd(GL, 16 * 2, 1).reshape(16, 2), 1, 7)
+    buf = ctx.upload(lde)
+    x = 7 * pow(int(O.two_adic_generator(GL, 5)), 3, O.GL_P) % O.GL_P
+    with pytest.raises(Z.ZkgError):
+        Z.HipDeep(ctx, GL, buf, 32, 2, 1, 7).open_values(x)

@@ -150,3 +150,65 @@ def test_reduce_offset_and_accumulation():
     base = rng.integers(0, P, ((n << lb), 4)).astype(np.uint64)
     acc = O.deep_reduce(lde, 31, z, alpha, 0, ys, ro=base)
     assert (acc == (base + r0) % P).all()
+
+
+# ---------------------------------------------------------------- Goldilocks, base-field challenges
+GL, GP = O.GL, O.GL_P
+
+
+def gl_lde(rng, n, w, lb):
+    coeffs = (rng.integers(0, 2**63, (n, w), dtype=np.uint64) % np.uint64(GP)).astype(np.uint64)
+    return coeffs, O.coset_lde_batch(GL, O.dft_batch(GL, coeffs), lb, 7)
+
+
+@pytest.mark.parametrize("logn,w,lb", [(1, 2, 1), (5, 3, 1), (6, 2, 2), (4, 5, 3)])
+def test_gl_open_values_equal_horner(logn, w, lb):
+    rng = np.random.default_rng(500 + logn)
+    coeffs, lde = gl_lde(rng, 1 << logn, w, lb)
+    z = int(rng.integers(0, 2**62))
+    ys = O.deep_gl_open_values(lde, lb, 7, z)
+    for k in range(w):
+        acc = 0
+        for c in reversed(coeffs[:, k]):
+            acc = (acc * z + int(c)) % GP
+        assert int(ys[k]) == acc
+
+
+@pytest.mark.parametrize("logn,w,lb", [(4, 3, 1), (5, 2, 2)])
+def test_gl_reduced_codeword_is_low_degree(logn, w, lb):
+    rng = np.random.default_rng(600 + logn)
+    n = 1 << logn
+    _, lde = gl_lde(rng, n, w, lb)
+    z, alpha = int(rng.integers(0, 2**62)), int(rng.integers(0, 2**62))
+    ys = O.deep_gl_open_values(lde, lb, 7, z)
+    ro = O.deep_gl_reduce(lde, 7, z, alpha, 0, ys)
+    h = n << lb
+    lg = h.bit_length() - 1
+    nat = np.empty_like(ro)
+    for i in range(h):
+        nat[rev(i, lg)] = ro[i]
+    c = O.dft_batch(GL, nat.reshape(h, 1), inverse=True).reshape(h)
+    assert (c[n - 1:] == 0).all() and (c[: n - 1] != 0).any()
+    bad = ys.copy()
+    bad[0] = (int(bad[0]) + 1) % GP
+    nat_bad = np.empty_like(ro)
+    ro_bad = O.deep_gl_reduce(lde, 7, z, alpha, 0, bad)
+    for i in range(h):
+        nat_bad[rev(i, lg)] = ro_bad[i]
+    assert (O.dft_batch(GL, nat_bad.reshape(h, 1), inverse=True).reshape(h)[n - 1:] != 0).any()
+
+
+def test_gl_reduce_row_explicit_and_offset():
+    rng = np.random.default_rng(77)
+    n, w, lb = 8, 4, 1
+    _, lde = gl_lde(rng, n, w, lb)
+    z, alpha = int(rng.integers(0, 2**62)), int(rng.integers(0, 2**62))
+    ys = O.deep_gl_open_values(lde, lb, 7, z)
+    ro = O.deep_gl_reduce(lde, 7, z, alpha, 2, ys)
+    h = n << lb
+    lg = h.bit_length() - 1
+    wh = int(O.two_adic_generator(GL, lg))
+    for i in (0, 3, h - 1):
+        x = 7 * pow(wh, rev(i, lg), GP) % GP
+        num = sum(pow(alpha, k, GP) * (int(lde[i, k]) - int(ys[k])) for k in range(w)) % GP
+        assert int(ro[i]) == num * pow((x - z) % GP, GP - 2, GP) * alpha * alpha % GP

@@ -285,6 +285,148 @@ __global__ __launch_bounds__(DEEP_THREADS) void k_deep_reduce(const uint32_t *__
     }
 }
 
+// ---------------------------------------------------------------------------
+// Goldilocks with base-field challenges (z, alpha in Goldilocks): the same
+// three passes in canonical u64 arithmetic.
+constexpr int INV_RUN_GL = 16;
+
+__device__ __forceinline__ uint64_t gl_root_pow(const uint64_t *__restrict__ lo, const uint64_t *__restrict__ hi,
+                                                uint32_t e, int logN)
+{
+    uint32_t ep = (e << (LOG_TW_TOP - logN)) & ((1u << LOG_TW_TOP) - 1u);
+    return Gl::mul(lo[ep & ((1u << TW_SPLIT) - 1u)], hi[ep >> TW_SPLIT]);
+}
+
+__device__ uint64_t gl_inv(uint64_t a)
+{
+    uint64_t r = 1, b = a, e = Gl::P - 2;
+#pragma unroll 1
+    while (e) {
+        if (e & 1) r = Gl::mul(r, b);
+        b = Gl::mul(b, b);
+        e >>= 1;
+    }
+    return r;
+}
+
+__global__ __launch_bounds__(DEEP_THREADS) void k_deep_inv_gl(uint64_t *__restrict__ inv, uint64_t height, int log_h,
+                                                              uint64_t z, uint64_t g, const uint64_t *__restrict__ lo,
+                                                              const uint64_t *__restrict__ hi)
+{
+    const uint64_t r0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * INV_RUN_GL;
+    if (r0 >= height) return;
+    const int run = height - r0 < (uint64_t)INV_RUN_GL ? (int)(height - r0) : INV_RUN_GL;
+    uint64_t d[INV_RUN_GL], pre[INV_RUN_GL];
+#pragma unroll
+    for (int j = 0; j < INV_RUN_GL; j++) {
+        d[j] = j < run ? Gl::sub(z, Gl::mul(g, gl_root_pow(lo, hi, brev((uint32_t)(r0 + j), log_h), log_h))) : 1;
+        pre[j] = j ? Gl::mul(pre[j - 1], d[j]) : d[j];
+    }
+    uint64_t acc = gl_inv(pre[INV_RUN_GL - 1]);
+#pragma unroll
+    for (int j = INV_RUN_GL - 1; j >= 0; j--) {
+        const uint64_t v = j ? Gl::mul(acc, pre[j - 1]) : acc;
+        if (j < run) inv[r0 + j] = v;
+        if (j) acc = Gl::mul(acc, d[j]);
+    }
+}
+
+__global__ __launch_bounds__(DEEP_THREADS) void k_deep_colsum_gl(const uint64_t *__restrict__ lde, uint32_t width,
+                                                                 uint64_t n, int log_h,
+                                                                 const uint64_t *__restrict__ inv, uint64_t g,
+                                                                 const uint64_t *__restrict__ lo,
+                                                                 const uint64_t *__restrict__ hi,
+                                                                 uint64_t *__restrict__ partial)
+{
+    __shared__ uint64_t w[COLSUM_ROWS];  // x_i inv_i
+    __shared__ uint64_t red[DEEP_THREADS];
+    const uint64_t base = (uint64_t)blockIdx.x * COLSUM_ROWS;
+    const int rows = n - base < (uint64_t)COLSUM_ROWS ? (int)(n - base) : COLSUM_ROWS;
+    for (int r = threadIdx.x; r < rows; r += DEEP_THREADS)
+        w[r] = Gl::mul(Gl::mul(g, gl_root_pow(lo, hi, brev((uint32_t)(base + r), log_h), log_h)), inv[base + r]);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t col = blockIdx.y * 64 + lane;
+    uint64_t acc = 0;
+    if (col < width) {
+        const uint64_t *e = lde + base * width + col;
+        constexpr int STEP = DEEP_THREADS / 64, BATCH = 16;
+        int r = wave;
+        for (; r + STEP * (BATCH - 1) < rows; r += STEP * BATCH) {
+            uint64_t v[BATCH];
+#pragma unroll
+            for (int u = 0; u < BATCH; u++) v[u] = e[(uint64_t)(r + STEP * u) * width];
+#pragma unroll
+            for (int u = 0; u < BATCH; u++) acc = Gl::mul_add_lazy(v[u], w[r + STEP * u], acc);
+        }
+        for (; r < rows; r += STEP) acc = Gl::mul_add_lazy(e[(uint64_t)r * width], w[r], acc);
+    }
+    red[threadIdx.x] = Gl::canon(acc);
+    __syncthreads();
+    if (wave == 0 && col < width) {
+        acc = red[lane];
+#pragma unroll
+        for (int q = 1; q < DEEP_THREADS / 64; q++) acc = Gl::add(acc, red[q * 64 + lane]);
+        partial[(uint64_t)blockIdx.x * width + col] = acc;
+    }
+}
+
+__global__ __launch_bounds__(DEEP_THREADS) void k_deep_colsum_final_gl(const uint64_t *__restrict__ partial,
+                                                                       uint32_t nblk, uint32_t width, uint64_t scale,
+                                                                       uint64_t *__restrict__ ys)
+{
+    __shared__ uint64_t red[DEEP_THREADS];
+    const uint32_t col = blockIdx.x;
+    uint64_t acc = 0;
+    for (uint32_t b = threadIdx.x; b < nblk; b += DEEP_THREADS) acc = Gl::add(acc, partial[(uint64_t)b * width + col]);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = DEEP_THREADS / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] = Gl::add(red[threadIdx.x], red[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ys[col] = Gl::mul(red[0], scale);
+}
+
+// ro_i += (ry - sum_k c_k e_ik) * inv_i, c_k = alpha^off alpha^k (wave-uniform
+// scalar loads); one lane per row over an LDS tile of 64 rows x REDUCE_COLS.
+__global__ __launch_bounds__(DEEP_THREADS) void k_deep_reduce_gl(const uint64_t *__restrict__ lde, uint64_t height,
+                                                                 uint32_t width, const uint64_t *__restrict__ inv,
+                                                                 const uint64_t *__restrict__ coef, uint64_t ry,
+                                                                 uint64_t *__restrict__ ro)
+{
+    __shared__ uint64_t tile[REDUCE_WAVES][64][REDUCE_COLS + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t(*t)[REDUCE_COLS + 1] = tile[wave];
+    const uint64_t step = (uint64_t)gridDim.x * REDUCE_WAVES * 64;
+    for (uint64_t r0 = ((uint64_t)blockIdx.x * REDUCE_WAVES + wave) * 64; r0 - wave * 64 < height; r0 += step) {
+        const uint64_t row = r0 + lane;
+        const int nrows = r0 >= height ? 0 : (height - r0 < 64 ? (int)(height - r0) : 64);
+        uint64_t acc = 0;
+        for (uint32_t c0 = 0; c0 < width; c0 += REDUCE_COLS) {
+            const int kc = width - c0 < (uint32_t)REDUCE_COLS ? (int)(width - c0) : REDUCE_COLS;
+            if (nrows && lane < 2 * kc) {
+                const int half = lane >= kc, c = lane - half * kc;
+                uint64_t v[32];
+#pragma unroll
+                for (int i = 0; i < 32; i++) {
+                    const int r = 2 * i + half < nrows ? 2 * i + half : nrows - 1;
+                    v[i] = lde[(r0 + r) * width + c0 + c];
+                }
+#pragma unroll
+                for (int i = 0; i < 32; i++) t[2 * i + half][c] = v[i];
+            }
+            __syncthreads();
+            if (lane < nrows) {
+#pragma unroll 4
+                for (int k = 0; k < kc; k++) acc = Gl::mul_add_lazy(t[lane][k], coef[c0 + k], acc);
+            }
+            __syncthreads();
+        }
+        if (lane < nrows) ro[row] = Gl::add(ro[row], Gl::mul(Gl::sub(ry, Gl::canon(acc)), inv[row]));
+    }
+}
+
 }  // namespace
 
 hipError_t launch_deep_inv(const TwiddleSet &tw, void *inv, uint64_t height, const uint32_t *z_mont,
@@ -331,6 +473,53 @@ hipError_t launch_deep_reduce(const void *lde, uint64_t height, uint32_t width, 
     if (blocks > 4096) blocks = 4096;  // grid-stride beyond 2^20 rows
     hipLaunchKernelGGL(k_deep_reduce, dim3((unsigned)blocks), dim3(DEEP_THREADS), 0, st, (const uint32_t *)lde, height,
                        width, (const uint32_t *)inv, (const double *)coef, r, w11, (uint32_t *)ro);
+    return hipGetLastError();
+}
+
+}  // namespace zkg
+
+namespace zkg {
+
+static int log2_of(uint64_t height)
+{
+    int log_h = 0;
+    while ((1ull << log_h) < height) log_h++;
+    return log_h;
+}
+
+hipError_t launch_deep_gl_inv(const TwiddleSet &tw, void *inv, uint64_t height, uint64_t z, uint64_t g,
+                              hipStream_t st)
+{
+    const uint64_t lanes = (height + INV_RUN_GL - 1) / INV_RUN_GL;
+    hipLaunchKernelGGL(k_deep_inv_gl, dim3((unsigned)((lanes + DEEP_THREADS - 1) / DEEP_THREADS)), dim3(DEEP_THREADS),
+                       0, st, (uint64_t *)inv, height, log2_of(height), z, g, (const uint64_t *)tw.top_lo_fwd,
+                       (const uint64_t *)tw.top_hi_fwd);
+    return hipGetLastError();
+}
+
+hipError_t launch_deep_gl_open_values(const TwiddleSet &tw, const void *lde, uint64_t height, uint32_t width,
+                                      uint64_t n, const void *inv, uint64_t g, uint64_t scale, void *partial, void *ys,
+                                      hipStream_t st)
+{
+    const uint64_t nblk = deep_partial_blocks(n);
+    hipLaunchKernelGGL(k_deep_colsum_gl, dim3((unsigned)nblk, (width + 63) / 64), dim3(DEEP_THREADS), 0, st,
+                       (const uint64_t *)lde, width, n, log2_of(height), (const uint64_t *)inv, g,
+                       (const uint64_t *)tw.top_lo_fwd, (const uint64_t *)tw.top_hi_fwd, (uint64_t *)partial);
+    hipError_t e = hipGetLastError();
+    if (e) return e;
+    hipLaunchKernelGGL(k_deep_colsum_final_gl, dim3(width), dim3(DEEP_THREADS), 0, st, (const uint64_t *)partial,
+                       (uint32_t)nblk, width, scale, (uint64_t *)ys);
+    return hipGetLastError();
+}
+
+hipError_t launch_deep_gl_reduce(const void *lde, uint64_t height, uint32_t width, const void *inv, const void *coef,
+                                 uint64_t ry, void *ro, hipStream_t st)
+{
+    const uint64_t rows_per_block = REDUCE_WAVES * 64;
+    uint64_t blocks = (height + rows_per_block - 1) / rows_per_block;
+    if (blocks > 4096) blocks = 4096;  // grid-stride beyond 2^20 rows
+    hipLaunchKernelGGL(k_deep_reduce_gl, dim3((unsigned)blocks), dim3(DEEP_THREADS), 0, st, (const uint64_t *)lde,
+                       height, width, (const uint64_t *)inv, (const uint64_t *)coef, ry, (uint64_t *)ro);
     return hipGetLastError();
 }
 

@@ -139,6 +139,17 @@ hipError_t launch_deep_open_values(const TwiddleSet &tw, const void *lde, uint64
 hipError_t launch_deep_reduce(const void *lde, uint64_t height, uint32_t width, const void *inv,
                               const void *coef, const uint32_t *ry, uint32_t w11, void *ro, hipStream_t st);
 
+// Goldilocks with base-field z (canonical u64 throughout): inv (height u64),
+// opened values ys (width u64) = scale * sum_{i<n} e_ik x_i inv_i, and
+// ro_i += (ry - sum_k coef_k e_ik) * inv_i.
+hipError_t launch_deep_gl_inv(const TwiddleSet &tw, void *inv, uint64_t height, uint64_t z, uint64_t g,
+                              hipStream_t st);
+hipError_t launch_deep_gl_open_values(const TwiddleSet &tw, const void *lde, uint64_t height, uint32_t width,
+                                      uint64_t n, const void *inv, uint64_t g, uint64_t scale, void *partial,
+                                      void *ys, hipStream_t st);
+hipError_t launch_deep_gl_reduce(const void *lde, uint64_t height, uint32_t width, const void *inv,
+                                 const void *coef, uint64_t ry, void *ro, hipStream_t st);
+
 // Register-form conversion helpers on the host.
 uint32_t bb_to_mont_host(uint32_t x);
 uint32_t bb_from_mont_host(uint32_t x);

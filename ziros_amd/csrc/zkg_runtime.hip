@@ -1110,62 +1110,101 @@ uint32_t bbh_pow(uint32_t a, uint64_t e)
     return r;
 }
 
+uint64_t glh_pow(uint64_t a, uint64_t e)
+{
+    uint64_t r = 1;
+    while (e) {
+        if (e & 1) r = Gl::mul(r, a);
+        a = Gl::mul(a, a);
+        e >>= 1;
+    }
+    return r;
+}
+
+// One DEEP request: BabyBear with EF4 challenges (4 x u32 each) or Goldilocks
+// with base-field challenges (one u64 each).
 struct DeepJob {
+    int field = FIELD_BB;
     uint64_t height = 0;
     uint32_t width = 0;
-    Ef4H z;        // Montgomery
-    uint32_t g = 0;  // Montgomery
+    Ef4H z;              // BabyBear: Montgomery
+    uint32_t g = 0;      // BabyBear shift, Montgomery
+    uint64_t zg = 0, gg = 0;  // Goldilocks z and shift
     void *inv = nullptr;
+    uint64_t es() const { return field == FIELD_GL ? 8 : 16; }  // bytes per challenge-field value
 };
 
-int deep_check(zkg_ctx *ctx, const zkg_buf *lde, uint64_t height, uint32_t width, uint64_t shift, const uint32_t *z,
-               DeepJob &j)
+int deep_check(zkg_ctx *ctx, int field, const zkg_buf *lde, uint64_t height, uint32_t width, uint64_t shift,
+               const void *z, DeepJob &j)
 {
     if (!ctx || !lde || !z) return set_err(ZKG_EINVAL, "null argument");
+    if (int rc = check_field(field)) return rc;
     if (!is_pow2(height) || ilog2u(height) > LOG_TW_TOP) return set_err(ZKG_EINVAL, "LDE height must be 2^k <= 2^24");
     if (!width) return set_err(ZKG_EINVAL, "width 0");
-    if (height * width * 4 > lde->bytes) return set_err(ZKG_EINVAL, "buffer too small");
-    if (shift % Bb::P == 0) return set_err(ZKG_EINVAL, "zero coset shift");
-    for (int i = 0; i < 4; i++)
-        if (z[i] >= Bb::P) return set_err(ZKG_EINVAL, "z is not canonical");
+    j.field = field;
     j.height = height;
     j.width = width;
-    j.z = ef4h_mont(z);
+    if (height * width * esize(field) > lde->bytes) return set_err(ZKG_EINVAL, "buffer too small");
+    if (field == FIELD_GL) {
+        const uint64_t zz = *(const uint64_t *)z;
+        if (zz >= Gl::P) return set_err(ZKG_EINVAL, "z is not canonical");
+        if (shift % Gl::P == 0) return set_err(ZKG_EINVAL, "zero coset shift");
+        j.zg = zz;
+        j.gg = shift % Gl::P;
+        if (glh_pow(j.zg, height) == glh_pow(j.gg, height)) return set_err(ZKG_EINVAL, "z lies on the LDE domain");
+        return ZKG_OK;
+    }
+    const uint32_t *zw = (const uint32_t *)z;
+    if (shift % Bb::P == 0) return set_err(ZKG_EINVAL, "zero coset shift");
+    for (int i = 0; i < 4; i++)
+        if (zw[i] >= Bb::P) return set_err(ZKG_EINVAL, "z is not canonical");
+    j.z = ef4h_mont(zw);
     j.g = Bb::from_canon((uint32_t)(shift % Bb::P));
     // z must avoid the evaluation domain g H_height (x^height = g^height there)
-    if (!z[1] && !z[2] && !z[3] && bbh_pow(j.z.c[0], height) == bbh_pow(j.g, height))
+    if (!zw[1] && !zw[2] && !zw[3] && bbh_pow(j.z.c[0], height) == bbh_pow(j.g, height))
         return set_err(ZKG_EINVAL, "z lies on the LDE domain");
     return ZKG_OK;
 }
 
 int deep_inv(zkg_ctx *ctx, DeepJob &j)
 {
-    int rc = dev_alloc(ctx, j.height * 16, &j.inv);
+    int rc = dev_alloc(ctx, j.height * j.es(), &j.inv);
     if (rc) return rc;
-    HIP_TRY(launch_deep_inv(ctx->tw[FIELD_BB], j.inv, j.height, j.z.c, j.g, Bb::from_canon(11), ctx->stream),
-            "deep inverse denominators");
+    if (j.field == FIELD_GL)
+        HIP_TRY(launch_deep_gl_inv(ctx->tw[FIELD_GL], j.inv, j.height, j.zg, j.gg, ctx->stream),
+                "deep inverse denominators");
+    else
+        HIP_TRY(launch_deep_inv(ctx->tw[FIELD_BB], j.inv, j.height, j.z.c, j.g, Bb::from_canon(11), ctx->stream),
+                "deep inverse denominators");
     return ZKG_OK;
 }
 
-int deep_values(zkg_ctx *ctx, const zkg_buf *lde, uint32_t log_blowup, DeepJob &j, uint32_t *ys_out)
+int deep_values(zkg_ctx *ctx, const zkg_buf *lde, uint32_t log_blowup, DeepJob &j, void *ys_out)
 {
     if (log_blowup > ilog2u(j.height)) return set_err(ZKG_EINVAL, "log_blowup exceeds log height");
     const uint64_t n = j.height >> log_blowup;
-    // (z^n - g^n) / (n g^n)
-    Ef4H zn = ef4h_pow(j.z, n);
-    const uint32_t gn = bbh_pow(j.g, n);
-    zn.c[0] = Bb::sub(zn.c[0], gn);
-    const uint32_t den = Bb::mul(Bb::from_canon((uint32_t)(n % Bb::P)), gn);
-    const uint32_t s = bbh_pow(den, Bb::P - 2);
-    Ef4H scale{{Bb::mul(zn.c[0], s), Bb::mul(zn.c[1], s), Bb::mul(zn.c[2], s), Bb::mul(zn.c[3], s)}};
     void *partial = nullptr, *ys = nullptr;
-    int rc = dev_alloc(ctx, deep_partial_blocks(n) * j.width * 16, &partial);
-    if (!rc) rc = dev_alloc(ctx, (uint64_t)j.width * 16, &ys);
+    int rc = dev_alloc(ctx, deep_partial_blocks(n) * j.width * j.es(), &partial);
+    if (!rc) rc = dev_alloc(ctx, (uint64_t)j.width * j.es(), &ys);
     hipError_t e = hipSuccess;
     if (!rc) {
-        e = launch_deep_open_values(ctx->tw[FIELD_BB], lde->ptr, j.height, j.width, n, j.inv, j.z.c, j.g,
-                                    Bb::from_canon(11), scale.c, partial, ys, ctx->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(ys_out, ys, (uint64_t)j.width * 16, hipMemcpyDeviceToHost, ctx->stream);
+        if (j.field == FIELD_GL) {
+            // (z^n - g^n) / (n g^n)
+            const uint64_t gn = glh_pow(j.gg, n);
+            const uint64_t scale = Gl::mul(Gl::sub(glh_pow(j.zg, n), gn), glh_pow(Gl::mul(n % Gl::P, gn), Gl::P - 2));
+            e = launch_deep_gl_open_values(ctx->tw[FIELD_GL], lde->ptr, j.height, j.width, n, j.inv, j.gg, scale,
+                                           partial, ys, ctx->stream);
+        } else {
+            Ef4H zn = ef4h_pow(j.z, n);
+            const uint32_t gn = bbh_pow(j.g, n);
+            zn.c[0] = Bb::sub(zn.c[0], gn);
+            const uint32_t sc = bbh_pow(Bb::mul(Bb::from_canon((uint32_t)(n % Bb::P)), gn), Bb::P - 2);
+            Ef4H scale{{Bb::mul(zn.c[0], sc), Bb::mul(zn.c[1], sc), Bb::mul(zn.c[2], sc), Bb::mul(zn.c[3], sc)}};
+            e = launch_deep_open_values(ctx->tw[FIELD_BB], lde->ptr, j.height, j.width, n, j.inv, j.z.c, j.g,
+                                        Bb::from_canon(11), scale.c, partial, ys, ctx->stream);
+        }
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(ys_out, ys, (uint64_t)j.width * j.es(), hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     }
     if (partial) dev_free(ctx, partial);
@@ -1175,90 +1214,111 @@ int deep_values(zkg_ctx *ctx, const zkg_buf *lde, uint32_t log_blowup, DeepJob &
     return ZKG_OK;
 }
 
-int deep_reduce(zkg_ctx *ctx, const zkg_buf *lde, const uint32_t *alpha, uint64_t alpha_offset, const uint32_t *ys,
+int deep_reduce(zkg_ctx *ctx, const zkg_buf *lde, const void *alpha, uint64_t alpha_offset, const void *ys,
                 zkg_buf *ro, DeepJob &j)
 {
-    for (int i = 0; i < 4; i++)
-        if (alpha[i] >= Bb::P) return set_err(ZKG_EINVAL, "alpha is not canonical");
-    for (uint64_t i = 0; i < (uint64_t)j.width * 4; i++)
-        if (ys[i] >= Bb::P) return set_err(ZKG_EINVAL, "opened values are not canonical");
-    if (j.height * 16 > ro->bytes) return set_err(ZKG_EINVAL, "ro buffer too small");
-    const Ef4H a = ef4h_mont(alpha);
-    // c_k = alpha^off alpha^k (canonical, as 16-bit halves in doubles) and
-    // ry = sum_k c_k y_k; Montgomery x canonical products come out canonical.
-    Ef4H p = ef4h_pow(a, alpha_offset), ry{{0, 0, 0, 0}};
-    std::vector<double> coef((size_t)j.width * 8);
-    for (uint32_t k = 0; k < j.width; k++) {
-        const Ef4H ck = ef4h_mul(p, Ef4H{{1, 0, 0, 0}});  // Montgomery -> canonical
-        for (int c = 0; c < 4; c++) {
-            coef[8 * k + c] = (double)(ck.c[c] & 0xFFFFu);
-            coef[8 * k + 4 + c] = (double)(ck.c[c] >> 16);
-        }
-        Ef4H t = ef4h_mul(p, Ef4H{{ys[4 * k], ys[4 * k + 1], ys[4 * k + 2], ys[4 * k + 3]}});
-        for (int c = 0; c < 4; c++) ry.c[c] = Bb::add(ry.c[c], t.c[c]);
-        p = ef4h_mul(p, a);
-    }
+    if (j.height * j.es() > ro->bytes) return set_err(ZKG_EINVAL, "ro buffer too small");
     void *d_coef = nullptr;
-    int rc = dev_alloc(ctx, coef.size() * sizeof(double), &d_coef);
-    if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(d_coef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice,
-                                  ctx->stream);
-    if (e == hipSuccess)
-        e = launch_deep_reduce(lde->ptr, j.height, j.width, j.inv, d_coef, ry.c, Bb::from_canon(11), ro->ptr,
-                               ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // coef is host memory
+    hipError_t e = hipSuccess;
+    if (j.field == FIELD_GL) {
+        const uint64_t a = *(const uint64_t *)alpha, *y = (const uint64_t *)ys;
+        if (a >= Gl::P) return set_err(ZKG_EINVAL, "alpha is not canonical");
+        for (uint32_t k = 0; k < j.width; k++)
+            if (y[k] >= Gl::P) return set_err(ZKG_EINVAL, "opened values are not canonical");
+        // c_k = alpha^off alpha^k, ry = sum_k c_k y_k
+        std::vector<uint64_t> coef(j.width);
+        uint64_t p = glh_pow(a, alpha_offset), ry = 0;
+        for (uint32_t k = 0; k < j.width; k++) {
+            coef[k] = p;
+            ry = Gl::add(ry, Gl::mul(p, y[k]));
+            p = Gl::mul(p, a);
+        }
+        int rc = dev_alloc(ctx, coef.size() * 8, &d_coef);
+        if (rc) return rc;
+        e = hipMemcpyAsync(d_coef, coef.data(), coef.size() * 8, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = launch_deep_gl_reduce(lde->ptr, j.height, j.width, j.inv, d_coef, ry, ro->ptr, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // coef is host memory
+    } else {
+        const uint32_t *aw = (const uint32_t *)alpha, *yw = (const uint32_t *)ys;
+        for (int i = 0; i < 4; i++)
+            if (aw[i] >= Bb::P) return set_err(ZKG_EINVAL, "alpha is not canonical");
+        for (uint64_t i = 0; i < (uint64_t)j.width * 4; i++)
+            if (yw[i] >= Bb::P) return set_err(ZKG_EINVAL, "opened values are not canonical");
+        const Ef4H a = ef4h_mont(aw);
+        // c_k = alpha^off alpha^k (canonical, as 16-bit halves in doubles) and
+        // ry = sum_k c_k y_k; Montgomery x canonical products come out canonical.
+        Ef4H p = ef4h_pow(a, alpha_offset), ry{{0, 0, 0, 0}};
+        std::vector<double> coef((size_t)j.width * 8);
+        for (uint32_t k = 0; k < j.width; k++) {
+            const Ef4H ck = ef4h_mul(p, Ef4H{{1, 0, 0, 0}});  // Montgomery -> canonical
+            for (int c = 0; c < 4; c++) {
+                coef[8 * k + c] = (double)(ck.c[c] & 0xFFFFu);
+                coef[8 * k + 4 + c] = (double)(ck.c[c] >> 16);
+            }
+            Ef4H t = ef4h_mul(p, Ef4H{{yw[4 * k], yw[4 * k + 1], yw[4 * k + 2], yw[4 * k + 3]}});
+            for (int c = 0; c < 4; c++) ry.c[c] = Bb::add(ry.c[c], t.c[c]);
+            p = ef4h_mul(p, a);
+        }
+        int rc = dev_alloc(ctx, coef.size() * sizeof(double), &d_coef);
+        if (rc) return rc;
+        e = hipMemcpyAsync(d_coef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess)
+            e = launch_deep_reduce(lde->ptr, j.height, j.width, j.inv, d_coef, ry.c, Bb::from_canon(11), ro->ptr,
+                                   ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // coef is host memory
+    }
     dev_free(ctx, d_coef);
     if (e != hipSuccess) return hip_err(e, "deep reduce");
     return ZKG_OK;
 }
 
-}  // namespace
-
-extern "C" int zkg_deep_open_values(zkg_ctx *ctx, const zkg_buf *lde, uint64_t height, uint32_t width,
-                                    uint32_t log_blowup, uint64_t shift, const uint32_t *z, uint32_t *ys_out)
+void deep_release(zkg_ctx *ctx, DeepJob &j)
 {
-    DeepJob j;
-    if (!ys_out) return set_err(ZKG_EINVAL, "null argument");
-    int rc = deep_check(ctx, lde, height, width, shift, z, j);
-    if (!rc) rc = deep_inv(ctx, j);
-    if (!rc) rc = deep_values(ctx, lde, log_blowup, j, ys_out);
     if (j.inv) {
         (void)hipStreamSynchronize(ctx->stream);
         dev_free(ctx, j.inv);
+        j.inv = nullptr;
     }
+}
+
+}  // namespace
+
+extern "C" int zkg_deep_open_values(zkg_ctx *ctx, int field, const zkg_buf *lde, uint64_t height, uint32_t width,
+                                    uint32_t log_blowup, uint64_t shift, const void *z, void *ys_out)
+{
+    DeepJob j;
+    if (!ys_out) return set_err(ZKG_EINVAL, "null argument");
+    int rc = deep_check(ctx, field, lde, height, width, shift, z, j);
+    if (!rc) rc = deep_inv(ctx, j);
+    if (!rc) rc = deep_values(ctx, lde, log_blowup, j, ys_out);
+    if (ctx) deep_release(ctx, j);
     return rc;
 }
 
-extern "C" int zkg_deep_reduce(zkg_ctx *ctx, const zkg_buf *lde, uint64_t height, uint32_t width, uint64_t shift,
-                               const uint32_t *z, const uint32_t *alpha, uint64_t alpha_offset, const uint32_t *ys,
+extern "C" int zkg_deep_reduce(zkg_ctx *ctx, int field, const zkg_buf *lde, uint64_t height, uint32_t width,
+                               uint64_t shift, const void *z, const void *alpha, uint64_t alpha_offset, const void *ys,
                                zkg_buf *ro)
 {
     DeepJob j;
     if (!alpha || !ys || !ro) return set_err(ZKG_EINVAL, "null argument");
-    int rc = deep_check(ctx, lde, height, width, shift, z, j);
+    int rc = deep_check(ctx, field, lde, height, width, shift, z, j);
     if (!rc) rc = deep_inv(ctx, j);
     if (!rc) rc = deep_reduce(ctx, lde, alpha, alpha_offset, ys, ro, j);
-    if (j.inv) {
-        (void)hipStreamSynchronize(ctx->stream);
-        dev_free(ctx, j.inv);
-    }
+    if (ctx) deep_release(ctx, j);
     return rc;
 }
 
-extern "C" int zkg_deep_open(zkg_ctx *ctx, const zkg_buf *lde, uint64_t height, uint32_t width, uint32_t log_blowup,
-                             uint64_t shift, const uint32_t *z, const uint32_t *alpha, uint64_t alpha_offset,
-                             uint32_t *ys_out, zkg_buf *ro)
+extern "C" int zkg_deep_open(zkg_ctx *ctx, int field, const zkg_buf *lde, uint64_t height, uint32_t width,
+                             uint32_t log_blowup, uint64_t shift, const void *z, const void *alpha,
+                             uint64_t alpha_offset, void *ys_out, zkg_buf *ro)
 {
     DeepJob j;
     if (!alpha || !ys_out || !ro) return set_err(ZKG_EINVAL, "null argument");
-    int rc = deep_check(ctx, lde, height, width, shift, z, j);
+    int rc = deep_check(ctx, field, lde, height, width, shift, z, j);
     if (!rc) rc = deep_inv(ctx, j);
     if (!rc) rc = deep_values(ctx, lde, log_blowup, j, ys_out);
     if (!rc) rc = deep_reduce(ctx, lde, alpha, alpha_offset, ys_out, ro, j);
-    if (j.inv) {
-        (void)hipStreamSynchronize(ctx->stream);
-        dev_free(ctx, j.inv);
-    }
+    if (ctx) deep_release(ctx, j);
     return rc;
 }
 

@@ -284,30 +284,38 @@ class HipMerkleTreeMmcs:
 
 
 class HipDeep:
-    """DEEP quotient of TwoAdicFriPcs::open over a device-resident BabyBear LDE
-    (bit-reversed rows on the coset shift * H); EF4 values as 4 canonical u32."""
+    """DEEP quotient of TwoAdicFriPcs::open over a device-resident LDE
+    (bit-reversed rows on the coset shift * H).  Challenge values: BabyBear ->
+    EF4, 4 canonical u32 (shape (..., 4)); Goldilocks -> the base field, one
+    canonical u64."""
 
-    def __init__(self, ctx, lde_buf, height, width, log_blowup, shift=None):
-        self.ctx, self.lde, self.height, self.width, self.log_blowup = ctx, lde_buf, height, width, log_blowup
-        self.shift = GENERATOR[BABYBEAR] if shift is None else shift
+    def __init__(self, ctx, field, lde_buf, height, width, log_blowup, shift=None):
+        self.ctx, self.field, self.lde = ctx, field, lde_buf
+        self.height, self.width, self.log_blowup = height, width, log_blowup
+        self.shift = GENERATOR[field] if shift is None else shift
+        self.ext = (4,) if field == BABYBEAR else ()
+
+    def _val(self, v):
+        return np.ascontiguousarray(np.asarray(v).reshape(self.ext) if self.ext else np.asarray([v]).reshape(1),
+                                    dtype=DTYPE[self.field])
 
     def open_values(self, z):
-        z = np.ascontiguousarray(z, dtype=np.uint32)
-        ys = np.empty((self.width, 4), dtype=np.uint32)
-        check(self.ctx.lib.zkg_deep_open_values(self.ctx.h, self.lde.h, self.height, self.width, self.log_blowup,
-                                                self.shift, _ptr(z), _ptr(ys)))
+        z = self._val(z)
+        ys = np.empty((self.width,) + self.ext, dtype=DTYPE[self.field])
+        check(self.ctx.lib.zkg_deep_open_values(self.ctx.h, self.field, self.lde.h, self.height, self.width,
+                                                self.log_blowup, self.shift, _ptr(z), _ptr(ys)))
         return ys
 
     def reduce(self, z, alpha, alpha_offset, ys, ro_buf):
-        z, alpha = np.ascontiguousarray(z, dtype=np.uint32), np.ascontiguousarray(alpha, dtype=np.uint32)
-        ys = np.ascontiguousarray(ys, dtype=np.uint32)
-        check(self.ctx.lib.zkg_deep_reduce(self.ctx.h, self.lde.h, self.height, self.width, self.shift, _ptr(z),
-                                           _ptr(alpha), alpha_offset, _ptr(ys), ro_buf.h))
+        z, alpha = self._val(z), self._val(alpha)
+        ys = np.ascontiguousarray(ys, dtype=DTYPE[self.field])
+        check(self.ctx.lib.zkg_deep_reduce(self.ctx.h, self.field, self.lde.h, self.height, self.width, self.shift,
+                                           _ptr(z), _ptr(alpha), alpha_offset, _ptr(ys), ro_buf.h))
 
     def open(self, z, alpha, alpha_offset, ro_buf):
-        z, alpha = np.ascontiguousarray(z, dtype=np.uint32), np.ascontiguousarray(alpha, dtype=np.uint32)
-        ys = np.empty((self.width, 4), dtype=np.uint32)
-        check(self.ctx.lib.zkg_deep_open(self.ctx.h, self.lde.h, self.height, self.width, self.log_blowup,
+        z, alpha = self._val(z), self._val(alpha)
+        ys = np.empty((self.width,) + self.ext, dtype=DTYPE[self.field])
+        check(self.ctx.lib.zkg_deep_open(self.ctx.h, self.field, self.lde.h, self.height, self.width, self.log_blowup,
                                          self.shift, _ptr(z), _ptr(alpha), alpha_offset, _ptr(ys), ro_buf.h))
         return ys
 
