@@ -30,9 +30,9 @@ EXTRA = {"zkg_probe.hip": ["-ffp-contract=off"], "zkg_hash.hip": ["-ffp-contract
          "zkg_deep.hip": ["-ffp-contract=off"]}
 
 
-def _compile(src):
-    obj = os.path.join(CSRC, "_obj", src.replace(".hip", ".o"))
-    cmd = [HIPCC, *FLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
+def _compile(src, obj_dir=None, defines=()):
+    obj = os.path.join(obj_dir or os.path.join(CSRC, "_obj"), src.replace(".hip", ".o"))
+    cmd = [HIPCC, *FLAGS, *EXTRA.get(src, []), *defines, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     msgs = [l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l]
     if r.returncode != 0:
@@ -62,6 +62,18 @@ def build(verbose=False):
     _link(objs[: len(SOURCES)], LIB)
     _link(objs[len(SOURCES):], PROBE_LIB)
     return LIB
+
+
+def build_variant(out_dir, defines):
+    """libzkg.so with extra -D definitions (tuning candidates) into out_dir."""
+    os.makedirs(out_dir, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+        results = list(ex.map(lambda src: _compile(src, out_dir, defines), SOURCES))
+    lib = os.path.join(out_dir, "libzkg.so")
+    _link([o for o, _ in results], lib)
+    for o, _ in results:
+        os.remove(o)
+    return lib
 
 
 if __name__ == "__main__":

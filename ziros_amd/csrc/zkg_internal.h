@@ -13,11 +13,22 @@ constexpr int LOG_TW_TOP = 24;    // four-step / coset tables cover orders <= 2^
 constexpr int TW_SPLIT = 12;      // omega^e = lo[e & 4095] * hi[e >> 12]
 
 // Radix split of a 2^L-point in-LDS transform: NR rounds of at most 2^rmax
-// points per lane (rmax 6 for 32-bit, 4 for 64-bit elements), larger first.
+// points per lane (ZKG_PLAN_RMAX32 / ZKG_PLAN_RMAX64), larger first.
+// Plan parameters chosen by the autotuner (ziros_amd/tune.py) when it has
+// written zkg_tuned.h; the defaults below otherwise.
+#if __has_include("zkg_tuned.h")
+#include "zkg_tuned.h"
+#endif
+#ifndef ZKG_PLAN_RMAX32
+#define ZKG_PLAN_RMAX32 6
+#endif
 #ifndef ZKG_PLAN_RMAX64
 #define ZKG_PLAN_RMAX64 4
 #endif
-__host__ __device__ constexpr int plan_rmax(int elem_bytes) { return elem_bytes == 4 ? 6 : ZKG_PLAN_RMAX64; }
+__host__ __device__ constexpr int plan_rmax(int elem_bytes)
+{
+    return elem_bytes == 4 ? ZKG_PLAN_RMAX32 : ZKG_PLAN_RMAX64;
+}
 __host__ __device__ constexpr int plan_nr(int rmax, int L)
 {
     return ((L + rmax - 1) / rmax < 2 && L >= 4) ? 2 : ((L + rmax - 1) / rmax < 1 ? 1 : (L + rmax - 1) / rmax);
