@@ -481,13 +481,15 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_final(const typename F::T *
     __syncthreads();
     tile_dft<F, L>(lds, tile_fwd, round_fwd);
     const uint64_t obase = base + ((uint64_t)brev(j2, g.logN2) << L);
-    // output row obase + r holds j1 = rev_L(r): read it from tile row pos_of(j1)
+    // tile row pos holds j1 = index_of(pos), which goes to output row
+    // obase + rev_L(j1).  Walking pos in order keeps a wave's LDS reads on
+    // distinct banks; the stores are one row segment per row either way.
     const int c = threadIdx.x & (P::C - 1);
     if (c0 + c >= g.width) return;
     T *d = out + obase * g.width + c0 + c;
-    TILE_STORE_LOOP(P, r)
+    TILE_STORE_LOOP(P, pos)
     {
-        d[(uint64_t)r * g.width] = lds[P::slot(P::pos_of(brev((uint32_t)r, L)), c)];
+        d[(uint64_t)brev(P::index_of(pos), L) * g.width] = lds[P::slot(pos, c)];
     }
 }
 
@@ -522,9 +524,9 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_small(const typename F::T *
         tile_dft<F, L, CREG ? SRC_REGS : SRC_COEF>(lds, tile_fwd, round_fwd, CREG ? creg : coef, wb);
         if (col_ok) {
             T *d = out + ((uint64_t)b << L) * g.width + c0 + c;
-            TILE_STORE_LOOP(P, r)
+            TILE_STORE_LOOP(P, pos)
             {
-                d[(uint64_t)r * g.width] = lds[P::slot(P::pos_of(brev((uint32_t)r, L)), c)];
+                d[(uint64_t)brev(P::index_of(pos), L) * g.width] = lds[P::slot(pos, c)];
             }
         }
         __syncthreads();
