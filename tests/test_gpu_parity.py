@@ -338,6 +338,66 @@ def test_serve_results_identical_in_mix(ctx):
         srv.stop()
 
 
+def test_commit_batch_trees_open(ctx):
+    """Trees handed out by a (replayed) batch graph are device copies: their
+    openings verify against the oracle's paths."""
+    specs = [(BB, 16, 10, 8, 1), (GL, 16, 9, 5, 1)]
+    jobs, keep, expect = _jobs(ctx, specs)
+    for rep in range(2):
+        trees = Z.commit_batch(ctx, jobs)
+    for k, ((field, p2w, logn, width, lb), t) in enumerate(zip(specs, trees)):
+        n = 1 << logn
+        tr = rnd(field, n * width, 100 + k).reshape(n, width)
+        lde = O.coset_lde_batch(field, tr, lb, Z.GENERATOR[field])
+        exp = O.MerkleTree(O.Poseidon2(field, p2w), [lde])
+        idx = [0, 5, (n << lb) - 1]
+        rows, paths = t.open_batch(idx, width)
+        for q, i in enumerate(idx):
+            assert (rows[q].astype(np.uint64) == lde[i]).all()
+            assert (paths[q].astype(np.uint64) == exp.path(i).reshape(paths[q].shape)).all()
+
+
+def test_serve_under_a_tight_hbm_budget():
+    """Admission against the context's HBM budget: small requests complete and
+    are bit-exact; a request whose tree and scratch can never fit fails with an
+    error instead of waiting forever."""
+    import time
+
+    c2 = Z.Context(0, hbm_budget=96 << 20)
+    try:
+        c2.load_constants(BB, 16, 42)
+        small = [(BB, 16, 12, 8, 1)] * 4
+        jobs, keep, expect = _jobs(c2, small)
+        n = 1 << 18
+        big_tr = c2.upload(rnd(BB, n * 16, 7).reshape(n, 16))
+        big_lde = c2.alloc((n << 1) * 16 * 4)
+        big = Z.make_job(BB, big_tr, n, 16, 1, big_lde, 16)
+        srv = Z.Server(c2, slots=2, max_batch=2)
+        try:
+            tickets = [srv.submit(j) for j in jobs] + [srv.submit(big)]
+            done, failed = {}, []
+            t0 = time.time()
+            while len(done) + len(failed) < len(tickets) and time.time() - t0 < 60:
+                for i, t in enumerate(tickets):
+                    if i in done or i in failed:
+                        continue
+                    try:
+                        r = srv.poll(t)
+                    except Z.ZkgError:
+                        failed.append(i)
+                        continue
+                    if r is not None:
+                        done[i] = r
+                time.sleep(0.001)
+            assert failed == [len(tickets) - 1] and sorted(done) == list(range(len(jobs)))
+            for i, (h, _) in done.items():
+                assert (Z.MerkleTree(c2, h, BB).root().astype(np.uint64) == expect[i]).all()
+        finally:
+            srv.stop()
+    finally:
+        c2.close()
+
+
 def test_errors_are_reported(ctx):
     with pytest.raises(Z.ZkgError):
         Z.HipDft(ctx, BB).coset_lde_batch(np.zeros((3, 2), dtype=np.uint32), 1)  # not a power of two
