@@ -182,6 +182,42 @@ int zkg_deep_reduce(zkg_ctx *ctx, int field, const zkg_buf *lde, uint64_t height
 int zkg_deep_open(zkg_ctx *ctx, int field, const zkg_buf *lde, uint64_t height, uint32_t width, uint32_t log_blowup,
                   uint64_t shift, const void *z, const void *alpha, uint64_t alpha_offset, void *ys_out, zkg_buf *ro);
 
+/* ---- constraint evaluation ---------------------------------------------------
+ * Replaces ConstraintEvalAccelerator::eval_trace_goldilocks (zkf-core/src/
+ * acceleration.rs:219-242; Metal body zkf-metal/src/constraint_eval.rs:115-190
+ * and shaders/constraint_eval.metal:1-74).  The bytecode is the reference's:
+ * word = op << 24 | operand (24 bits), op 0 CONST (push constants[operand]),
+ * 1 LOAD (push trace[row][operand]), 2 ADD, 3 SUB, 4 MUL (pop b, pop a, push
+ * a op b), 5 DUP, 6 EMIT (pop into out[row][operand]); other opcodes do
+ * nothing.  Every row of the row-major Goldilocks trace (n_rows x width,
+ * canonical) runs the whole program; out is n_rows x n_constraints and
+ * constraints the program never emits read zero.  Unlike the reference,
+ * programs are checked before they run: a stack underflow, a stack deeper
+ * than the reference's 32 entries, or a column, constant or constraint index
+ * out of range is ZKG_EINVAL (the reference would read or write out of
+ * bounds), and so is a non-canonical constant. */
+typedef struct zkg_constraint_program zkg_constraint_program;
+/* Check and translate a program once, upload it to the context's device. */
+int zkg_constraint_program_create(zkg_ctx *ctx, uint32_t width, const uint32_t *bytecode, uint32_t n_instructions,
+                                  const uint64_t *constants, uint32_t n_constants, uint32_t n_constraints,
+                                  zkg_constraint_program **out);
+/* Evaluate over n_rows >= 1 rows of a device trace into a device output;
+ * stream-ordered on the context's stream (zkg_buf_download waits for it). */
+int zkg_constraint_program_eval(zkg_constraint_program *prog, const zkg_buf *trace, uint64_t n_rows, zkg_buf *out);
+/* info: device words, panel slots, panel stages per wave, LDS stack entries,
+ * LDS bytes per wave, columns per panel. */
+int zkg_constraint_program_info(const zkg_constraint_program *prog, uint32_t info[6]);
+void zkg_constraint_program_free(zkg_constraint_program *prog);
+/* One-shot form of the three calls above; returns when out is written. */
+int zkg_constraint_eval_gl(zkg_ctx *ctx, const zkg_buf *trace, uint64_t n_rows, uint32_t width,
+                           const uint32_t *bytecode, uint32_t n_instructions, const uint64_t *constants,
+                           uint32_t n_constants, uint32_t n_constraints, zkg_buf *out);
+/* The translation alone, on the host with no device: writes up to cap device
+ * words and the info array of zkg_constraint_program_info (info[0] = words
+ * needed; ZKG_EINVAL if cap is smaller). */
+int zkg_constraint_translate(uint32_t width, const uint32_t *bytecode, uint32_t n_instructions, uint32_t n_constants,
+                             uint32_t n_constraints, uint32_t *words, uint32_t cap, uint32_t info[6]);
+
 /* Fused commit of one trace: coset LDE at log_blowup with the field's
  * generator shift (p3 Val::GENERATOR: 7 or 31), row hash, tree.  lde_out
  * (height << log_blowup rows) receives the LDE and backs the tree's openings. */

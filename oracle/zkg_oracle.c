@@ -985,6 +985,56 @@ void or_deep_gl_reduce(const uint64_t *lde, uint64_t height, uint32_t width, uin
     free(x);
 }
 
+/* ======================================================================
+ * Constraint evaluation: the stack machine of
+ * zkf-metal/src/shaders/constraint_eval.metal:1-74 (opcodes :7-13, one row
+ * per evaluation, 32-entry stack :29-30), Goldilocks arithmetic.
+ * ====================================================================== */
+static int constraint_check(const uint32_t *bc, uint32_t n_instr, uint32_t width, uint32_t n_constants,
+                            uint32_t n_constraints)
+{
+    int sp = 0;
+    for (uint32_t pc = 0; pc < n_instr; pc++) {
+        uint32_t op = bc[pc] >> 24, arg = bc[pc] & 0xFFFFFFu;
+        switch (op) {
+        case 0: if (arg >= n_constants || sp >= 32) return -1; sp++; break;
+        case 1: if (arg >= width || sp >= 32) return -1; sp++; break;
+        case 2: case 3: case 4: if (sp < 2) return -1; sp--; break;
+        case 5: if (sp < 1 || sp >= 32) return -1; sp++; break;
+        case 6: if (sp < 1 || arg >= n_constraints) return -1; sp--; break;
+        default: break;
+        }
+    }
+    return 0;
+}
+
+int or_constraint_eval_gl(const uint64_t *trace, uint64_t n_rows, uint32_t width, const uint32_t *bytecode,
+                          uint32_t n_instr, const uint64_t *constants, uint32_t n_constants,
+                          uint32_t n_constraints, uint64_t *out)
+{
+    if (constraint_check(bytecode, n_instr, width, n_constants, n_constraints)) return -1;
+    memset(out, 0, n_rows * n_constraints * sizeof(uint64_t));
+#pragma omp parallel for schedule(static)
+    for (uint64_t row = 0; row < n_rows; row++) {
+        uint64_t st[32], a, b;
+        int sp = 0;
+        for (uint32_t pc = 0; pc < n_instr; pc++) {
+            uint32_t op = bytecode[pc] >> 24, arg = bytecode[pc] & 0xFFFFFFu;
+            switch (op) {
+            case 0: st[sp++] = constants[arg]; break;
+            case 1: st[sp++] = trace[row * width + arg]; break;
+            case 2: b = st[--sp]; a = st[--sp]; st[sp++] = or_gl_add(a, b); break;
+            case 3: b = st[--sp]; a = st[--sp]; st[sp++] = or_gl_sub(a, b); break;
+            case 4: b = st[--sp]; a = st[--sp]; st[sp++] = or_gl_mul(a, b); break;
+            case 5: st[sp] = st[sp - 1]; sp++; break;
+            case 6: out[row * n_constraints + arg] = st[--sp]; break;
+            default: break;
+            }
+        }
+    }
+    return 0;
+}
+
 void or_set_threads(int n)
 {
 #ifdef _OPENMP

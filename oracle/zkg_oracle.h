@@ -160,6 +160,20 @@ void or_deep_gl_open_values(const uint64_t *lde, uint64_t height, uint32_t width
 void or_deep_gl_reduce(const uint64_t *lde, uint64_t height, uint32_t width, uint64_t shift, uint64_t z,
                        uint64_t alpha, uint64_t alpha_off, const uint64_t *ys, uint64_t *ro);
 
+/* ---- Constraint evaluation (ConstraintEvalAccelerator::eval_trace_goldilocks,
+ * zkf-core/src/acceleration.rs:219-242; interpreter semantics of
+ * zkf-metal/src/shaders/constraint_eval.metal:1-74) ----
+ * Each instruction word is opcode << 24 | operand (24 bits).  Opcodes:
+ * 0 CONST (push constants[operand]), 1 LOAD (push trace[row][operand]),
+ * 2 ADD, 3 SUB, 4 MUL (pop b, pop a, push a op b), 5 DUP, 6 EMIT (pop into
+ * out[row][operand]); any other opcode does nothing.  Every row runs the
+ * whole program; out (n_rows x n_constraints) is zeroed first.  Returns -1 if
+ * the program underflows or overflows the 32-entry stack or indexes out of
+ * range (the reference would read or write out of bounds). */
+int or_constraint_eval_gl(const uint64_t *trace, uint64_t n_rows, uint32_t width, const uint32_t *bytecode,
+                          uint32_t n_instr, const uint64_t *constants, uint32_t n_constants,
+                          uint32_t n_constraints, uint64_t *out);
+
 /* ---- threading ---- */
 void or_set_threads(int n);
 int or_get_threads(void);

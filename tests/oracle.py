@@ -102,6 +102,8 @@ def lib():
         L.or_deep_reduce.argtypes = [c_vp, c_u64, c_u32, c_u64, c_vp, c_vp, c_u64, c_vp, c_vp]
         L.or_deep_gl_open_values.argtypes = [c_vp, c_u64, c_u32, c_u32, c_u64, c_u64, c_vp]
         L.or_deep_gl_reduce.argtypes = [c_vp, c_u64, c_u32, c_u64, c_u64, c_u64, c_u64, c_vp, c_vp]
+        L.or_constraint_eval_gl.restype = c_int
+        L.or_constraint_eval_gl.argtypes = [c_vp, c_u64, c_u32, c_vp, c_u32, c_vp, c_u32, c_u32, c_vp]
         L.or_set_threads.argtypes = [c_int]
         L.or_get_threads.restype = c_int
         _lib = L
@@ -189,6 +191,26 @@ def deep_gl_reduce(lde, shift, z, alpha, alpha_off, ys, ro=None):
     out = np.zeros(h, dtype=np.uint64) if ro is None else u64(ro).copy()
     lib().or_deep_gl_reduce(_p(m), h, w, shift, z, alpha, alpha_off, _p(u64(ys)), _p(out))
     return out
+
+
+# ---------------------------------------------------------------- constraint evaluation
+OP_CONST, OP_LOAD, OP_ADD, OP_SUB, OP_MUL, OP_DUP, OP_EMIT = range(7)
+
+
+def instr(op, operand=0):
+    return (op << 24) | (operand & 0xFFFFFF)
+
+
+def constraint_eval_gl(trace, bytecode, constants, n_constraints):
+    """Goldilocks stack-machine evaluation of every row; None if the program is invalid."""
+    t = u64(trace)
+    rows, width = t.shape
+    bc = np.ascontiguousarray(bytecode, dtype=np.uint32)
+    cs = u64(constants) if len(constants) else np.zeros(1, dtype=np.uint64)
+    out = np.zeros((rows, max(n_constraints, 1)), dtype=np.uint64)
+    rc = lib().or_constraint_eval_gl(_p(t), rows, width, _p(bc), bc.size, _p(cs), len(constants), n_constraints,
+                                     _p(out))
+    return None if rc else out[:, :n_constraints]
 
 
 # ---------------------------------------------------------------- generator

@@ -1,0 +1,173 @@
+"""GPU parity of constraint evaluation (HIP interpreter vs the oracle).
+
+Semantics: ConstraintEvalAccelerator::eval_trace_goldilocks
+(zkf-core/src/acceleration.rs:219-242) running the bytecode of
+zkf-metal/src/shaders/constraint_eval.metal:1-74.  Bit-exact against the
+oracle on every row.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from test_oracle_constraint import random_program
+
+import ziros_amd as Z
+from ziros_amd._lib import check
+
+pytestmark = pytest.mark.gpu
+P = O.GL_P
+I = O.instr
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Z.Context(0)
+    yield c
+    c.close()
+
+
+def _trace(rng, rows, width):
+    t = (rng.integers(0, 2**63, (rows, width), dtype=np.uint64) % np.uint64(P)).astype(np.uint64)
+    t[0, :] = P - 1
+    return t
+
+
+def _run(ctx, trace, bc, consts, n_c):
+    rows, w = trace.shape
+    prog = Z.ConstraintProgram(ctx, w, bc, consts, n_c)
+    tb = ctx.upload(trace)
+    ob = ctx.alloc(rows * n_c * 8)
+    ob.upload(np.full(rows * n_c, 0xDEADBEEF, dtype=np.uint64))  # every output must be written
+    prog.eval(tb, rows, ob)
+    got = ob.download(np.uint64, rows * n_c).reshape(rows, n_c)
+    info = prog.info()
+    prog.free(); tb.free(); ob.free()
+    return got, info
+
+
+def air_program(width, n_c, rng):
+    """Degree-3 constraints of a round-function AIR: for constraint k,
+    x^3 + a_k y + b_k - u v over four columns."""
+    cc = Z.ConstraintCompiler()
+    for k in range(n_c):
+        x, y, u, v = (2 * k) % width, (2 * k + 1) % width, (2 * k + 2) % width, (2 * k + 3) % width
+        cc.load_column(x); cc.dup(); cc.dup(); cc.mul(); cc.mul()
+        cc.push_const(int(rng.integers(0, 2**63, dtype=np.uint64)) % P); cc.load_column(y); cc.mul(); cc.add()
+        cc.push_const(int(rng.integers(0, 2**63, dtype=np.uint64)) % P); cc.add()
+        cc.load_column(u); cc.load_column(v); cc.mul(); cc.sub()
+        cc.emit(k)
+    return cc.finish()
+
+
+def test_reference_add_mul_vector(ctx):
+    """zkf-metal/src/constraint_eval.rs:224-269 through the accelerator mirror."""
+    n, w = 2048, 2
+    trace = np.array([(i * 7 + 1) % P for i in range(n * w)], dtype=np.uint64)
+    cc = Z.ConstraintCompiler()
+    cc.load_column(0); cc.load_column(1); cc.add(); cc.emit(0)
+    cc.load_column(0); cc.load_column(1); cc.mul(); cc.emit(1)
+    bc, consts, n_c = cc.finish()
+    ev = Z.HipConstraintEval(ctx)
+    got = ev.eval_trace_goldilocks(trace, w, bc, consts, n_c)
+    assert got.size == n * n_c
+    t = trace.reshape(n, w)
+    for r in range(n):
+        a, b = int(t[r, 0]), int(t[r, 1])
+        assert int(got[2 * r]) == (a + b) % P and int(got[2 * r + 1]) == a * b % P
+    assert (got.reshape(n, n_c) == O.constraint_eval_gl(t, bc, consts, n_c)).all()
+
+
+def test_accelerator_refuses_what_the_reference_refuses(ctx):
+    ev = Z.HipConstraintEval(ctx)
+    ok = [I(O.OP_LOAD, 0), I(O.OP_EMIT, 0)]
+    t = np.arange(2048 * 2, dtype=np.uint64)
+    assert ev.min_rows() == 1024
+    for args in [(t[:1023 * 2], 2, ok, [], 1), (t, 0, ok, [], 1), (t[:5], 2, ok, [], 1), (t, 2, [], [], 1),
+                 (t, 2, ok, [], 0), (t[:0], 2, ok, [], 1)]:
+        with pytest.raises(Z.ZkgError):
+            ev.eval_trace_goldilocks(*args)
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_random_programs(ctx, seed):
+    rng = np.random.default_rng(500 + seed)
+    w = int(rng.choice([1, 2, 15, 16, 33, 64, 100]))
+    rows = int(rng.choice([1, 63, 64, 65, 255, 256, 257, 3000]))
+    n_const, n_c = int(rng.integers(1, 8)), int(rng.integers(1, 9))
+    trace = _trace(rng, rows, w)
+    consts = [int(v) % P for v in rng.integers(0, 2**63, n_const, dtype=np.uint64)]
+    consts[0] = P - 1
+    bc = random_program(rng, w, n_const, n_c, int(rng.integers(3, 300)))
+    if seed % 2:
+        bc = [x for i in bc for x in ((i,) if rng.random() < 0.8 else (i, (int(rng.integers(7, 256)) << 24) | 9))]
+    got, _ = _run(ctx, trace, bc, consts, n_c)
+    assert (got == O.constraint_eval_gl(trace, bc, consts, n_c)).all()
+
+
+def test_air_program_large_trace(ctx):
+    """2^18 rows x 64 columns, 32 degree-3 constraints: every row vs the oracle."""
+    rng = np.random.default_rng(77)
+    rows, w, n_c = 1 << 18, 64, 32
+    trace = _trace(rng, rows, w)
+    bc, consts, n_c = air_program(w, n_c, rng)
+    got, info = _run(ctx, trace, bc, consts, n_c)
+    assert info["slots"] == 1 and info["stages"] == 5  # the last constraint wraps to panel 0
+    assert (got == O.constraint_eval_gl(trace, bc, consts, n_c)).all()
+
+
+def test_deep_stack_many_slots_large_lds(ctx):
+    """Depth-32 stack plus interleaved panels: more than 64 KiB of LDS per block."""
+    rng = np.random.default_rng(3)
+    w, rows = 64, 1000
+    trace = _trace(rng, rows, w)
+    bc = []
+    for c in range(8):
+        bc += [I(O.OP_LOAD, c), I(O.OP_LOAD, 16 + c), I(O.OP_LOAD, 32 + c), I(O.OP_LOAD, 48 + c)]
+        bc += [I(O.OP_ADD), I(O.OP_MUL), I(O.OP_SUB), I(O.OP_EMIT, c)]
+    bc += [I(O.OP_LOAD, (5 * i) % w) for i in range(32)] + [I(O.OP_MUL), I(O.OP_SUB)] * 15 + [I(O.OP_ADD)]
+    bc += [I(O.OP_EMIT, 8)]
+    got, info = _run(ctx, trace, bc, [], 10)
+    assert info["stack_slots"] == 31 and info["slots"] >= 2
+    assert info["wave_bytes"] * 4 > 64 * 1024
+    assert (got == O.constraint_eval_gl(trace, bc, [], 10)).all()
+    assert (got[:, 9] == 0).all()
+
+
+def test_program_reuse_and_one_shot_entry(ctx):
+    rng = np.random.default_rng(21)
+    w, n_c = 20, 6
+    bc, consts, n_c = air_program(w, n_c, rng)
+    prog = Z.ConstraintProgram(ctx, w, bc, consts, n_c)
+    for rows in (5000, 333):
+        trace = _trace(rng, rows, w)
+        tb, ob = ctx.upload(trace), ctx.alloc(rows * n_c * 8)
+        prog.eval(tb, rows, ob)
+        got = ob.download(np.uint64, rows * n_c).reshape(rows, n_c)
+        assert (got == O.constraint_eval_gl(trace, bc, consts, n_c)).all()
+        tb2, ob2 = ctx.upload(trace), ctx.alloc(rows * n_c * 8)
+        bca = np.asarray(bc, dtype=np.uint32)
+        csa = np.asarray(consts, dtype=np.uint64)
+        check(ctx.lib.zkg_constraint_eval_gl(ctx.h, tb2.h, rows, w, bca.ctypes.data, bca.size, csa.ctypes.data,
+                                               csa.size, n_c, ob2.h))
+        assert (ob2.download(np.uint64, rows * n_c).reshape(rows, n_c) == got).all()
+        for b in (tb, ob, tb2, ob2):
+            b.free()
+    prog.free()
+
+
+def test_eval_checks_buffers(ctx):
+    bc = [I(O.OP_LOAD, 3), I(O.OP_EMIT, 1)]
+    prog = Z.ConstraintProgram(ctx, 4, bc, [], 2)
+    tb, ob = ctx.alloc(100 * 4 * 8), ctx.alloc(100 * 2 * 8)
+    with pytest.raises(Z.ZkgError):
+        prog.eval(tb, 101, ob)      # trace too small
+    small = ctx.alloc(100 * 2 * 8 - 8)
+    with pytest.raises(Z.ZkgError):
+        prog.eval(tb, 100, small)   # output too small
+    with pytest.raises(Z.ZkgError):
+        prog.eval(tb, 0, ob)
+    with pytest.raises(Z.ZkgError):
+        Z.ConstraintProgram(ctx, 4, bc, [P], 2)  # non-canonical constant
+    for b in (tb, ob, small):
+        b.free()
+    prog.free()

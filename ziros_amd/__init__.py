@@ -1,5 +1,6 @@
 """ziros_amd -- MI355X-native STARK trace commitment (coset LDE, Poseidon2 row
-hashing, Merkle trees, FRI folding, the DEEP quotient of the PCS opening)
+hashing, Merkle trees, FRI folding, the DEEP quotient of the PCS opening,
+constraint evaluation)
 behind the libzkg C ABI (include/zkg.h).
 
 The compute path is hand-written HIP for gfx950 in ``csrc/``; this package
@@ -11,9 +12,12 @@ from .host import (  # noqa: F401
     BABYBEAR,
     GENERATOR,
     GOLDILOCKS,
+    ConstraintCompiler,
+    ConstraintProgram,
     Context,
     DeviceBuffer,
     FriCommitment,
+    HipConstraintEval,
     HipDeep,
     HipDft,
     HipFri,
@@ -24,4 +28,5 @@ from .host import (  # noqa: F401
     commit_batch,
     commit_trace,
     make_job,
+    translate_constraints,
 )

@@ -66,6 +66,12 @@ SIGNATURES = {
     "zkg_deep_open_values": (c_int, [c_vp, c_int, c_vp, c_u64, c_u32, c_u32, c_u64, c_vp, c_vp]),
     "zkg_deep_reduce": (c_int, [c_vp, c_int, c_vp, c_u64, c_u32, c_u64, c_vp, c_vp, c_u64, c_vp, c_vp]),
     "zkg_deep_open": (c_int, [c_vp, c_int, c_vp, c_u64, c_u32, c_u32, c_u64, c_vp, c_vp, c_u64, c_vp, c_vp]),
+    "zkg_constraint_program_create": (c_int, [c_vp, c_u32, c_vp, c_u32, c_vp, c_u32, c_u32, P(c_vp)]),
+    "zkg_constraint_program_eval": (c_int, [c_vp, c_vp, c_u64, c_vp]),
+    "zkg_constraint_program_info": (c_int, [c_vp, c_vp]),
+    "zkg_constraint_program_free": (None, [c_vp]),
+    "zkg_constraint_eval_gl": (c_int, [c_vp, c_vp, c_u64, c_u32, c_vp, c_u32, c_vp, c_u32, c_u32, c_vp]),
+    "zkg_constraint_translate": (c_int, [c_u32, c_vp, c_u32, c_u32, c_u32, c_vp, c_u32, c_vp]),
     "zkg_tree_free": (c_int, [c_vp]),
     "zkg_commit": (c_int, [c_vp, c_int, c_int, c_vp, c_u64, c_u32, c_u32, c_vp, P(c_vp)]),
     "zkg_commit_plain": (c_int, [c_vp, c_int, c_int, c_vp, c_u64, c_u32, c_u32, c_vp, P(c_vp)]),

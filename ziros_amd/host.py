@@ -11,6 +11,10 @@ Names follow the reference so that call sites read the same:
   (zkf-metal/src/merkle.rs:55-138).
 * ``HipFri`` -- ``FriAccelerator::fold_*`` (zkf-core/src/acceleration.rs:196-217)
   plus the FRI commit phase over the BabyBear degree-4 extension.
+* ``HipConstraintEval`` -- ``ConstraintEvalAccelerator::eval_trace_goldilocks``
+  (zkf-core/src/acceleration.rs:219-242) with the reference's
+  ``ConstraintCompiler`` bytecode builder; ``ConstraintProgram`` keeps a
+  translated program resident for device traces.
 
 Arrays are numpy (uint64 for Goldilocks, uint32 for BabyBear), row-major.
 Everything runs on the GPU through libzkg; errors raise ``ZkgError``.
@@ -318,6 +322,144 @@ class HipDeep:
         check(self.ctx.lib.zkg_deep_open(self.ctx.h, self.field, self.lde.h, self.height, self.width, self.log_blowup,
                                          self.shift, _ptr(z), _ptr(alpha), alpha_offset, _ptr(ys), ro_buf.h))
         return ys
+
+
+# ---------------------------------------------------------------- constraint evaluation
+OP_CONST, OP_LOAD, OP_ADD, OP_SUB, OP_MUL, OP_DUP, OP_EMIT = range(7)
+
+
+def encode_instruction(op, operand):
+    """Opcode in the top 8 bits, operand in the low 24 (zkf-metal/src/constraint_eval.rs:24-27)."""
+    return ((op & 0xFF) << 24) | (operand & 0x00FFFFFF)
+
+
+class ConstraintCompiler:
+    """Bytecode builder of zkf-metal/src/constraint_eval.rs:29-98: push_const,
+    load_column, add, sub, mul, dup, emit; finish() -> (bytecode, constants,
+    n_constraints)."""
+
+    def __init__(self):
+        self.bytecode, self.constants, self.n_constraints = [], [], 0
+
+    def push_const(self, value):
+        self.bytecode.append(encode_instruction(OP_CONST, len(self.constants)))
+        self.constants.append(value)
+
+    def load_column(self, col):
+        self.bytecode.append(encode_instruction(OP_LOAD, col))
+
+    def add(self):
+        self.bytecode.append(encode_instruction(OP_ADD, 0))
+
+    def sub(self):
+        self.bytecode.append(encode_instruction(OP_SUB, 0))
+
+    def mul(self):
+        self.bytecode.append(encode_instruction(OP_MUL, 0))
+
+    def dup(self):
+        self.bytecode.append(encode_instruction(OP_DUP, 0))
+
+    def emit(self, constraint_idx):
+        self.bytecode.append(encode_instruction(OP_EMIT, constraint_idx))
+        self.n_constraints = max(self.n_constraints, constraint_idx + 1)
+
+    def finish(self):
+        return self.bytecode, self.constants, self.n_constraints
+
+
+def translate_constraints(width, bytecode, n_constants, n_constraints):
+    """The host translation libzkg runs before evaluation (no device needed):
+    (device words, info dict)."""
+    lib = _lib.load()
+    bc = np.ascontiguousarray(bytecode, dtype=np.uint32)
+    info = np.zeros(6, dtype=np.uint32)
+    rc = lib.zkg_constraint_translate(width, _ptr(bc), bc.size, n_constants, n_constraints, None, 0, _ptr(info))
+    if rc != 0 and info[0] == 0:
+        check(rc)
+    words = np.zeros(int(info[0]), dtype=np.uint32)
+    check(lib.zkg_constraint_translate(width, _ptr(bc), bc.size, n_constants, n_constraints, _ptr(words),
+                                       words.size, _ptr(info)))
+    keys = ("words", "slots", "stages", "stack_slots", "wave_bytes", "panel")
+    return words, dict(zip(keys, (int(v) for v in info)))
+
+
+class ConstraintProgram:
+    """A checked, translated program resident on the device; evaluates any
+    number of device traces of its width."""
+
+    def __init__(self, ctx, width, bytecode, constants, n_constraints):
+        self.ctx, self.width, self.n_constraints = ctx, width, n_constraints
+        bc = np.ascontiguousarray(bytecode, dtype=np.uint32)
+        cs = np.ascontiguousarray(constants, dtype=np.uint64) if len(constants) else np.zeros(1, np.uint64)
+        h = ctypes.c_void_p()
+        check(ctx.lib.zkg_constraint_program_create(ctx.h, width, _ptr(bc), bc.size, _ptr(cs), len(constants),
+                                                    n_constraints, ctypes.byref(h)))
+        self.h = h
+
+    def info(self):
+        a = np.zeros(6, dtype=np.uint32)
+        check(self.ctx.lib.zkg_constraint_program_info(self.h, _ptr(a)))
+        return dict(zip(("words", "slots", "stages", "stack_slots", "wave_bytes", "panel"), (int(v) for v in a)))
+
+    def eval(self, trace_buf, n_rows, out_buf):
+        check(self.ctx.lib.zkg_constraint_program_eval(self.h, trace_buf.h, n_rows, out_buf.h))
+
+    def free(self):
+        if self.h:
+            self.ctx.lib.zkg_constraint_program_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+class HipConstraintEval:
+    """ConstraintEvalAccelerator counterpart (zkf-core/src/acceleration.rs:
+    219-242, MetalConstraintEvalAccelerator of zkf-metal/src/registry.rs:
+    213-237): host trace in, host constraint values out (n_rows x
+    n_constraints, flat like the reference's Vec<u64>).  Refuses the inputs
+    the reference's dispatch refuses (zkf-metal/src/constraint_eval.rs:
+    123-130): width 0, an empty or ragged trace, fewer than min_rows() rows,
+    empty bytecode, zero constraints."""
+
+    MIN_ROWS = 1024
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def name(self):
+        return "hip-gfx950-constraint-eval"
+
+    def is_available(self):
+        return True
+
+    def min_rows(self):
+        return self.MIN_ROWS
+
+    def eval_trace_goldilocks(self, trace, width, bytecode, constants, n_constraints):
+        t = np.ascontiguousarray(trace, dtype=np.uint64).reshape(-1)
+        if width == 0 or t.size == 0 or t.size % width:
+            raise ZkgError(-1, "trace of %d elements is not a non-empty multiple of width %d" % (t.size, width))
+        n_rows = t.size // width
+        if n_rows < self.MIN_ROWS or len(bytecode) == 0 or n_constraints == 0:
+            raise ZkgError(-1, "constraint eval dispatch refused (%d rows, %d instructions, %d constraints)"
+                           % (n_rows, len(bytecode), n_constraints))
+        tb = self.ctx.upload(t)
+        ob = self.ctx.alloc(n_rows * n_constraints * 8)
+        try:
+            bc = np.ascontiguousarray(bytecode, dtype=np.uint32)
+            cs = np.ascontiguousarray(constants, dtype=np.uint64) if len(constants) else np.zeros(1, np.uint64)
+            check(self.ctx.lib.zkg_constraint_eval_gl(self.ctx.h, tb.h, n_rows, width, _ptr(bc), bc.size, _ptr(cs),
+                                                      len(constants), n_constraints, ob.h))
+            return ob.download(np.uint64, n_rows * n_constraints)
+        finally:
+            tb.free()
+            ob.free()
 
 
 def commit_trace(ctx, field, trace_buf, height, width, log_blowup, p2_width=16, plain=False):
