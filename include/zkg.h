@@ -195,7 +195,8 @@ int zkg_deep_open(zkg_ctx *ctx, int field, const zkg_buf *lde, uint64_t height, 
  * programs are checked before they run: a stack underflow, a stack deeper
  * than the reference's 32 entries, or a column, constant or constraint index
  * out of range is ZKG_EINVAL (the reference would read or write out of
- * bounds), and so is a non-canonical constant. */
+ * bounds), and so is a non-canonical constant.  Constant and constraint
+ * indices are limited to 2^22, column indices to the reference's 2^24. */
 typedef struct zkg_constraint_program zkg_constraint_program;
 /* Check and translate a program once, upload it to the context's device. */
 int zkg_constraint_program_create(zkg_ctx *ctx, uint32_t width, const uint32_t *bytecode, uint32_t n_instructions,
@@ -205,8 +206,13 @@ int zkg_constraint_program_create(zkg_ctx *ctx, uint32_t width, const uint32_t *
  * stream-ordered on the context's stream (zkg_buf_download waits for it). */
 int zkg_constraint_program_eval(zkg_constraint_program *prog, const zkg_buf *trace, uint64_t n_rows, zkg_buf *out);
 /* info: device words, panel slots, panel stages per wave, LDS stack entries,
- * LDS bytes per wave, columns per panel. */
-int zkg_constraint_program_info(const zkg_constraint_program *prog, uint32_t info[6]);
+ * LDS bytes per wave, columns per panel, rows per lane, LDS bytes per panel
+ * column. */
+int zkg_constraint_program_info(const zkg_constraint_program *prog, uint32_t info[8]);
+/* Compile the program once into a straight-line kernel for gfx950 (hiprtc,
+ * about a second); later evals of this handle run it instead of the
+ * interpreter, with identical results.  ZKG_EDEVICE carries the compiler log. */
+int zkg_constraint_program_compile(zkg_constraint_program *prog);
 void zkg_constraint_program_free(zkg_constraint_program *prog);
 /* One-shot form of the three calls above; returns when out is written. */
 int zkg_constraint_eval_gl(zkg_ctx *ctx, const zkg_buf *trace, uint64_t n_rows, uint32_t width,
@@ -216,7 +222,17 @@ int zkg_constraint_eval_gl(zkg_ctx *ctx, const zkg_buf *trace, uint64_t n_rows, 
  * words and the info array of zkg_constraint_program_info (info[0] = words
  * needed; ZKG_EINVAL if cap is smaller). */
 int zkg_constraint_translate(uint32_t width, const uint32_t *bytecode, uint32_t n_instructions, uint32_t n_constants,
-                             uint32_t n_constraints, uint32_t *words, uint32_t cap, uint32_t info[6]);
+                             uint32_t n_constraints, uint32_t *words, uint32_t cap, uint32_t info[8]);
+/* The HIP source zkg_constraint_program_compile would compile (*size bytes
+ * including the terminator; up to cap written). */
+int zkg_constraint_jit_source(uint32_t width, const uint32_t *bytecode, uint32_t n_instructions,
+                              const uint64_t *constants, uint32_t n_constants, uint32_t n_constraints, char *src,
+                              uint64_t cap, uint64_t *size);
+/* Translate and compile on the host with no device (the compiler's log, if
+ * any, into log). */
+int zkg_constraint_compile_check(uint32_t width, const uint32_t *bytecode, uint32_t n_instructions,
+                                 const uint64_t *constants, uint32_t n_constants, uint32_t n_constraints, char *log,
+                                 uint32_t log_cap);
 
 /* Fused commit of one trace: coset LDE at log_blowup with the field's
  * generator shift (p3 Val::GENERATOR: 7 or 31), row hash, tree.  lde_out

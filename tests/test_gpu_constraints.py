@@ -88,8 +88,13 @@ def test_accelerator_refuses_what_the_reference_refuses(ctx):
             ev.eval_trace_goldilocks(*args)
 
 
+SHAPES = ["1,16", "2,4", "2,8", "2,16", "4,4", "4,8"]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("seed", range(10))
-def test_random_programs(ctx, seed):
+def test_random_programs(ctx, seed, shape, monkeypatch):
+    monkeypatch.setenv("ZKG_CE_SHAPE", shape)
     rng = np.random.default_rng(500 + seed)
     w = int(rng.choice([1, 2, 15, 16, 33, 64, 100]))
     rows = int(rng.choice([1, 63, 64, 65, 255, 256, 257, 3000]))
@@ -100,7 +105,8 @@ def test_random_programs(ctx, seed):
     bc = random_program(rng, w, n_const, n_c, int(rng.integers(3, 300)))
     if seed % 2:
         bc = [x for i in bc for x in ((i,) if rng.random() < 0.8 else (i, (int(rng.integers(7, 256)) << 24) | 9))]
-    got, _ = _run(ctx, trace, bc, consts, n_c)
+    got, info = _run(ctx, trace, bc, consts, n_c)
+    assert "%d,%d" % (info["rows_per_lane"], info["panel"]) == shape or info["rows_per_lane"] == 1
     assert (got == O.constraint_eval_gl(trace, bc, consts, n_c)).all()
 
 
@@ -111,12 +117,13 @@ def test_air_program_large_trace(ctx):
     trace = _trace(rng, rows, w)
     bc, consts, n_c = air_program(w, n_c, rng)
     got, info = _run(ctx, trace, bc, consts, n_c)
-    assert info["slots"] == 1 and info["stages"] == 5  # the last constraint wraps to panel 0
+    assert info["slots"] == 1 and info["stages"] == 64 // info["panel"] + 1  # the last constraint wraps to panel 0
     assert (got == O.constraint_eval_gl(trace, bc, consts, n_c)).all()
 
 
-def test_deep_stack_many_slots_large_lds(ctx):
+def test_deep_stack_many_slots_large_lds(ctx, monkeypatch):
     """Depth-32 stack plus interleaved panels: more than 64 KiB of LDS per block."""
+    monkeypatch.setenv("ZKG_CE_SHAPE", "1,16")
     rng = np.random.default_rng(3)
     w, rows = 64, 1000
     trace = _trace(rng, rows, w)
@@ -171,3 +178,63 @@ def test_eval_checks_buffers(ctx):
     for b in (tb, ob, small):
         b.free()
     prog.free()
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_output_panel_sessions(ctx, shape, monkeypatch):
+    """Emits that revisit an output panel, constraints never emitted, a panel
+    with no emits at all, and a constraint emitted twice (the last wins)."""
+    monkeypatch.setenv("ZKG_CE_SHAPE", shape)
+    rng = np.random.default_rng(31)
+    rows = 777
+    trace = _trace(rng, rows, 6)
+    order = [0, 1, 20, 2, 21, 0, 40, 3, 1]
+    bc = []
+    for k, j in enumerate(order):
+        bc += [I(O.OP_LOAD, k % 6), I(O.OP_CONST, 0), I(O.OP_MUL), I(O.OP_EMIT, j)]
+    got, _ = _run(ctx, trace, bc, [5], 50)
+    exp = O.constraint_eval_gl(trace, bc, [5], 50)
+    assert (got == exp).all()
+    assert (got[:, 10:20] == 0).all() and (got[:, 41:] == 0).all()
+
+
+def _run_compiled(ctx, trace, bc, consts, n_c):
+    rows, w = trace.shape
+    prog = Z.ConstraintProgram(ctx, w, bc, consts, n_c, compiled=True)
+    tb = ctx.upload(trace)
+    ob = ctx.alloc(rows * n_c * 8)
+    ob.upload(np.full(rows * n_c, 0xDEADBEEF, dtype=np.uint64))
+    prog.eval(tb, rows, ob)
+    got = ob.download(np.uint64, rows * n_c).reshape(rows, n_c)
+    prog.free(); tb.free(); ob.free()
+    return got
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_compiled_random_programs(ctx, seed, monkeypatch):
+    """The straight-line kernel compiled for one program against the oracle."""
+    monkeypatch.setenv("ZKG_CE_SHAPE", SHAPES[seed % len(SHAPES)])
+    rng = np.random.default_rng(900 + seed)
+    w = int(rng.choice([1, 15, 33, 64, 100]))
+    rows = int(rng.choice([1, 65, 257, 3000]))
+    n_const, n_c = int(rng.integers(1, 8)), int(rng.integers(1, 9))
+    trace = _trace(rng, rows, w)
+    consts = [int(v) % P for v in rng.integers(0, 2**63, n_const, dtype=np.uint64)]
+    consts[0] = P - 1
+    bc = random_program(rng, w, n_const, n_c, int(rng.integers(3, 200)))
+    got = _run_compiled(ctx, trace, bc, consts, n_c)
+    assert (got == O.constraint_eval_gl(trace, bc, consts, n_c)).all()
+
+
+def test_compiled_air_program_and_sessions(ctx):
+    rng = np.random.default_rng(78)
+    rows, w = (1 << 16) + 37, 64
+    trace = _trace(rng, rows, w)
+    bc, consts, n_c = air_program(w, 32, rng)
+    assert (_run_compiled(ctx, trace, bc, consts, n_c) == O.constraint_eval_gl(trace, bc, consts, n_c)).all()
+    order = [0, 1, 20, 2, 21, 0, 40, 3, 1]
+    bc = []
+    for k, j in enumerate(order):
+        bc += [I(O.OP_LOAD, k % 6), I(O.OP_CONST, 0), I(O.OP_MUL), I(O.OP_EMIT, j)]
+    trace = _trace(rng, 777, 6)
+    assert (_run_compiled(ctx, trace, bc, [5], 50) == O.constraint_eval_gl(trace, bc, [5], 50)).all()

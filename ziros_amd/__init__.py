@@ -27,6 +27,8 @@ from .host import (  # noqa: F401
     Server,
     commit_batch,
     commit_trace,
+    compile_check_constraints,
+    constraint_source,
     make_job,
     translate_constraints,
 )

@@ -31,6 +31,27 @@ EXTRA = {"zkg_probe.hip": ["-ffp-contract=off"], "zkg_hash.hip": ["-ffp-contract
          "zkg_deep.hip": ["-ffp-contract=off"]}
 
 
+JIT_PRELUDE = os.path.join(CSRC, "_gen", "zkg_jit_prelude.inc")
+JIT_HEADERS = ["zkg_field.h", "zkg_ce_device.h"]
+
+
+def write_jit_prelude():
+    """The source that constraint programs compiled at run time start with:
+    the field arithmetic and evaluator helpers, as one C++ raw string."""
+    parts = ["typedef unsigned int uint32_t;", "typedef unsigned long long uint64_t;",
+             "typedef int int32_t;", "typedef long long int64_t;"]
+    for h in JIT_HEADERS:
+        with open(os.path.join(CSRC, h)) as f:
+            parts += [l for l in f.read().splitlines() if not l.startswith(("#include", "#pragma once"))]
+    text = "\n".join(parts)
+    assert ")ZKGJIT" not in text
+    os.makedirs(os.path.dirname(JIT_PRELUDE), exist_ok=True)
+    body = 'R"ZKGJIT(' + text + '\n)ZKGJIT"\n'
+    if not os.path.exists(JIT_PRELUDE) or open(JIT_PRELUDE).read() != body:
+        with open(JIT_PRELUDE, "w") as f:
+            f.write(body)
+
+
 def _compile(src, obj_dir=None, defines=()):
     obj = os.path.join(obj_dir or os.path.join(CSRC, "_obj"), src.replace(".hip", ".o"))
     cmd = [HIPCC, *FLAGS, *EXTRA.get(src, []), *defines, "-c", os.path.join(CSRC, src), "-o", obj]
@@ -43,7 +64,7 @@ def _compile(src, obj_dir=None, defines=()):
 
 def _link(objs, out):
     tmp = out + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread", "-lhiprtc"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
@@ -53,6 +74,7 @@ def _link(objs, out):
 def build(verbose=False):
     """Compile every translation unit and link libzkg.so and libzkg_probe.so."""
     os.makedirs(os.path.join(CSRC, "_obj"), exist_ok=True)
+    write_jit_prelude()
     with ThreadPoolExecutor(max_workers=6) as ex:
         results = list(ex.map(_compile, SOURCES + PROBE_SOURCES))
     if verbose:
@@ -68,6 +90,7 @@ def build(verbose=False):
 def build_variant(out_dir, defines):
     """libzkg.so with extra -D definitions (tuning candidates) into out_dir."""
     os.makedirs(out_dir, exist_ok=True)
+    write_jit_prelude()
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         results = list(ex.map(lambda src: _compile(src, out_dir, defines), SOURCES))
     lib = os.path.join(out_dir, "libzkg.so")

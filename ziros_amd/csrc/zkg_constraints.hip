@@ -27,11 +27,17 @@
 // One lane evaluates one row; a 256-thread block holds four independent
 // waves with private LDS regions and no block-level barriers.
 #include "../../include/zkg.h"
+#include "zkg_ce_device.h"
 #include "zkg_field.h"
 #include "zkg_internal.h"
 #include "zkg_runtime.h"
 
+#include <hip/hiprtc.h>
+
 #include <algorithm>
+#include <cstdio>
+#include <string>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -39,15 +45,22 @@ using namespace zkg;
 
 namespace {
 
-// Device program opcodes (word = op << 24 | arg).
+// Device program: one word per instruction, op << 27 | slot << 22 | arg.
+// `slot` is the LDS stack entry the instruction spills to or reloads from
+// (the host knows the stack depth at every instruction), or the panel slot of
+// CE_STAGE; `arg` is an LDS byte offset, a constant index, a constraint index
+// or a panel index.
 enum : uint32_t {
-    CE_PUSH_L = 0,  // push; top = lds[arg]                 (LOAD col)
-    CE_PUSH_K,      // push; top = consts[arg]              (CONST k)
-    CE_ADD_S,       // top = pop() + top                    (ADD)
-    CE_SUB_S,       // top = pop() - top                    (SUB)
-    CE_MUL_S,       // top = pop() * top                    (MUL)
-    CE_DUP,         // push (top stays)                     (DUP)
-    CE_EMIT,        // out[row][arg] = top; top = pop()     (EMIT j)
+    CE_PUSH_L = 0,  // stk[slot] = top; top = lds[arg]      (LOAD col)
+    CE_PUSH_L0,     // top = lds[arg] (empty stack)         (LOAD col)
+    CE_PUSH_K,      // stk[slot] = top; top = consts[arg]   (CONST k)
+    CE_PUSH_K0,     // top = consts[arg] (empty stack)      (CONST k)
+    CE_ADD_S,       // top = stk[slot] + top                (ADD)
+    CE_SUB_S,       // top = stk[slot] - top                (SUB)
+    CE_MUL_S,       // top = stk[slot] * top                (MUL)
+    CE_DUP,         // stk[slot] = top                      (DUP)
+    CE_EMIT,        // lds[arg] = top; top = stk[slot]      (EMIT j, into the output panel)
+    CE_EMIT0,       // lds[arg] = top (stack empties)       (EMIT j)
     CE_ADD_L,       // top = top + lds[arg]                 (LOAD col; ADD)
     CE_SUB_L,       // top = top - lds[arg]                 (LOAD col; SUB)
     CE_MUL_L,       // top = top * lds[arg]                 (LOAD col; MUL)
@@ -57,117 +70,168 @@ enum : uint32_t {
     CE_DBL,         // top = top + top                      (DUP; ADD)
     CE_ZERO_T,      // top = 0                              (DUP; SUB)
     CE_SQR,         // top = top * top                      (DUP; MUL)
-    CE_STAGE,       // slot arg >> 20 <- panel arg & 0xFFFFF
-    CE_ZERO,        // out[row][arg] = 0 (never emitted)
+    CE_STAGE,       // panel slot `slot` <- panel `arg`
+    CE_OCLEAR,      // zero the output panel at LDS offset arg
+    CE_OFLUSH,      // output panel -> constraints arg*PC.., columns in the next word's mask
     CE_END,
 };
 
-constexpr int CE_PANEL = 16;                             // columns per panel
-constexpr int CE_ROWS = 64;                              // rows per wave
-constexpr int CE_WAVES = 4;                              // waves per block
-constexpr uint32_t CE_COL_STRIDE = CE_ROWS * 8 + 8;      // LDS bytes per panel column (padded)
-constexpr uint32_t CE_SLOT_BYTES = CE_PANEL * CE_COL_STRIDE;
-constexpr uint32_t CE_STACK_ROW = CE_ROWS * 8;           // LDS bytes per stack depth
-constexpr int CE_MAX_STACK = 32;                         // the reference's stack depth
-constexpr int CE_MAX_SLOTS = 4;
-constexpr uint32_t CE_MAX_PANELS = 1u << 20;
-constexpr uint32_t CE_LDS_BYTES = 160u * 1024;            // LDS per CU on gfx950
-static_assert(CE_WAVES * (CE_SLOT_BYTES + (CE_MAX_STACK - 1) * CE_STACK_ROW) <= CE_LDS_BYTES,
-              "one panel slot and a full stack must fit");
-
-__device__ __forceinline__ uint64_t lds_u64(const unsigned char *p) { return *(const uint64_t *)p; }
-
-// Stage panel `panel` (columns panel*16 .. +15) of this wave's 64 rows into
-// the LDS slot at `dst`, column-major with a padded column stride: each
-// 16-lane quarter reads one row's 128-byte segment.
-__device__ __forceinline__ void stage_panel(const uint64_t *__restrict__ trace, uint64_t n_rows, uint32_t width,
-                                            uint64_t row0, uint32_t panel, uint32_t lane, unsigned char *dst)
+__host__ __device__ constexpr uint32_t ce_word(uint32_t op, uint32_t slot, uint32_t arg)
 {
-    const uint32_t c = lane % CE_PANEL;
-    const uint32_t col = min(panel * CE_PANEL + c, width - 1);  // columns past the width: clamped, never read
-    constexpr int PASSES = CE_ROWS * CE_PANEL / 64;
-    uint64_t v[PASSES];
-#pragma unroll
-    for (int k = 0; k < PASSES; k++) {
-        const uint64_t r = min(row0 + lane / CE_PANEL + (uint64_t)k * (64 / CE_PANEL), n_rows - 1);
-        v[k] = trace[r * width + col];
-    }
-    // Lanes of this wave may still read the slot's previous panel.
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int k = 0; k < PASSES; k++)
-        *(uint64_t *)(dst + c * CE_COL_STRIDE + (lane / CE_PANEL + k * (64 / CE_PANEL)) * 8) = v[k];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return op << 27 | slot << 22 | arg;
 }
 
+
+constexpr int CE_WAVES = 4;                   // waves per block
+constexpr int CE_MAX_STACK = 32;              // the reference's stack depth
+constexpr int CE_MAX_SLOTS = 4;
+constexpr uint32_t CE_ARG_LIMIT = 1u << 22;
+constexpr uint32_t CE_LDS_BYTES = 160u * 1024;  // LDS per CU on gfx950
+
+// Work shape of a wave: each lane evaluates R rows (rows lane + 64 j), so one
+// decoded instruction drives R independent field operations; panels hold PC
+// columns of the wave's 64 R rows, column-major with a padded column stride.
+struct CeShape {
+    uint32_t rows_per_lane, panel;
+    uint32_t wave_rows() const { return 64 * rows_per_lane; }
+    uint32_t col_stride() const { return wave_rows() * 8 + 8; }
+    uint32_t slot_bytes() const { return panel * col_stride(); }
+    uint32_t stack_row() const { return wave_rows() * 8; }
+};
+
+#ifndef ZKG_CE_ROWS_PER_LANE
+#define ZKG_CE_ROWS_PER_LANE 2
+#endif
+#ifndef ZKG_CE_PANEL
+#define ZKG_CE_PANEL 8
+#endif
+
+#define CE_EACH for (int j = 0; j < R; j++)
+
+template <int R, int PC>
 __global__ __launch_bounds__(CE_WAVES * 64) void k_constraint_eval_gl(
     const uint64_t *__restrict__ trace, uint64_t n_rows, uint32_t width, const uint32_t *__restrict__ prog,
     const uint64_t *__restrict__ consts, uint32_t n_out, uint32_t wave_bytes, uint32_t stack_off,
-    uint64_t *__restrict__ out)
+    uint32_t out_off, uint64_t *__restrict__ out)
 {
+    constexpr uint32_t WR = 64 * R, COLS = WR * 8 + 8, SLOT = PC * COLS, SROW = WR * 8;
     extern __shared__ __align__(16) unsigned char ce_lds[];
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint64_t row0 = ((uint64_t)blockIdx.x * CE_WAVES + wave) * CE_ROWS;
+    const uint64_t row0 = ((uint64_t)blockIdx.x * CE_WAVES + wave) * WR;
     if (row0 >= n_rows) return;
     unsigned char *const base = ce_lds + wave * wave_bytes;
-    unsigned char *const mine = base + lane * 8;  // this lane's column in panels and stack
+    unsigned char *const mine = base + lane * 8;  // row lane + 64 j sits at mine + 512 j
     unsigned char *const stk = mine + stack_off;
-    const uint64_t row = row0 + lane;
-    const bool live = row < n_rows;
-    uint64_t *const orow = out + row * n_out;
-    uint64_t top = 0;
-    uint32_t d = 0;  // stack depth; the top is in `top`, entries 0..d-2 in LDS
+    // Lanes past the last row evaluate the last row again (staging clamps
+    // their loads to it); flushes store those copies over the same values.
+    uint64_t top[R];
+    CE_EACH top[j] = 0;
 
-    for (uint32_t pc = 0;; pc += 4) {
-        const uint4 w4 = *(const uint4 *)(prog + pc);
-        const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
+    uint32_t pc = 0, w = prog[0];
+    for (;;) {
+        const uint32_t op = w >> 27, arg = w & (CE_ARG_LIMIT - 1);
+        if (op == CE_END) break;
+        const uint32_t slot = (w >> 22) & 31;
+        unsigned char *const sp = stk + slot * SROW;
+        w = prog[++pc];  // the next word loads while this one executes
+        switch (op) {
+        case CE_PUSH_L: CE_EACH *(uint64_t *)(sp + 512 * j) = top[j]; [[fallthrough]];
+        case CE_PUSH_L0: CE_EACH top[j] = lds_u64(mine + arg + 512 * j); break;
+        case CE_PUSH_K: CE_EACH *(uint64_t *)(sp + 512 * j) = top[j]; [[fallthrough]];
+        case CE_PUSH_K0: {
+            const uint64_t k = consts[arg];
+            CE_EACH top[j] = k;
+            break;
+        }
+        case CE_ADD_S: CE_EACH top[j] = Gl::add(lds_u64(sp + 512 * j), top[j]); break;
+        case CE_SUB_S: CE_EACH top[j] = Gl::sub(lds_u64(sp + 512 * j), top[j]); break;
+        case CE_MUL_S: CE_EACH top[j] = Gl::mul(lds_u64(sp + 512 * j), top[j]); break;
+        case CE_DUP: CE_EACH *(uint64_t *)(sp + 512 * j) = top[j]; break;
+        case CE_EMIT:
+            CE_EACH *(uint64_t *)(mine + arg + 512 * j) = top[j];
+            CE_EACH top[j] = lds_u64(sp + 512 * j);
+            break;
+        case CE_EMIT0: CE_EACH *(uint64_t *)(mine + arg + 512 * j) = top[j]; break;
+        case CE_ADD_L: CE_EACH top[j] = Gl::add(top[j], lds_u64(mine + arg + 512 * j)); break;
+        case CE_SUB_L: CE_EACH top[j] = Gl::sub(top[j], lds_u64(mine + arg + 512 * j)); break;
+        case CE_MUL_L: CE_EACH top[j] = Gl::mul(top[j], lds_u64(mine + arg + 512 * j)); break;
+        case CE_ADD_K: { const uint64_t k = consts[arg]; CE_EACH top[j] = Gl::add(top[j], k); break; }
+        case CE_SUB_K: { const uint64_t k = consts[arg]; CE_EACH top[j] = Gl::sub(top[j], k); break; }
+        case CE_MUL_K: { const uint64_t k = consts[arg]; CE_EACH top[j] = Gl::mul(top[j], k); break; }
+        case CE_DBL: CE_EACH top[j] = Gl::add(top[j], top[j]); break;
+        case CE_ZERO_T: CE_EACH top[j] = 0; break;
+        case CE_SQR: CE_EACH top[j] = Gl::mul(top[j], top[j]); break;
+        case CE_STAGE: stage_panel<R, PC>(trace, n_rows, width, row0, arg, lane, base + slot * SLOT); break;
+        case CE_OCLEAR:
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const uint32_t op = ws[i] >> 24, arg = ws[i] & 0xFFFFFFu;
-            switch (op) {
-            case CE_PUSH_L:
-                if (d) *(uint64_t *)(stk + (d - 1) * CE_STACK_ROW) = top;
-                d++;
-                top = lds_u64(mine + arg);
-                break;
-            case CE_PUSH_K:
-                if (d) *(uint64_t *)(stk + (d - 1) * CE_STACK_ROW) = top;
-                d++;
-                top = consts[arg];
-                break;
-            case CE_ADD_S: top = Gl::add(lds_u64(stk + (--d - 1) * CE_STACK_ROW), top); break;
-            case CE_SUB_S: top = Gl::sub(lds_u64(stk + (--d - 1) * CE_STACK_ROW), top); break;
-            case CE_MUL_S: top = Gl::mul(lds_u64(stk + (--d - 1) * CE_STACK_ROW), top); break;
-            case CE_DUP: *(uint64_t *)(stk + (d++ - 1) * CE_STACK_ROW) = top; break;
-            case CE_EMIT:
-                if (live) orow[arg] = top;
-                if (--d) top = lds_u64(stk + (d - 1) * CE_STACK_ROW);
-                break;
-            case CE_ADD_L: top = Gl::add(top, lds_u64(mine + arg)); break;
-            case CE_SUB_L: top = Gl::sub(top, lds_u64(mine + arg)); break;
-            case CE_MUL_L: top = Gl::mul(top, lds_u64(mine + arg)); break;
-            case CE_ADD_K: top = Gl::add(top, consts[arg]); break;
-            case CE_SUB_K: top = Gl::sub(top, consts[arg]); break;
-            case CE_MUL_K: top = Gl::mul(top, consts[arg]); break;
-            case CE_DBL: top = Gl::add(top, top); break;
-            case CE_ZERO_T: top = 0; break;
-            case CE_SQR: top = Gl::mul(top, top); break;
-            case CE_STAGE:
-                stage_panel(trace, n_rows, width, row0, arg & (CE_MAX_PANELS - 1), lane,
-                            base + (arg >> 20) * CE_SLOT_BYTES);
-                break;
-            case CE_ZERO:
-                if (live) orow[arg] = 0;
-                break;
-            default: return;  // CE_END
-            }
+            for (int c = 0; c < PC; c++) CE_EACH *(uint64_t *)(mine + arg + c * COLS + 512 * j) = 0;
+            break;
+        case CE_OFLUSH: {
+            const uint32_t mask = w;  // the mask word
+            w = prog[++pc];
+            flush_panel<R, PC>(out, n_rows, n_out, row0, arg, mask, lane, base + out_off);
+            break;
+        }
+        default: break;
         }
     }
 }
+
+#undef CE_EACH
+
+static_assert(CE_WAVES * (2 * 16 * (64 * 8 + 8) + (CE_MAX_STACK - 1) * 64 * 8) <= CE_LDS_BYTES,
+              "at one row per lane, an input and an output panel of 16 columns and a full stack must fit");
+
+// Every work shape the library is built with.
+#define CE_SHAPES(X) X(1, 16) X(2, 4) X(2, 8) X(2, 16) X(4, 4) X(4, 8)
+
+bool ce_shape_built(CeShape s)
+{
+#define X(R, PC) if (s.rows_per_lane == R && s.panel == PC) return true;
+    CE_SHAPES(X)
+#undef X
+    return false;
+}
+
+hipError_t ce_launch(CeShape s, dim3 grid, uint32_t lds, hipStream_t st, const uint64_t *trace, uint64_t n_rows,
+                     uint32_t width, const uint32_t *prog, const uint64_t *consts, uint32_t n_out, uint32_t wave_bytes,
+                     uint32_t stack_off, uint32_t out_off, uint64_t *out, uint32_t lds_limit)
+{
+#define X(R, PC)                                                                                                  \
+    if (s.rows_per_lane == R && s.panel == PC) {                                                                  \
+        if (lds > 64u * 1024) {                                                                                   \
+            /* above the default dynamic-LDS limit */                                                             \
+            hipError_t e = hipFuncSetAttribute((const void *)k_constraint_eval_gl<R, PC>,                         \
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit);       \
+            if (e != hipSuccess) return e;                                                                        \
+        }                                                                                                         \
+        hipLaunchKernelGGL((k_constraint_eval_gl<R, PC>), grid, dim3(CE_WAVES * 64), lds, st, trace, n_rows,     \
+                           width, prog, consts, n_out, wave_bytes, stack_off, out_off, out);                     \
+        return hipGetLastError();                                                                                 \
+    }
+    CE_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+// The work shape: `dflt`, or ZKG_CE_SHAPE="R,PC" (a tuning override that
+// applies to interpreted and compiled programs; it must name a built shape).
+int ce_pick_shape(CeShape dflt, CeShape *out)
+{
+    CeShape s = dflt;
+    if (const char *e = getenv("ZKG_CE_SHAPE")) {
+        unsigned r = 0, pc = 0;
+        if (sscanf(e, "%u,%u", &r, &pc) != 2 || !ce_shape_built(CeShape{r, pc}))
+            return set_err(ZKG_EINVAL, "ZKG_CE_SHAPE=%s is not a built work shape", e);
+        s = CeShape{r, pc};
+    }
+    *out = s;
+    return ZKG_OK;
+}
+
+constexpr CeShape CE_INTERP_SHAPE{ZKG_CE_ROWS_PER_LANE, ZKG_CE_PANEL};
+constexpr CeShape CE_COMPILED_SHAPE{1, 16};
 
 // ------------------------------------------------------------------ host translation
 struct SrcIns {
@@ -179,36 +243,38 @@ struct MidIns {
     int64_t col;  // trace column read by the instruction, -1 if none
 };
 
-}  // namespace
-
-struct zkg_constraint_program {
-    zkg_ctx *ctx = nullptr;
-    uint32_t width = 0, n_constraints = 0;
-    uint32_t slots = 0, stack_slots = 0, stages = 0, n_words = 0;
-    uint32_t wave_bytes = 0, stack_off = 0, lds_budget = 0;
-    void *d_prog = nullptr, *d_consts = nullptr;
+// A translated program: the device words and the per-wave LDS layout they
+// address (input panel slots, then the stack below the top, then the output
+// panel).
+struct CeLayout {
+    CeShape shape{1, 16};
+    uint32_t slots = 0, stages = 0, stack_slots = 0;
+    uint32_t stack_off = 0, out_off = 0, wave_bytes = 0;
+    std::vector<uint32_t> words;
 };
 
-namespace {
-
-// Validate the reference bytecode and translate it.  Returns ZKG_OK and fills
-// `p` (host side) and `words`.
-int ce_translate(const uint32_t *bc, uint32_t n, uint32_t width, uint32_t n_consts, uint32_t n_out,
-                 uint32_t lds_budget, zkg_constraint_program &p, std::vector<uint32_t> &words)
+// Validate the reference bytecode and translate it for `shape`.  The stack
+// below the top lives in LDS for the interpreter; compiled kernels keep it in
+// registers (stack_in_lds false).  A block holds `block_waves` waves and may
+// use `lds_budget` bytes.  ZKG_ENOMEM: does not fit at this shape.
+int ce_translate(const uint32_t *bc, uint32_t n, uint32_t width, uint32_t n_consts, uint32_t n_out, CeShape shape,
+                 bool stack_in_lds, uint32_t block_waves, uint32_t lds_budget, CeLayout &L)
 {
+    const uint32_t PC = shape.panel;
+    L = CeLayout{};
+    L.shape = shape;
     // 1. validate: reference opcodes 0..6, anything else is a no-op
     //    (constraint_eval.metal:38-73 has no default case)
     std::vector<SrcIns> src;
-    std::vector<int> depth_before;
     int sp = 0, maxd = 0;
     std::vector<bool> emitted(n_out, false);
     for (uint32_t pc = 0; pc < n; pc++) {
         const uint32_t op = bc[pc] >> 24, arg = bc[pc] & 0xFFFFFFu;
         if (op > 6) continue;
-        depth_before.push_back(sp);
         switch (op) {
         case 0:
             if (arg >= n_consts) return set_err(ZKG_EINVAL, "instruction %u: constant %u of %u", pc, arg, n_consts);
+            if (arg >= CE_ARG_LIMIT) return set_err(ZKG_EINVAL, "instruction %u: constant index beyond 2^22", pc);
             sp++;
             break;
         case 1:
@@ -264,10 +330,10 @@ int ce_translate(const uint32_t *bc, uint32_t n, uint32_t width, uint32_t n_cons
     //    the smallest K whose stage count is within 25% of the best
     std::vector<uint32_t> uses;  // panel of each column read, in order
     for (const MidIns &m : mid)
-        if (m.col >= 0) uses.push_back((uint32_t)(m.col / CE_PANEL));
+        if (m.col >= 0) uses.push_back((uint32_t)(m.col / PC));
     std::vector<size_t> next_use(uses.size());
     {
-        std::vector<size_t> last((width + CE_PANEL - 1) / CE_PANEL, SIZE_MAX);
+        std::vector<size_t> last((width + PC - 1) / PC, SIZE_MAX);
         for (size_t i = uses.size(); i-- > 0;) {
             next_use[i] = last[uses[i]];
             last[uses[i]] = i;
@@ -299,78 +365,303 @@ int ce_translate(const uint32_t *bc, uint32_t n, uint32_t width, uint32_t n_cons
         }
         return stages;
     };
-    const uint32_t n_panels = (width + CE_PANEL - 1) / CE_PANEL;
+    const uint32_t n_panels = (width + PC - 1) / PC;
     const int kmax = (int)std::max<uint32_t>(1, std::min<uint32_t>(CE_MAX_SLOTS, n_panels));
     const uint32_t best = simulate(kmax, nullptr, nullptr);
     int K = 1;
     while (K < kmax && (uint64_t)simulate(K, nullptr, nullptr) * 4 > (uint64_t)best * 5) K++;
-    const uint32_t stack_bytes = (uint32_t)std::max(0, maxd - 1) * CE_STACK_ROW;
-    while (K > 1 && CE_WAVES * (K * CE_SLOT_BYTES + stack_bytes) > lds_budget) K--;
-    if (CE_WAVES * ((uses.empty() ? 0 : CE_SLOT_BYTES) + stack_bytes) > lds_budget)
-        return set_err(ZKG_EINVAL, "program needs more than %u bytes of LDS per block", lds_budget);
+    // LDS per wave: K input panel slots, the stack below the top, one output panel
+    const uint32_t stack_bytes = stack_in_lds ? (uint32_t)std::max(0, maxd - 1) * shape.stack_row() : 0;
+    const uint32_t fixed = stack_bytes + shape.slot_bytes();
+    while (K > 1 && block_waves * (K * shape.slot_bytes() + fixed) > lds_budget) K--;
+    if (block_waves * ((uses.empty() ? 0 : shape.slot_bytes()) + fixed) > lds_budget)
+        return ZKG_ENOMEM;  // the caller tries a smaller shape
     std::vector<int> slot_of_use(uses.size());
     std::vector<bool> stage_at(uses.size());
-    p.stages = simulate(K, &slot_of_use, &stage_at);
-    p.slots = uses.empty() ? 0 : K;
+    L.stages = simulate(K, &slot_of_use, &stage_at);
+    L.slots = uses.empty() ? 0 : K;
 
     // 4. emit device words
-    p.stack_off = p.slots * CE_SLOT_BYTES;
-    p.stack_slots = (uint32_t)std::max(0, maxd - 1);
-    p.wave_bytes = (p.stack_off + stack_bytes + 15) & ~15u;
-    words.clear();
+    L.stack_off = L.slots * shape.slot_bytes();
+    L.stack_slots = stack_bytes / shape.stack_row();
+    L.out_off = L.stack_off + stack_bytes;
+    L.wave_bytes = (L.out_off + shape.slot_bytes() + 15) & ~15u;
+    std::vector<uint32_t> &words = L.words;
     size_t u = 0;
+    uint32_t d = 0;  // stack depth before each instruction
+    // Output panel sessions: EMITs land in the LDS output panel of constraint
+    // panel `cur`; switching panels flushes it.  A panel's first flush also
+    // writes zeros for its constraints that are never emitted.
+    const uint32_t n_opanels = (n_out + PC - 1) / PC;
+    std::vector<uint32_t> zero_mask(n_opanels, 0);
+    for (uint32_t j = 0; j < n_out; j++)
+        if (!emitted[j]) zero_mask[j / PC] |= 1u << (j % PC);
+    std::vector<bool> flushed(n_opanels, false);
+    int64_t cur = -1;
+    uint32_t cur_mask = 0;
+    auto flush = [&]() {
+        uint32_t mask = cur_mask;
+        if (!flushed[cur]) mask |= zero_mask[cur];
+        flushed[cur] = true;
+        words.push_back(ce_word(CE_OFLUSH, 0, (uint32_t)cur));
+        words.push_back(mask);
+        cur = -1;
+        cur_mask = 0;
+    };
+    auto open = [&](uint32_t q) {
+        if (!flushed[q] && zero_mask[q]) words.push_back(ce_word(CE_OCLEAR, 0, L.out_off));
+        cur = q;
+    };
     for (const MidIns &m : mid) {
-        uint32_t arg = m.arg;
+        uint32_t op = m.op, arg = m.arg, slot = 0;
         if (m.col >= 0) {
-            const uint32_t panel = (uint32_t)(m.col / CE_PANEL), slot = (uint32_t)slot_of_use[u];
-            if (stage_at[u]) words.push_back(CE_STAGE << 24 | slot << 20 | panel);
-            arg = slot * CE_SLOT_BYTES + (uint32_t)(m.col % CE_PANEL) * CE_COL_STRIDE;
+            const uint32_t panel = (uint32_t)(m.col / PC), ps = (uint32_t)slot_of_use[u];
+            if (stage_at[u]) words.push_back(ce_word(CE_STAGE, ps, panel));
+            arg = ps * shape.slot_bytes() + (uint32_t)(m.col % PC) * shape.col_stride();
             u++;
         }
-        words.push_back(m.op << 24 | arg);
+        switch (op) {
+        case CE_PUSH_L: case CE_PUSH_K:
+            if (d) slot = d - 1;
+            else op = op == CE_PUSH_L ? CE_PUSH_L0 : CE_PUSH_K0;
+            d++;
+            break;
+        case CE_ADD_S: case CE_SUB_S: case CE_MUL_S:
+            slot = d - 2;
+            d--;
+            break;
+        case CE_DUP:
+            slot = d - 1;
+            d++;
+            break;
+        case CE_EMIT: {
+            const uint32_t q = arg / PC, c = arg % PC;
+            if (cur != (int64_t)q) {
+                if (cur >= 0) flush();
+                open(q);
+            }
+            cur_mask |= 1u << c;
+            arg = L.out_off + c * shape.col_stride();
+            if (d >= 2) slot = d - 2;
+            else op = CE_EMIT0;
+            d--;
+            break;
+        }
+        }
+        words.push_back(ce_word(op, slot, arg));
     }
-    for (uint32_t j = 0; j < n_out; j++)
-        if (!emitted[j]) words.push_back(CE_ZERO << 24 | j);
-    do words.push_back(CE_END << 24); while (words.size() % 4);
-    p.n_words = (uint32_t)words.size();
+    if (cur >= 0) flush();
+    for (uint32_t q = 0; q < n_opanels; q++)
+        if (!flushed[q]) {  // never emitted into: all zeros
+            open(q);
+            flush();
+        }
+    words.push_back(ce_word(CE_END, 0, 0));
+    words.push_back(ce_word(CE_END, 0, 0));  // read ahead by the last instruction
     return ZKG_OK;
+}
+
+// Translate at the preferred shape; a program whose stack does not fit the
+// LDS there falls back to one row per lane.
+int ce_translate_fit(const uint32_t *bc, uint32_t n, uint32_t width, uint32_t n_consts, uint32_t n_out,
+                     CeShape preferred, bool stack_in_lds, uint32_t block_waves, uint32_t lds_budget, CeLayout &L)
+{
+    CeShape shape;
+    if (int rc = ce_pick_shape(preferred, &shape)) return rc;
+    int rc = ce_translate(bc, n, width, n_consts, n_out, shape, stack_in_lds, block_waves, lds_budget, L);
+    if (rc == ZKG_ENOMEM && shape.rows_per_lane > 1)
+        rc = ce_translate(bc, n, width, n_consts, n_out, CeShape{1, 16}, stack_in_lds, block_waves, lds_budget, L);
+    if (rc == ZKG_ENOMEM) rc = set_err(ZKG_EINVAL, "program needs more than %u bytes of LDS per block", lds_budget);
+    return rc;
 }
 
 }  // namespace
 
-static int ce_check_args(uint32_t width, const uint32_t *bytecode, uint32_t n_instructions, uint32_t n_constraints)
+struct zkg_constraint_program {
+    zkg_ctx *ctx = nullptr;
+    uint32_t width = 0, n_constraints = 0, lds_budget = 0;
+    std::vector<uint32_t> bytecode;  // the reference bytecode (host copy)
+    std::vector<uint64_t> consts;    // constant pool (host copy)
+    CeLayout interp;                 // what the interpreter runs
+    void *d_prog = nullptr, *d_consts = nullptr;
+    CeLayout jit;                    // what the compiled kernel was generated from
+    hipModule_t mod = nullptr;       // compiled straight-line kernel, if any
+    hipFunction_t fn = nullptr;
+};
+
+// ------------------------------------------------------------------ compiled programs
+// A program evaluated many times (one AIR, many traces) can be compiled to a
+// straight-line kernel: the same staging, output panels and field arithmetic
+// as the interpreter, with the stack resolved at compile time into registers,
+// constants as immediates and no per-instruction dispatch.  The source is the
+// device program above, rendered as HIP and compiled for gfx950 with hiprtc.
+// One wave per block.
+
+static const char kJitPrelude[] =
+#include "_gen/zkg_jit_prelude.inc"
+    ;
+
+namespace {
+
+constexpr uint32_t CE_JIT_LDS = 64u * 1024;  // per one-wave block
+
+std::string ce_jit_source(const CeLayout &L, const std::vector<uint64_t> &consts)
+{
+    const uint32_t R = L.shape.rows_per_lane, PC = L.shape.panel, COLS = L.shape.col_stride();
+    std::string b;
+    char line[320];
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wformat-security"  // every format below is a literal
+    auto emit = [&](const char *fmt, auto... a) {
+        snprintf(line, sizeof line, fmt, a...);
+        b += line;
+        b += '\n';
+    };
+#pragma clang diagnostic pop
+    b += kJitPrelude;
+    emit("extern \"C\" __global__ __launch_bounds__(64) void zkg_ce_jit(const uint64_t *__restrict__ trace, "
+         "uint64_t n_rows, uint32_t width, uint32_t n_out, uint64_t *__restrict__ out)");
+    emit("{");
+    emit("    constexpr int R = %u, PC = %u;", R, PC);
+    emit("    extern __shared__ __align__(16) unsigned char ce_lds[];");
+    emit("    const uint32_t lane = threadIdx.x;");
+    emit("    const uint64_t row0 = (uint64_t)blockIdx.x * (64 * R);");
+    emit("    if (row0 >= n_rows) return;");
+    emit("    unsigned char *const mine = ce_lds + lane * 8;");
+    emit("#define LD(off, j) (*(const uint64_t *)(mine + (off) + 512 * (j)))");
+    emit("#define EACH for (int j = 0; j < R; j++)");
+    std::vector<uint32_t> st;  // the stack, as variable numbers
+    uint32_t nv = 0;
+    static const char *BIN[3] = {"add", "sub", "mul"};
+    for (size_t i = 0; i < L.words.size(); i++) {
+        const uint32_t w = L.words[i], op = w >> 27, slot = (w >> 22) & 31, arg = w & (CE_ARG_LIMIT - 1);
+        if (op == CE_END) break;
+        switch (op) {
+        case CE_PUSH_L: case CE_PUSH_L0:
+            emit("    uint64_t v%u[R]; EACH v%u[j] = LD(%uu, j);", nv, nv, arg);
+            st.push_back(nv++);
+            break;
+        case CE_PUSH_K: case CE_PUSH_K0:
+            emit("    uint64_t v%u[R]; EACH v%u[j] = 0x%llxull;", nv, nv, (unsigned long long)consts[arg]);
+            st.push_back(nv++);
+            break;
+        case CE_ADD_S: case CE_SUB_S: case CE_MUL_S: {
+            const uint32_t y = st.back();
+            st.pop_back();
+            const uint32_t x = st.back();
+            st.pop_back();
+            emit("    uint64_t v%u[R]; EACH v%u[j] = zkg::Gl::%s(v%u[j], v%u[j]);", nv, nv, BIN[op - CE_ADD_S], x, y);
+            st.push_back(nv++);
+            break;
+        }
+        case CE_DUP: st.push_back(st.back()); break;
+        case CE_EMIT: case CE_EMIT0:
+            emit("    EACH *(uint64_t *)(mine + %uu + 512 * j) = v%u[j];", arg, st.back());
+            st.pop_back();
+            break;
+        case CE_ADD_L: case CE_SUB_L: case CE_MUL_L:
+            emit("    uint64_t v%u[R]; EACH v%u[j] = zkg::Gl::%s(v%u[j], LD(%uu, j));", nv, nv, BIN[op - CE_ADD_L],
+                 st.back(), arg);
+            st.back() = nv++;
+            break;
+        case CE_ADD_K: case CE_SUB_K: case CE_MUL_K:
+            emit("    uint64_t v%u[R]; EACH v%u[j] = zkg::Gl::%s(v%u[j], 0x%llxull);", nv, nv, BIN[op - CE_ADD_K],
+                 st.back(), (unsigned long long)consts[arg]);
+            st.back() = nv++;
+            break;
+        case CE_DBL: case CE_ZERO_T: case CE_SQR:
+            emit("    uint64_t v%u[R]; EACH v%u[j] = zkg::Gl::%s(v%u[j], v%u[j]);", nv, nv, BIN[op - CE_DBL], st.back(),
+                 st.back());
+            st.back() = nv++;
+            break;
+        case CE_STAGE:
+            emit("    zkg::stage_panel<R, PC>(trace, n_rows, width, row0, %uu, lane, ce_lds + %uu);", arg,
+                 slot * L.shape.slot_bytes());
+            break;
+        case CE_OCLEAR:
+            emit("    for (int c = 0; c < PC; c++) EACH *(uint64_t *)(mine + %uu + c * %uu + 512 * j) = 0;", arg, COLS);
+            break;
+        case CE_OFLUSH:
+            emit("    zkg::flush_panel<R, PC>(out, n_rows, n_out, row0, %uu, 0x%xu, lane, ce_lds + %uu);", arg,
+                 L.words[i + 1], L.out_off);
+            i++;
+            break;
+        }
+    }
+    emit("}");
+    return b;
+}
+
+int ce_jit_compile(const std::string &src, std::vector<char> &code, std::string &log)
+{
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "zkg_ce_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+        return set_err(ZKG_EDEVICE, "hiprtcCreateProgram failed");
+    const char *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
+                          "-Xclang", "-packed-fp32-ops", "-Wno-pragma-once-outside-header"};
+    const hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof *opts), opts);
+    size_t n = 0;
+    hiprtcGetProgramLogSize(prog, &n);
+    log.assign(n, '\0');
+    if (n) hiprtcGetProgramLog(prog, &log[0]);
+    int rc = ZKG_OK;
+    if (r != HIPRTC_SUCCESS) {
+        rc = set_err(ZKG_EDEVICE, "constraint program compile failed: %.400s", log.c_str());
+    } else {
+        size_t cs = 0;
+        hiprtcGetCodeSize(prog, &cs);
+        code.resize(cs);
+        hiprtcGetCode(prog, code.data());
+    }
+    hiprtcDestroyProgram(&prog);
+    return rc;
+}
+
+int ce_check_args(uint32_t width, const uint32_t *bytecode, uint32_t n_instructions, uint32_t n_constraints)
 {
     if (!bytecode && n_instructions) return set_err(ZKG_EINVAL, "null argument");
     if (!width || width > (1u << 24)) return set_err(ZKG_EINVAL, "width %u outside 1..2^24", width);
     if (!n_instructions) return set_err(ZKG_EINVAL, "empty bytecode");
-    if (!n_constraints || n_constraints > (1u << 24))
-        return set_err(ZKG_EINVAL, "n_constraints %u outside 1..2^24", n_constraints);
+    if (!n_constraints || n_constraints > CE_ARG_LIMIT)
+        return set_err(ZKG_EINVAL, "n_constraints %u outside 1..2^22", n_constraints);
     return ZKG_OK;
 }
 
-static void ce_info(const zkg_constraint_program &p, uint32_t info[6])
+void ce_info(const CeLayout &L, uint32_t info[8])
 {
-    info[0] = p.n_words;
-    info[1] = p.slots;
-    info[2] = p.stages;
-    info[3] = p.stack_slots;
-    info[4] = p.wave_bytes;
-    info[5] = CE_PANEL;
+    info[0] = (uint32_t)L.words.size();
+    info[1] = L.slots;
+    info[2] = L.stages;
+    info[3] = L.stack_slots;
+    info[4] = L.wave_bytes;
+    info[5] = L.shape.panel;
+    info[6] = L.shape.rows_per_lane;
+    info[7] = L.shape.col_stride();
 }
+
+// The compiled kernel's layout for a program (host only).
+int ce_jit_layout(const uint32_t *bc, uint32_t n, uint32_t width, uint32_t n_consts, uint32_t n_out,
+                  uint32_t lds_budget, CeLayout &L)
+{
+    return ce_translate_fit(bc, n, width, n_consts, n_out, CE_COMPILED_SHAPE, false, 1,
+                            std::min(CE_JIT_LDS, lds_budget), L);
+}
+
+}  // namespace
 
 extern "C" int zkg_constraint_translate(uint32_t width, const uint32_t *bytecode, uint32_t n_instructions,
                                         uint32_t n_constants, uint32_t n_constraints, uint32_t *words, uint32_t cap,
-                                        uint32_t info[6])
+                                        uint32_t info[8])
 {
     if (!info || (!words && cap)) return set_err(ZKG_EINVAL, "null argument");
     if (int rc = ce_check_args(width, bytecode, n_instructions, n_constraints)) return rc;
-    zkg_constraint_program p;
-    std::vector<uint32_t> w;
-    if (int rc = ce_translate(bytecode, n_instructions, width, n_constants, n_constraints, CE_LDS_BYTES, p, w))
+    CeLayout L;
+    if (int rc = ce_translate_fit(bytecode, n_instructions, width, n_constants, n_constraints, CE_INTERP_SHAPE, true,
+                                  CE_WAVES, CE_LDS_BYTES, L))
         return rc;
-    ce_info(p, info);
-    std::memcpy(words, w.data(), std::min<size_t>(cap, w.size()) * 4);
-    if (cap < w.size()) return set_err(ZKG_EINVAL, "%zu words needed, room for %u", w.size(), cap);
+    ce_info(L, info);
+    std::memcpy(words, L.words.data(), std::min<size_t>(cap, L.words.size()) * 4);
+    if (cap < L.words.size()) return set_err(ZKG_EINVAL, "%zu words needed, room for %u", L.words.size(), cap);
     return ZKG_OK;
 }
 
@@ -388,19 +679,23 @@ extern "C" int zkg_constraint_program_create(zkg_ctx *ctx, uint32_t width, const
     p->ctx = ctx;
     p->width = width;
     p->n_constraints = n_constraints;
-    std::vector<uint32_t> words;
+    p->bytecode.assign(bytecode, bytecode + n_instructions);
+    p->consts.assign(constants, constants + n_constants);
     int lds_max = 0;
     int rc = ZKG_OK;
     hipError_t e = hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
     if (e != hipSuccess) rc = hip_err(e, "device LDS size");
     p->lds_budget = std::min<uint32_t>(CE_LDS_BYTES, (uint32_t)std::max(lds_max, 0));
-    if (!rc) rc = ce_translate(bytecode, n_instructions, width, n_constants, n_constraints, p->lds_budget, *p, words);
+    if (!rc)
+        rc = ce_translate_fit(bytecode, n_instructions, width, n_constants, n_constraints, CE_INTERP_SHAPE, true,
+                              CE_WAVES, p->lds_budget, p->interp);
+    const std::vector<uint32_t> &words = p->interp.words;
     if (!rc) rc = dev_alloc(ctx, words.size() * 4, &p->d_prog);
     if (!rc) rc = dev_alloc(ctx, std::max<uint32_t>(n_constants, 1) * 8, &p->d_consts);
     if (!rc) e = hipMemcpyAsync(p->d_prog, words.data(), words.size() * 4, hipMemcpyHostToDevice, ctx->stream);
     if (!rc && e == hipSuccess && n_constants)
         e = hipMemcpyAsync(p->d_consts, constants, (uint64_t)n_constants * 8, hipMemcpyHostToDevice, ctx->stream);
-    if (!rc && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // host vectors go out of scope
+    if (!rc && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // host vectors may change after return
     if (!rc && e != hipSuccess) rc = hip_err(e, "constraint program upload");
     if (rc) {
         zkg_constraint_program_free(p);
@@ -410,10 +705,10 @@ extern "C" int zkg_constraint_program_create(zkg_ctx *ctx, uint32_t width, const
     return ZKG_OK;
 }
 
-extern "C" int zkg_constraint_program_info(const zkg_constraint_program *p, uint32_t info[6])
+extern "C" int zkg_constraint_program_info(const zkg_constraint_program *p, uint32_t info[8])
 {
     if (!p || !info) return set_err(ZKG_EINVAL, "null argument");
-    ce_info(*p, info);
+    ce_info(p->fn ? p->jit : p->interp, info);
     return ZKG_OK;
 }
 
@@ -428,31 +723,64 @@ extern "C" int zkg_constraint_program_eval(zkg_constraint_program *p, const zkg_
     if (n_rows > (UINT64_MAX / 8) / p->n_constraints || n_rows * p->n_constraints * 8 > out->bytes)
         return set_err(ZKG_EINVAL, "output buffer too small");
     if (trace->ctx != p->ctx || out->ctx != p->ctx) return set_err(ZKG_EINVAL, "buffers of another context");
-    const uint32_t lds = CE_WAVES * p->wave_bytes;
-    if (lds > 64u * 1024) {
-        // above the default dynamic-LDS limit; gfx950 has 160 KiB per CU
-        hipError_t e = hipFuncSetAttribute((const void *)k_constraint_eval_gl,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_budget);
-        if (e != hipSuccess) return hip_err(e, "constraint eval LDS limit");
+    const uint64_t *tr = (const uint64_t *)trace->ptr;
+    uint64_t *o = (uint64_t *)out->ptr;
+    if (p->fn) {  // compiled: one wave per block
+        const uint64_t waves = (n_rows + p->jit.shape.wave_rows() - 1) / p->jit.shape.wave_rows();
+        if (waves > 0x7FFFFFFFull) return set_err(ZKG_EINVAL, "too many rows");
+        uint32_t width = p->width, n_out = p->n_constraints;
+        void *args[] = {&tr, &n_rows, &width, &n_out, &o};
+        hipError_t e = hipModuleLaunchKernel(p->fn, (uint32_t)waves, 1, 1, 64, 1, 1, p->jit.wave_bytes,
+                                             p->ctx->stream, args, nullptr);
+        if (e != hipSuccess) return hip_err(e, "compiled constraint eval launch");
+        return ZKG_OK;
     }
-    const uint64_t waves = (n_rows + CE_ROWS - 1) / CE_ROWS;
+    const CeLayout &L = p->interp;
+    const uint64_t waves = (n_rows + L.shape.wave_rows() - 1) / L.shape.wave_rows();
     const uint64_t blocks = (waves + CE_WAVES - 1) / CE_WAVES;
     if (blocks > 0x7FFFFFFFull) return set_err(ZKG_EINVAL, "too many rows");
-    hipLaunchKernelGGL(k_constraint_eval_gl, dim3((uint32_t)blocks), dim3(CE_WAVES * 64), lds,
-                       p->ctx->stream, (const uint64_t *)trace->ptr, n_rows, p->width, (const uint32_t *)p->d_prog,
-                       (const uint64_t *)p->d_consts, p->n_constraints, p->wave_bytes, p->stack_off,
-                       (uint64_t *)out->ptr);
-    hipError_t e = hipGetLastError();
+    hipError_t e = ce_launch(L.shape, dim3((uint32_t)blocks), CE_WAVES * L.wave_bytes, p->ctx->stream, tr, n_rows,
+                             p->width, (const uint32_t *)p->d_prog, (const uint64_t *)p->d_consts, p->n_constraints,
+                             L.wave_bytes, L.stack_off, L.out_off, o, p->lds_budget);
     if (e != hipSuccess) return hip_err(e, "constraint eval launch");
+    return ZKG_OK;
+}
+
+extern "C" int zkg_constraint_program_compile(zkg_constraint_program *p)
+{
+    if (!p) return set_err(ZKG_EINVAL, "null argument");
+    if (p->fn) return ZKG_OK;
+    CeLayout L;
+    if (int rc = ce_jit_layout(p->bytecode.data(), (uint32_t)p->bytecode.size(), p->width, (uint32_t)p->consts.size(),
+                               p->n_constraints, p->lds_budget, L))
+        return rc;
+    std::vector<char> code;
+    std::string log;
+    if (int rc = ce_jit_compile(ce_jit_source(L, p->consts), code, log)) return rc;
+    hipError_t e = hipSetDevice(p->ctx->device);
+    if (e != hipSuccess) return hip_err(e, "hipSetDevice");
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
+    e = hipModuleLoadData(&mod, code.data());
+    if (e != hipSuccess) return hip_err(e, "load compiled constraint program");
+    e = hipModuleGetFunction(&fn, mod, "zkg_ce_jit");
+    if (e != hipSuccess) {
+        (void)hipModuleUnload(mod);
+        return hip_err(e, "compiled constraint program entry");
+    }
+    p->jit = std::move(L);
+    p->mod = mod;
+    p->fn = fn;
     return ZKG_OK;
 }
 
 extern "C" void zkg_constraint_program_free(zkg_constraint_program *p)
 {
     if (!p) return;
-    if (p->d_prog || p->d_consts) (void)hipStreamSynchronize(p->ctx->stream);
+    if (p->d_prog || p->d_consts || p->mod) (void)hipStreamSynchronize(p->ctx->stream);
     if (p->d_prog) dev_free(p->ctx, p->d_prog);
     if (p->d_consts) dev_free(p->ctx, p->d_consts);
+    if (p->mod) (void)hipModuleUnload(p->mod);
     delete p;
 }
 
@@ -469,5 +797,44 @@ extern "C" int zkg_constraint_eval_gl(zkg_ctx *ctx, const zkg_buf *trace, uint64
         if (e != hipSuccess) rc = hip_err(e, "constraint eval");
     }
     zkg_constraint_program_free(p);
+    return rc;
+}
+
+// Host-only views of the compiled path: its source, and a compile without a device.
+static int ce_host_jit(uint32_t width, const uint32_t *bytecode, uint32_t n_instructions, const uint64_t *constants,
+                       uint32_t n_constants, uint32_t n_constraints, std::string &src)
+{
+    if (!constants && n_constants) return set_err(ZKG_EINVAL, "null argument");
+    if (int rc = ce_check_args(width, bytecode, n_instructions, n_constraints)) return rc;
+    CeLayout L;
+    if (int rc = ce_jit_layout(bytecode, n_instructions, width, n_constants, n_constraints, CE_LDS_BYTES, L))
+        return rc;
+    src = ce_jit_source(L, std::vector<uint64_t>(constants, constants + n_constants));
+    return ZKG_OK;
+}
+
+extern "C" int zkg_constraint_jit_source(uint32_t width, const uint32_t *bytecode, uint32_t n_instructions,
+                                         const uint64_t *constants, uint32_t n_constants, uint32_t n_constraints,
+                                         char *src, uint64_t cap, uint64_t *size)
+{
+    if (!size || (!src && cap)) return set_err(ZKG_EINVAL, "null argument");
+    std::string text;
+    if (int rc = ce_host_jit(width, bytecode, n_instructions, constants, n_constants, n_constraints, text)) return rc;
+    *size = text.size() + 1;
+    if (cap) snprintf(src, cap, "%s", text.c_str());
+    if (cap && cap < text.size() + 1) return set_err(ZKG_EINVAL, "source needs %zu bytes", text.size() + 1);
+    return ZKG_OK;
+}
+
+extern "C" int zkg_constraint_compile_check(uint32_t width, const uint32_t *bytecode, uint32_t n_instructions,
+                                            const uint64_t *constants, uint32_t n_constants, uint32_t n_constraints,
+                                            char *log, uint32_t log_cap)
+{
+    if (!log && log_cap) return set_err(ZKG_EINVAL, "null argument");
+    std::string src, text;
+    if (int rc = ce_host_jit(width, bytecode, n_instructions, constants, n_constants, n_constraints, src)) return rc;
+    std::vector<char> code;
+    const int rc = ce_jit_compile(src, code, text);
+    if (log_cap) snprintf(log, log_cap, "%s", text.c_str());
     return rc;
 }

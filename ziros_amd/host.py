@@ -368,27 +368,56 @@ class ConstraintCompiler:
         return self.bytecode, self.constants, self.n_constraints
 
 
+_CE_INFO = ("words", "slots", "stages", "stack_slots", "wave_bytes", "panel", "rows_per_lane", "col_stride")
+
+
 def translate_constraints(width, bytecode, n_constants, n_constraints):
     """The host translation libzkg runs before evaluation (no device needed):
     (device words, info dict)."""
     lib = _lib.load()
     bc = np.ascontiguousarray(bytecode, dtype=np.uint32)
-    info = np.zeros(6, dtype=np.uint32)
+    info = np.zeros(8, dtype=np.uint32)
     rc = lib.zkg_constraint_translate(width, _ptr(bc), bc.size, n_constants, n_constraints, None, 0, _ptr(info))
     if rc != 0 and info[0] == 0:
         check(rc)
     words = np.zeros(int(info[0]), dtype=np.uint32)
     check(lib.zkg_constraint_translate(width, _ptr(bc), bc.size, n_constants, n_constraints, _ptr(words),
                                        words.size, _ptr(info)))
-    keys = ("words", "slots", "stages", "stack_slots", "wave_bytes", "panel")
-    return words, dict(zip(keys, (int(v) for v in info)))
+    return words, dict(zip(_CE_INFO, (int(v) for v in info)))
+
+
+def compile_check_constraints(width, bytecode, constants, n_constraints):
+    """Translate and compile a program on the host (no device); returns the
+    compiler log, raises ZkgError if it does not compile."""
+    lib = _lib.load()
+    bc = np.ascontiguousarray(bytecode, dtype=np.uint32)
+    cs = np.ascontiguousarray(constants, dtype=np.uint64) if len(constants) else np.zeros(1, np.uint64)
+    log = ctypes.create_string_buffer(4096)
+    check(lib.zkg_constraint_compile_check(width, _ptr(bc), bc.size, _ptr(cs), len(constants), n_constraints, log,
+                                           len(log)))
+    return log.value.decode()
+
+
+def constraint_source(width, bytecode, constants, n_constraints):
+    """The HIP source a compiled program is built from."""
+    lib = _lib.load()
+    bc = np.ascontiguousarray(bytecode, dtype=np.uint32)
+    cs = np.ascontiguousarray(constants, dtype=np.uint64) if len(constants) else np.zeros(1, np.uint64)
+    n = ctypes.c_uint64()
+    check(lib.zkg_constraint_jit_source(width, _ptr(bc), bc.size, _ptr(cs), len(constants), n_constraints, None, 0,
+                                        ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value)
+    check(lib.zkg_constraint_jit_source(width, _ptr(bc), bc.size, _ptr(cs), len(constants), n_constraints, buf,
+                                        n.value, ctypes.byref(n)))
+    return buf.value.decode()
 
 
 class ConstraintProgram:
     """A checked, translated program resident on the device; evaluates any
-    number of device traces of its width."""
+    number of device traces of its width.  compiled=True also compiles it to
+    a straight-line kernel (once, about a second) that later evals run."""
 
-    def __init__(self, ctx, width, bytecode, constants, n_constraints):
+    def __init__(self, ctx, width, bytecode, constants, n_constraints, compiled=False):
         self.ctx, self.width, self.n_constraints = ctx, width, n_constraints
         bc = np.ascontiguousarray(bytecode, dtype=np.uint32)
         cs = np.ascontiguousarray(constants, dtype=np.uint64) if len(constants) else np.zeros(1, np.uint64)
@@ -396,11 +425,16 @@ class ConstraintProgram:
         check(ctx.lib.zkg_constraint_program_create(ctx.h, width, _ptr(bc), bc.size, _ptr(cs), len(constants),
                                                     n_constraints, ctypes.byref(h)))
         self.h = h
+        if compiled:
+            self.compile()
+
+    def compile(self):
+        check(self.ctx.lib.zkg_constraint_program_compile(self.h))
 
     def info(self):
-        a = np.zeros(6, dtype=np.uint32)
+        a = np.zeros(8, dtype=np.uint32)
         check(self.ctx.lib.zkg_constraint_program_info(self.h, _ptr(a)))
-        return dict(zip(("words", "slots", "stages", "stack_slots", "wave_bytes", "panel"), (int(v) for v in a)))
+        return dict(zip(_CE_INFO, (int(v) for v in a)))
 
     def eval(self, trace_buf, n_rows, out_buf):
         check(self.ctx.lib.zkg_constraint_program_eval(self.h, trace_buf.h, n_rows, out_buf.h))
