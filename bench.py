@@ -20,6 +20,10 @@ Also reported on the same line:
                 heights 2^U{14..22}, widths U{16..128}, blowup 2 or 4,
                 Poisson arrivals): p50/p99 latency and cells/s vs one plain
                 call per request;
+  constraint_eval  the reference's constraint bytecode (32 degree-3
+                constraints) over a Goldilocks 2^21 x 64 LDE, interpreted and
+                compiled to a straight-line kernel, against HBM and the
+                measured Goldilocks product ceiling;
   cpu_baseline  the CPU oracle (C restatement of the reference CPU path),
                 multithreaded over the host cores, on a bounded sample.
 """
@@ -210,6 +214,66 @@ def bench_deep(ctx, steps):
                 achieved_GBps=alg / dt / 1e9, hbm_frac=alg / dt / HBM_PEAK)
 
 
+def air_program(width, n_c, seed):
+    """A degree-3 round-function AIR: constraint k is x^3 + a_k y + b_k - u v over
+    columns 2k .. 2k+3 (mod width), built with the reference's bytecode builder."""
+    rng = np.random.default_rng(seed)
+    p = 0xFFFFFFFF00000001
+    cc = Z.ConstraintCompiler()
+    for k in range(n_c):
+        x, y, u, v = (2 * k) % width, (2 * k + 1) % width, (2 * k + 2) % width, (2 * k + 3) % width
+        cc.load_column(x); cc.dup(); cc.dup(); cc.mul(); cc.mul()
+        cc.push_const(int(rng.integers(0, 2**63, dtype=np.uint64)) % p); cc.load_column(y); cc.mul(); cc.add()
+        cc.push_const(int(rng.integers(0, 2**63, dtype=np.uint64)) % p); cc.add()
+        cc.load_column(u); cc.load_column(v); cc.mul(); cc.sub()
+        cc.emit(k)
+    return cc.finish()
+
+
+CE_SHAPE = (1 << 20, 64, 1, 32)  # trace rows, width, log blowup, constraints
+
+
+def bench_constraints(ctx, steps, peak_gl):
+    """Constraint evaluation over a committed Goldilocks LDE (2^20 x 64 trace at
+    blowup 2: 2^21 rows, bit-reversed on the coset 7*H), 32 degree-3
+    constraints, interpreted and compiled.  Algorithmic bytes per evaluation:
+    every LDE row read once and every constraint value written once,
+    rows x (64 + 32) x 8; field products: 4 per constraint per row."""
+    n, w, lb, n_c = CE_SHAPE
+    h = n << lb
+    bc, consts, n_c = air_program(w, n_c, SEED_C2 + 7)
+    tr = ctx.random(Z.GOLDILOCKS, n * w, SEED_C2 + 8)
+    lde = ctx.alloc(h * w * 8)
+    check(ctx.lib.zkg_coset_lde_batch(ctx.h, Z.GOLDILOCKS, tr.h, n, w, lb, 7, lde.h))
+    out = ctx.alloc(h * n_c * 8)
+    alg = h * (w + n_c) * 8
+    products = h * n_c * 4
+    res = dict(lde_rows=h, width=w, constraints=n_c, instructions=len(bc), algorithmic_bytes=alg,
+               gl_products=products)
+    for mode in ("interpreted", "compiled"):
+        t0 = time.perf_counter()
+        prog = Z.ConstraintProgram(ctx, w, bc, consts, n_c, compiled=(mode == "compiled"))
+        prep = time.perf_counter() - t0
+        for _ in range(2):
+            prog.eval(lde, h, out)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            prog.eval(lde, h, out)
+        ctx.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        info = prog.info()
+        prog.free()
+        res[mode] = dict(ms=dt * 1e3, rows_per_s=h / dt, achieved_GBps=alg / dt / 1e9, hbm_frac=alg / dt / HBM_PEAK,
+                         gl_products_per_s=products / dt, gl_product_frac=products / dt / peak_gl,
+                         prepare_s=prep, shape="%d rows/lane, %d-column panels" % (info["rows_per_lane"],
+                                                                                   info["panel"]))
+    for b in (tr, lde, out):
+        b.free()
+    res["compiled_vs_interpreted"] = res["interpreted"]["ms"] / res["compiled"]["ms"]
+    return res
+
+
 def c3_shapes():
     rng = np.random.default_rng(SEED_C3)
     logs = rng.integers(12, 21, 48)
@@ -370,11 +434,22 @@ def cpu_baseline(log_rows=20, width=64):
                 break
     except OSError:
         pass
+    # constraint evaluation: the oracle's stack machine on 1/8 of the LDE rows
+    ce_rows = (CE_SHAPE[0] << CE_SHAPE[2]) // 8
+    bc, consts, n_c = air_program(CE_SHAPE[1], CE_SHAPE[3], SEED_C2 + 7)
+    ce_tr = O.gen(O.GL, SEED_C2 + 9, ce_rows * CE_SHAPE[1]).reshape(ce_rows, CE_SHAPE[1])
+    t1 = time.perf_counter()
+    O.constraint_eval_gl(ce_tr, bc, consts, n_c)
+    ce_dt = time.perf_counter() - t1
     cores = O.get_threads()
     share = "the whole config-2 workload" if n == 1 << 20 else "1/%d of the config-2 workload" % ((1 << 20) // n)
     return dict(value=n * width / dt, unit="cells/s", cores=cores, kind="port",
                 sample="one BabyBear %d x %d commit (coset LDE blowup 2, Poseidon2-16 rows, Merkle) = %s; %s; "
-                       "%.1f s wall, about %.0f core-seconds" % (n, width, share, cpu, dt, dt * cores))
+                       "%.1f s wall, about %.0f core-seconds" % (n, width, share, cpu, dt, dt * cores),
+                constraint_eval=dict(rows_per_s=ce_rows / ce_dt, cores=cores, kind="port",
+                                     sample="%d of the %d LDE rows x %d columns, %d constraints (%d instructions); "
+                                            "%.2f s wall" % (ce_rows, CE_SHAPE[0] << CE_SHAPE[2], CE_SHAPE[1], n_c,
+                                                             len(bc), ce_dt)))
 
 
 def main():
@@ -385,6 +460,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--only-constraints", action="store_true",
+                    help="run only the constraint-evaluation leg (for profiling) and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-load", type=float, default=1.5,
                     help="Poisson arrival rate as a multiple of the plain path's capacity")
@@ -398,12 +475,17 @@ def main():
         ctx.load_constants(field, width, 42)
     log("probe: VALU ceilings")
     peak_bb = probe_peak(0)
+    peak_gl = probe_peak(1)
     peak_mul32 = probe_peak(2)
     peak_bbf = probe_peak(4)
     copy_gbps = probe_copy()
-    log("probe: %.3e BabyBear Montgomery products/s, %.3e BabyBear FP64 products/s, %.3e mul_lo_u32/s, "
-        "copy %.0f GB/s" % (peak_bb, peak_bbf, peak_mul32, copy_gbps))
+    log("probe: %.3e BabyBear Montgomery products/s, %.3e BabyBear FP64 products/s, %.3e Goldilocks products/s, "
+        "%.3e mul_lo_u32/s, copy %.0f GB/s" % (peak_bb, peak_bbf, peak_gl, peak_mul32, copy_gbps))
     peak_bb = max(peak_bb, peak_bbf)
+    if args.only_constraints:
+        print(json.dumps({"constraint_eval": bench_constraints(ctx, max(2, args.steps), peak_gl)}))
+        ctx.close()
+        return
     log("c2: %d warmup + %d timed steps" % (args.warmup, args.steps))
     c2 = bench_c2(ctx, args.steps, args.warmup)
     n, w, lb = c2["n"], c2["w"], c2["lb"]
@@ -447,6 +529,7 @@ def main():
         "plain": {"cells_per_s": cells / c2["plain_sec"], "ms_per_commit": c2["plain_sec"] * 1e3,
                   "fused_vs_plain": c2["plain_sec"] / c2["sec_per_step"]},
         "probe": {"bb_product_ceiling_per_s": peak_bb, "bb_fp64_products_per_s": peak_bbf,
+                  "gl_products_per_s": peak_gl,
                   "mul_lo_u32_per_s": peak_mul32, "copy_1GiB_GBps": copy_gbps},
     }
     log("c1: goldilocks 2^14 x 8")
@@ -458,6 +541,9 @@ def main():
     log("deep: PCS opening DEEP quotient over the config-2 LDE")
     out["pcs_open_deep"] = bench_deep(ctx, max(5, args.steps))
     log("deep: %s" % out["pcs_open_deep"])
+    log("constraints: 32 degree-3 constraints over a Goldilocks 2^21 x 64 LDE")
+    out["constraint_eval"] = bench_constraints(ctx, max(5, args.steps), peak_gl)
+    log("constraints: %s" % out["constraint_eval"])
     if not args.skip_batch:
         log("c3: batched 48 traces")
         out["batched"] = bench_c3(ctx, max(1, args.steps // 4), 1)
