@@ -102,11 +102,30 @@ struct Gl {
         hi = (uint64_t)a1 * b1 + ((t >> 32) + (u >> 32));
         lo = ((uint64_t)(uint32_t)u << 32) | (uint32_t)p00;
     }
+    // 128-bit square of any u64 with three partial products:
+    // a0^2 + a0 a1 2^33 + a1^2 2^64.  lo = a0^2 + (a0 a1 << 33) mod 2^64 and
+    // hi = a1^2 + (a0 a1 >> 31) + carry is the exact high word (< 2^64).
+    __host__ __device__ static inline void sqr_wide(T a, uint64_t &lo, uint64_t &hi)
+    {
+        const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+        const uint64_t p00 = (uint64_t)a0 * a0;
+        const uint64_t m = (uint64_t)a0 * a1;
+        uint32_t c;
+        lo = add_c(p00, m << 33, c);
+        hi = (uint64_t)a1 * a1 + ((m >> 31) + c);
+    }
     __host__ __device__ static inline T mul(T a, T b)
     {
         uint64_t lo, hi;
         mul_wide(a, b, lo, hi);
         return reduce128(lo, hi);
+    }
+    // a^2 for any u64 representative, result a u64 representative
+    __host__ __device__ static inline T sqr_lazy(T a)
+    {
+        uint64_t lo, hi;
+        sqr_wide(a, lo, hi);
+        return reduce128_lazy(lo, hi);
     }
     // a b for any u64 representatives, result a u64 representative
     __host__ __device__ static inline T mul_lazy(T a, T b)

@@ -143,9 +143,9 @@ __host__ __device__ __forceinline__ void mds_light_gl(uint64_t (&s)[W])
 
 __host__ __device__ __forceinline__ uint64_t sbox7_gl(uint64_t x)
 {
-    const uint64_t x2 = Gl::mul_lazy(x, x);
+    const uint64_t x2 = Gl::sqr_lazy(x);
     const uint64_t x3 = Gl::mul_lazy(x2, x);
-    const uint64_t x4 = Gl::mul_lazy(x2, x2);
+    const uint64_t x4 = Gl::sqr_lazy(x2);
     return Gl::mul_lazy(x3, x4);
 }
 
