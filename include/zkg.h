@@ -214,10 +214,20 @@ int zkg_constraint_program_info(const zkg_constraint_program *prog, uint32_t inf
  * interpreter, with identical results.  ZKG_EDEVICE carries the compiler log. */
 int zkg_constraint_program_compile(zkg_constraint_program *prog);
 void zkg_constraint_program_free(zkg_constraint_program *prog);
-/* One-shot form of the three calls above; returns when out is written. */
+/* One-shot form of the calls above; returns when out is written.  The
+ * context remembers the programs it is given (by content), so a program passed
+ * again skips the check and upload, and is compiled on its second use; results
+ * are the same either way. */
 int zkg_constraint_eval_gl(zkg_ctx *ctx, const zkg_buf *trace, uint64_t n_rows, uint32_t width,
                            const uint32_t *bytecode, uint32_t n_instructions, const uint64_t *constants,
                            uint32_t n_constants, uint32_t n_constraints, zkg_buf *out);
+/* The context's program memory for zkg_constraint_eval_gl: keep up to
+ * max_programs (least recently used dropped; 0 turns it off) and compile a
+ * program on its compile_after_uses-th use (0: never).  Default 16 and 2.
+ * Reconfiguring drops the remembered programs. */
+int zkg_constraint_cache_config(zkg_ctx *ctx, uint32_t max_programs, uint32_t compile_after_uses);
+/* info: programs remembered, of those compiled, total uses. */
+int zkg_constraint_cache_info(const zkg_ctx *ctx, uint32_t info[3]);
 /* The translation alone, on the host with no device: writes up to cap device
  * words and the info array of zkg_constraint_program_info (info[0] = words
  * needed; ZKG_EINVAL if cap is smaller). */

@@ -238,3 +238,33 @@ def test_compiled_air_program_and_sessions(ctx):
         bc += [I(O.OP_LOAD, k % 6), I(O.OP_CONST, 0), I(O.OP_MUL), I(O.OP_EMIT, j)]
     trace = _trace(rng, 777, 6)
     assert (_run_compiled(ctx, trace, bc, [5], 50) == O.constraint_eval_gl(trace, bc, [5], 50)).all()
+
+
+def test_one_shot_entry_remembers_and_compiles(ctx):
+    """The reference-shaped call with the same program again: the context
+    compiles it on the second use; every result matches the oracle."""
+    ev = Z.HipConstraintEval(ctx)
+    ev.cache_config(4, 2)
+    rng = np.random.default_rng(44)
+    w = 12
+    bc, consts, n_c = air_program(w, 5, rng)
+    other, oconsts, on_c = air_program(w, 3, rng)
+    for k in range(3):
+        trace = _trace(rng, 1500 + 100 * k, w)
+        got = ev.eval_trace_goldilocks(trace.reshape(-1), w, bc, consts, n_c).reshape(-1, n_c)
+        assert (got == O.constraint_eval_gl(trace, bc, consts, n_c)).all()
+        info = ev.cache_info()
+        assert info["programs"] == 1 and info["uses"] == k + 1 and info["compiled"] == (1 if k >= 1 else 0)
+    for _ in range(6):  # other programs push it out (4 kept, least recently used dropped)
+        other = other + [I(O.OP_LOAD, 0), I(O.OP_EMIT, 0)]
+        trace = _trace(rng, 1024, w)
+        got = ev.eval_trace_goldilocks(trace.reshape(-1), w, other, oconsts, on_c).reshape(-1, on_c)
+        assert (got == O.constraint_eval_gl(trace, other, oconsts, on_c)).all()
+    assert ev.cache_info()["programs"] == 4
+    ev.cache_config(0, 0)
+    assert ev.cache_info()["programs"] == 0
+    trace = _trace(rng, 2000, w)
+    got = ev.eval_trace_goldilocks(trace.reshape(-1), w, bc, consts, n_c).reshape(-1, n_c)
+    assert (got == O.constraint_eval_gl(trace, bc, consts, n_c)).all()
+    assert ev.cache_info()["programs"] == 0
+    ev.cache_config()

@@ -74,6 +74,8 @@ SIGNATURES = {
     "zkg_constraint_jit_source": (c_int, [c_u32, c_vp, c_u32, c_vp, c_u32, c_u32, ctypes.c_char_p, c_u64, P(c_u64)]),
     "zkg_constraint_compile_check": (c_int, [c_u32, c_vp, c_u32, c_vp, c_u32, c_u32, ctypes.c_char_p, c_u32]),
     "zkg_constraint_eval_gl": (c_int, [c_vp, c_vp, c_u64, c_u32, c_vp, c_u32, c_vp, c_u32, c_u32, c_vp]),
+    "zkg_constraint_cache_config": (c_int, [c_vp, c_u32, c_u32]),
+    "zkg_constraint_cache_info": (c_int, [c_vp, c_vp]),
     "zkg_constraint_translate": (c_int, [c_u32, c_vp, c_u32, c_u32, c_u32, c_vp, c_u32, c_vp]),
     "zkg_tree_free": (c_int, [c_vp]),
     "zkg_commit": (c_int, [c_vp, c_int, c_int, c_vp, c_u64, c_u32, c_u32, c_vp, P(c_vp)]),

@@ -784,19 +784,120 @@ extern "C" void zkg_constraint_program_free(zkg_constraint_program *p)
     delete p;
 }
 
+// ------------------------------------------------------------------ per-context program cache
+// zkg_constraint_eval_gl takes the program with every call, as the reference's
+// accelerator does; a prover passes the same AIR program for every proof.  The
+// context keeps the last `max_programs` programs (least recently used out) and
+// compiles one once it has been seen `compile_after` times.  A program that
+// fails to compile keeps running interpreted.
+struct zkg::CeCache {
+    struct Entry {
+        uint32_t width = 0, n_out = 0;
+        zkg_constraint_program *prog = nullptr;
+        uint32_t uses = 0;
+        uint64_t last = 0;
+        bool no_compile = false;
+    };
+    uint32_t max_programs = 16, compile_after = 2;
+    uint64_t tick = 0;
+    std::vector<Entry> entries;
+};
+
+void zkg::ce_cache_release(zkg_ctx *ctx)
+{
+    if (!ctx->ce_cache) return;
+    for (auto &e : ctx->ce_cache->entries) zkg_constraint_program_free(e.prog);
+    delete ctx->ce_cache;
+    ctx->ce_cache = nullptr;
+}
+
+extern "C" int zkg_constraint_cache_config(zkg_ctx *ctx, uint32_t max_programs, uint32_t compile_after_uses)
+{
+    if (!ctx) return set_err(ZKG_EINVAL, "null ctx");
+    (void)hipStreamSynchronize(ctx->stream);
+    ce_cache_release(ctx);
+    ctx->ce_cache_set = true;
+    if (max_programs) {
+        ctx->ce_cache = new CeCache;
+        ctx->ce_cache->max_programs = max_programs;
+        ctx->ce_cache->compile_after = compile_after_uses;
+    }
+    return ZKG_OK;
+}
+
+extern "C" int zkg_constraint_cache_info(const zkg_ctx *ctx, uint32_t info[3])
+{
+    if (!ctx || !info) return set_err(ZKG_EINVAL, "null argument");
+    info[0] = info[1] = info[2] = 0;
+    if (const CeCache *c = ctx->ce_cache) {
+        info[0] = (uint32_t)c->entries.size();
+        for (const auto &e : c->entries) {
+            info[1] += e.prog->fn ? 1 : 0;
+            info[2] += e.uses;
+        }
+    }
+    return ZKG_OK;
+}
+
+static zkg_constraint_program *ce_cache_find(CeCache &c, uint32_t width, const uint32_t *bc, uint32_t n,
+                                             const uint64_t *k, uint32_t nk, uint32_t n_out)
+{
+    for (auto &e : c.entries) {
+        const zkg_constraint_program *p = e.prog;
+        if (e.width == width && e.n_out == n_out && p->bytecode.size() == n && p->consts.size() == nk &&
+            std::equal(bc, bc + n, p->bytecode.begin()) && std::equal(k, k + nk, p->consts.begin())) {
+            e.uses++;
+            e.last = ++c.tick;
+            if (!e.prog->fn && !e.no_compile && c.compile_after && e.uses >= c.compile_after &&
+                zkg_constraint_program_compile(e.prog) != ZKG_OK)
+                e.no_compile = true;
+            return e.prog;
+        }
+    }
+    return nullptr;
+}
+
 extern "C" int zkg_constraint_eval_gl(zkg_ctx *ctx, const zkg_buf *trace, uint64_t n_rows, uint32_t width,
                                       const uint32_t *bytecode, uint32_t n_instructions, const uint64_t *constants,
                                       uint32_t n_constants, uint32_t n_constraints, zkg_buf *out)
 {
+    if (!ctx) return set_err(ZKG_EINVAL, "null ctx");
+    if (!ctx->ce_cache_set) {
+        ctx->ce_cache = new CeCache;  // 16 programs, compiled on the second use
+        ctx->ce_cache_set = true;
+    }
     zkg_constraint_program *p = nullptr;
-    int rc = zkg_constraint_program_create(ctx, width, bytecode, n_instructions, constants, n_constants,
-                                           n_constraints, &p);
+    bool owned = true;
+    int rc = ZKG_OK;
+    if (ctx->ce_cache && bytecode && (constants || !n_constants)) {
+        p = ce_cache_find(*ctx->ce_cache, width, bytecode, n_instructions, constants, n_constants, n_constraints);
+        owned = p == nullptr;
+    }
+    if (!p) rc = zkg_constraint_program_create(ctx, width, bytecode, n_instructions, constants, n_constants,
+                                               n_constraints, &p);
+    if (!rc && owned && ctx->ce_cache) {
+        CeCache &c = *ctx->ce_cache;
+        if (c.entries.size() >= c.max_programs) {
+            auto lru = std::min_element(c.entries.begin(), c.entries.end(),
+                                        [](const CeCache::Entry &a, const CeCache::Entry &b) { return a.last < b.last; });
+            zkg_constraint_program_free(lru->prog);
+            c.entries.erase(lru);
+        }
+        CeCache::Entry e;
+        e.width = width;
+        e.n_out = n_constraints;
+        e.prog = p;
+        e.uses = 1;
+        e.last = ++c.tick;
+        c.entries.push_back(e);
+        owned = false;
+    }
     if (!rc) rc = zkg_constraint_program_eval(p, trace, n_rows, out);
     if (!rc) {
         hipError_t e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) rc = hip_err(e, "constraint eval");
     }
-    zkg_constraint_program_free(p);
+    if (owned) zkg_constraint_program_free(p);
     return rc;
 }
 

@@ -30,6 +30,7 @@ struct CosetKey {
 
 struct BatchState;  // zkg_batch.hip
 struct ServeState;  // zkg_batch.hip
+struct CeCache;     // zkg_constraints.hip
 
 }  // namespace zkg
 
@@ -53,6 +54,8 @@ struct zkg_ctx {
     double stage_ms[4] = {0, 0, 0, 0};
     zkg::BatchState *batch = nullptr;
     zkg::ServeState *serve = nullptr;
+    zkg::CeCache *ce_cache = nullptr;  // constraint programs seen by zkg_constraint_eval_gl
+    bool ce_cache_set = false;          // the default cache is made on first use unless configured
 };
 
 struct zkg_buf {
@@ -107,5 +110,6 @@ void stage_mark(zkg_ctx *ctx, hipStream_t st, int i);
 void stage_collect(zkg_ctx *ctx, int nmarks);
 void batch_release(zkg_ctx *ctx);
 void serve_shutdown(zkg_ctx *ctx);
+void ce_cache_release(zkg_ctx *ctx);
 
 }  // namespace zkg

@@ -474,6 +474,7 @@ extern "C" int zkg_ctx_destroy(zkg_ctx *ctx)
     serve_shutdown(ctx);
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    ce_cache_release(ctx);
     batch_release(ctx);
     std::vector<void *> ptrs;
     for (auto &kv : ctx->alloc_sizes) ptrs.push_back(kv.first);

@@ -475,6 +475,15 @@ class HipConstraintEval:
     def min_rows(self):
         return self.MIN_ROWS
 
+    def cache_config(self, max_programs=16, compile_after_uses=2):
+        """Program memory of the context (zkg_constraint_cache_config)."""
+        check(self.ctx.lib.zkg_constraint_cache_config(self.ctx.h, max_programs, compile_after_uses))
+
+    def cache_info(self):
+        a = np.zeros(3, dtype=np.uint32)
+        check(self.ctx.lib.zkg_constraint_cache_info(self.ctx.h, _ptr(a)))
+        return dict(programs=int(a[0]), compiled=int(a[1]), uses=int(a[2]))
+
     def eval_trace_goldilocks(self, trace, width, bytecode, constants, n_constraints):
         t = np.ascontiguousarray(trace, dtype=np.uint64).reshape(-1)
         if width == 0 or t.size == 0 or t.size % width:
