@@ -216,8 +216,9 @@ int zkg_constraint_program_compile(zkg_constraint_program *prog);
 void zkg_constraint_program_free(zkg_constraint_program *prog);
 /* One-shot form of the calls above; returns when out is written.  The
  * context remembers the programs it is given (by content), so a program passed
- * again skips the check and upload, and is compiled on its second use; results
- * are the same either way. */
+ * again skips the check and upload, and is compiled on its second use (up to
+ * 16384 instructions; longer ones stay interpreted unless compiled through a
+ * handle); results are the same either way. */
 int zkg_constraint_eval_gl(zkg_ctx *ctx, const zkg_buf *trace, uint64_t n_rows, uint32_t width,
                            const uint32_t *bytecode, uint32_t n_instructions, const uint64_t *constants,
                            uint32_t n_constants, uint32_t n_constraints, zkg_buf *out);

@@ -268,3 +268,23 @@ def test_one_shot_entry_remembers_and_compiles(ctx):
     assert (got == O.constraint_eval_gl(trace, bc, consts, n_c)).all()
     assert ev.cache_info()["programs"] == 0
     ev.cache_config()
+
+
+@pytest.mark.parametrize("compiled", [False, True])
+def test_wide_trace_scattered_columns(ctx, compiled):
+    """1000 columns read in a scattered order: panels cycle through the slots."""
+    rng = np.random.default_rng(55)
+    w, rows = 1000, 700
+    trace = _trace(rng, rows, w)
+    cols = rng.permutation(w)[:200]
+    bc = []
+    for k in range(40):
+        a, b, c = (int(x) for x in cols[5 * k:5 * k + 3])
+        bc += [I(O.OP_LOAD, a), I(O.OP_LOAD, b), I(O.OP_MUL), I(O.OP_LOAD, c), I(O.OP_SUB), I(O.OP_EMIT, k)]
+    bc += [I(O.OP_LOAD, int(cols[(7 * i) % 200])) for i in range(20)] + [I(O.OP_ADD)] * 19 + [I(O.OP_EMIT, 40)]
+    exp = O.constraint_eval_gl(trace, bc, [], 41)
+    got = _run_compiled(ctx, trace, bc, [], 41) if compiled else _run(ctx, trace, bc, [], 41)[0]
+    assert (got == exp).all()
+    if not compiled:
+        info = Z.translate_constraints(w, bc, 0, 41)[1]
+        assert info["stages"] >= 100  # scattered columns: almost every read stages a panel

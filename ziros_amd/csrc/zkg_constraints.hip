@@ -788,8 +788,12 @@ extern "C" void zkg_constraint_program_free(zkg_constraint_program *p)
 // zkg_constraint_eval_gl takes the program with every call, as the reference's
 // accelerator does; a prover passes the same AIR program for every proof.  The
 // context keeps the last `max_programs` programs (least recently used out) and
-// compiles one once it has been seen `compile_after` times.  A program that
-// fails to compile keeps running interpreted.
+// compiles one once it has been seen `compile_after` times, if it has at most
+// CE_AUTO_COMPILE_MAX instructions (compile time grows with the program; a
+// longer one can still be compiled explicitly).  A program that fails to
+// compile keeps running interpreted.
+constexpr size_t CE_AUTO_COMPILE_MAX = 16384;
+
 struct zkg::CeCache {
     struct Entry {
         uint32_t width = 0, n_out = 0;
@@ -849,7 +853,7 @@ static zkg_constraint_program *ce_cache_find(CeCache &c, uint32_t width, const u
             e.uses++;
             e.last = ++c.tick;
             if (!e.prog->fn && !e.no_compile && c.compile_after && e.uses >= c.compile_after &&
-                zkg_constraint_program_compile(e.prog) != ZKG_OK)
+                (n > CE_AUTO_COMPILE_MAX || zkg_constraint_program_compile(e.prog) != ZKG_OK))
                 e.no_compile = true;
             return e.prog;
         }
