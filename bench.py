@@ -238,7 +238,9 @@ def bench_constraints(ctx, steps, peak_gl):
     blowup 2: 2^21 rows, bit-reversed on the coset 7*H), 32 degree-3
     constraints, interpreted and compiled.  Algorithmic bytes per evaluation:
     every LDE row read once and every constraint value written once,
-    rows x (64 + 32) x 8; field products: 4 per constraint per row."""
+    rows x (64 + 32) x 8; field products: 4 per constraint per row.  The
+    quotient (constraints combined with 32 coefficients and divided by the
+    vanishing polynomial, one value per row) moves rows x (64 + 1) x 8 bytes."""
     n, w, lb, n_c = CE_SHAPE
     h = n << lb
     bc, consts, n_c = air_program(w, n_c, SEED_C2 + 7)
@@ -250,6 +252,10 @@ def bench_constraints(ctx, steps, peak_gl):
     products = h * n_c * 4
     res = dict(lde_rows=h, width=w, constraints=n_c, instructions=len(bc), algorithmic_bytes=alg,
                gl_products=products)
+    q = ctx.alloc(h * 8)
+    rng = np.random.default_rng(SEED_C2 + 10)
+    coef = [int(v) % 0xFFFFFFFF00000001 for v in rng.integers(0, 2**63, n_c, dtype=np.uint64)]
+    alg_q = h * (w + 1) * 8
     for mode in ("interpreted", "compiled"):
         t0 = time.perf_counter()
         prog = Z.ConstraintProgram(ctx, w, bc, consts, n_c, compiled=(mode == "compiled"))
@@ -262,14 +268,23 @@ def bench_constraints(ctx, steps, peak_gl):
             prog.eval(lde, h, out)
         ctx.synchronize()
         dt = (time.perf_counter() - t0) / steps
+        # quotient: the same constraints combined with coefficients and divided
+        # by Z_H, one value per row (each call returns when q is written)
+        prog.quotient(lde, h, lb, 7, coef, q)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            prog.quotient(lde, h, lb, 7, coef, q)
+        dq = (time.perf_counter() - t0) / steps
         info = prog.info()
         prog.free()
         res[mode] = dict(ms=dt * 1e3, rows_per_s=h / dt, achieved_GBps=alg / dt / 1e9, hbm_frac=alg / dt / HBM_PEAK,
                          gl_products_per_s=products / dt, gl_product_frac=products / dt / peak_gl,
                          prepare_s=prep, shape="%d rows/lane, %d-column panels" % (info["rows_per_lane"],
-                                                                                   info["panel"]))
-    for b in (tr, lde, out):
+                                                                                   info["panel"]),
+                         quotient_ms=dq * 1e3, quotient_rows_per_s=h / dq, quotient_GBps=alg_q / dq / 1e9)
+    for b in (tr, lde, out, q):
         b.free()
+    res["quotient_algorithmic_bytes"] = alg_q
     res["compiled_vs_interpreted"] = res["interpreted"]["ms"] / res["compiled"]["ms"]
     return res
 

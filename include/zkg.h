@@ -209,9 +209,22 @@ int zkg_constraint_program_eval(zkg_constraint_program *prog, const zkg_buf *tra
  * LDS bytes per wave, columns per panel, rows per lane, LDS bytes per panel
  * column. */
 int zkg_constraint_program_info(const zkg_constraint_program *prog, uint32_t info[8]);
-/* Compile the program once into a straight-line kernel for gfx950 (hiprtc,
- * about a second); later evals of this handle run it instead of the
- * interpreter, with identical results.  ZKG_EDEVICE carries the compiler log. */
+/* Quotient values over a committed Goldilocks LDE (height H rows, row i the
+ * trace polynomials at x_i = shift * w_H^rev(i), as zkg_coset_lde_batch /
+ * zkg_commit leave it): q_out[i] = (sum_k coef[k] C_k(row i)) / (x_i^N - 1),
+ * N = H >> log_blowup, C_k the constraint values above, coef the caller's
+ * combination (canonical, n_constraints of them; a prover passes powers of its
+ * challenge alpha in its own order).  Dividing by the trace domain's vanishing
+ * polynomial Z_H is the quotient step of the STARK prover that precedes the
+ * quotient commitment.  A constraint emitted more than once counts with its
+ * last value.  ZKG_EINVAL if the coset meets the trace domain.  Returns when
+ * q_out is written. */
+int zkg_constraint_program_quotient(zkg_constraint_program *prog, const zkg_buf *lde, uint64_t height,
+                                    uint32_t log_blowup, uint64_t shift, const uint64_t *coef, zkg_buf *q_out);
+/* Compile the program once into straight-line kernels for gfx950 (hiprtc,
+ * about a second each, evaluation and quotient); later evals and quotients of
+ * this handle run them instead of the interpreter, with identical results.
+ * ZKG_EDEVICE carries the compiler log. */
 int zkg_constraint_program_compile(zkg_constraint_program *prog);
 void zkg_constraint_program_free(zkg_constraint_program *prog);
 /* One-shot form of the calls above; returns when out is written.  The
@@ -229,18 +242,19 @@ int zkg_constraint_eval_gl(zkg_ctx *ctx, const zkg_buf *trace, uint64_t n_rows, 
 int zkg_constraint_cache_config(zkg_ctx *ctx, uint32_t max_programs, uint32_t compile_after_uses);
 /* info: programs remembered, of those compiled, total uses. */
 int zkg_constraint_cache_info(const zkg_ctx *ctx, uint32_t info[3]);
-/* The translation alone, on the host with no device: writes up to cap device
- * words and the info array of zkg_constraint_program_info (info[0] = words
- * needed; ZKG_EINVAL if cap is smaller). */
+/* The translation alone, on the host with no device (for evaluation, or for
+ * the quotient when quotient != 0): writes up to cap device words and the
+ * info array of zkg_constraint_program_info (info[0] = words needed;
+ * ZKG_EINVAL if cap is smaller). */
 int zkg_constraint_translate(uint32_t width, const uint32_t *bytecode, uint32_t n_instructions, uint32_t n_constants,
-                             uint32_t n_constraints, uint32_t *words, uint32_t cap, uint32_t info[8]);
+                             uint32_t n_constraints, int quotient, uint32_t *words, uint32_t cap, uint32_t info[8]);
 /* The HIP source zkg_constraint_program_compile would compile (*size bytes
  * including the terminator; up to cap written). */
 int zkg_constraint_jit_source(uint32_t width, const uint32_t *bytecode, uint32_t n_instructions,
                               const uint64_t *constants, uint32_t n_constants, uint32_t n_constraints, char *src,
                               uint64_t cap, uint64_t *size);
-/* Translate and compile on the host with no device (the compiler's log, if
- * any, into log). */
+/* Translate and compile both kernels on the host with no device (the
+ * compiler's log, if any, into log). */
 int zkg_constraint_compile_check(uint32_t width, const uint32_t *bytecode, uint32_t n_instructions,
                                  const uint64_t *constants, uint32_t n_constants, uint32_t n_constraints, char *log,
                                  uint32_t log_cap);

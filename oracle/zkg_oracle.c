@@ -1035,6 +1035,33 @@ int or_constraint_eval_gl(const uint64_t *trace, uint64_t n_rows, uint32_t width
     return 0;
 }
 
+int or_constraint_quotient_gl(const uint64_t *lde, uint64_t height, uint32_t width, uint32_t log_blowup,
+                              uint64_t shift, const uint32_t *bytecode, uint32_t n_instr, const uint64_t *constants,
+                              uint32_t n_constants, uint32_t n_constraints, const uint64_t *coef, uint64_t *q)
+{
+    uint64_t *c = (uint64_t *)malloc(height * n_constraints * sizeof(uint64_t));
+    if (or_constraint_eval_gl(lde, height, width, bytecode, n_instr, constants, n_constants, n_constraints, c)) {
+        free(c);
+        return -1;
+    }
+    uint64_t *x = deep_points_f(OR_GOLDILOCKS, height, shift);
+    const uint64_t n = height >> log_blowup;
+    int rc = 0;
+    for (uint64_t i = 0; i < height && !rc; i++) {
+        const uint64_t v = or_gl_sub(or_gl_pow(x[i], n), 1);
+        if (!v) {
+            rc = -1;
+            break;
+        }
+        uint64_t acc = 0;
+        for (uint32_t k = 0; k < n_constraints; k++) acc = or_gl_add(acc, or_gl_mul(coef[k], c[i * n_constraints + k]));
+        q[i] = or_gl_mul(acc, or_f_inv(OR_GOLDILOCKS, v));
+    }
+    free(x);
+    free(c);
+    return rc;
+}
+
 void or_set_threads(int n)
 {
 #ifdef _OPENMP

@@ -173,6 +173,15 @@ void or_deep_gl_reduce(const uint64_t *lde, uint64_t height, uint32_t width, uin
 int or_constraint_eval_gl(const uint64_t *trace, uint64_t n_rows, uint32_t width, const uint32_t *bytecode,
                           uint32_t n_instr, const uint64_t *constants, uint32_t n_constants,
                           uint32_t n_constraints, uint64_t *out);
+/* Quotient values over a committed Goldilocks LDE (height H rows, row i holding
+ * the trace polynomials at x_i = shift * w_H^rev(i), as or_coset_lde_batch
+ * leaves them): q_i = (sum_k coef[k] C_k(row i)) / (x_i^N - 1), N = H >> log_blowup,
+ * with C_k the constraint values above.  The division by the vanishing
+ * polynomial of the trace domain is the quotient step of a STARK prover.
+ * Returns -1 for an invalid program or a vanishing value of zero. */
+int or_constraint_quotient_gl(const uint64_t *lde, uint64_t height, uint32_t width, uint32_t log_blowup,
+                              uint64_t shift, const uint32_t *bytecode, uint32_t n_instr, const uint64_t *constants,
+                              uint32_t n_constants, uint32_t n_constraints, const uint64_t *coef, uint64_t *q);
 
 /* ---- threading ---- */
 void or_set_threads(int n);

@@ -104,6 +104,9 @@ def lib():
         L.or_deep_gl_reduce.argtypes = [c_vp, c_u64, c_u32, c_u64, c_u64, c_u64, c_u64, c_vp, c_vp]
         L.or_constraint_eval_gl.restype = c_int
         L.or_constraint_eval_gl.argtypes = [c_vp, c_u64, c_u32, c_vp, c_u32, c_vp, c_u32, c_u32, c_vp]
+        L.or_constraint_quotient_gl.restype = c_int
+        L.or_constraint_quotient_gl.argtypes = [c_vp, c_u64, c_u32, c_u32, c_u64, c_vp, c_u32, c_vp, c_u32, c_u32,
+                                                c_vp, c_vp]
         L.or_set_threads.argtypes = [c_int]
         L.or_get_threads.restype = c_int
         _lib = L
@@ -211,6 +214,19 @@ def constraint_eval_gl(trace, bytecode, constants, n_constraints):
     rc = lib().or_constraint_eval_gl(_p(t), rows, width, _p(bc), bc.size, _p(cs), len(constants), n_constraints,
                                      _p(out))
     return None if rc else out[:, :n_constraints]
+
+
+def constraint_quotient_gl(lde, log_blowup, shift, bytecode, constants, n_constraints, coef):
+    """q_i = sum_k coef[k] C_k(row i) / (x_i^N - 1) over a bit-reversed coset LDE; None if invalid."""
+    t = u64(lde)
+    rows, width = t.shape
+    bc = np.ascontiguousarray(bytecode, dtype=np.uint32)
+    cs = u64(constants) if len(constants) else np.zeros(1, dtype=np.uint64)
+    cf = u64(coef)
+    q = np.zeros(rows, dtype=np.uint64)
+    rc = lib().or_constraint_quotient_gl(_p(t), rows, width, log_blowup, shift, _p(bc), bc.size, _p(cs),
+                                         len(constants), n_constraints, _p(cf), _p(q))
+    return None if rc else q
 
 
 # ---------------------------------------------------------------- generator

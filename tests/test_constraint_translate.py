@@ -20,29 +20,32 @@ import ziros_amd as Z
 P = O.GL_P
 SHAPES = ["1,16", "2,4", "2,8", "2,16", "4,4", "4,8"]
 (PUSH_L, PUSH_L0, PUSH_K, PUSH_K0, ADD_S, SUB_S, MUL_S, DUP, EMIT, EMIT0, ADD_L, SUB_L, MUL_L, ADD_K, SUB_K, MUL_K,
- DBL, ZERO_T, SQR, STAGE, OCLEAR, OFLUSH, END) = range(23)
+ DBL, ZERO_T, SQR, STAGE, OCLEAR, OFLUSH, ACC, ACC0, DROP, DROP0, END) = range(27)
 
 
 def _bin(k, a, b):
     return (a + b) % P if k == 0 else (a - b) % P if k == 1 else a * b % P
 
 
-def emulate(words, info, trace, consts, n_c):
+def emulate(words, info, trace, consts, n_c, coef=None):
     """Run the device program of the translation on the CPU: word =
     op << 27 | slot << 22 | arg, the stack below the top in LDS entries
     addressed by `slot`, trace panels in LDS slots, EMITs into an LDS output
-    panel that OFLUSH (followed by a column-mask word) writes out."""
+    panel that OFLUSH (followed by a column-mask word) writes out.  With coef
+    (a quotient program) returns the per-row accumulation sum coef[k] C_k."""
     assert words[-1] >> 27 == END
     PANEL, COL_STRIDE, WROWS = info["panel"], info["col_stride"], 64 * info["rows_per_lane"]
     assert COL_STRIDE == WROWS * 8 + 8
     SLOT_BYTES = PANEL * COL_STRIDE
     OUT_OFF = info["slots"] * SLOT_BYTES + info["stack_slots"] * WROWS * 8
-    assert info["wave_bytes"] == (OUT_OFF + SLOT_BYTES + 15) & ~15
+    assert info["wave_bytes"] == (OUT_OFF + (0 if coef is not None else SLOT_BYTES) + 15) & ~15
     rows, width = trace.shape
     out = np.full((rows, n_c), -1, dtype=object)  # -1: never written
+    accs = np.zeros(rows, dtype=object)
     for r0 in range(0, rows, WROWS):
         rr = range(r0, min(r0 + WROWS, rows))
         panels, stack, top = {}, {}, None
+        acc = [0] * len(rr)
         opanel, fresh = {}, set()  # output panel columns; those written since the last flush
 
         def lds(arg):
@@ -80,6 +83,11 @@ def emulate(words, info, trace, consts, n_c):
                 opanel[c] = top
                 fresh.add(c)
                 top = stack.pop(slot) if op == EMIT else None
+            elif op in (ACC, ACC0, DROP, DROP0):
+                assert coef is not None
+                if op in (ACC, ACC0):
+                    acc = [(a + int(coef[arg]) * t) % P for a, t in zip(acc, top)]
+                top = stack.pop(slot) if op in (ACC, DROP) else None
             elif op == OCLEAR:
                 assert arg == OUT_OFF
                 for c in range(PANEL):
@@ -98,7 +106,8 @@ def emulate(words, info, trace, consts, n_c):
             else:
                 raise AssertionError("unknown device op %d" % op)
         assert not stack
-    return out
+        accs[r0:r0 + len(rr)] = acc
+    return accs if coef is not None else out
 
 
 def _trace(rng, rows, width):
@@ -247,3 +256,25 @@ def test_output_panels_flush_in_sessions(shape):
         bc += [I(O.OP_LOAD, k % 6), I(O.OP_CONST, 0), I(O.OP_MUL), I(O.OP_EMIT, j)]
     info = _check(trace, bc, [5], 50)
     assert info["words"] > 0
+
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_quotient_programs(seed, shape):
+    """Quotient-mode translation: the last EMIT of each constraint accumulates
+    coef[k] * value (earlier ones only pop), no output panels."""
+    rng = np.random.default_rng(300 + seed)
+    w = int(rng.choice([2, 17, 40]))
+    n_const, n_c = int(rng.integers(1, 5)), int(rng.integers(1, 6))
+    trace = _trace(rng, int(rng.choice([1, 64, 150])), w)
+    consts = [int(v) % P for v in rng.integers(0, 2**63, n_const, dtype=np.uint64)]
+    bc = random_program(rng, w, n_const, n_c, int(rng.integers(3, 100)))
+    if seed % 2:  # emit some constraints twice: the last value counts
+        bc = bc + [O.instr(O.OP_LOAD, 0), O.instr(O.OP_EMIT, 0)]
+    coef = [int(v) % P for v in rng.integers(0, 2**63, n_c, dtype=np.uint64)]
+    words, info = Z.translate_constraints(w, bc, n_const, n_c, quotient=True)
+    assert not any((int(x) >> 27) in (EMIT, EMIT0, OCLEAR, OFLUSH) for x in words)
+    got = emulate([int(x) for x in words], info, trace, consts, n_c, coef)
+    c = O.constraint_eval_gl(trace, bc, consts, n_c).astype(object)
+    exp = [sum(int(k) * int(v) for k, v in zip(coef, row)) % P for row in c]
+    assert list(got) == exp

@@ -288,3 +288,72 @@ def test_wide_trace_scattered_columns(ctx, compiled):
     if not compiled:
         info = Z.translate_constraints(w, bc, 0, 41)[1]
         assert info["stages"] >= 100  # scattered columns: almost every read stages a panel
+
+
+# ---------------------------------------------------------------- quotient over the LDE
+def _quotient(ctx, lde, log_blowup, shift, bc, consts, n_c, coef, compiled=False):
+    h, w = lde.shape
+    prog = Z.ConstraintProgram(ctx, w, bc, consts, n_c, compiled=compiled)
+    lb, qb = ctx.upload(lde), ctx.alloc(h * 8)
+    qb.upload(np.full(h, 0xDEADBEEF, dtype=np.uint64))
+    prog.quotient(lb, h, log_blowup, shift, coef, qb)
+    q = qb.download(np.uint64, h)
+    prog.free(); lb.free(); qb.free()
+    return q
+
+
+@pytest.mark.parametrize("compiled", [False, True])
+@pytest.mark.parametrize("seed", range(5))
+def test_quotient_random_programs(ctx, seed, compiled, monkeypatch):
+    monkeypatch.setenv("ZKG_CE_SHAPE", SHAPES[seed % len(SHAPES)])
+    rng = np.random.default_rng(700 + seed)
+    log_h, lb = int(rng.integers(6, 15)), int(rng.integers(1, 4))
+    w = int(rng.choice([3, 20, 64]))
+    n_const, n_c = int(rng.integers(1, 5)), int(rng.integers(1, 7))
+    lde = _trace(rng, 1 << log_h, w)
+    consts = [int(v) % P for v in rng.integers(0, 2**63, n_const, dtype=np.uint64)]
+    bc = random_program(rng, w, n_const, n_c, int(rng.integers(3, 120)))
+    if seed % 2:
+        bc = bc + [I(O.OP_LOAD, 0), I(O.OP_EMIT, n_c - 1)]  # emitted twice: the last value counts
+    coef = [int(v) % P for v in rng.integers(0, 2**63, n_c, dtype=np.uint64)]
+    got = _quotient(ctx, lde, lb, 7, bc, consts, n_c, coef, compiled)
+    assert (got == O.constraint_quotient_gl(lde, lb, 7, bc, consts, n_c, coef)).all()
+
+
+@pytest.mark.parametrize("compiled", [False, True])
+def test_quotient_of_a_satisfied_air_is_low_degree(ctx, compiled):
+    """a, b = a^2, c = a^3 on a 2^10-row domain, LDE on the GPU at blowup 2:
+    the quotient of a*a - b and b*a - c has degree N - 2."""
+    from test_oracle_constraint import _degree_bound_holds, _square_air
+    rng = np.random.default_rng(91)
+    log_n, lb = 10, 1
+    trace, bc = _square_air(rng, log_n)
+    n = 1 << log_n
+    tb, lde_b = ctx.upload(trace), ctx.alloc((n << lb) * 3 * 8)
+    check(ctx.lib.zkg_coset_lde_batch(ctx.h, Z.GOLDILOCKS, tb.h, n, 3, lb, 7, lde_b.h))
+    prog = Z.ConstraintProgram(ctx, 3, bc, [], 2, compiled=compiled)
+    qb = ctx.alloc((n << lb) * 8)
+    coef = [123456789, 987654321]
+    prog.quotient(lde_b, n << lb, lb, 7, coef, qb)
+    q = qb.download(np.uint64, n << lb)
+    lde = lde_b.download(np.uint64, (n << lb) * 3).reshape(n << lb, 3)
+    assert (q == O.constraint_quotient_gl(lde, lb, 7, bc, [], 2, coef)).all()
+    assert _degree_bound_holds(q, log_n + lb, n - 1)
+    prog.free(); tb.free(); lde_b.free(); qb.free()
+
+
+def test_quotient_refusals(ctx):
+    bc = [I(O.OP_LOAD, 0), I(O.OP_EMIT, 0)]
+    prog = Z.ConstraintProgram(ctx, 1, bc, [], 1)
+    lb, qb = ctx.alloc(64 * 8), ctx.alloc(64 * 8)
+    with pytest.raises(Z.ZkgError):
+        prog.quotient(lb, 64, 1, 1, [1], qb)       # shift 1: the coset meets the trace domain
+    with pytest.raises(Z.ZkgError):
+        prog.quotient(lb, 48, 1, 7, [1], qb)       # height not a power of two
+    with pytest.raises(Z.ZkgError):
+        prog.quotient(lb, 64, 7, 7, [1], qb)       # blowup above the height
+    with pytest.raises(Z.ZkgError):
+        prog.quotient(lb, 64, 1, 7, [P], qb)       # coefficient not canonical
+    with pytest.raises(Z.ZkgError):
+        prog.quotient(lb, 128, 1, 7, [1], qb)      # buffers too small
+    prog.free(); lb.free(); qb.free()

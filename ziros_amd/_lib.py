@@ -76,7 +76,8 @@ SIGNATURES = {
     "zkg_constraint_eval_gl": (c_int, [c_vp, c_vp, c_u64, c_u32, c_vp, c_u32, c_vp, c_u32, c_u32, c_vp]),
     "zkg_constraint_cache_config": (c_int, [c_vp, c_u32, c_u32]),
     "zkg_constraint_cache_info": (c_int, [c_vp, c_vp]),
-    "zkg_constraint_translate": (c_int, [c_u32, c_vp, c_u32, c_u32, c_u32, c_vp, c_u32, c_vp]),
+    "zkg_constraint_translate": (c_int, [c_u32, c_vp, c_u32, c_u32, c_u32, c_int, c_vp, c_u32, c_vp]),
+    "zkg_constraint_program_quotient": (c_int, [c_vp, c_vp, c_u64, c_u32, c_u64, c_vp, c_vp]),
     "zkg_tree_free": (c_int, [c_vp]),
     "zkg_commit": (c_int, [c_vp, c_int, c_int, c_vp, c_u64, c_u32, c_u32, c_vp, P(c_vp)]),
     "zkg_commit_plain": (c_int, [c_vp, c_int, c_int, c_vp, c_u64, c_u32, c_u32, c_vp, P(c_vp)]),

@@ -73,6 +73,10 @@ enum : uint32_t {
     CE_STAGE,       // panel slot `slot` <- panel `arg`
     CE_OCLEAR,      // zero the output panel at LDS offset arg
     CE_OFLUSH,      // output panel -> constraints arg*PC.., columns in the next word's mask
+    CE_ACC,         // acc += coef[arg] * top; top = stk[slot]  (quotient: last EMIT of constraint arg)
+    CE_ACC0,        // acc += coef[arg] * top (stack empties)
+    CE_DROP,        // top = stk[slot]                           (quotient: an EMIT overwritten later)
+    CE_DROP0,       // stack empties
     CE_END,
 };
 
@@ -112,7 +116,8 @@ template <int R, int PC>
 __global__ __launch_bounds__(CE_WAVES * 64) void k_constraint_eval_gl(
     const uint64_t *__restrict__ trace, uint64_t n_rows, uint32_t width, const uint32_t *__restrict__ prog,
     const uint64_t *__restrict__ consts, uint32_t n_out, uint32_t wave_bytes, uint32_t stack_off,
-    uint32_t out_off, uint64_t *__restrict__ out)
+    uint32_t out_off, uint64_t *__restrict__ out, const uint64_t *__restrict__ coef,
+    const uint64_t *__restrict__ invz, uint32_t log_block, uint64_t *__restrict__ qout)
 {
     constexpr uint32_t WR = 64 * R, COLS = WR * 8 + 8, SLOT = PC * COLS, SROW = WR * 8;
     extern __shared__ __align__(16) unsigned char ce_lds[];
@@ -125,8 +130,8 @@ __global__ __launch_bounds__(CE_WAVES * 64) void k_constraint_eval_gl(
     unsigned char *const stk = mine + stack_off;
     // Lanes past the last row evaluate the last row again (staging clamps
     // their loads to it); flushes store those copies over the same values.
-    uint64_t top[R];
-    CE_EACH top[j] = 0;
+    uint64_t top[R], acc[R];
+    CE_EACH top[j] = acc[j] = 0;
 
     uint32_t pc = 0, w = prog[0];
     for (;;) {
@@ -173,7 +178,26 @@ __global__ __launch_bounds__(CE_WAVES * 64) void k_constraint_eval_gl(
             flush_panel<R, PC>(out, n_rows, n_out, row0, arg, mask, lane, base + out_off);
             break;
         }
+        case CE_ACC: {
+            const uint64_t k = coef[arg];
+            CE_EACH acc[j] = Gl::add(acc[j], Gl::mul(top[j], k));
+            CE_EACH top[j] = lds_u64(sp + 512 * j);
+            break;
+        }
+        case CE_ACC0: {
+            const uint64_t k = coef[arg];
+            CE_EACH acc[j] = Gl::add(acc[j], Gl::mul(top[j], k));
+            break;
+        }
+        case CE_DROP: CE_EACH top[j] = lds_u64(sp + 512 * j); break;
+        case CE_DROP0: break;
         default: break;
+        }
+    }
+    if (qout) {  // quotient: one value per row, divided by the vanishing polynomial
+        CE_EACH {
+            const uint64_t r = min(row0 + lane + 64 * j, n_rows - 1);
+            qout[r] = Gl::mul(acc[j], invz[r >> log_block]);
         }
     }
 }
@@ -194,9 +218,15 @@ bool ce_shape_built(CeShape s)
     return false;
 }
 
+struct CeQuot {  // quotient mode operands (all null for plain evaluation)
+    const uint64_t *coef = nullptr, *invz = nullptr;
+    uint32_t log_block = 0;
+    uint64_t *qout = nullptr;
+};
+
 hipError_t ce_launch(CeShape s, dim3 grid, uint32_t lds, hipStream_t st, const uint64_t *trace, uint64_t n_rows,
                      uint32_t width, const uint32_t *prog, const uint64_t *consts, uint32_t n_out, uint32_t wave_bytes,
-                     uint32_t stack_off, uint32_t out_off, uint64_t *out, uint32_t lds_limit)
+                     uint32_t stack_off, uint32_t out_off, uint64_t *out, uint32_t lds_limit, const CeQuot &q)
 {
 #define X(R, PC)                                                                                                  \
     if (s.rows_per_lane == R && s.panel == PC) {                                                                  \
@@ -207,7 +237,8 @@ hipError_t ce_launch(CeShape s, dim3 grid, uint32_t lds, hipStream_t st, const u
             if (e != hipSuccess) return e;                                                                        \
         }                                                                                                         \
         hipLaunchKernelGGL((k_constraint_eval_gl<R, PC>), grid, dim3(CE_WAVES * 64), lds, st, trace, n_rows,     \
-                           width, prog, consts, n_out, wave_bytes, stack_off, out_off, out);                     \
+                           width, prog, consts, n_out, wave_bytes, stack_off, out_off, out, q.coef, q.invz,     \
+                           q.log_block, q.qout);                                                                  \
         return hipGetLastError();                                                                                 \
     }
     CE_SHAPES(X)
@@ -256,9 +287,11 @@ struct CeLayout {
 // Validate the reference bytecode and translate it for `shape`.  The stack
 // below the top lives in LDS for the interpreter; compiled kernels keep it in
 // registers (stack_in_lds false).  A block holds `block_waves` waves and may
-// use `lds_budget` bytes.  ZKG_ENOMEM: does not fit at this shape.
+// use `lds_budget` bytes.  In quotient mode the last EMIT of each constraint
+// accumulates coef[k] * value instead (earlier EMITs of it only pop) and
+// there is no output panel.  ZKG_ENOMEM: does not fit at this shape.
 int ce_translate(const uint32_t *bc, uint32_t n, uint32_t width, uint32_t n_consts, uint32_t n_out, CeShape shape,
-                 bool stack_in_lds, uint32_t block_waves, uint32_t lds_budget, CeLayout &L)
+                 bool stack_in_lds, uint32_t block_waves, uint32_t lds_budget, bool quotient, CeLayout &L)
 {
     const uint32_t PC = shape.panel;
     L = CeLayout{};
@@ -372,7 +405,7 @@ int ce_translate(const uint32_t *bc, uint32_t n, uint32_t width, uint32_t n_cons
     while (K < kmax && (uint64_t)simulate(K, nullptr, nullptr) * 4 > (uint64_t)best * 5) K++;
     // LDS per wave: K input panel slots, the stack below the top, one output panel
     const uint32_t stack_bytes = stack_in_lds ? (uint32_t)std::max(0, maxd - 1) * shape.stack_row() : 0;
-    const uint32_t fixed = stack_bytes + shape.slot_bytes();
+    const uint32_t fixed = stack_bytes + (quotient ? 0 : shape.slot_bytes());
     while (K > 1 && block_waves * (K * shape.slot_bytes() + fixed) > lds_budget) K--;
     if (block_waves * ((uses.empty() ? 0 : shape.slot_bytes()) + fixed) > lds_budget)
         return ZKG_ENOMEM;  // the caller tries a smaller shape
@@ -385,7 +418,7 @@ int ce_translate(const uint32_t *bc, uint32_t n, uint32_t width, uint32_t n_cons
     L.stack_off = L.slots * shape.slot_bytes();
     L.stack_slots = stack_bytes / shape.stack_row();
     L.out_off = L.stack_off + stack_bytes;
-    L.wave_bytes = (L.out_off + shape.slot_bytes() + 15) & ~15u;
+    L.wave_bytes = (L.out_off + (quotient ? 0 : shape.slot_bytes()) + 15) & ~15u;
     std::vector<uint32_t> &words = L.words;
     size_t u = 0;
     uint32_t d = 0;  // stack depth before each instruction
@@ -412,7 +445,11 @@ int ce_translate(const uint32_t *bc, uint32_t n, uint32_t width, uint32_t n_cons
         if (!flushed[q] && zero_mask[q]) words.push_back(ce_word(CE_OCLEAR, 0, L.out_off));
         cur = q;
     };
-    for (const MidIns &m : mid) {
+    std::vector<size_t> last_emit(n_out, SIZE_MAX);
+    for (size_t i = 0; i < mid.size(); i++)
+        if (mid[i].op == CE_EMIT) last_emit[mid[i].arg] = i;
+    for (size_t mi = 0; mi < mid.size(); mi++) {
+        const MidIns &m = mid[mi];
         uint32_t op = m.op, arg = m.arg, slot = 0;
         if (m.col >= 0) {
             const uint32_t panel = (uint32_t)(m.col / PC), ps = (uint32_t)slot_of_use[u];
@@ -435,6 +472,17 @@ int ce_translate(const uint32_t *bc, uint32_t n, uint32_t width, uint32_t n_cons
             d++;
             break;
         case CE_EMIT: {
+            if (quotient) {
+                const bool acc = last_emit[arg] == mi;
+                if (d >= 2) {
+                    slot = d - 2;
+                    op = acc ? CE_ACC : CE_DROP;
+                } else {
+                    op = acc ? CE_ACC0 : CE_DROP0;
+                }
+                d--;
+                break;
+            }
             const uint32_t q = arg / PC, c = arg % PC;
             if (cur != (int64_t)q) {
                 if (cur >= 0) flush();
@@ -451,7 +499,7 @@ int ce_translate(const uint32_t *bc, uint32_t n, uint32_t width, uint32_t n_cons
         words.push_back(ce_word(op, slot, arg));
     }
     if (cur >= 0) flush();
-    for (uint32_t q = 0; q < n_opanels; q++)
+    for (uint32_t q = 0; q < n_opanels && !quotient; q++)
         if (!flushed[q]) {  // never emitted into: all zeros
             open(q);
             flush();
@@ -464,13 +512,15 @@ int ce_translate(const uint32_t *bc, uint32_t n, uint32_t width, uint32_t n_cons
 // Translate at the preferred shape; a program whose stack does not fit the
 // LDS there falls back to one row per lane.
 int ce_translate_fit(const uint32_t *bc, uint32_t n, uint32_t width, uint32_t n_consts, uint32_t n_out,
-                     CeShape preferred, bool stack_in_lds, uint32_t block_waves, uint32_t lds_budget, CeLayout &L)
+                     CeShape preferred, bool stack_in_lds, uint32_t block_waves, uint32_t lds_budget, bool quotient,
+                     CeLayout &L)
 {
     CeShape shape;
     if (int rc = ce_pick_shape(preferred, &shape)) return rc;
-    int rc = ce_translate(bc, n, width, n_consts, n_out, shape, stack_in_lds, block_waves, lds_budget, L);
+    int rc = ce_translate(bc, n, width, n_consts, n_out, shape, stack_in_lds, block_waves, lds_budget, quotient, L);
     if (rc == ZKG_ENOMEM && shape.rows_per_lane > 1)
-        rc = ce_translate(bc, n, width, n_consts, n_out, CeShape{1, 16}, stack_in_lds, block_waves, lds_budget, L);
+        rc = ce_translate(bc, n, width, n_consts, n_out, CeShape{1, 16}, stack_in_lds, block_waves, lds_budget,
+                          quotient, L);
     if (rc == ZKG_ENOMEM) rc = set_err(ZKG_EINVAL, "program needs more than %u bytes of LDS per block", lds_budget);
     return rc;
 }
@@ -482,11 +532,11 @@ struct zkg_constraint_program {
     uint32_t width = 0, n_constraints = 0, lds_budget = 0;
     std::vector<uint32_t> bytecode;  // the reference bytecode (host copy)
     std::vector<uint64_t> consts;    // constant pool (host copy)
-    CeLayout interp;                 // what the interpreter runs
-    void *d_prog = nullptr, *d_consts = nullptr;
-    CeLayout jit;                    // what the compiled kernel was generated from
-    hipModule_t mod = nullptr;       // compiled straight-line kernel, if any
-    hipFunction_t fn = nullptr;
+    CeLayout interp, interp_q;       // what the interpreter runs (evaluation, quotient)
+    void *d_prog = nullptr, *d_prog_q = nullptr, *d_consts = nullptr;
+    CeLayout jit, jit_q;             // what the compiled kernels were generated from
+    hipModule_t mod = nullptr, mod_q = nullptr;  // compiled straight-line kernels, if any
+    hipFunction_t fn = nullptr, fn_q = nullptr;
 };
 
 // ------------------------------------------------------------------ compiled programs
@@ -505,7 +555,9 @@ namespace {
 
 constexpr uint32_t CE_JIT_LDS = 64u * 1024;  // per one-wave block
 
-std::string ce_jit_source(const CeLayout &L, const std::vector<uint64_t> &consts)
+// The kernel for layout L: zkg_ce_jit (constraint values through output
+// panels) or, for a quotient layout, zkg_ce_jit_q (one quotient value per row).
+std::string ce_jit_source(const CeLayout &L, const std::vector<uint64_t> &consts, bool quotient)
 {
     const uint32_t R = L.shape.rows_per_lane, PC = L.shape.panel, COLS = L.shape.col_stride();
     std::string b;
@@ -519,8 +571,13 @@ std::string ce_jit_source(const CeLayout &L, const std::vector<uint64_t> &consts
     };
 #pragma clang diagnostic pop
     b += kJitPrelude;
-    emit("extern \"C\" __global__ __launch_bounds__(64) void zkg_ce_jit(const uint64_t *__restrict__ trace, "
-         "uint64_t n_rows, uint32_t width, uint32_t n_out, uint64_t *__restrict__ out)");
+    if (quotient)
+        emit("extern \"C\" __global__ __launch_bounds__(64) void zkg_ce_jit_q(const uint64_t *__restrict__ trace, "
+             "uint64_t n_rows, uint32_t width, const uint64_t *__restrict__ coef, const uint64_t *__restrict__ invz, "
+             "uint32_t log_block, uint64_t *__restrict__ qout)");
+    else
+        emit("extern \"C\" __global__ __launch_bounds__(64) void zkg_ce_jit(const uint64_t *__restrict__ trace, "
+             "uint64_t n_rows, uint32_t width, uint32_t n_out, uint64_t *__restrict__ out)");
     emit("{");
     emit("    constexpr int R = %u, PC = %u;", R, PC);
     emit("    extern __shared__ __align__(16) unsigned char ce_lds[];");
@@ -530,6 +587,7 @@ std::string ce_jit_source(const CeLayout &L, const std::vector<uint64_t> &consts
     emit("    unsigned char *const mine = ce_lds + lane * 8;");
     emit("#define LD(off, j) (*(const uint64_t *)(mine + (off) + 512 * (j)))");
     emit("#define EACH for (int j = 0; j < R; j++)");
+    if (quotient) emit("    uint64_t acc[R]; EACH acc[j] = 0;");
     std::vector<uint32_t> st;  // the stack, as variable numbers
     uint32_t nv = 0;
     static const char *BIN[3] = {"add", "sub", "mul"};
@@ -559,6 +617,12 @@ std::string ce_jit_source(const CeLayout &L, const std::vector<uint64_t> &consts
             emit("    EACH *(uint64_t *)(mine + %uu + 512 * j) = v%u[j];", arg, st.back());
             st.pop_back();
             break;
+        case CE_ACC: case CE_ACC0:
+            emit("    { const uint64_t k = coef[%uu]; EACH acc[j] = zkg::Gl::add(acc[j], zkg::Gl::mul(v%u[j], k)); }",
+                 arg, st.back());
+            st.pop_back();
+            break;
+        case CE_DROP: case CE_DROP0: st.pop_back(); break;
         case CE_ADD_L: case CE_SUB_L: case CE_MUL_L:
             emit("    uint64_t v%u[R]; EACH v%u[j] = zkg::Gl::%s(v%u[j], LD(%uu, j));", nv, nv, BIN[op - CE_ADD_L],
                  st.back(), arg);
@@ -588,6 +652,9 @@ std::string ce_jit_source(const CeLayout &L, const std::vector<uint64_t> &consts
             break;
         }
     }
+    if (quotient)
+        emit("    EACH { const uint64_t r = min(row0 + lane + 64 * j, n_rows - 1); "
+             "qout[r] = zkg::Gl::mul(acc[j], invz[r >> log_block]); }");
     emit("}");
     return b;
 }
@@ -641,23 +708,52 @@ void ce_info(const CeLayout &L, uint32_t info[8])
 
 // The compiled kernel's layout for a program (host only).
 int ce_jit_layout(const uint32_t *bc, uint32_t n, uint32_t width, uint32_t n_consts, uint32_t n_out,
-                  uint32_t lds_budget, CeLayout &L)
+                  uint32_t lds_budget, bool quotient, CeLayout &L)
 {
     return ce_translate_fit(bc, n, width, n_consts, n_out, CE_COMPILED_SHAPE, false, 1,
-                            std::min(CE_JIT_LDS, lds_budget), L);
+                            std::min(CE_JIT_LDS, lds_budget), quotient, L);
+}
+
+// Compile one kernel of the program (evaluation or quotient) and load it.
+int ce_compile_kind(zkg_constraint_program *p, bool quotient)
+{
+    hipFunction_t &fn = quotient ? p->fn_q : p->fn;
+    if (fn) return ZKG_OK;
+    CeLayout L;
+    if (int rc = ce_jit_layout(p->bytecode.data(), (uint32_t)p->bytecode.size(), p->width, (uint32_t)p->consts.size(),
+                               p->n_constraints, p->lds_budget, quotient, L))
+        return rc;
+    std::vector<char> code;
+    std::string log;
+    if (int rc = ce_jit_compile(ce_jit_source(L, p->consts, quotient), code, log)) return rc;
+    hipError_t e = hipSetDevice(p->ctx->device);
+    if (e != hipSuccess) return hip_err(e, "hipSetDevice");
+    hipModule_t mod = nullptr;
+    hipFunction_t f = nullptr;
+    e = hipModuleLoadData(&mod, code.data());
+    if (e != hipSuccess) return hip_err(e, "load compiled constraint program");
+    e = hipModuleGetFunction(&f, mod, quotient ? "zkg_ce_jit_q" : "zkg_ce_jit");
+    if (e != hipSuccess) {
+        (void)hipModuleUnload(mod);
+        return hip_err(e, "compiled constraint program entry");
+    }
+    (quotient ? p->jit_q : p->jit) = std::move(L);
+    (quotient ? p->mod_q : p->mod) = mod;
+    fn = f;
+    return ZKG_OK;
 }
 
 }  // namespace
 
 extern "C" int zkg_constraint_translate(uint32_t width, const uint32_t *bytecode, uint32_t n_instructions,
-                                        uint32_t n_constants, uint32_t n_constraints, uint32_t *words, uint32_t cap,
-                                        uint32_t info[8])
+                                        uint32_t n_constants, uint32_t n_constraints, int quotient, uint32_t *words,
+                                        uint32_t cap, uint32_t info[8])
 {
     if (!info || (!words && cap)) return set_err(ZKG_EINVAL, "null argument");
     if (int rc = ce_check_args(width, bytecode, n_instructions, n_constraints)) return rc;
     CeLayout L;
     if (int rc = ce_translate_fit(bytecode, n_instructions, width, n_constants, n_constraints, CE_INTERP_SHAPE, true,
-                                  CE_WAVES, CE_LDS_BYTES, L))
+                                  CE_WAVES, CE_LDS_BYTES, quotient != 0, L))
         return rc;
     ce_info(L, info);
     std::memcpy(words, L.words.data(), std::min<size_t>(cap, L.words.size()) * 4);
@@ -688,11 +784,17 @@ extern "C" int zkg_constraint_program_create(zkg_ctx *ctx, uint32_t width, const
     p->lds_budget = std::min<uint32_t>(CE_LDS_BYTES, (uint32_t)std::max(lds_max, 0));
     if (!rc)
         rc = ce_translate_fit(bytecode, n_instructions, width, n_constants, n_constraints, CE_INTERP_SHAPE, true,
-                              CE_WAVES, p->lds_budget, p->interp);
-    const std::vector<uint32_t> &words = p->interp.words;
+                              CE_WAVES, p->lds_budget, false, p->interp);
+    if (!rc)
+        rc = ce_translate_fit(bytecode, n_instructions, width, n_constants, n_constraints, CE_INTERP_SHAPE, true,
+                              CE_WAVES, p->lds_budget, true, p->interp_q);
+    const std::vector<uint32_t> &words = p->interp.words, &words_q = p->interp_q.words;
     if (!rc) rc = dev_alloc(ctx, words.size() * 4, &p->d_prog);
+    if (!rc) rc = dev_alloc(ctx, words_q.size() * 4, &p->d_prog_q);
     if (!rc) rc = dev_alloc(ctx, std::max<uint32_t>(n_constants, 1) * 8, &p->d_consts);
     if (!rc) e = hipMemcpyAsync(p->d_prog, words.data(), words.size() * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (!rc && e == hipSuccess)
+        e = hipMemcpyAsync(p->d_prog_q, words_q.data(), words_q.size() * 4, hipMemcpyHostToDevice, ctx->stream);
     if (!rc && e == hipSuccess && n_constants)
         e = hipMemcpyAsync(p->d_consts, constants, (uint64_t)n_constants * 8, hipMemcpyHostToDevice, ctx->stream);
     if (!rc && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // host vectors may change after return
@@ -741,46 +843,97 @@ extern "C" int zkg_constraint_program_eval(zkg_constraint_program *p, const zkg_
     if (blocks > 0x7FFFFFFFull) return set_err(ZKG_EINVAL, "too many rows");
     hipError_t e = ce_launch(L.shape, dim3((uint32_t)blocks), CE_WAVES * L.wave_bytes, p->ctx->stream, tr, n_rows,
                              p->width, (const uint32_t *)p->d_prog, (const uint64_t *)p->d_consts, p->n_constraints,
-                             L.wave_bytes, L.stack_off, L.out_off, o, p->lds_budget);
+                             L.wave_bytes, L.stack_off, L.out_off, o, p->lds_budget, CeQuot{});
     if (e != hipSuccess) return hip_err(e, "constraint eval launch");
+    return ZKG_OK;
+}
+
+extern "C" int zkg_constraint_program_quotient(zkg_constraint_program *p, const zkg_buf *lde, uint64_t height,
+                                               uint32_t log_blowup, uint64_t shift, const uint64_t *coef,
+                                               zkg_buf *q_out)
+{
+    if (!p || !lde || !coef || !q_out) return set_err(ZKG_EINVAL, "null argument");
+    if (!height || (height & (height - 1)) || height > (1ull << 32))
+        return set_err(ZKG_EINVAL, "height %llu is not a power of two <= 2^32", (unsigned long long)height);
+    int log_h = 0;
+    while ((1ull << log_h) < height) log_h++;
+    if ((int)log_blowup > log_h) return set_err(ZKG_EINVAL, "log_blowup %u exceeds log height %d", log_blowup, log_h);
+    if (height * p->width * 8 > lde->bytes) return set_err(ZKG_EINVAL, "LDE buffer too small");
+    if (height * 8 > q_out->bytes) return set_err(ZKG_EINVAL, "quotient buffer too small");
+    if (lde->ctx != p->ctx || q_out->ctx != p->ctx) return set_err(ZKG_EINVAL, "buffers of another context");
+    if (shift % Gl::P == 0) return set_err(ZKG_EINVAL, "zero coset shift");
+    for (uint32_t k = 0; k < p->n_constraints; k++)
+        if (coef[k] >= Gl::P) return set_err(ZKG_EINVAL, "coefficient %u is not canonical", k);
+    // Row i sits at x_i = shift w_H^rev(i); with N = H / B, x_i^N = shift^N w_B^t,
+    // t the bit reversal (log_blowup bits) of the top log_blowup bits of i.  So
+    // 1 / Z_H(x_i) = 1 / (x_i^N - 1) takes B values, one per block of H / B rows.
+    const uint32_t B = 1u << log_blowup;
+    const uint64_t n = height >> log_blowup;
+    auto gpow = [](uint64_t a, uint64_t e) {
+        uint64_t r = 1;
+        for (; e; e >>= 1, a = Gl::mul(a, a))
+            if (e & 1) r = Gl::mul(r, a);
+        return r;
+    };
+    uint64_t w_b = 0x185629dcda58878cULL;  // p3 Goldilocks two_adic_generator(32), squared down to order B
+    for (uint32_t i = log_blowup; i < 32; i++) w_b = Gl::mul(w_b, w_b);
+    const uint64_t sn = gpow(shift % Gl::P, n);
+    std::vector<uint64_t> host(p->n_constraints + B);
+    std::copy(coef, coef + p->n_constraints, host.begin());
+    for (uint32_t blk = 0; blk < B; blk++) {
+        uint32_t t = 0;
+        for (uint32_t b = 0; b < log_blowup; b++) t |= ((blk >> b) & 1u) << (log_blowup - 1 - b);
+        const uint64_t v = Gl::sub(Gl::mul(sn, gpow(w_b, t)), 1);
+        if (!v) return set_err(ZKG_EINVAL, "the coset meets the trace domain (shift^N is a root of unity of order B)");
+        host[p->n_constraints + blk] = gpow(v, Gl::P - 2);
+    }
+    void *d = nullptr;
+    if (int rc = dev_alloc(p->ctx, host.size() * 8, &d)) return rc;
+    hipError_t e = hipMemcpyAsync(d, host.data(), host.size() * 8, hipMemcpyHostToDevice, p->ctx->stream);
+    CeQuot q;
+    q.coef = (const uint64_t *)d;
+    q.invz = q.coef + p->n_constraints;
+    q.log_block = (uint32_t)(log_h - (int)log_blowup);
+    q.qout = (uint64_t *)q_out->ptr;
+    const uint64_t *tr = (const uint64_t *)lde->ptr;
+    if (e == hipSuccess && p->fn_q) {
+        const uint64_t waves = (height + p->jit_q.shape.wave_rows() - 1) / p->jit_q.shape.wave_rows();
+        uint32_t width = p->width, lb = q.log_block;
+        const uint64_t *cf = q.coef, *iz = q.invz;
+        uint64_t *qo = q.qout;
+        void *args[] = {&tr, &height, &width, &cf, &iz, &lb, &qo};
+        e = hipModuleLaunchKernel(p->fn_q, (uint32_t)waves, 1, 1, 64, 1, 1, p->jit_q.wave_bytes, p->ctx->stream, args,
+                                  nullptr);
+    } else if (e == hipSuccess) {
+        const CeLayout &L = p->interp_q;
+        const uint64_t waves = (height + L.shape.wave_rows() - 1) / L.shape.wave_rows();
+        const uint64_t blocks = (waves + CE_WAVES - 1) / CE_WAVES;
+        e = ce_launch(L.shape, dim3((uint32_t)blocks), CE_WAVES * L.wave_bytes, p->ctx->stream, tr, height, p->width,
+                      (const uint32_t *)p->d_prog_q, (const uint64_t *)p->d_consts, p->n_constraints, L.wave_bytes,
+                      L.stack_off, L.out_off, nullptr, p->lds_budget, q);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(p->ctx->stream);  // host coefficients, temporary buffer
+    dev_free(p->ctx, d);
+    if (e != hipSuccess) return hip_err(e, "constraint quotient");
     return ZKG_OK;
 }
 
 extern "C" int zkg_constraint_program_compile(zkg_constraint_program *p)
 {
     if (!p) return set_err(ZKG_EINVAL, "null argument");
-    if (p->fn) return ZKG_OK;
-    CeLayout L;
-    if (int rc = ce_jit_layout(p->bytecode.data(), (uint32_t)p->bytecode.size(), p->width, (uint32_t)p->consts.size(),
-                               p->n_constraints, p->lds_budget, L))
-        return rc;
-    std::vector<char> code;
-    std::string log;
-    if (int rc = ce_jit_compile(ce_jit_source(L, p->consts), code, log)) return rc;
-    hipError_t e = hipSetDevice(p->ctx->device);
-    if (e != hipSuccess) return hip_err(e, "hipSetDevice");
-    hipModule_t mod = nullptr;
-    hipFunction_t fn = nullptr;
-    e = hipModuleLoadData(&mod, code.data());
-    if (e != hipSuccess) return hip_err(e, "load compiled constraint program");
-    e = hipModuleGetFunction(&fn, mod, "zkg_ce_jit");
-    if (e != hipSuccess) {
-        (void)hipModuleUnload(mod);
-        return hip_err(e, "compiled constraint program entry");
-    }
-    p->jit = std::move(L);
-    p->mod = mod;
-    p->fn = fn;
-    return ZKG_OK;
+    if (int rc = ce_compile_kind(p, false)) return rc;
+    return ce_compile_kind(p, true);
 }
 
 extern "C" void zkg_constraint_program_free(zkg_constraint_program *p)
 {
     if (!p) return;
-    if (p->d_prog || p->d_consts || p->mod) (void)hipStreamSynchronize(p->ctx->stream);
+    if (p->d_prog || p->d_prog_q || p->d_consts || p->mod || p->mod_q) (void)hipStreamSynchronize(p->ctx->stream);
     if (p->d_prog) dev_free(p->ctx, p->d_prog);
+    if (p->d_prog_q) dev_free(p->ctx, p->d_prog_q);
     if (p->d_consts) dev_free(p->ctx, p->d_consts);
     if (p->mod) (void)hipModuleUnload(p->mod);
+    if (p->mod_q) (void)hipModuleUnload(p->mod_q);
     delete p;
 }
 
@@ -853,7 +1006,7 @@ static zkg_constraint_program *ce_cache_find(CeCache &c, uint32_t width, const u
             e.uses++;
             e.last = ++c.tick;
             if (!e.prog->fn && !e.no_compile && c.compile_after && e.uses >= c.compile_after &&
-                (n > CE_AUTO_COMPILE_MAX || zkg_constraint_program_compile(e.prog) != ZKG_OK))
+                (n > CE_AUTO_COMPILE_MAX || ce_compile_kind(e.prog, false) != ZKG_OK))
                 e.no_compile = true;
             return e.prog;
         }
@@ -907,14 +1060,14 @@ extern "C" int zkg_constraint_eval_gl(zkg_ctx *ctx, const zkg_buf *trace, uint64
 
 // Host-only views of the compiled path: its source, and a compile without a device.
 static int ce_host_jit(uint32_t width, const uint32_t *bytecode, uint32_t n_instructions, const uint64_t *constants,
-                       uint32_t n_constants, uint32_t n_constraints, std::string &src)
+                       uint32_t n_constants, uint32_t n_constraints, bool quotient, std::string &src)
 {
     if (!constants && n_constants) return set_err(ZKG_EINVAL, "null argument");
     if (int rc = ce_check_args(width, bytecode, n_instructions, n_constraints)) return rc;
     CeLayout L;
-    if (int rc = ce_jit_layout(bytecode, n_instructions, width, n_constants, n_constraints, CE_LDS_BYTES, L))
+    if (int rc = ce_jit_layout(bytecode, n_instructions, width, n_constants, n_constraints, CE_LDS_BYTES, quotient, L))
         return rc;
-    src = ce_jit_source(L, std::vector<uint64_t>(constants, constants + n_constants));
+    src = ce_jit_source(L, std::vector<uint64_t>(constants, constants + n_constants), quotient);
     return ZKG_OK;
 }
 
@@ -924,7 +1077,8 @@ extern "C" int zkg_constraint_jit_source(uint32_t width, const uint32_t *bytecod
 {
     if (!size || (!src && cap)) return set_err(ZKG_EINVAL, "null argument");
     std::string text;
-    if (int rc = ce_host_jit(width, bytecode, n_instructions, constants, n_constants, n_constraints, text)) return rc;
+    if (int rc = ce_host_jit(width, bytecode, n_instructions, constants, n_constants, n_constraints, false, text))
+        return rc;
     *size = text.size() + 1;
     if (cap) snprintf(src, cap, "%s", text.c_str());
     if (cap && cap < text.size() + 1) return set_err(ZKG_EINVAL, "source needs %zu bytes", text.size() + 1);
@@ -937,9 +1091,12 @@ extern "C" int zkg_constraint_compile_check(uint32_t width, const uint32_t *byte
 {
     if (!log && log_cap) return set_err(ZKG_EINVAL, "null argument");
     std::string src, text;
-    if (int rc = ce_host_jit(width, bytecode, n_instructions, constants, n_constants, n_constraints, src)) return rc;
+    int rc = ZKG_OK;
     std::vector<char> code;
-    const int rc = ce_jit_compile(src, code, text);
+    for (int quotient = 0; quotient < 2 && !rc; quotient++) {
+        rc = ce_host_jit(width, bytecode, n_instructions, constants, n_constants, n_constraints, quotient != 0, src);
+        if (!rc) rc = ce_jit_compile(src, code, text);
+    }
     if (log_cap) snprintf(log, log_cap, "%s", text.c_str());
     return rc;
 }

@@ -371,17 +371,18 @@ class ConstraintCompiler:
 _CE_INFO = ("words", "slots", "stages", "stack_slots", "wave_bytes", "panel", "rows_per_lane", "col_stride")
 
 
-def translate_constraints(width, bytecode, n_constants, n_constraints):
-    """The host translation libzkg runs before evaluation (no device needed):
-    (device words, info dict)."""
+def translate_constraints(width, bytecode, n_constants, n_constraints, quotient=False):
+    """The host translation libzkg runs before evaluation (or, quotient=True,
+    before a quotient) with no device: (device words, info dict)."""
     lib = _lib.load()
     bc = np.ascontiguousarray(bytecode, dtype=np.uint32)
     info = np.zeros(8, dtype=np.uint32)
-    rc = lib.zkg_constraint_translate(width, _ptr(bc), bc.size, n_constants, n_constraints, None, 0, _ptr(info))
+    q = 1 if quotient else 0
+    rc = lib.zkg_constraint_translate(width, _ptr(bc), bc.size, n_constants, n_constraints, q, None, 0, _ptr(info))
     if rc != 0 and info[0] == 0:
         check(rc)
     words = np.zeros(int(info[0]), dtype=np.uint32)
-    check(lib.zkg_constraint_translate(width, _ptr(bc), bc.size, n_constants, n_constraints, _ptr(words),
+    check(lib.zkg_constraint_translate(width, _ptr(bc), bc.size, n_constants, n_constraints, q, _ptr(words),
                                        words.size, _ptr(info)))
     return words, dict(zip(_CE_INFO, (int(v) for v in info)))
 
@@ -438,6 +439,14 @@ class ConstraintProgram:
 
     def eval(self, trace_buf, n_rows, out_buf):
         check(self.ctx.lib.zkg_constraint_program_eval(self.h, trace_buf.h, n_rows, out_buf.h))
+
+    def quotient(self, lde_buf, height, log_blowup, shift, coef, q_buf):
+        """q[i] = sum_k coef[k] C_k(row i) / Z_H(x_i) over a committed bit-reversed coset LDE."""
+        cf = np.ascontiguousarray(coef, dtype=np.uint64)
+        if cf.size != self.n_constraints:
+            raise ZkgError(-1, "%d coefficients for %d constraints" % (cf.size, self.n_constraints))
+        check(self.ctx.lib.zkg_constraint_program_quotient(self.h, lde_buf.h, height, log_blowup, shift, _ptr(cf),
+                                                           q_buf.h))
 
     def free(self):
         if self.h:
