@@ -154,6 +154,17 @@ int zkg_mmcs_commit(zkg_ctx *ctx, int field, int p2_width, uint32_t n_mats, cons
  * order (sum of widths elements), and log2_ceil(max_h) sibling digests.
  * Also accepts single-matrix trees. */
 int zkg_mmcs_open(const zkg_tree *tree, const uint64_t *indices, uint32_t n, void *rows_out, void *paths_out);
+/* Mmcs::verify_batch (vendor/p3-merkle-tree-gpu/src/mmcs.rs:210-330) for n
+ * openings at once, on the device: ok[q] = 1 if opening q (its rows and
+ * path, laid out as zkg_mmcs_open writes them: every matrix's row in the given
+ * order, then log2_ceil(max height) sibling digests) hashes to `root`, else 0.
+ * The matrices are given by heights and widths, in commit order.  ZKG_EINVAL
+ * for what p3 refuses with an error (heights that differ but round up to the
+ * same power of two, an index at or past the tallest height) and for
+ * non-canonical values. */
+int zkg_mmcs_verify_batch(zkg_ctx *ctx, int field, int p2_width, const void *root, uint32_t n_mats,
+                          const uint64_t *heights, const uint32_t *widths, uint32_t n, const uint64_t *indices,
+                          const void *rows, const void *paths, uint8_t *ok);
 
 /* ---- PCS opening: DEEP quotient --------------------------------------------
  * Replaces the reduced-opening step of p3-fri 0.4.2 TwoAdicFriPcs::open and

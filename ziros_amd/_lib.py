@@ -63,6 +63,7 @@ SIGNATURES = {
     "zkg_tree_layer_len": (c_int, [c_vp, c_u32, P(c_u64)]),
     "zkg_mmcs_commit": (c_int, [c_vp, c_int, c_int, c_u32, c_vp, c_vp, c_vp, c_vp]),
     "zkg_mmcs_open": (c_int, [c_vp, c_vp, c_u32, c_vp, c_vp]),
+    "zkg_mmcs_verify_batch": (c_int, [c_vp, c_int, c_int, c_vp, c_u32, c_vp, c_vp, c_u32, c_vp, c_vp, c_vp, c_vp]),
     "zkg_deep_open_values": (c_int, [c_vp, c_int, c_vp, c_u64, c_u32, c_u32, c_u64, c_vp, c_vp]),
     "zkg_deep_reduce": (c_int, [c_vp, c_int, c_vp, c_u64, c_u32, c_u64, c_vp, c_vp, c_u64, c_vp, c_vp]),
     "zkg_deep_open": (c_int, [c_vp, c_int, c_vp, c_u64, c_u32, c_u32, c_u64, c_vp, c_vp, c_u64, c_vp, c_vp]),

@@ -599,7 +599,77 @@ __global__ __launch_bounds__(HASH_THREADS) void k_mmcs_layer(const typename F::T
     for (int e = 0; e < D; e++) next[i * D + e] = F::to_canon(s[e]);
 }
 
+// Mmcs::verify_batch (vendor/p3-merkle-tree-gpu/src/mmcs.rs:210-330), one
+// lane per opening q.  Matrices are sorted tallest first; group g (descriptors
+// gstart[g] .. gstart[g+1]) holds the rows of the matrices whose heights round
+// up to the same power of two, each laid out [opening][width], and is hashed
+// in after glevel[g] compressions (group 0 gives the leaf digest).  ok[q] = 1
+// if the recomputed root equals `root`.
+template <class F, int W>
+__global__ __launch_bounds__(HASH_THREADS) void k_mmcs_verify(
+    const MatDesc *__restrict__ desc, const int *__restrict__ gstart, const int *__restrict__ glevel, int n_groups,
+    uint32_t n_q, const uint64_t *__restrict__ idx, const typename F::T *__restrict__ proof, int log_h,
+    const typename F::T *__restrict__ root, uint32_t *__restrict__ ok, const typename F::T *__restrict__ rc,
+    const typename F::T *__restrict__ diag)
+{
+    using T = typename F::T;
+    constexpr int D = W / 2;
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_q) return;
+    T s[W];
+    group_row_hash<F, W>(desc + gstart[0], gstart[1] - gstart[0], q, s, rc, diag);
+    uint64_t index = idx[q];
+    int g = 1;
+    for (int l = 0; l < log_h; l++) {
+        const T *sib = proof + ((uint64_t)q * log_h + l) * D;
+        const bool right = index & 1;  // the current node is the right child
+#pragma unroll
+        for (int e = 0; e < D; e++) {
+            const T node = s[e], other = F::from_canon(sib[e]);
+            s[e] = right ? other : node;
+            s[D + e] = right ? node : other;
+        }
+        p2_permute<F, W>(s, rc, diag);
+        index >>= 1;
+        if (g < n_groups && glevel[g] == l + 1) {  // compress the injected rows' digest in
+            T r[W];
+            group_row_hash<F, W>(desc + gstart[g], gstart[g + 1] - gstart[g], q, r, rc, diag);
+#pragma unroll
+            for (int e = 0; e < D; e++) s[D + e] = r[e];
+            p2_permute<F, W>(s, rc, diag);
+            g++;
+        }
+    }
+    uint32_t good = 1;
+#pragma unroll
+    for (int e = 0; e < D; e++) good &= F::to_canon(s[e]) == root[e] ? 1u : 0u;
+    ok[q] = good;
+}
+
 }  // namespace
+
+hipError_t launch_mmcs_verify(int field, int width, const P2Set &p2, const MatDesc *desc, const int *gstart,
+                              const int *glevel, int n_groups, uint32_t n_q, const uint64_t *idx, const void *proof,
+                              int log_h, const void *root, uint32_t *ok, hipStream_t st)
+{
+    if (!n_q) return hipSuccess;
+    dim3 g((n_q + HASH_THREADS - 1) / HASH_THREADS), b(HASH_THREADS);
+    if (field == FIELD_BB && width == 16)
+        hipLaunchKernelGGL((k_mmcs_verify<Bb, 16>), g, b, 0, st, desc, gstart, glevel, n_groups, n_q, idx,
+                           (const uint32_t *)proof, log_h, (const uint32_t *)root, ok, (const uint32_t *)p2.rc,
+                           (const uint32_t *)p2.diag);
+    else if (field == FIELD_GL && width == 16)
+        hipLaunchKernelGGL((k_mmcs_verify<Gl, 16>), g, b, 0, st, desc, gstart, glevel, n_groups, n_q, idx,
+                           (const uint64_t *)proof, log_h, (const uint64_t *)root, ok, (const uint64_t *)p2.rc,
+                           (const uint64_t *)p2.diag);
+    else if (field == FIELD_GL && width == 8)
+        hipLaunchKernelGGL((k_mmcs_verify<Gl, 8>), g, b, 0, st, desc, gstart, glevel, n_groups, n_q, idx,
+                           (const uint64_t *)proof, log_h, (const uint64_t *)root, ok, (const uint64_t *)p2.rc,
+                           (const uint64_t *)p2.diag);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
 
 hipError_t launch_mmcs_leaves(int field, int width, const P2Set &p2, const MatDesc *group, int n, uint64_t rows,
                               void *digests, hipStream_t st)

@@ -115,6 +115,9 @@ hipError_t launch_gen(int field, void *out, uint64_t seed, uint64_t start, uint6
 // the group (device array of n MatDesc), rows concatenated.  Layer: node i of
 // the next layer (i < half) = compress(prev[2i], prev[2i+1]), then, if n_inj,
 // compress(node, group hash of row i) for i < inj_len, compress(node, 0) above.
+hipError_t launch_mmcs_verify(int field, int width, const P2Set &p2, const MatDesc *desc, const int *gstart,
+                              const int *glevel, int n_groups, uint32_t n_q, const uint64_t *idx, const void *proof,
+                              int log_h, const void *root, uint32_t *ok, hipStream_t st);
 hipError_t launch_mmcs_leaves(int field, int width, const P2Set &p2, const MatDesc *group, int n,
                               uint64_t rows, void *digests, hipStream_t st);
 hipError_t launch_mmcs_layer(int field, int width, const P2Set &p2, const void *prev, uint64_t half,

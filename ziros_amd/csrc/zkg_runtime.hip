@@ -1073,6 +1073,109 @@ extern "C" int zkg_mmcs_open(const zkg_tree *t, const uint64_t *indices, uint32_
     return ZKG_OK;
 }
 
+// Mmcs::verify_batch (vendor/p3-merkle-tree-gpu/src/mmcs.rs:210-330) for n
+// openings on the device.  The host checks what p3 refuses with an error and
+// regroups the rows per matrix ([opening][width], tallest matrix first, ties
+// in the given order) for the verify kernel.
+extern "C" int zkg_mmcs_verify_batch(zkg_ctx *ctx, int field, int p2_width, const void *root, uint32_t n_mats,
+                                     const uint64_t *heights, const uint32_t *widths, uint32_t n,
+                                     const uint64_t *indices, const void *rows, const void *paths, uint8_t *ok)
+{
+    if (!ctx || !root || !heights || !widths || (n && (!indices || !rows || !paths || !ok)))
+        return set_err(ZKG_EINVAL, "null argument");
+    if (!n_mats) return set_err(ZKG_EINVAL, "empty batch");
+    if (int rc = check_field(field)) return rc;
+    const P2Set *p2 = get_p2(ctx, field, p2_width);
+    if (!p2) return set_err(ZKG_ESTATE, "Poseidon2 constants for field %d width %d not loaded", field, p2_width);
+    if (!n) return ZKG_OK;
+    const uint64_t es = esize(field), D = (uint64_t)p2_width / 2, P = field == FIELD_GL ? Gl::P : (uint64_t)Bb::P;
+    for (uint32_t i = 0; i < n_mats; i++)
+        if (!heights[i] || heights[i] > (1ull << 40)) return set_err(ZKG_EINVAL, "matrix %u: height %llu", i,
+                                                                    (unsigned long long)heights[i]);
+    std::vector<uint32_t> ord(n_mats);
+    for (uint32_t i = 0; i < n_mats; i++) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return heights[a] > heights[b]; });
+    for (uint32_t i = 1; i < n_mats; i++) {
+        const uint64_t a = heights[ord[i - 1]], b = heights[ord[i]];
+        if (a != b && next_pow2_u64(a) == next_pow2_u64(b))
+            return set_err(ZKG_EINVAL, "heights %llu and %llu round up to the same power of two",
+                           (unsigned long long)a, (unsigned long long)b);
+    }
+    const uint64_t maxh = heights[ord[0]];
+    const int logmax = log2_ceil_u64(maxh);
+    // groups of equal padded height, tallest first, and the layer each enters at
+    std::vector<int> gstart, glevel;
+    for (uint32_t i = 0; i < n_mats; i++)
+        if (!i || next_pow2_u64(heights[ord[i]]) != next_pow2_u64(heights[ord[i - 1]])) {
+            gstart.push_back((int)i);
+            glevel.push_back(logmax - log2_ceil_u64(next_pow2_u64(heights[ord[i]])));
+        }
+    const int n_groups = (int)gstart.size();
+    gstart.push_back((int)n_mats);
+    uint64_t row_elems = 0;
+    std::vector<uint64_t> col_off(n_mats);  // offset of matrix i within one opening's row block
+    for (uint32_t i = 0; i < n_mats; i++) {
+        col_off[i] = row_elems;
+        row_elems += widths[i];
+    }
+    auto value = [&](const void *base, uint64_t k) -> uint64_t {
+        return es == 8 ? ((const uint64_t *)base)[k] : ((const uint32_t *)base)[k];
+    };
+    for (uint32_t q = 0; q < n; q++)
+        if (indices[q] >= maxh)
+            return set_err(ZKG_EINVAL, "opening %u: index %llu out of bounds for height %llu", q,
+                           (unsigned long long)indices[q], (unsigned long long)maxh);
+    for (uint64_t k = 0; k < (uint64_t)n * row_elems; k++)
+        if (value(rows, k) >= P) return set_err(ZKG_EINVAL, "opened value %llu is not canonical", (unsigned long long)k);
+    for (uint64_t k = 0; k < (uint64_t)n * logmax * D; k++)
+        if (value(paths, k) >= P) return set_err(ZKG_EINVAL, "sibling word %llu is not canonical", (unsigned long long)k);
+    for (uint64_t k = 0; k < D; k++)
+        if (value(root, k) >= P) return set_err(ZKG_EINVAL, "root is not canonical");
+    // device layout: per-matrix row blocks, proofs, indices, root, flags, descriptors, group tables
+    auto al = [](uint64_t x) { return (x + 255) & ~255ull; };
+    std::vector<uint64_t> mat_off(n_mats);
+    uint64_t off = 0;
+    for (uint32_t i = 0; i < n_mats; i++) {
+        mat_off[i] = off;
+        off = al(off + (uint64_t)n * widths[ord[i]] * es);
+    }
+    const uint64_t proof_off = off, idx_off = al(proof_off + (uint64_t)n * logmax * D * es);
+    const uint64_t root_off = al(idx_off + (uint64_t)n * 8), ok_off = al(root_off + D * es);
+    const uint64_t desc_off = al(ok_off + (uint64_t)n * 4), gs_off = al(desc_off + n_mats * sizeof(MatDesc));
+    const uint64_t gl_off = al(gs_off + gstart.size() * sizeof(int)), total = al(gl_off + glevel.size() * sizeof(int));
+    std::vector<char> host(total, 0);
+    for (uint32_t i = 0; i < n_mats; i++) {
+        const uint32_t m = ord[i];
+        const uint64_t rb = (uint64_t)widths[m] * es;
+        for (uint32_t q = 0; q < n; q++)
+            std::memcpy(&host[mat_off[i] + q * rb], (const char *)rows + (q * row_elems + col_off[m]) * es, rb);
+    }
+    std::memcpy(&host[proof_off], paths, (uint64_t)n * logmax * D * es);
+    std::memcpy(&host[idx_off], indices, (uint64_t)n * 8);
+    std::memcpy(&host[root_off], root, D * es);
+    std::memcpy(&host[gs_off], gstart.data(), gstart.size() * sizeof(int));
+    std::memcpy(&host[gl_off], glevel.data(), glevel.size() * sizeof(int));
+    void *d = nullptr;
+    if (int rc = dev_alloc(ctx, total, &d)) return rc;
+    char *db = (char *)d;
+    std::vector<MatDesc> desc(n_mats);
+    for (uint32_t i = 0; i < n_mats; i++) desc[i] = MatDesc{db + mat_off[i], widths[ord[i]]};
+    std::memcpy(&host[desc_off], desc.data(), desc.size() * sizeof(MatDesc));
+    hipStream_t st = ctx->stream;
+    std::vector<uint32_t> flags(n);
+    hipError_t e = hipMemcpyAsync(d, host.data(), total, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = launch_mmcs_verify(field, p2_width, *p2, (const MatDesc *)(db + desc_off), (const int *)(db + gs_off),
+                               (const int *)(db + gl_off), n_groups, n, (const uint64_t *)(db + idx_off),
+                               db + proof_off, logmax, db + root_off, (uint32_t *)(db + ok_off), st);
+    if (e == hipSuccess) e = hipMemcpyAsync(flags.data(), db + ok_off, (uint64_t)n * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    dev_free(ctx, d);
+    if (e != hipSuccess) return hip_err(e, "mmcs verify");
+    for (uint32_t q = 0; q < n; q++) ok[q] = flags[q] ? 1 : 0;
+    return ZKG_OK;
+}
+
 // ------------------------------------------------------------------ DEEP quotient
 namespace {
 

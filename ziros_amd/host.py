@@ -286,6 +286,22 @@ class HipMerkleTreeMmcs:
         rows, paths = tree.open_batch([index], width)
         return rows[0], paths[0]
 
+    def verify_batch(self, root, heights, widths, indices, rows, paths):
+        """Mmcs::verify_batch for many openings on the device: rows (n, sum of
+        widths) and paths (n, log2_ceil(max height), digest) as open_mmcs
+        returns them; a bool per opening."""
+        dt = DTYPE[self.field]
+        hs = np.ascontiguousarray(heights, dtype=np.uint64)
+        ws = np.ascontiguousarray(widths, dtype=np.uint32)
+        idx = np.ascontiguousarray(indices, dtype=np.uint64)
+        r = np.ascontiguousarray(rows, dtype=dt)
+        p = np.ascontiguousarray(paths, dtype=dt)
+        rt = np.ascontiguousarray(root, dtype=dt)
+        ok = np.zeros(idx.size, dtype=np.uint8)
+        check(self.ctx.lib.zkg_mmcs_verify_batch(self.ctx.h, self.field, self.p2_width, _ptr(rt), hs.size, _ptr(hs),
+                                                 _ptr(ws), idx.size, _ptr(idx), _ptr(r), _ptr(p), _ptr(ok)))
+        return ok.astype(bool)
+
 
 class HipDeep:
     """DEEP quotient of TwoAdicFriPcs::open over a device-resident LDE
