@@ -36,6 +36,7 @@ enum zkg_status {
     ZKG_ENOMEM = -3,   /* device allocation failed or HBM budget exceeded */
     ZKG_ESTATE = -4,   /* constants not loaded, serving loop not running, ... */
     ZKG_EBUSY = -5,    /* serve ticket not finished yet */
+    ZKG_EFALLBACK = -6, /* the dispatch driver does not take this node: the CPU may */
 };
 
 typedef struct zkg_ctx zkg_ctx;
@@ -339,6 +340,64 @@ int zkg_serve_submit(zkg_ctx *ctx, const zkg_job *job, uint64_t *ticket);
 /* ZKG_OK and *tree set when done, ZKG_EBUSY while queued or running. */
 int zkg_serve_poll(zkg_ctx *ctx, uint64_t ticket, zkg_tree **tree, double *latency_ms);
 int zkg_serve_stop(zkg_ctx *ctx);
+
+/* ---- runtime dispatch driver ------------------------------------------------
+ * The UMPG GPU lane over libzkg: zkf-runtime's GpuDispatchDriver
+ * (zkf-runtime/src/metal_driver.rs:55-93) executing commit-path ProverOps
+ * (graph.rs:69-108) with the node payloads of execution.rs:51-106, and the
+ * BufferBridge slots allocated as device buffers (GpuBufferAllocator::
+ * alloc_shared / free_shared, buffer_bridge.rs:77-84).  Each arm keeps the
+ * semantics of the Metal driver's arm for that op (metal_dispatch_macos.rs:
+ * 305-779); LDE and the BabyBear Poseidon batch and Merkle layer are payloads
+ * the reference lacks, with the same conventions. */
+typedef struct zkg_bridge zkg_bridge;
+int zkg_bridge_create(zkg_ctx *ctx, zkg_bridge **out);
+void zkg_bridge_destroy(zkg_bridge *bridge);
+/* alloc_shared: slot (nonzero; 0 means "no buffer") gets `bytes` of HBM,
+ * replacing what it held; free_shared releases it. */
+int zkg_bridge_alloc(zkg_bridge *bridge, uint32_t slot, uint64_t bytes);
+int zkg_bridge_free(zkg_bridge *bridge, uint32_t slot);
+/* Host bytes into a slot (sized to them) / a slot's bytes out (*bytes = its
+ * size; nothing copied when cap is 0). */
+int zkg_bridge_write(zkg_bridge *bridge, uint32_t slot, const void *src, uint64_t bytes);
+int zkg_bridge_read(const zkg_bridge *bridge, uint32_t slot, void *dst, uint64_t cap, uint64_t *bytes);
+/* The device buffer behind a slot, for the other entry points (NULL if none). */
+zkg_buf *zkg_bridge_buffer(zkg_bridge *bridge, uint32_t slot);
+
+enum zkg_op_kind {
+    ZKG_OP_NTT = 1,          /* ProverOp::Ntt, NttGoldilocks { values_slot } */
+    ZKG_OP_LDE = 2,          /* ProverOp::Lde, Lde{Goldilocks,BabyBear} { values_slot, blowup } */
+    ZKG_OP_POSEIDON_BATCH = 3, /* ProverOp::PoseidonBatch, Poseidon{Goldilocks,BabyBear} */
+    ZKG_OP_MERKLE_LAYER = 4, /* ProverOp::MerkleLayer, Merkle{Goldilocks,BabyBear} (Poseidon2) */
+    ZKG_OP_FRI_FOLD = 5,     /* ProverOp::FriFold, FriFold{Goldilocks,BabyBear} */
+};
+typedef struct zkg_node {
+    uint32_t op;        /* zkg_op_kind */
+    int32_t field;      /* payload variant: ZKG_GOLDILOCKS or ZKG_BABYBEAR */
+    uint64_t size;      /* Ntt/FriFold size, Lde rows, PoseidonBatch count, MerkleLayer leaf_count */
+    uint32_t width;     /* PoseidonBatch width; columns of an Ntt or Lde (1 for a vector) */
+    uint32_t blowup;    /* Lde */
+    uint32_t inverse;   /* Ntt */
+    uint32_t n_ext, n_int; /* PoseidonBatch rounds (with constants in aux_slot) */
+    uint64_t seed;      /* MerkleLayer: the runtime's Poseidon seed */
+    uint32_t in_slot;   /* values / states / leaves / evals */
+    uint32_t aux_slot;  /* PoseidonBatch round constants (0: seed 42); FriFold alpha */
+    uint32_t aux2_slot; /* FriFold inverse twiddles */
+    uint32_t out_slot;  /* the primary output */
+    uint64_t out_bytes; /* MerkleLayer: the output buffer's size, which sets the digest width */
+} zkg_node;
+typedef struct zkg_telemetry {
+    char kernel[48];
+    double ms;
+    uint64_t input_bytes, output_bytes;
+} zkg_telemetry;
+/* GpuDispatchDriver::execute: ZKG_OK with telemetry; ZKG_EFALLBACK (the
+ * FallbackSignal, reason in zkg_last_error) when this lane does not take the
+ * node; another negative code for a hard failure.  Returns when the output
+ * slot is written. */
+int zkg_driver_execute(zkg_ctx *ctx, const zkg_node *node, zkg_bridge *bridge, zkg_telemetry *telemetry);
+/* should_try_on_either: 1 if the node is large enough to be worth a launch. */
+int zkg_driver_should_try(const zkg_node *node);
 
 /* ---- telemetry ------------------------------------------------------------- */
 /* Per-stage device time of the most recent commit / batch on the context's

@@ -12,13 +12,16 @@ from .host import (  # noqa: F401
     BABYBEAR,
     GENERATOR,
     GOLDILOCKS,
+    BufferBridge,
     ConstraintCompiler,
     ConstraintProgram,
     Context,
     DeviceBuffer,
+    FallbackSignal,
     FriCommitment,
     HipConstraintEval,
     HipDeep,
+    HipDispatchDriver,
     HipDft,
     HipFri,
     HipMerkleTreeMmcs,

@@ -18,6 +18,33 @@ c_dbl = ctypes.c_double
 P = ctypes.POINTER
 
 
+class Node(ctypes.Structure):
+    """zkg_node"""
+
+    _fields_ = [
+        ("op", c_u32),
+        ("field", ctypes.c_int32),
+        ("size", c_u64),
+        ("width", c_u32),
+        ("blowup", c_u32),
+        ("inverse", c_u32),
+        ("n_ext", c_u32),
+        ("n_int", c_u32),
+        ("seed", c_u64),
+        ("in_slot", c_u32),
+        ("aux_slot", c_u32),
+        ("aux2_slot", c_u32),
+        ("out_slot", c_u32),
+        ("out_bytes", c_u64),
+    ]
+
+
+class Telemetry(ctypes.Structure):
+    """zkg_telemetry"""
+
+    _fields_ = [("kernel", ctypes.c_char * 48), ("ms", c_dbl), ("input_bytes", c_u64), ("output_bytes", c_u64)]
+
+
 class Job(ctypes.Structure):
     """zkg_job"""
 
@@ -95,6 +122,15 @@ SIGNATURES = {
     "zkg_serve_submit": (c_int, [c_vp, P(Job), P(c_u64)]),
     "zkg_serve_poll": (c_int, [c_vp, c_u64, P(c_vp), P(c_dbl)]),
     "zkg_serve_stop": (c_int, [c_vp]),
+    "zkg_bridge_create": (c_int, [c_vp, P(c_vp)]),
+    "zkg_bridge_destroy": (None, [c_vp]),
+    "zkg_bridge_alloc": (c_int, [c_vp, c_u32, c_u64]),
+    "zkg_bridge_free": (c_int, [c_vp, c_u32]),
+    "zkg_bridge_write": (c_int, [c_vp, c_u32, c_vp, c_u64]),
+    "zkg_bridge_read": (c_int, [c_vp, c_u32, c_vp, c_u64, P(c_u64)]),
+    "zkg_bridge_buffer": (c_vp, [c_vp, c_u32]),
+    "zkg_driver_execute": (c_int, [c_vp, P(Node), c_vp, P(Telemetry)]),
+    "zkg_driver_should_try": (c_int, [P(Node)]),
     "zkg_ctx_set_timing": (c_int, [c_vp, c_int]),
     "zkg_ctx_stage_ms": (c_int, [c_vp, P(c_dbl)]),
 }
@@ -126,6 +162,7 @@ class ZkgError(RuntimeError):
 
 
 EBUSY = -5
+EFALLBACK = -6
 
 
 def check(rc):

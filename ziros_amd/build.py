@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libzkg.so")
 PROBE_LIB = os.path.join(HERE, "libzkg_probe.so")
 SOURCES = ["zkg_ntt.hip", "zkg_hash.hip", "zkg_deep.hip", "zkg_constraints.hip", "zkg_runtime.hip",
-           "zkg_batch.hip"]
+           "zkg_batch.hip", "zkg_driver.hip"]
 PROBE_SOURCES = ["zkg_probe.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -75,7 +75,7 @@ def build(verbose=False):
     """Compile every translation unit and link libzkg.so and libzkg_probe.so."""
     os.makedirs(os.path.join(CSRC, "_obj"), exist_ok=True)
     write_jit_prelude()
-    with ThreadPoolExecutor(max_workers=6) as ex:
+    with ThreadPoolExecutor(max_workers=8) as ex:
         results = list(ex.map(_compile, SOURCES + PROBE_SOURCES))
     if verbose:
         for _, msgs in results:

@@ -111,5 +111,8 @@ void stage_collect(zkg_ctx *ctx, int nmarks);
 void batch_release(zkg_ctx *ctx);
 void serve_shutdown(zkg_ctx *ctx);
 void ce_cache_release(zkg_ctx *ctx);
+// Poseidon2 constants drawn from SmallRng(seed), flat [initial][internal][terminal].
+int p2_seed_constants(int field, int width, uint64_t seed, std::vector<uint64_t> &flat);
+int p2_internal_rounds(int field);
 
 }  // namespace zkg

@@ -590,23 +590,32 @@ extern "C" int zkg_fill_random(zkg_ctx *ctx, int field, zkg_buf *buf, uint64_t s
 }
 
 // ------------------------------------------------------------------ constants
-extern "C" int zkg_constants_load(zkg_ctx *ctx, int field, int width, uint64_t seed)
+// RNG order of Poseidon2::new_from_rng_128 / flatten_round_constants:
+// initial rf/2 x W, terminal rf/2 x W, internal rp; flat layout
+// [initial][internal][terminal].
+int zkg::p2_seed_constants(int field, int width, uint64_t seed, std::vector<uint64_t> &flat)
 {
-    if (!ctx) return set_err(ZKG_EINVAL, "null ctx");
     if (p2_index(field, width) < 0) return set_err(ZKG_EINVAL, "unsupported Poseidon2 field %d width %d", field, width);
-    // RNG order of Poseidon2::new_from_rng_128 / flatten_round_constants:
-    // initial rf/2 x W, terminal rf/2 x W, internal rp; flat layout
-    // [initial][internal][terminal].
     SmallRng r(seed);
     const int half = 4, rp = rounds_p_of(field);
     std::vector<uint64_t> init(half * width), term(half * width), internal(rp);
     for (auto &v : init) v = sample_canon(r, field);
     for (auto &v : term) v = sample_canon(r, field);
     for (auto &v : internal) v = sample_canon(r, field);
-    std::vector<uint64_t> flat;
+    flat.clear();
     flat.insert(flat.end(), init.begin(), init.end());
     flat.insert(flat.end(), internal.begin(), internal.end());
     flat.insert(flat.end(), term.begin(), term.end());
+    return ZKG_OK;
+}
+
+int zkg::p2_internal_rounds(int field) { return rounds_p_of(field); }
+
+extern "C" int zkg_constants_load(zkg_ctx *ctx, int field, int width, uint64_t seed)
+{
+    if (!ctx) return set_err(ZKG_EINVAL, "null ctx");
+    std::vector<uint64_t> flat;
+    if (int rc = p2_seed_constants(field, width, seed, flat)) return rc;
     HIP_TRY(hipStreamSynchronize(ctx->stream), "sync");
     int rc = load_p2_flat(ctx, field, width, flat);
     if (!rc) ctx->p2[p2_index(field, width)].seed = seed;

@@ -530,6 +530,81 @@ class HipConstraintEval:
             ob.free()
 
 
+# ---------------------------------------------------------------- runtime dispatch driver
+OP_NTT, OP_LDE, OP_POSEIDON_BATCH, OP_MERKLE_LAYER, OP_FRI_FOLD = 1, 2, 3, 4, 5
+
+
+class FallbackSignal(ZkgError):
+    """The driver does not take the node; the runtime may run it on the CPU
+    (zkf-runtime/src/metal_driver.rs:40-48)."""
+
+
+class BufferBridge:
+    """BufferBridge slots held in HBM (GpuBufferAllocator over zkg buffers)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        check(ctx.lib.zkg_bridge_create(ctx.h, ctypes.byref(h)))
+        self.h = h
+
+    def alloc_shared(self, slot, nbytes):
+        check(self.ctx.lib.zkg_bridge_alloc(self.h, slot, nbytes))
+
+    def free_shared(self, slot):
+        check(self.ctx.lib.zkg_bridge_free(self.h, slot))
+
+    def write(self, slot, arr):
+        a = np.ascontiguousarray(arr)
+        check(self.ctx.lib.zkg_bridge_write(self.h, slot, _ptr(a) if a.nbytes else None, a.nbytes))
+
+    def read(self, slot, dtype):
+        n = ctypes.c_uint64()
+        check(self.ctx.lib.zkg_bridge_read(self.h, slot, None, 0, ctypes.byref(n)))
+        out = np.empty(n.value // np.dtype(dtype).itemsize, dtype=dtype)
+        check(self.ctx.lib.zkg_bridge_read(self.h, slot, _ptr(out) if out.nbytes else None, out.nbytes,
+                                           ctypes.byref(n)))
+        return out
+
+    def destroy(self):
+        if self.h:
+            self.ctx.lib.zkg_bridge_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                self.destroy()
+        except Exception:
+            pass
+
+
+class HipDispatchDriver:
+    """GpuDispatchDriver (zkf-runtime/src/metal_driver.rs:55-93) over libzkg.
+    execute(node, bridge) runs a commit-path ProverOp whose payload slots live
+    in the bridge; it returns the telemetry, or raises FallbackSignal when
+    this lane does not take the node."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def is_available(self):
+        return True
+
+    def should_try_on_either(self, **node):
+        return bool(self.ctx.lib.zkg_driver_should_try(ctypes.byref(_lib.Node(**node))))
+
+    def execute(self, bridge, **node):
+        nd = _lib.Node(**node)
+        tel = _lib.Telemetry()
+        rc = self.ctx.lib.zkg_driver_execute(self.ctx.h, ctypes.byref(nd), bridge.h, ctypes.byref(tel))
+        if rc == _lib.EFALLBACK:
+            raise FallbackSignal(rc, (self.ctx.lib.zkg_last_error() or b"").decode())
+        check(rc)
+        return dict(kernel=tel.kernel.decode(), ms=tel.ms, input_bytes=tel.input_bytes,
+                    output_bytes=tel.output_bytes)
+
+
 def commit_trace(ctx, field, trace_buf, height, width, log_blowup, p2_width=16, plain=False):
     """Fused (or plain reference-structured) commit of a device-resident trace."""
     es = np.dtype(DTYPE[field]).itemsize
