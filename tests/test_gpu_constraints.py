@@ -357,3 +357,21 @@ def test_quotient_refusals(ctx):
     with pytest.raises(Z.ZkgError):
         prog.quotient(lb, 128, 1, 7, [1], qb)      # buffers too small
     prog.free(); lb.free(); qb.free()
+
+
+@pytest.mark.parametrize("log_n,lb", [(14, 1), (13, 2)])
+def test_quotient_low_degree_at_size(ctx, log_n, lb):
+    """A satisfied AIR at 2^14 / 2^13 trace rows: GPU LDE, compiled quotient,
+    and an inverse DFT on the host show degree exactly N - 2."""
+    from test_oracle_constraint import _degree_bound_holds, _square_air
+    rng = np.random.default_rng(log_n)
+    trace, bc = _square_air(rng, log_n)
+    n = 1 << log_n
+    tb, lde_b, qb = ctx.upload(trace), ctx.alloc((n << lb) * 3 * 8), ctx.alloc((n << lb) * 8)
+    check(ctx.lib.zkg_coset_lde_batch(ctx.h, Z.GOLDILOCKS, tb.h, n, 3, lb, 7, lde_b.h))
+    prog = Z.ConstraintProgram(ctx, 3, bc, [], 2, compiled=True)
+    prog.quotient(lde_b, n << lb, lb, 7, [3, 5], qb)
+    q = qb.download(np.uint64, n << lb)
+    assert _degree_bound_holds(q, log_n + lb, n - 1)
+    assert not _degree_bound_holds(q, log_n + lb, n - 2)
+    prog.free(); tb.free(); lde_b.free(); qb.free()
