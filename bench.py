@@ -477,6 +477,8 @@ def main():
     ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--only-constraints", action="store_true",
                     help="run only the constraint-evaluation leg (for profiling) and print its JSON")
+    ap.add_argument("--only-c1", action="store_true",
+                    help="run only the Goldilocks 2^14 x 8 single-commit leg (for profiling) and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-load", type=float, default=1.5,
                     help="Poisson arrival rate as a multiple of the plain path's capacity")
@@ -499,6 +501,10 @@ def main():
     peak_bb = max(peak_bb, peak_bbf)
     if args.only_constraints:
         print(json.dumps({"constraint_eval": bench_constraints(ctx, max(2, args.steps), peak_gl)}))
+        ctx.close()
+        return
+    if args.only_c1:
+        print(json.dumps({"c1": bench_c1(ctx, max(5, args.steps))}))
         ctx.close()
         return
     log("c2: %d warmup + %d timed steps" % (args.warmup, args.steps))

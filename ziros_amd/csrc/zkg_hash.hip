@@ -13,8 +13,9 @@
 //
 // Design: one Poseidon2 state per lane, held in VGPRs for the whole
 // permutation (constants are wave-uniform scalar loads).  Merkle levels are
-// fused: a block compresses 512 digests down to one, passing each level
-// through LDS, so a 2^21-leaf tree takes three launches.  The FRI fold of a
+// fused: a block compresses its inputs 3 levels down, passing each level
+// through LDS.  The small levels at the top of a tree spread each compression
+// over W lanes, 5 (W = 16) or 6 (W = 8) levels per launch.  The FRI fold of a
 // pair of outputs is fused with the Poseidon2 hash of the next layer's leaf
 // (one leaf = two folded EF4 values = 8 base elements = one absorption).
 #include "zkg_field.h"
@@ -29,6 +30,23 @@ namespace {
 constexpr int HASH_THREADS = 256;
 constexpr int MERKLE_THREADS = 256;
 constexpr int MAX_FUSE = 9;  // 2 * MERKLE_THREADS leaves -> 1 root
+
+// Merkle levels of at most this many digests run with W lanes per
+// compression: below it one compression per lane leaves the chip mostly idle.
+// ZKG_MERKLE_LANES_LOG2=<k> (a tuning override, 0..30) sets 2^k for both
+// widths.
+template <int W>
+uint64_t lanes_split_max()
+{
+    static const int override_log2 = [] {
+        const char *e = getenv("ZKG_MERKLE_LANES_LOG2");
+        int k = -1;
+        if (e && (sscanf(e, "%d", &k) != 1 || k < 0 || k > 30)) k = -1;
+        return k;
+    }();
+    if (override_log2 >= 0) return 1ull << override_log2;
+    return 1ull << (W == 16 ? 17 : 16);
+}
 
 // BabyBear width-16 permutations run in exact FP64 (Bf); the Montgomery form
 // serves the 16-lane tree-top kernel and the FRI fold.
@@ -220,43 +238,51 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_fused(const typename 
     }
 }
 
-// Tree top: the same level fusion with each compression spread over a
-// 16-lane row (width-16 permutations only).  A block (16 rows) compresses 2^k
-// input digests, k <= 5, into levels 1..k.
-constexpr int LANES_FUSE = 5;
+// Tree top: the same level fusion with each compression spread over W lanes
+// (one state element per lane).  A block holds 256 / W compressions and
+// compresses 2^k input digests, k <= log2(256 / W) + 1, into levels 1..k.
+template <int W>
+constexpr int lanes_fuse()
+{
+    return W == 16 ? 5 : 6;
+}
 
-template <class F>
+template <class F, int W>
 __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_lanes(const typename F::T *__restrict__ in,
                                                                  LayerPtrs<typename F::T> out, int k,
                                                                  const typename F::T *__restrict__ rc,
                                                                  const typename F::T *__restrict__ diag)
 {
     using T = typename F::T;
-    __shared__ T lds[16 * 8];
-    const int g = threadIdx.x >> 4, lane = threadIdx.x & 15;
+    constexpr int D = W / 2;
+    using M = LaneMap<W>;
+    static_assert(M::PER_BLOCK * W == MERKLE_THREADS && (2 << (lanes_fuse<W>() - 1)) == 2 * M::PER_BLOCK,
+                  "a block's first level holds 256 / W compressions");
+    __shared__ T lds[M::PER_BLOCK * D];
+    const int g = M::slot(threadIdx.x), e = M::elem(threadIdx.x);
     LaneConsts<F> kc;
-    load_lane_consts<F>(kc, lane, rc, diag);
+    load_lane_consts<F, W>(kc, e, rc, diag);
     int nodes = 1 << (k - 1);
     T x = 0;
     if (g < nodes) {
         const uint64_t i = (uint64_t)blockIdx.x * nodes + g;
-        x = p2_permute_lanes<F>(F::from_canon(in[2 * i * 8 + lane]), lane, kc, rc);
-        if (lane < 8) {
-            out.p[0][i * 8 + lane] = F::to_canon(x);
-            lds[g * 8 + lane] = x;
+        x = p2_permute_lanes<F, W>(F::from_canon(in[2 * i * D + e]), e, kc, rc);
+        if (e < D) {
+            out.p[0][i * D + e] = F::to_canon(x);
+            lds[g * D + e] = x;
         }
     }
     for (int lvl = 1; lvl < k; lvl++) {
         __syncthreads();
         nodes >>= 1;
-        if (g < nodes) x = lds[2 * g * 8 + lane];  // lanes 8..15 read the right child
+        if (g < nodes) x = lds[2 * g * D + e];  // elements D..W-1 read the right child
         __syncthreads();
         if (g < nodes) {
             const uint64_t i = (uint64_t)blockIdx.x * nodes + g;
-            x = p2_permute_lanes<F>(x, lane, kc, rc);
-            if (lane < 8) {
-                out.p[lvl][i * 8 + lane] = F::to_canon(x);
-                lds[g * 8 + lane] = x;
+            x = p2_permute_lanes<F, W>(x, e, kc, rc);
+            if (e < D) {
+                out.p[lvl][i * D + e] = F::to_canon(x);
+                lds[g * D + e] = x;
             }
         }
     }
@@ -471,30 +497,28 @@ hipError_t merkle_impl(const P2Set &p2, void *const *layers, int nlayers, uint64
         return hipSuccess;
     }
     // Large levels: one compression per lane, 3 levels per launch (every
-    // level keeps full waves).  Levels of <= 2^17 digests are latency-bound:
-    // width-16 trees finish with 16-lane compressions, 5 levels per launch;
-    // width-8 trees fuse up to 9 levels per launch.
-    constexpr uint64_t LANE_SPLIT_MAX = 1ull << 17;
+    // level keeps full waves).  Levels of at most lanes_split_max<W>() digests
+    // are latency-bound and finish with W-lane compressions, lanes_fuse<W>()
+    // levels per launch.
+    const uint64_t split = lanes_split_max<W>();
     while (lvl < nlayers - 1) {
         const int left = nlayers - 1 - lvl;
         const uint64_t level_n = n >> lvl;
-        const bool lanes = W == 16 && level_n <= LANE_SPLIT_MAX;
-        int k = lanes ? LANES_FUSE : (level_n <= 512 ? MAX_FUSE : 3);
+        const bool lanes = level_n <= split;
+        int k = lanes ? lanes_fuse<W>() : 3;
         if (k > left) k = left;
         if (k > fuse) k = fuse;
         LayerPtrs<T> lp;
         for (int i = 0; i < MAX_FUSE; i++) lp.p[i] = i < k ? (T *)layers[lvl + 1 + i] : nullptr;
-        if constexpr (W == 16) {
-            if (lanes) {
-                // a block = 16 rows of 16 lanes = 2^k inputs
-                hipLaunchKernelGGL((k_merkle_lanes<F>), dim3((unsigned)(level_n >> k)), dim3(MERKLE_THREADS), 0, st,
-                                   (const T *)layers[lvl], lp, k, (const T *)p2.rc, (const T *)p2.diag);
-                hipError_t e = hipGetLastError();
-                if (e == hipSuccess && sync_each) e = hipStreamSynchronize(st);
-                if (e) return e;
-                lvl += k;
-                continue;
-            }
+        if (lanes) {
+            // a block = 256 / W compressions = up to 2^k inputs
+            hipLaunchKernelGGL((k_merkle_lanes<F, W>), dim3((unsigned)(level_n >> k)), dim3(MERKLE_THREADS), 0, st,
+                               (const T *)layers[lvl], lp, k, (const T *)p2.rc, (const T *)p2.diag);
+            hipError_t e = hipGetLastError();
+            if (e == hipSuccess && sync_each) e = hipStreamSynchronize(st);
+            if (e) return e;
+            lvl += k;
+            continue;
         }
         // a block = nb1 first-level nodes (up to one per lane)
         const uint64_t nb1 = level_n / 2 < MERKLE_THREADS ? level_n / 2 : MERKLE_THREADS;

@@ -1,17 +1,26 @@
-// zkg_poseidon2_lanes.h -- width-16 Poseidon2 with the state spread over 16
-// lanes (lane l of a 16-lane row holds s_l).
+// zkg_poseidon2_lanes.h -- Poseidon2 with the state spread over lanes, one
+// state element per lane (widths 8 and 16).
 //
 // Same permutation as zkg_poseidon2.h (schedule of zkf-metal/src/shaders/
 // poseidon2.metal:17-264); this form trades throughput for latency and is used
-// where few permutations are independent, at the top of a Merkle tree.  The
-// cross-lane steps are DPP moves inside a 16-lane row:
+// where few permutations are independent, at the top of a Merkle tree.
+//
+// Lane layout inside a 16-lane DPP row (lane bits b3 b2 b1 b0):
+//   * width 16: one permutation per row, element e on lane e;
+//   * width 8: two permutations per row, element e = (b3 << 2) | (b1 b0) of
+//     permutation b2.  Each 4-chunk is one quad, and the two chunks of a
+//     permutation sit 8 lanes apart.
+// The cross-lane steps are DPP moves inside the row:
 //   * M4 on each 4-chunk: new_k = (x0+x1+x2+x3) + x_k + 2 x_{k+1} (indices mod
 //     4), the circulant [[2,3,1,1],[1,2,3,1],[1,1,2,3],[3,1,1,2]], with quad
 //     permutes fetching x_{k+1}, x_{k+2}, x_{k+3};
-//   * the sum over chunks of each position: two row rotations by 8 and 4;
-//   * the internal-round sum of all 16 lanes: row rotations by 8, 4, 2, 1.
+//   * the sum over chunks of each position: row rotations by 8 and 4 (width
+//     16), or by 8 (width 8);
+//   * the internal-round sum of a permutation's lanes: row rotations by 8, 4,
+//     2, 1 (width 16), or two quad butterflies and a rotation by 8 (width 8).
 // Per-lane constants (the lane's external round constants and internal
-// diagonal entry) sit in VGPRs; internal round constants are wave-uniform.
+// diagonal entry) sit in VGPRs; internal round constants are wave-uniform and
+// loaded one round ahead.
 #pragma once
 #include "zkg_field.h"
 #include "zkg_poseidon2.h"
@@ -22,15 +31,18 @@ namespace zkg {
 constexpr int DPP_QUAD_ROT1 = 0x39;  // [1,2,3,0]
 constexpr int DPP_QUAD_ROT2 = 0x4E;  // [2,3,0,1]
 constexpr int DPP_QUAD_ROT3 = 0x93;  // [3,0,1,2]
+constexpr int DPP_QUAD_SWAP1 = 0xB1;  // [1,0,3,2]
 constexpr int DPP_ROW_ROR1 = 0x121;
 constexpr int DPP_ROW_ROR2 = 0x122;
 constexpr int DPP_ROW_ROR4 = 0x124;
 constexpr int DPP_ROW_ROR8 = 0x128;
 
+// Every control above reads a lane of the same row, so no lane takes the
+// bound_ctrl zero; setting it spares the copy of an "old" value.
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp32(uint32_t v)
 {
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, true);
 }
 
 template <int CTRL>
@@ -49,50 +61,76 @@ __device__ __forceinline__ typename F::T dpp(typename F::T v)
         return dpp64<CTRL>(v);
 }
 
-template <class F>
+// Element index and permutation slot of a lane (thread index t) within a
+// 256-thread block: 256 / W permutations per block.
+template <int W>
+struct LaneMap {
+    static_assert(W == 8 || W == 16, "lanes form: widths 8 and 16");
+    static constexpr int PER_BLOCK = 256 / W;
+    __device__ static int elem(int t)
+    {
+        if constexpr (W == 16) return t & 15;
+        return (t & 3) | ((t >> 1) & 4);
+    }
+    __device__ static int slot(int t)
+    {
+        if constexpr (W == 16) return t >> 4;
+        return ((t >> 4) << 1) | ((t >> 2) & 1);
+    }
+};
+
+template <class F, int W>
 __device__ __forceinline__ typename F::T mds_light_lanes(typename F::T x)
 {
     using T = typename F::T;
     T r1 = dpp<F, DPP_QUAD_ROT1>(x), r2 = dpp<F, DPP_QUAD_ROT2>(x), r3 = dpp<F, DPP_QUAD_ROT3>(x);
     T quad = F::add(F::add(x, r1), F::add(r2, r3));
     T y = F::add(F::add(quad, x), F::dbl(r1));
-    T t = F::add(y, dpp<F, DPP_ROW_ROR8>(y));
-    T col = F::add(t, dpp<F, DPP_ROW_ROR4>(t));
+    T col = F::add(y, dpp<F, DPP_ROW_ROR8>(y));
+    if constexpr (W == 16) col = F::add(col, dpp<F, DPP_ROW_ROR4>(col));
     return F::add(y, col);
 }
 
-template <class F>
-__device__ __forceinline__ typename F::T row_sum16(typename F::T x)
+template <class F, int W>
+__device__ __forceinline__ typename F::T lanes_sum(typename F::T x)
 {
-    x = F::add(x, dpp<F, DPP_ROW_ROR8>(x));
-    x = F::add(x, dpp<F, DPP_ROW_ROR4>(x));
-    x = F::add(x, dpp<F, DPP_ROW_ROR2>(x));
-    return F::add(x, dpp<F, DPP_ROW_ROR1>(x));
+    if constexpr (W == 16) {
+        x = F::add(x, dpp<F, DPP_ROW_ROR8>(x));
+        x = F::add(x, dpp<F, DPP_ROW_ROR4>(x));
+        x = F::add(x, dpp<F, DPP_ROW_ROR2>(x));
+        return F::add(x, dpp<F, DPP_ROW_ROR1>(x));
+    } else {
+        x = F::add(x, dpp<F, DPP_QUAD_ROT2>(x));
+        x = F::add(x, dpp<F, DPP_QUAD_SWAP1>(x));
+        return F::add(x, dpp<F, DPP_ROW_ROR8>(x));
+    }
 }
 
-// Per-lane constants of one width-16 permutation.
+// Per-lane constants of one permutation.
 template <class F>
 struct LaneConsts {
     typename F::T ext[8];  // initial rounds 0..3, terminal rounds 0..3 (this lane)
     typename F::T diag;    // internal diagonal entry of this lane
 };
 
-template <class F>
-__device__ __forceinline__ void load_lane_consts(LaneConsts<F> &k, int lane, const typename F::T *__restrict__ rc,
+template <class F, int W>
+__device__ __forceinline__ void load_lane_consts(LaneConsts<F> &k, int e, const typename F::T *__restrict__ rc,
                                                  const typename F::T *__restrict__ diag)
 {
-    constexpr int RP = P2Spec<F, 16>::RP;
+    constexpr int RP = P2Spec<F, W>::RP;
+    static_assert(P2Spec<F, W>::RF == 8, "4 + 4 external rounds");
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-        k.ext[r] = rc[r * 16 + lane];
-        k.ext[4 + r] = rc[4 * 16 + RP + r * 16 + lane];
+        k.ext[r] = rc[r * W + e];
+        k.ext[4 + r] = rc[4 * W + RP + r * W + e];
     }
     if constexpr (std::is_same<F, Bb>::value) {
+        static_assert(W == 16, "BabyBear: width 16");
         // diag in register form: V = [-2, 1, 2, 1/2, 3, 4, -1/2, -3, -4, 2^-8,
         // 1/4, 1/8, 2^-27, -2^-8, -1/16, -2^-27] (zkf-metal/src/poseidon2/babybear.rs:14-58)
         const uint32_t one = Bb::one();
         uint32_t v;
-        switch (lane) {
+        switch (e) {
         case 0: v = Bb::neg(Bb::dbl(one)); break;
         case 1: v = one; break;
         case 2: v = Bb::dbl(one); break;
@@ -112,30 +150,34 @@ __device__ __forceinline__ void load_lane_consts(LaneConsts<F> &k, int lane, con
         }
         k.diag = v;
     } else {
-        k.diag = diag[lane];
+        k.diag = diag[e];
     }
 }
 
-// x is this lane's state element; lane = lane index within its 16-lane row.
-template <class F>
-__device__ __forceinline__ typename F::T p2_permute_lanes(typename F::T x, int lane, const LaneConsts<F> &k,
+// x is this lane's state element; e = its element index.
+template <class F, int W>
+__device__ __forceinline__ typename F::T p2_permute_lanes(typename F::T x, int e, const LaneConsts<F> &k,
                                                           const typename F::T *__restrict__ rc)
 {
     using T = typename F::T;
-    constexpr int RP = P2Spec<F, 16>::RP;
-    x = mds_light_lanes<F>(x);
+    constexpr int RP = P2Spec<F, W>::RP;
+    x = mds_light_lanes<F, W>(x);
 #pragma unroll
-    for (int r = 0; r < 4; r++) x = mds_light_lanes<F>(sbox7<F>(F::add(x, k.ext[r])));
-    const T *ir = rc + 4 * 16;
+    for (int r = 0; r < 4; r++) x = mds_light_lanes<F, W>(sbox7<F>(F::add(x, k.ext[r])));
+    // ir[RP] is the first terminal constant, so the look-ahead stays in bounds
+    const T *ir = rc + 4 * W;
+    T c = ir[0];
 #pragma unroll 1
     for (int r = 0; r < RP; r++) {
-        T y = sbox7<F>(F::add(x, ir[r]));
-        x = lane == 0 ? y : x;
-        T sum = row_sum16<F>(x);
+        const T next = ir[r + 1];
+        T y = sbox7<F>(F::add(x, c));
+        x = e == 0 ? y : x;
+        T sum = lanes_sum<F, W>(x);
         x = F::add(F::mul(x, k.diag), sum);
+        c = next;
     }
 #pragma unroll
-    for (int r = 0; r < 4; r++) x = mds_light_lanes<F>(sbox7<F>(F::add(x, k.ext[4 + r])));
+    for (int r = 0; r < 4; r++) x = mds_light_lanes<F, W>(sbox7<F>(F::add(x, k.ext[4 + r])));
     return x;
 }
 
