@@ -280,6 +280,20 @@ struct Bf {
     }
     // x mod p into [-p/2 - 1, p/2 + 1] for |x| < 2^52 (exact).
     __host__ __device__ static inline double red(double x) { return __builtin_fma(-quot(x), P, x); }
+    // x * (+-2^-K mod p) for an integer |x| < 2^52, 1 <= K <= 27, in 4 operations
+    // instead of a product's 6.  y = +-x 2^-K is exact, f = y - floor(y) = r 2^-K
+    // with r = (+-x) mod 2^K, and y - f p = floor(y) - r 15 2^(27-K) is an integer
+    // (p - 1 = 15 2^27) whose 2^K-multiple is +-x - r p.  Every step is exact;
+    // |result| < |x| 2^-K + 2^31.
+    template <int K, bool NEG>
+    __host__ __device__ static inline double mul_inv_pow2(double x)
+    {
+        static_assert(K >= 1 && K <= 27, "shift range");
+        constexpr double c = (NEG ? -1.0 : 1.0) / (double)(1u << K);
+        double y = x * c;
+        double f = y - __builtin_floor(y);
+        return __builtin_fma(-f, P, y);
+    }
     // canonical u32 in [0, p)
     __host__ __device__ static inline uint32_t canon(double x)
     {

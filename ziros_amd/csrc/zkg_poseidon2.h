@@ -221,9 +221,10 @@ __host__ __device__ __forceinline__ void p2_permute(typename F::T (&s)[W], const
 // (capacity lanes carry unreduced values between sponge absorptions); MDS-light
 // multiplies a bound by at most 35, S-box outputs are < 1.6 p, so external
 // rounds see |s| < 57 p < 2^37; internal rounds reduce the sum and the lanes
-// with |V_i| >= 2 before use and the V = 1 lane grows by < p/2 per round, so
-// every product input stays below 2^37 (the product routine needs < 2^41) and
-// every sum below 2^41 (exactness needs < 2^53).
+// with integer |V_i| >= 2 before use, scale the V = +-2^-k lanes exactly
+// (Bf::mul_inv_pow2: |x| 2^-k + 2^31, so they shrink) and the V = 1 lane grows
+// by < p/2 per round, so every product input stays below 2^37 (the product
+// routine needs < 2^41) and every sum below 2^41 (exactness needs < 2^53).
 __host__ __device__ __forceinline__ void mds_light_bf(double (&s)[16])
 {
 #pragma unroll
@@ -248,14 +249,6 @@ __host__ __device__ __forceinline__ double sbox7_bf(double x)
     double x3 = Bf::mul(x2, x);
     double x4 = Bf::mul(x2, x2);
     return Bf::mul(x3, x4);
-}
-
-// canonical representatives of 2^-k (k <= 27) mod p: ((p + 1) / 2)^k
-__host__ __device__ constexpr double bf_inv_pow2(int k)
-{
-    uint64_t r = 1, h = (2013265921ull + 1) / 2;
-    for (int i = 0; i < k; i++) r = r * h % 2013265921ull;
-    return (double)r;
 }
 
 // rc: flat [initial 4x16][internal 13][terminal 4x16], canonical, as doubles.
@@ -283,19 +276,19 @@ __host__ __device__ __forceinline__ void p2_permute_bf(double (&s)[16], const do
         s[0] = __builtin_fma(-2.0, s[0], sum);
         s[1] = s[1] + sum;
         s[2] = __builtin_fma(2.0, Bf::red(s[2]), sum);
-        s[3] = Bf::mul(s[3], bf_inv_pow2(1)) + sum;
+        s[3] = Bf::mul_inv_pow2<1, false>(s[3]) + sum;
         s[4] = __builtin_fma(3.0, Bf::red(s[4]), sum);
         s[5] = __builtin_fma(4.0, Bf::red(s[5]), sum);
-        s[6] = Bf::mul(s[6], -bf_inv_pow2(1)) + sum;
+        s[6] = Bf::mul_inv_pow2<1, true>(s[6]) + sum;
         s[7] = __builtin_fma(-3.0, Bf::red(s[7]), sum);
         s[8] = __builtin_fma(-4.0, Bf::red(s[8]), sum);
-        s[9] = Bf::mul(s[9], bf_inv_pow2(8)) + sum;
-        s[10] = Bf::mul(s[10], bf_inv_pow2(2)) + sum;
-        s[11] = Bf::mul(s[11], bf_inv_pow2(3)) + sum;
-        s[12] = Bf::mul(s[12], bf_inv_pow2(27)) + sum;
-        s[13] = Bf::mul(s[13], -bf_inv_pow2(8)) + sum;
-        s[14] = Bf::mul(s[14], -bf_inv_pow2(4)) + sum;
-        s[15] = Bf::mul(s[15], -bf_inv_pow2(27)) + sum;
+        s[9] = Bf::mul_inv_pow2<8, false>(s[9]) + sum;
+        s[10] = Bf::mul_inv_pow2<2, false>(s[10]) + sum;
+        s[11] = Bf::mul_inv_pow2<3, false>(s[11]) + sum;
+        s[12] = Bf::mul_inv_pow2<27, false>(s[12]) + sum;
+        s[13] = Bf::mul_inv_pow2<8, true>(s[13]) + sum;
+        s[14] = Bf::mul_inv_pow2<4, true>(s[14]) + sum;
+        s[15] = Bf::mul_inv_pow2<27, true>(s[15]) + sum;
     }
     const double *tr = ir + 13;
 #pragma unroll 1
