@@ -405,8 +405,30 @@ __device__ __forceinline__ void load_coefficients(const typename F::T *lds, type
 // matrix.  The coset weights (by k2) and omega_N^(j2 k1) (by j2) are vectors
 // in LDS shared by the tile's columns; the weighting is fused into the first
 // forward round.
+// Occupancy of pass B: at least ZKG_LDE_MID_WAVES32 (BabyBear) or
+// ZKG_LDE_MID_WAVES64 (Goldilocks) waves per SIMD for tiles of up to 2^10
+// points.  These are autotuner knobs; 0 leaves the register budget to the
+// compiler.  2^11-point tiles hold too much LDS for more than one block per CU.
+#ifndef ZKG_LDE_MID_WAVES32
+#define ZKG_LDE_MID_WAVES32 0
+#endif
+#ifndef ZKG_LDE_MID_WAVES64
+#define ZKG_LDE_MID_WAVES64 0
+#endif
 template <class F, int L>
-__global__ __launch_bounds__(NTT_THREADS) void k_lde_mid(const typename F::T *__restrict__ src,
+constexpr int lde_mid_waves()
+{
+    const int w = sizeof(typename F::T) == 4 ? ZKG_LDE_MID_WAVES32 : ZKG_LDE_MID_WAVES64;
+    return L <= 10 && w > 0 ? w : 1;
+}
+#if ZKG_LDE_MID_WAVES32 > 0 || ZKG_LDE_MID_WAVES64 > 0
+#define ZKG_LDE_MID_OCC __attribute__((amdgpu_waves_per_eu(lde_mid_waves<F, L>())))
+#else
+#define ZKG_LDE_MID_OCC
+#endif
+
+template <class F, int L>
+__global__ __launch_bounds__(NTT_THREADS) ZKG_LDE_MID_OCC void k_lde_mid(const typename F::T *__restrict__ src,
                                                          typename F::T *__restrict__ dst, Geo g,
                                                          const typename F::T *__restrict__ tile_inv,
                                                          const typename F::T *__restrict__ tile_fwd,

@@ -4,7 +4,9 @@ The in-LDS transforms split a 2^L-point tile into radix rounds of at most
 2^ZKG_PLAN_RMAX32 (BabyBear) / 2^ZKG_PLAN_RMAX64 (Goldilocks) points per lane.
 The radix fixes the tile shape: columns per tile = 256 * radix / 2^L.  It also
 fixes the kernels' registers and LDS, and with them the blocks per CU.
-ZKG_TILE_STORE_UNROLL sets how far the tile store loops unroll.  The tuner
+ZKG_TILE_STORE_UNROLL sets how far the tile store loops unroll, and
+ZKG_LDE_MID_WAVES32/64 the waves per SIMD (blocks per CU) that LDE pass B is
+compiled for, BabyBear and Goldilocks.  The tuner
 builds every candidate, times the coset LDE of each on shapes that cover the
 batched and served workloads (heights 2^12..2^22, widths 16..256, blowup 2 and
 4), and writes the fastest candidate into csrc/zkg_tuned.h, which the build
@@ -32,6 +34,10 @@ CANDIDATES = {
     "r6_r4_u4": {"ZKG_PLAN_RMAX32": 6, "ZKG_PLAN_RMAX64": 4, "ZKG_TILE_STORE_UNROLL": 4},
     "r6_r4_u16": {"ZKG_PLAN_RMAX32": 6, "ZKG_PLAN_RMAX64": 4, "ZKG_TILE_STORE_UNROLL": 16},
     "r5_r5_u8": {"ZKG_PLAN_RMAX32": 5, "ZKG_PLAN_RMAX64": 5, "ZKG_TILE_STORE_UNROLL": 8},
+    # pass B held to 3 waves per SIMD (168 registers) instead of the compiler's 2
+    "r6_r4_u4_o3": {"ZKG_PLAN_RMAX32": 6, "ZKG_PLAN_RMAX64": 4, "ZKG_TILE_STORE_UNROLL": 4, "ZKG_LDE_MID_WAVES32": 3},
+    "r6_r4_u4_o33": {"ZKG_PLAN_RMAX32": 6, "ZKG_PLAN_RMAX64": 4, "ZKG_TILE_STORE_UNROLL": 4, "ZKG_LDE_MID_WAVES32": 3,
+                     "ZKG_LDE_MID_WAVES64": 3},
 }
 
 # (field, log2 height, width, log_blowup); field 1 = BabyBear, 0 = Goldilocks
