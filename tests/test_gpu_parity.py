@@ -134,7 +134,8 @@ def test_hash_rows_vs_oracle(ctx, field, p2w, width):
 
 @pytest.mark.parametrize("field,p2w,logh,width", [(BB, 16, 0, 5), (BB, 16, 1, 5), (BB, 16, 10, 24), (GL, 16, 12, 8),
                                                   (GL, 8, 11, 8), (BB, 16, 13, 8), (BB, 16, 9, 1),
-                                                  (BB, 16, 19, 8), (GL, 16, 18, 4), (GL, 8, 16, 4)])
+                                                  (BB, 16, 19, 8), (GL, 16, 18, 4), (GL, 8, 16, 4),
+                                                  (GL, 8, 1, 3), (GL, 8, 7, 5), (GL, 8, 18, 2)])
 def test_merkle_layers_and_openings(ctx, field, p2w, logh, width):
     p2 = O.Poseidon2(field, p2w)
     h = 1 << logh
