@@ -62,6 +62,22 @@ def test_lde(ctx, field, blowup):
     br.destroy()
 
 
+@pytest.mark.parametrize("field", [GL, BB])
+def test_lde_in_place(ctx, field):
+    # the output slot is the input slot: the input stays valid until the LDE is done
+    drv, br = Z.HipDispatchDriver(ctx), Z.BufferBridge(ctx)
+    n, w = 1 << 11, 2
+    m = _rand(field, n * w, 23).reshape(n, w)
+    br.write(3, m)
+    drv.execute(br, op=Z.host.OP_LDE, field=field, size=n, width=w, blowup=4, in_slot=3, out_slot=3)
+    exp = O.coset_lde_batch(field, m.astype(np.uint64), 2, 7 if field == GL else 31)
+    assert (br.read(3, DT[field]).reshape(n * 4, w).astype(np.uint64) == exp).all()
+    for bad in (3, 0, 1 << 20):  # not a power of two, zero, beyond any LDE
+        with pytest.raises(Z.ZkgError):
+            drv.execute(br, op=Z.host.OP_LDE, field=field, size=n, width=w, blowup=bad, in_slot=3, out_slot=4)
+    br.destroy()
+
+
 @pytest.mark.parametrize("field,count,width,explicit", [(GL, 37, 12, True), (GL, 20, 16, False), (BB, 25, 5, False),
                                                         (BB, 9, 16, True)])
 def test_poseidon_batch(ctx, field, count, width, explicit):
@@ -119,6 +135,37 @@ def test_fri_fold(ctx, field):
                 out_slot=13)
     exp = O.fri_fold_base(field, ev.astype(np.uint64), alpha, tw.astype(np.uint64))
     assert (br.read(13, DT[field]).astype(np.uint64) == exp).all()
+    br.destroy()
+
+
+@pytest.mark.parametrize("alias", ["evals", "alpha", "twiddles"])
+def test_fri_fold_output_aliases_an_input(ctx, alias):
+    drv, br = Z.HipDispatchDriver(ctx), Z.BufferBridge(ctx)
+    n = 1 << 11
+    ev, tw = _rand(BB, 2 * n, 54), _rand(BB, n, 55)
+    alpha = int(_rand(BB, 1, 56)[0])
+    br.write(10, ev)
+    br.write(11, np.array([alpha], dtype=np.uint32))
+    br.write(12, tw)
+    out = {"evals": 10, "alpha": 11, "twiddles": 12}[alias]
+    drv.execute(br, op=Z.host.OP_FRI_FOLD, field=BB, size=2 * n, in_slot=10, aux_slot=11, aux2_slot=12, out_slot=out)
+    exp = O.fri_fold_base(BB, ev.astype(np.uint64), alpha, tw.astype(np.uint64))
+    assert (br.read(out, np.uint32).astype(np.uint64) == exp).all()
+    br.destroy()
+
+
+def test_fri_fold_needs_a_resident_alpha_slot(ctx):
+    drv, br = Z.HipDispatchDriver(ctx), Z.BufferBridge(ctx)
+    n = 1 << 10
+    br.write(10, _rand(GL, 2 * n, 57))
+    br.write(12, _rand(GL, n, 58))
+    with pytest.raises(Z.ZkgError):
+        drv.execute(br, op=Z.host.OP_FRI_FOLD, field=GL, size=2 * n, in_slot=10, aux_slot=11, aux2_slot=12,
+                    out_slot=13)
+    br.write(11, np.zeros(0, dtype=np.uint64))  # resident but empty: alpha = 0 (the reference's unwrap_or(0))
+    drv.execute(br, op=Z.host.OP_FRI_FOLD, field=GL, size=2 * n, in_slot=10, aux_slot=11, aux2_slot=12, out_slot=13)
+    ev, tw = br.read(10, np.uint64), br.read(12, np.uint64)
+    assert (br.read(13, np.uint64) == O.fri_fold_base(GL, ev, 0, tw)).all()
     br.destroy()
 
 

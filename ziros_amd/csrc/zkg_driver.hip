@@ -56,6 +56,24 @@ int slot_alloc(zkg_bridge *b, uint32_t slot, uint64_t bytes, zkg_buf **out)
     return ZKG_OK;
 }
 
+// The output buffer of a node whose output slot is also one of its inputs:
+// a fresh buffer, put into the slot (freeing the input) only after the node's
+// kernels are done.  Otherwise the slot's own buffer.
+int out_buffer(zkg_bridge *b, uint32_t slot, uint64_t bytes, bool aliases_input, zkg_buf **out, zkg_buf **pending)
+{
+    *pending = nullptr;
+    if (!aliases_input) return slot_alloc(b, slot, bytes, out);
+    if (int rc = zkg_buf_alloc(b->ctx, std::max<uint64_t>(bytes, 1), out)) return rc;
+    (*out)->bytes = bytes;
+    *pending = *out;
+    return ZKG_OK;
+}
+
+void install(zkg_bridge *b, uint32_t slot, zkg_buf *nb)
+{
+    if (zkg_buf *old = slot_buf(b, slot)) zkg_buf_free(old);
+    b->slots[slot] = nb;
+}
 
 const char *field_name(int f) { return f == FIELD_GL ? "goldilocks" : "babybear"; }
 
@@ -246,6 +264,7 @@ extern "C" int zkg_driver_execute(zkg_ctx *ctx, const zkg_node *nd, zkg_bridge *
     uint64_t in_bytes = 0, out_bytes = 0;
     const char *kernel = "";
     int rc = ZKG_OK;
+    zkg_buf *pending = nullptr;  // output to install once the kernels are done
     switch (nd->op) {
     case ZKG_OP_NTT: {  // values transformed in natural order (NttAccelerator::forward_ntt / inverse_ntt)
         const zkg_buf *in = slot_buf(b, nd->in_slot);
@@ -271,11 +290,11 @@ extern "C" int zkg_driver_execute(zkg_ctx *ctx, const zkg_node *nd, zkg_bridge *
         const zkg_buf *in = slot_buf(b, nd->in_slot);
         if (!in) return set_err(ZKG_EINVAL, "slot %u is not resident", nd->in_slot);
         const uint64_t w = std::max<uint32_t>(nd->width, 1), n = in->bytes / es / w;
-        uint32_t lb = 0;
-        while ((1u << lb) < nd->blowup) lb++;
-        if (!is_pow2(nd->blowup)) return set_err(ZKG_EINVAL, "blowup %u is not a power of two", nd->blowup);
+        if (!is_pow2(nd->blowup) || nd->blowup > 256)
+            return set_err(ZKG_EINVAL, "blowup %u is not a power of two <= 256", nd->blowup);
+        const uint32_t lb = (uint32_t)__builtin_ctz(nd->blowup);
         zkg_buf *out = nullptr;
-        rc = slot_alloc(b, nd->out_slot, (n << lb) * w * es, &out);
+        rc = out_buffer(b, nd->out_slot, (n << lb) * w * es, nd->out_slot == nd->in_slot, &out, &pending);
         if (!rc) rc = zkg_coset_lde_batch(ctx, f, in, n, (uint32_t)w, lb, f == FIELD_GL ? 7 : 31, out);
         in_bytes = n * w * es;
         out_bytes = (n << lb) * w * es;
@@ -299,11 +318,14 @@ extern "C" int zkg_driver_execute(zkg_ctx *ctx, const zkg_node *nd, zkg_bridge *
         if (!n_out || tw->bytes / es < n_out)
             return set_err(ZKG_EFALLBACK, "FriFold %s: %llu evaluations, %llu inverse twiddles", field_name(f),
                            (unsigned long long)(ev->bytes / es), (unsigned long long)(tw->bytes / es));
+        // every input slot must be resident; an empty alpha slot reads as 0
+        const zkg_buf *a = slot_buf(b, nd->aux_slot);
+        if (!a) return set_err(ZKG_EINVAL, "FriFold alpha slot %u is not resident", nd->aux_slot);
         uint64_t alpha = 0;
-        if (const zkg_buf *a = slot_buf(b, nd->aux_slot))
-            if (a->bytes >= es) rc = zkg_buf_download(a, 0, &alpha, es);
+        if (a->bytes >= es) rc = zkg_buf_download(a, 0, &alpha, es);
+        const bool alias = nd->out_slot == nd->in_slot || nd->out_slot == nd->aux_slot || nd->out_slot == nd->aux2_slot;
         zkg_buf *out = nullptr;
-        if (!rc) rc = slot_alloc(b, nd->out_slot, n_out * es, &out);
+        if (!rc) rc = out_buffer(b, nd->out_slot, n_out * es, alias, &out, &pending);
         if (!rc) rc = zkg_fri_fold(ctx, f, ev, n_out, alpha, tw, out);
         in_bytes = ev->bytes;
         out_bytes = n_out * es;
@@ -314,6 +336,12 @@ extern "C" int zkg_driver_execute(zkg_ctx *ctx, const zkg_node *nd, zkg_bridge *
         return set_err(ZKG_EFALLBACK, "op %u is not on the HIP lane", nd->op);
     }
     if (!rc) rc = zkg_ctx_synchronize(ctx);
+    if (pending) {
+        if (rc)
+            zkg_buf_free(pending);
+        else
+            install(b, nd->out_slot, pending);
+    }
     if (rc) return rc;
     if (tel) {
         snprintf(tel->kernel, sizeof tel->kernel, "%s", kernel);
