@@ -31,21 +31,63 @@ constexpr int HASH_THREADS = 256;
 constexpr int MERKLE_THREADS = 256;
 constexpr int MAX_FUSE = 9;  // 2 * MERKLE_THREADS leaves -> 1 root
 
-// Merkle levels of at most this many digests run with W lanes per
-// compression: below it one compression per lane leaves the chip mostly idle.
-// ZKG_MERKLE_LANES_LOG2=<k> (a tuning override, 0..30) sets 2^k for both
-// widths.
-template <int W>
+// Goldilocks work-form thresholds (log2; see lanes_split_max below).
+#ifndef ZKG_GL16_LANES_LOG2
+#define ZKG_GL16_LANES_LOG2 13
+#endif
+#ifndef ZKG_GL8_LANES_LOG2
+#define ZKG_GL8_LANES_LOG2 14
+#endif
+#ifndef ZKG_GL16_CHUNKS_LOG2
+#define ZKG_GL16_CHUNKS_LOG2 16
+#endif
+#ifndef ZKG_GL8_CHUNKS_LOG2
+#define ZKG_GL8_CHUNKS_LOG2 16
+#endif
+#ifndef ZKG_GL16_HASH_LOG2
+#define ZKG_GL16_HASH_LOG2 15
+#endif
+#ifndef ZKG_GL8_HASH_LOG2
+#define ZKG_GL8_HASH_LOG2 15
+#endif
+
+// A tuning override from the environment: 2^k for NAME=<k>, 0 <= k <= 30,
+// else `dflt`.
+uint64_t env_pow2(const char *name, uint64_t dflt)
+{
+    const char *e = getenv(name);
+    int k = -1;
+    if (!e || sscanf(e, "%d", &k) != 1 || k < 0 || k > 30) return dflt;
+    return 1ull << k;
+}
+
+// Merkle levels of at most lanes_split_max digests run with W lanes per
+// compression (one element each); Goldilocks levels above that and of at most
+// chunks_split_max digests with W/4 lanes per compression (one 4-chunk each);
+// larger levels with one compression per lane.  Row hashes of Goldilocks
+// matrices of at most hash_chunks_max rows run with W/4 lanes per row.  Below
+// these sizes one compression (or row) per lane leaves the chip mostly idle,
+// while the W-lane form repeats the internal S-box in every lane and has the
+// most work per compression.  ZKG_MERKLE_LANES_LOG2, ZKG_MERKLE_CHUNKS_LOG2
+// and ZKG_HASH_CHUNKS_LOG2 are tuning overrides (for every field and width).
+template <class F, int W>
 uint64_t lanes_split_max()
 {
-    static const int override_log2 = [] {
-        const char *e = getenv("ZKG_MERKLE_LANES_LOG2");
-        int k = -1;
-        if (e && (sscanf(e, "%d", &k) != 1 || k < 0 || k > 30)) k = -1;
-        return k;
-    }();
-    if (override_log2 >= 0) return 1ull << override_log2;
-    return 1ull << (W == 16 ? 17 : 16);
+    constexpr int dflt = std::is_same<F, Bb>::value ? 17 : (W == 16 ? ZKG_GL16_LANES_LOG2 : ZKG_GL8_LANES_LOG2);
+    static const uint64_t v = env_pow2("ZKG_MERKLE_LANES_LOG2", 1ull << dflt);
+    return v;
+}
+template <int W>
+uint64_t chunks_split_max()
+{
+    static const uint64_t v = env_pow2("ZKG_MERKLE_CHUNKS_LOG2", 1ull << (W == 16 ? ZKG_GL16_CHUNKS_LOG2 : ZKG_GL8_CHUNKS_LOG2));
+    return v;
+}
+template <int W>
+uint64_t hash_chunks_max()
+{
+    static const uint64_t v = env_pow2("ZKG_HASH_CHUNKS_LOG2", 1ull << (W == 16 ? ZKG_GL16_HASH_LOG2 : ZKG_GL8_HASH_LOG2));
+    return v;
 }
 
 // BabyBear width-16 permutations run in exact FP64 (Bf); the Montgomery form
@@ -141,6 +183,51 @@ __global__ __launch_bounds__(HASH_THREADS) void k_hash_rows(const typename F::T 
     }
 #pragma unroll
     for (int k = 0; k < R; k++) digests[row * R + k] = F::to_canon(s[k]);
+}
+
+// The same sponge for Goldilocks with one row per L = W/4 lanes (one 4-chunk
+// of the state per lane, zkg_poseidon2_lanes.h): for matrices too short to
+// give every SIMD several waves at one row per lane.  Lanes j < R/4 hold the
+// rate.
+template <int W>
+__global__ __launch_bounds__(HASH_THREADS) void k_hash_rows_chunks(const uint64_t *__restrict__ mat, uint64_t height,
+                                                                   uint32_t width, uint64_t *__restrict__ digests,
+                                                                   const uint64_t *__restrict__ rc,
+                                                                   const uint64_t *__restrict__ diag)
+{
+    constexpr int R = W / 2, L = ChunkMap<W>::L;
+    const int j = threadIdx.x & (L - 1);
+    const uint64_t row = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / L;
+    if (row >= height) return;  // a whole lane group leaves together
+    uint64_t d[4], s[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        d[k] = diag[4 * j + k];
+        s[k] = 0;
+    }
+    const uint64_t *src = mat + row * width + 4 * j;
+    const bool rate_lane = 4 * j < R;
+    uint32_t pos = 0;
+    for (; pos + R <= width; pos += R) {
+        if (rate_lane) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) s[k] = src[pos + k];
+        }
+        p2_permute_gl_chunk<W>(s, j, d, rc);
+    }
+    if (pos < width) {
+        const uint32_t take = width - pos;
+        if (rate_lane) {
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if ((uint32_t)(4 * j + k) < take) s[k] = src[pos + k];
+        }
+        p2_permute_gl_chunk<W>(s, j, d, rc);
+    }
+    if (rate_lane) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) digests[row * R + 4 * j + k] = s[k];
+    }
 }
 
 // ---------------------------------------------------------------- Merkle
@@ -283,6 +370,68 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_lanes(const typename 
             if (e < D) {
                 out.p[lvl][i * D + e] = F::to_canon(x);
                 lds[g * D + e] = x;
+            }
+        }
+    }
+}
+
+// Middle levels of a Goldilocks tree: level fusion as in k_merkle_fused with
+// one 4-chunk of the state per lane (L = W/4 lanes per compression).  Block b
+// compresses input digests [2 b nb1, 2 (b+1) nb1) into k levels, nb1 <=
+// 256 / L first-level compressions and nb1 >= 2^(k-1); level l node i goes to
+// out.p[l-1][i].
+template <int W>
+constexpr int chunks_fuse()
+{
+    return W == 16 ? 7 : 8;
+}
+
+template <int W>
+__global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_chunks(const uint64_t *__restrict__ in,
+                                                                  LayerPtrs<uint64_t> out, int k, int nb1,
+                                                                  const uint64_t *__restrict__ rc,
+                                                                  const uint64_t *__restrict__ diag)
+{
+    constexpr int D = W / 2, L = ChunkMap<W>::L, PER = ChunkMap<W>::PER_BLOCK;
+    static_assert((2 << (chunks_fuse<W>() - 1)) == 2 * PER && chunks_fuse<W>() <= MAX_FUSE,
+                  "a block's first level holds 256 / L compressions");
+    __shared__ uint64_t lds[PER * D];
+    const int j = threadIdx.x & (L - 1), g = threadIdx.x / L;
+    const bool digest_lane = 4 * j < D;
+    uint64_t d[4], s[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) d[e] = diag[4 * j + e];
+    int nodes = nb1;
+    if (g < nodes) {
+        const uint64_t i = (uint64_t)blockIdx.x * nodes + g;
+#pragma unroll
+        for (int e = 0; e < 4; e++) s[e] = in[2 * i * D + 4 * j + e];
+        p2_permute_gl_chunk<W>(s, j, d, rc);
+        if (digest_lane) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                out.p[0][i * D + 4 * j + e] = s[e];
+                lds[g * D + 4 * j + e] = s[e];
+            }
+        }
+    }
+    for (int lvl = 1; lvl < k; lvl++) {
+        __syncthreads();
+        nodes >>= 1;
+        if (g < nodes) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) s[e] = lds[2 * g * D + 4 * j + e];  // chunks D/4.. read the right child
+        }
+        __syncthreads();
+        if (g < nodes) {
+            const uint64_t i = (uint64_t)blockIdx.x * nodes + g;
+            p2_permute_gl_chunk<W>(s, j, d, rc);
+            if (digest_lane) {
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    out.p[lvl][i * D + 4 * j + e] = s[e];
+                    lds[g * D + 4 * j + e] = s[e];
+                }
             }
         }
     }
@@ -497,19 +646,46 @@ hipError_t merkle_impl(const P2Set &p2, void *const *layers, int nlayers, uint64
         return hipSuccess;
     }
     // Large levels: one compression per lane, 3 levels per launch (every
-    // level keeps full waves).  Levels of at most lanes_split_max<W>() digests
-    // are latency-bound and finish with W-lane compressions, lanes_fuse<W>()
-    // levels per launch.
-    const uint64_t split = lanes_split_max<W>();
+    // level keeps full waves).  Goldilocks levels of at most chunks_split_max
+    // digests: W/4 lanes per compression, down to the lanes split.  Levels of
+    // at most lanes_split_max digests are latency-bound and finish with W-lane
+    // compressions, lanes_fuse<W>() levels per launch.
+    const uint64_t split = lanes_split_max<F, W>();
+    constexpr bool GL = std::is_same<F, Gl>::value;
+    const uint64_t csplit = GL ? chunks_split_max<W>() : 0;
     while (lvl < nlayers - 1) {
         const int left = nlayers - 1 - lvl;
         const uint64_t level_n = n >> lvl;
         const bool lanes = level_n <= split;
+        const bool chunks = !lanes && level_n <= csplit;
         int k = lanes ? lanes_fuse<W>() : 3;
+        if (chunks) {
+            // stop where the W-lane form takes over: a fused level with few
+            // compressions costs the chunk form's longer chain
+            k = chunks_fuse<W>();
+            int to_split = 0;
+            while ((level_n >> to_split) > split) to_split++;
+            if (k > to_split) k = to_split;
+        }
         if (k > left) k = left;
         if (k > fuse) k = fuse;
         LayerPtrs<T> lp;
         for (int i = 0; i < MAX_FUSE; i++) lp.p[i] = i < k ? (T *)layers[lvl + 1 + i] : nullptr;
+        if constexpr (GL) {
+            if (chunks) {
+                // a block = nb1 <= 256 / (W/4) first-level compressions
+                const uint64_t per = ChunkMap<W>::PER_BLOCK;
+                const uint64_t nb1 = level_n / 2 < per ? level_n / 2 : per;
+                hipLaunchKernelGGL((k_merkle_chunks<W>), dim3((unsigned)(level_n / 2 / nb1)), dim3(MERKLE_THREADS), 0,
+                                   st, (const uint64_t *)layers[lvl], lp, k, (int)nb1, (const uint64_t *)p2.rc,
+                                   (const uint64_t *)p2.diag);
+                hipError_t e = hipGetLastError();
+                if (e == hipSuccess && sync_each) e = hipStreamSynchronize(st);
+                if (e) return e;
+                lvl += k;
+                continue;
+            }
+        }
         if (lanes) {
             // a block = 256 / W compressions = up to 2^k inputs
             hipLaunchKernelGGL((k_merkle_lanes<F, W>), dim3((unsigned)(level_n >> k)), dim3(MERKLE_THREADS), 0, st,
@@ -760,6 +936,14 @@ hipError_t launch_hash_rows(int field, int width, const P2Set &p2, const void *m
         hipLaunchKernelGGL((k_hash_rows<Bb, 16>), g, b, 0, st, (const uint32_t *)mat, height, mat_width,
                            (uint32_t *)digests, (const uint32_t *)p2.rc, (const uint32_t *)p2.diag,
                            (const double *)p2.rc_f64);
+    else if (field == FIELD_GL && width == 16 && height <= hash_chunks_max<16>())
+        hipLaunchKernelGGL((k_hash_rows_chunks<16>), dim3((unsigned)((height * 4 + HASH_THREADS - 1) / HASH_THREADS)),
+                           b, 0, st, (const uint64_t *)mat, height, mat_width, (uint64_t *)digests,
+                           (const uint64_t *)p2.rc, (const uint64_t *)p2.diag);
+    else if (field == FIELD_GL && width == 8 && height <= hash_chunks_max<8>())
+        hipLaunchKernelGGL((k_hash_rows_chunks<8>), dim3((unsigned)((height * 2 + HASH_THREADS - 1) / HASH_THREADS)),
+                           b, 0, st, (const uint64_t *)mat, height, mat_width, (uint64_t *)digests,
+                           (const uint64_t *)p2.rc, (const uint64_t *)p2.diag);
     else if (field == FIELD_GL && width == 16)
         hipLaunchKernelGGL((k_hash_rows<Gl, 16>), g, b, 0, st, (const uint64_t *)mat, height, mat_width,
                            (uint64_t *)digests, (const uint64_t *)p2.rc, (const uint64_t *)p2.diag, nullptr);

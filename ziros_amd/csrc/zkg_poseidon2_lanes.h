@@ -181,4 +181,100 @@ __device__ __forceinline__ typename F::T p2_permute_lanes(typename F::T x, int e
     return x;
 }
 
+// ---------------------------------------------------------------------------
+// Goldilocks Poseidon2 (widths 8 and 16) with one 4-chunk of the state per
+// lane: L = W / 4 adjacent lanes of a quad hold one permutation, lane j
+// elements 4j..4j+3.  Between one state per lane (least work, longest chain)
+// and one element per lane (shortest chain, each lane repeating the internal
+// S-box): about 1.3x the first's work at a quarter of its chain for W = 16.
+// M4 is local to a lane; the sums over chunks (MDS-light) and over the state
+// (internal rounds) are quad butterflies on the lazy three-word sums (GlWide),
+// with the same lazy reduction as p2_permute_gl.
+template <int W>
+struct ChunkMap {
+    static_assert(W == 8 || W == 16, "chunk form: widths 8 and 16");
+    static constexpr int L = W / 4;  // lanes per permutation
+    static constexpr int PER_BLOCK = 256 / L;
+};
+
+template <int CTRL>
+__device__ __forceinline__ GlWide glw_dpp(GlWide v)
+{
+    return GlWide{dpp32<CTRL>(v.w0), dpp32<CTRL>(v.w1), dpp32<CTRL>(v.top)};
+}
+
+// sum of v over the L lanes of a permutation
+template <int W>
+__device__ __forceinline__ GlWide chunk_sum(GlWide v)
+{
+    v = glw_add(v, glw_dpp<DPP_QUAD_SWAP1>(v));
+    if constexpr (W == 16) v = glw_add(v, glw_dpp<DPP_QUAD_ROT2>(v));
+    return v;
+}
+
+template <int W>
+__device__ __forceinline__ void mds_light_gl_chunk(uint64_t (&s)[4])
+{
+    const uint64_t a = s[0], b = s[1], cc = s[2], d = s[3];
+    const GlWide t = glw_add(glw_add(glw(a), b), glw_add(glw(cc), d));
+    GlWide y[4];
+    y[0] = glw_add(glw_add(t, a), glw_dbl(b));   // 2a + 3b + c + d
+    y[1] = glw_add(glw_add(t, b), glw_dbl(cc));  // a + 2b + 3c + d
+    y[2] = glw_add(glw_add(t, cc), glw_dbl(d));  // a + b + 2c + 3d
+    y[3] = glw_add(glw_add(t, d), glw_dbl(a));   // 3a + b + c + 2d
+#pragma unroll
+    for (int k = 0; k < 4; k++) s[k] = glw_fold(glw_add(y[k], chunk_sum<W>(y[k])));
+}
+
+// External rounds r0..r0+3 over this lane's chunk; er = the round block's
+// constants (4 rounds x W) offset to the chunk.  Each round's constants are
+// loaded one round ahead.
+template <int W>
+__device__ __forceinline__ void gl_chunk_external(uint64_t (&s)[4], const uint64_t *__restrict__ er)
+{
+    uint64_t c[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) c[k] = er[k];
+#pragma unroll 1
+    for (int r = 0; r < 4; r++) {
+        const int rn = r + 1 < 4 ? r + 1 : r;
+        uint64_t cn[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) cn[k] = er[rn * W + k];
+#pragma unroll
+        for (int k = 0; k < 4; k++) s[k] = sbox7_gl(Gl::add_const_lazy(s[k], c[k]));
+        mds_light_gl_chunk<W>(s);
+#pragma unroll
+        for (int k = 0; k < 4; k++) c[k] = cn[k];
+    }
+}
+
+// s = this lane's chunk (j = chunk index), d = diag[4j..4j+3]; rc flat
+// [initial 4W][internal 22][terminal 4W], canonical.  Canonical out.
+template <int W>
+__device__ __forceinline__ void p2_permute_gl_chunk(uint64_t (&s)[4], int j, const uint64_t (&d)[4],
+                                                    const uint64_t *__restrict__ rc)
+{
+    constexpr int RP = P2Spec<Gl, W>::RP;
+    mds_light_gl_chunk<W>(s);
+    gl_chunk_external<W>(s, rc + 4 * j);
+    // ir[RP] is the first terminal constant, so the look-ahead stays in bounds
+    const uint64_t *ir = rc + 4 * W;
+    uint64_t ci = ir[0];
+#pragma unroll 1
+    for (int r = 0; r < RP; r++) {
+        const uint64_t next = ir[r + 1];
+        const uint64_t y = sbox7_gl(Gl::add_const_lazy(s[0], ci));
+        s[0] = j == 0 ? y : s[0];
+        const GlWide acc = glw_add(glw_add(glw(s[0]), s[1]), glw_add(glw(s[2]), s[3]));
+        const uint64_t sum = glw_fold(chunk_sum<W>(acc));
+#pragma unroll
+        for (int k = 0; k < 4; k++) s[k] = Gl::mul_add_lazy(s[k], d[k], sum);
+        ci = next;
+    }
+    gl_chunk_external<W>(s, ir + RP + 4 * j);
+#pragma unroll
+    for (int k = 0; k < 4; k++) s[k] = Gl::canon(s[k]);
+}
+
 }  // namespace zkg
