@@ -15,7 +15,9 @@
 // permutation (constants are wave-uniform scalar loads).  Merkle levels are
 // fused: a block compresses its inputs 3 levels down, passing each level
 // through LDS.  The small levels at the top of a tree spread each compression
-// over W lanes, 5 (W = 16) or 6 (W = 8) levels per launch.  The FRI fold of a
+// over W lanes, 5 (W = 16) or 6 (W = 8) levels per launch.  Goldilocks work
+// of middle size (short matrices, the first levels below the fused ones) runs
+// with one 4-chunk of the state per lane.  The FRI fold of a
 // pair of outputs is fused with the Poseidon2 hash of the next layer's leaf
 // (one leaf = two folded EF4 values = 8 base elements = one absorption).
 #include "zkg_field.h"
