@@ -61,6 +61,12 @@ __device__ __forceinline__ typename F::T dpp(typename F::T v)
         return dpp64<CTRL>(v);
 }
 
+template <int CTRL>
+__device__ __forceinline__ GlWide glw_dpp(GlWide v)
+{
+    return GlWide{dpp32<CTRL>(v.w0), dpp32<CTRL>(v.w1), dpp32<CTRL>(v.top)};
+}
+
 // Element index and permutation slot of a lane (thread index t) within a
 // 256-thread block: 256 / W permutations per block.
 template <int W>
@@ -154,11 +160,68 @@ __device__ __forceinline__ void load_lane_consts(LaneConsts<F> &k, int e, const 
     }
 }
 
+// Goldilocks in the lazy representation of p2_permute_gl (any u64
+// representative; one reduction per product; sums as three-word GlWide, folded
+// once), with the cross-lane steps of mds_light_lanes / lanes_sum on all three
+// words.  MDS-light outputs sum at most 35 terms, as in mds_light_gl.
+template <int W>
+__device__ __forceinline__ uint64_t mds_light_lanes_gl(uint64_t x)
+{
+    const uint64_t r1 = dpp64<DPP_QUAD_ROT1>(x), r2 = dpp64<DPP_QUAD_ROT2>(x), r3 = dpp64<DPP_QUAD_ROT3>(x);
+    const GlWide quad = glw_add(glw_add(glw(x), r1), glw_add(glw(r2), r3));
+    const GlWide y = glw_add(glw_add(quad, x), glw_dbl(r1));
+    GlWide col = glw_add(y, glw_dpp<DPP_ROW_ROR8>(y));
+    if constexpr (W == 16) col = glw_add(col, glw_dpp<DPP_ROW_ROR4>(col));
+    return glw_fold(glw_add(y, col));
+}
+
+template <int W>
+__device__ __forceinline__ uint64_t lanes_sum_gl(uint64_t x)
+{
+    GlWide v = glw(x);
+    if constexpr (W == 16) {
+        v = glw_add(v, glw_dpp<DPP_ROW_ROR8>(v));
+        v = glw_add(v, glw_dpp<DPP_ROW_ROR4>(v));
+        v = glw_add(v, glw_dpp<DPP_ROW_ROR2>(v));
+        v = glw_add(v, glw_dpp<DPP_ROW_ROR1>(v));
+    } else {
+        v = glw_add(v, glw_dpp<DPP_QUAD_ROT2>(v));
+        v = glw_add(v, glw_dpp<DPP_QUAD_SWAP1>(v));
+        v = glw_add(v, glw_dpp<DPP_ROW_ROR8>(v));
+    }
+    return glw_fold(v);
+}
+
+template <int W>
+__device__ __forceinline__ uint64_t p2_permute_lanes_gl(uint64_t x, int e, const LaneConsts<Gl> &k,
+                                                        const uint64_t *__restrict__ rc)
+{
+    constexpr int RP = P2Spec<Gl, W>::RP;
+    x = mds_light_lanes_gl<W>(x);
+#pragma unroll
+    for (int r = 0; r < 4; r++) x = mds_light_lanes_gl<W>(sbox7_gl(Gl::add_const_lazy(x, k.ext[r])));
+    // ir[RP] is the first terminal constant, so the look-ahead stays in bounds
+    const uint64_t *ir = rc + 4 * W;
+    uint64_t c = ir[0];
+#pragma unroll 1
+    for (int r = 0; r < RP; r++) {
+        const uint64_t next = ir[r + 1];
+        const uint64_t y = sbox7_gl(Gl::add_const_lazy(x, c));
+        x = e == 0 ? y : x;
+        x = Gl::mul_add_lazy(x, k.diag, lanes_sum_gl<W>(x));
+        c = next;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) x = mds_light_lanes_gl<W>(sbox7_gl(Gl::add_const_lazy(x, k.ext[4 + r])));
+    return Gl::canon(x);
+}
+
 // x is this lane's state element; e = its element index.
 template <class F, int W>
 __device__ __forceinline__ typename F::T p2_permute_lanes(typename F::T x, int e, const LaneConsts<F> &k,
                                                           const typename F::T *__restrict__ rc)
 {
+    if constexpr (std::is_same<F, Gl>::value) return p2_permute_lanes_gl<W>(x, e, k, rc);
     using T = typename F::T;
     constexpr int RP = P2Spec<F, W>::RP;
     x = mds_light_lanes<F, W>(x);
@@ -196,12 +259,6 @@ struct ChunkMap {
     static constexpr int L = W / 4;  // lanes per permutation
     static constexpr int PER_BLOCK = 256 / L;
 };
-
-template <int CTRL>
-__device__ __forceinline__ GlWide glw_dpp(GlWide v)
-{
-    return GlWide{dpp32<CTRL>(v.w0), dpp32<CTRL>(v.w1), dpp32<CTRL>(v.top)};
-}
 
 // sum of v over the L lanes of a permutation
 template <int W>
