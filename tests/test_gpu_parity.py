@@ -163,6 +163,10 @@ MMCS_SHAPES = [
     [(1, 5)],  # a single row: the root is the row digest
     [(33, 1), (17, 12), (9, 8), (5, 3), (3, 2), (2, 6)],
     [(64, 17), (64, 0), (32, 1)],  # a zero-width matrix contributes nothing
+    # larger trees: plain runs between injections take the single-matrix tree kernels
+    [(4096, 5), (4096, 3), (512, 7), (64, 2)],
+    [(16384, 4)],  # one matrix: the single-matrix row hash and tree
+    [(8192, 3), (1000, 2)],  # a non-power-of-two height injected at 1024
 ]
 
 
@@ -182,7 +186,11 @@ def test_mmcs_mixed_heights(ctx, field, p2w, shape_id):
     maxh = max(h for h, _ in shapes)
     logmax = (maxh - 1).bit_length()
     tw = sum(w for _, w in shapes)
-    idx = np.array(sorted({0, maxh - 1, maxh // 2, (maxh * 3) // 7}), dtype=np.uint64)
+    shift = [logmax - (h - 1).bit_length() for h, _ in shapes]
+    cand = {0, maxh - 1, maxh // 2, (maxh * 3) // 7} | {(h - 1) << sh for (h, _), sh in zip(shapes, shift)}
+    # an index whose row lies in the padding of a shorter matrix has no opening (p3 panics on it)
+    idx = np.array(sorted(i for i in cand if all((i >> sh) < h for (h, _), sh in zip(shapes, shift))),
+                   dtype=np.uint64)
     rows, paths = tree.open_mmcs(idx, tw)
     for k, i in enumerate(idx):
         i = int(i)

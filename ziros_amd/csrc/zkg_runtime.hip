@@ -1010,12 +1010,30 @@ extern "C" int zkg_mmcs_commit(zkg_ctx *ctx, int field, int p2_width, uint32_t n
     hipStream_t st = ctx->stream;
     hipError_t e = hipMemcpyAsync(t->desc, desc.data(), desc.size() * sizeof(MatDesc), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemsetAsync(t->base, 0, off, st);  // zero-digest padding
-    if (e == hipSuccess) e = launch_mmcs_leaves(field, p2_width, *p2, d + group[0].first, (int)group[0].second, maxh,
-                                                t->layers[0], st);
-    for (int i = 1; i < t->nlayers && e == hipSuccess; i++) {
+    // A tallest group of one matrix is the single-matrix row hash; a run of
+    // layers with nothing injected and power-of-two lengths is a plain tree
+    // segment.  Both take the single-matrix kernels; other layers compress and
+    // inject one node per lane.
+    if (e == hipSuccess) {
+        if (group[0].second == 1)
+            e = launch_hash_rows(field, p2_width, *p2, mats[ord[0]]->ptr, maxh, widths[ord[0]], t->layers[0], st);
+        else
+            e = launch_mmcs_leaves(field, p2_width, *p2, d + group[0].first, (int)group[0].second, maxh,
+                                   t->layers[0], st);
+    }
+    auto plain = [&](int i) { return group[i].second == 0 && is_pow2(len[i - 1]) && len[i] * 2 == len[i - 1]; };
+    for (int i = 1; i < t->nlayers && e == hipSuccess;) {
+        if (plain(i)) {
+            int j = i + 1;
+            while (j < t->nlayers && plain(j)) j++;
+            e = launch_merkle_tree(field, p2_width, *p2, &t->layers[i - 1], j - i + 1, len[i - 1], 9, st, false);
+            i = j;
+            continue;
+        }
         const uint32_t g0 = group[i].first, gn = group[i].second;
         e = launch_mmcs_layer(field, p2_width, *p2, t->layers[i - 1], len[i - 1] / 2, t->layers[i], d + g0, (int)gn,
                               gn ? heights[ord[g0]] : 0, st);
+        i++;
     }
     // the host descriptor vector goes out of scope: finish the upload first
     if (e == hipSuccess) e = hipStreamSynchronize(st);
