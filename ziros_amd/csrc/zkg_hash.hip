@@ -754,16 +754,60 @@ __device__ void group_row_hash(const MatDesc *__restrict__ g, int n, uint64_t ro
     }
 }
 
+// The same sponge for BabyBear width 16 in exact FP64 (p2_permute_bf reduces
+// the carried state on entry).
+__device__ void group_row_hash_bf(const MatDesc *__restrict__ g, int n, uint64_t row, double (&s)[16],
+                                  const double *__restrict__ rcf)
+{
+#pragma unroll
+    for (int e = 0; e < 16; e++) s[e] = 0.0;
+    int m = 0;
+    uint64_t k = 0;
+    auto next_elem = [&](double &v) -> bool {
+        while (m < n && k >= g[m].width) {
+            m++;
+            k = 0;
+        }
+        if (m >= n) return false;
+        v = Bf::from_u32(((const uint32_t *)g[m].ptr)[row * g[m].width + k]);
+        k++;
+        return true;
+    };
+    bool more = true;
+    while (more) {
+        int filled = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            double v;
+            if (more && next_elem(v)) {
+                s[j] = v;
+                filled++;
+            } else {
+                more = false;
+            }
+        }
+        if (filled) p2_permute_bf(s, rcf);
+    }
+}
+
 template <class F, int W>
 __global__ __launch_bounds__(HASH_THREADS) void k_mmcs_leaves(const MatDesc *__restrict__ g, int n, uint64_t rows,
                                                               typename F::T *__restrict__ digests,
                                                               const typename F::T *__restrict__ rc,
-                                                              const typename F::T *__restrict__ diag)
+                                                              const typename F::T *__restrict__ diag,
+                                                              const double *__restrict__ rcf)
 {
     using T = typename F::T;
     constexpr int D = W / 2;
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= rows) return;
+    if constexpr (IS_BB16<F, W>) {
+        double s[16];
+        group_row_hash_bf(g, n, i, s, rcf);
+#pragma unroll
+        for (int e = 0; e < 8; e++) digests[i * 8 + e] = Bf::canon(s[e]);
+        return;
+    }
     T s[W];
     group_row_hash<F, W>(g, n, i, s, rc, diag);
 #pragma unroll
@@ -775,12 +819,34 @@ __global__ __launch_bounds__(HASH_THREADS) void k_mmcs_layer(const typename F::T
                                                              typename F::T *__restrict__ next,
                                                              const MatDesc *__restrict__ inj, int n_inj,
                                                              uint64_t inj_len, const typename F::T *__restrict__ rc,
-                                                             const typename F::T *__restrict__ diag)
+                                                             const typename F::T *__restrict__ diag,
+                                                             const double *__restrict__ rcf)
 {
     using T = typename F::T;
     constexpr int D = W / 2;
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= half) return;
+    if constexpr (IS_BB16<F, W>) {
+        double s[16];
+#pragma unroll
+        for (int e = 0; e < 16; e++) s[e] = Bf::from_u32(prev[2 * i * 8 + e]);
+        p2_permute_bf(s, rcf);
+        if (n_inj) {
+            double r[16];
+            if (i < inj_len) {
+                group_row_hash_bf(inj, n_inj, i, r, rcf);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 16; e++) r[e] = 0.0;
+            }
+#pragma unroll
+            for (int e = 0; e < 8; e++) s[8 + e] = r[e];
+            p2_permute_bf(s, rcf);
+        }
+#pragma unroll
+        for (int e = 0; e < 8; e++) next[i * 8 + e] = Bf::canon(s[e]);
+        return;
+    }
     T s[W];
 #pragma unroll
     for (int e = 0; e < W; e++) s[e] = F::from_canon(prev[2 * i * D + e]);
@@ -880,13 +946,13 @@ hipError_t launch_mmcs_leaves(int field, int width, const P2Set &p2, const MatDe
     dim3 g((unsigned)((rows + HASH_THREADS - 1) / HASH_THREADS)), b(HASH_THREADS);
     if (field == FIELD_BB && width == 16)
         hipLaunchKernelGGL((k_mmcs_leaves<Bb, 16>), g, b, 0, st, group, n, rows, (uint32_t *)digests,
-                           (const uint32_t *)p2.rc, (const uint32_t *)p2.diag);
+                           (const uint32_t *)p2.rc, (const uint32_t *)p2.diag, (const double *)p2.rc_f64);
     else if (field == FIELD_GL && width == 16)
         hipLaunchKernelGGL((k_mmcs_leaves<Gl, 16>), g, b, 0, st, group, n, rows, (uint64_t *)digests,
-                           (const uint64_t *)p2.rc, (const uint64_t *)p2.diag);
+                           (const uint64_t *)p2.rc, (const uint64_t *)p2.diag, nullptr);
     else if (field == FIELD_GL && width == 8)
         hipLaunchKernelGGL((k_mmcs_leaves<Gl, 8>), g, b, 0, st, group, n, rows, (uint64_t *)digests,
-                           (const uint64_t *)p2.rc, (const uint64_t *)p2.diag);
+                           (const uint64_t *)p2.rc, (const uint64_t *)p2.diag, nullptr);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -899,13 +965,14 @@ hipError_t launch_mmcs_layer(int field, int width, const P2Set &p2, const void *
     dim3 g((unsigned)((half + HASH_THREADS - 1) / HASH_THREADS)), b(HASH_THREADS);
     if (field == FIELD_BB && width == 16)
         hipLaunchKernelGGL((k_mmcs_layer<Bb, 16>), g, b, 0, st, (const uint32_t *)prev, half, (uint32_t *)next, inj,
-                           n_inj, inj_len, (const uint32_t *)p2.rc, (const uint32_t *)p2.diag);
+                           n_inj, inj_len, (const uint32_t *)p2.rc, (const uint32_t *)p2.diag,
+                           (const double *)p2.rc_f64);
     else if (field == FIELD_GL && width == 16)
         hipLaunchKernelGGL((k_mmcs_layer<Gl, 16>), g, b, 0, st, (const uint64_t *)prev, half, (uint64_t *)next, inj,
-                           n_inj, inj_len, (const uint64_t *)p2.rc, (const uint64_t *)p2.diag);
+                           n_inj, inj_len, (const uint64_t *)p2.rc, (const uint64_t *)p2.diag, nullptr);
     else if (field == FIELD_GL && width == 8)
         hipLaunchKernelGGL((k_mmcs_layer<Gl, 8>), g, b, 0, st, (const uint64_t *)prev, half, (uint64_t *)next, inj,
-                           n_inj, inj_len, (const uint64_t *)p2.rc, (const uint64_t *)p2.diag);
+                           n_inj, inj_len, (const uint64_t *)p2.rc, (const uint64_t *)p2.diag, nullptr);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
