@@ -25,9 +25,18 @@ constexpr int TW_SPLIT = 12;      // omega^e = lo[e & 4095] * hi[e >> 12]
 #ifndef ZKG_PLAN_RMAX64
 #define ZKG_PLAN_RMAX64 4
 #endif
-__host__ __device__ constexpr int plan_rmax(int elem_bytes)
+// Goldilocks tiles of at most 2^ZKG_PLAN_SMALL_L64 points use radix rounds of
+// at most 2^ZKG_PLAN_RMAX64_SMALL points per lane: more lanes per column, so
+// narrow matrices fill the block (columns per tile = 256 * radix / 2^L).
+#ifndef ZKG_PLAN_RMAX64_SMALL
+#define ZKG_PLAN_RMAX64_SMALL ZKG_PLAN_RMAX64
+#endif
+#ifndef ZKG_PLAN_SMALL_L64
+#define ZKG_PLAN_SMALL_L64 0
+#endif
+__host__ __device__ constexpr int plan_rmax(int elem_bytes, int L)
 {
-    return elem_bytes == 4 ? ZKG_PLAN_RMAX32 : ZKG_PLAN_RMAX64;
+    return elem_bytes == 4 ? ZKG_PLAN_RMAX32 : (L <= ZKG_PLAN_SMALL_L64 ? ZKG_PLAN_RMAX64_SMALL : ZKG_PLAN_RMAX64);
 }
 __host__ __device__ constexpr int plan_nr(int rmax, int L)
 {

@@ -69,7 +69,7 @@ __device__ __forceinline__ typename F::T top_pow(const typename F::T *__restrict
 // digit first), at most 2^6 (BabyBear) / 2^4 (Goldilocks) points per lane.
 template <class F, int L>
 struct Plan {
-    static constexpr int RMAXLOG = plan_rmax(sizeof(typename F::T));
+    static constexpr int RMAXLOG = plan_rmax(sizeof(typename F::T), L);
     static constexpr int NR = plan_nr(RMAXLOG, L);
     static constexpr int logr(int i) { return plan_logr(RMAXLOG, L, i); }
     static constexpr int LOGRMAX = logr(0);
@@ -639,9 +639,9 @@ template <class F>
 void fill_round_twiddles_t(typename F::T w2048, typename F::T *tab)
 {
     using T = typename F::T;
-    constexpr int RM = plan_rmax(sizeof(T));
     for (int i = 0; i < RTW_ENTRIES; i++) tab[i] = 0;
     for (int L = 1; L <= LOG_TW_TILE; L++) {
+        const int RM = plan_rmax(sizeof(T), L);
         const int nr = plan_nr(RM, L);
         int shift = 0;
         for (int I = 0; I + 1 < nr; I++) {

@@ -4,13 +4,15 @@ The in-LDS transforms split a 2^L-point tile into radix rounds of at most
 2^ZKG_PLAN_RMAX32 (BabyBear) / 2^ZKG_PLAN_RMAX64 (Goldilocks) points per lane.
 The radix fixes the tile shape: columns per tile = 256 * radix / 2^L.  It also
 fixes the kernels' registers and LDS, and with them the blocks per CU.
-ZKG_TILE_STORE_UNROLL sets how far the tile store loops unroll, and
-ZKG_LDE_MID_WAVES32/64 the waves per SIMD (blocks per CU) that LDE pass B is
-compiled for, BabyBear and Goldilocks.  The tuner
-builds every candidate, times the coset LDE of each on shapes that cover the
-batched and served workloads (heights 2^12..2^22, widths 16..256, blowup 2 and
-4), and writes the fastest candidate into csrc/zkg_tuned.h, which the build
-picks up:
+ZKG_PLAN_RMAX64_SMALL applies to Goldilocks tiles of at most
+2^ZKG_PLAN_SMALL_L64 points.  ZKG_TILE_STORE_UNROLL sets how far the tile
+store loops unroll, and ZKG_LDE_MID_WAVES32/64 the waves per SIMD (blocks per
+CU) that LDE pass B is compiled for, BabyBear and Goldilocks.  The tuner builds
+every candidate and times the coset LDE of each on shapes that cover the
+single, batched and served workloads (heights 2^13..2^22, widths 8..256,
+blowup 2 and 4).  It writes the candidate with the least score (the sum over
+shapes of its time relative to the best candidate on that shape) into
+csrc/zkg_tuned.h, which the build picks up:
 
     python -m ziros_amd.tune --build     # compile the candidates (no GPU needed)
     python -m ziros_amd.tune --measure   # time them on the GPU, write zkg_tuned.h
@@ -38,12 +40,15 @@ CANDIDATES = {
     "r6_r4_u4_o3": {"ZKG_PLAN_RMAX32": 6, "ZKG_PLAN_RMAX64": 4, "ZKG_TILE_STORE_UNROLL": 4, "ZKG_LDE_MID_WAVES32": 3},
     "r6_r4_u4_o33": {"ZKG_PLAN_RMAX32": 6, "ZKG_PLAN_RMAX64": 4, "ZKG_TILE_STORE_UNROLL": 4, "ZKG_LDE_MID_WAVES32": 3,
                      "ZKG_LDE_MID_WAVES64": 3},
+    # Goldilocks tiles of up to 2^8 points with 2^3-point rounds: narrow small matrices fill the block
+    "r6_r4s3_u8": {"ZKG_PLAN_RMAX32": 6, "ZKG_PLAN_RMAX64": 4, "ZKG_PLAN_RMAX64_SMALL": 3, "ZKG_PLAN_SMALL_L64": 8,
+                   "ZKG_TILE_STORE_UNROLL": 8},
 }
 
 # (field, log2 height, width, log_blowup); field 1 = BabyBear, 0 = Goldilocks
 SHAPES = [
     (1, 20, 64, 1), (1, 22, 16, 1), (1, 16, 128, 2), (1, 13, 256, 1), (1, 18, 48, 1),
-    (0, 20, 64, 1), (0, 18, 64, 2), (0, 16, 32, 1), (0, 21, 16, 1), (0, 14, 96, 2),
+    (0, 20, 64, 1), (0, 18, 64, 2), (0, 16, 32, 1), (0, 21, 16, 1), (0, 14, 96, 2), (0, 14, 8, 1),
 ]
 
 
@@ -102,21 +107,29 @@ def measure(out_path=TUNED_H):
             raise SystemExit("timing %s failed:\n%s" % (name, r.stderr[-2000:]))
         results[name] = json.loads(r.stdout.strip().splitlines()[-1])
         print(name, " ".join("%.3f" % t for t in results[name]), flush=True)
-    best = min(results, key=lambda k: sum(results[k]))
+    best = min(results, key=lambda k: score(results, k))
     write_header(best, results, out_path)
     print("best", best, "->", out_path, flush=True)
+
+
+def score(results, name):
+    """Sum over shapes of the candidate's time relative to the best time on
+    that shape: every shape weighs the same, whatever its size."""
+    best = [min(ts[i] for ts in results.values()) for i in range(len(SHAPES))]
+    return sum(t / b for t, b in zip(results[name], best))
 
 
 def write_header(best, results, out_path):
     lines = [
         "// zkg_tuned.h -- LDE tile plan chosen by ziros_amd/tune.py (python -m ziros_amd.tune --measure)",
         "// on %s.  Coset LDE milliseconds per shape (field: 1 BabyBear, 0 Goldilocks;" % _device_name(),
-        "// log2 height, width, log_blowup), median of 3; the candidate with the least total wins.",
+        "// log2 height, width, log_blowup), median of 3.  Score = sum over shapes of the time relative to",
+        "// the best candidate on that shape; the least score wins.",
         "//",
-        "// %-10s %s  total" % ("candidate", " ".join("%d/%d/%d/%d" % s for s in SHAPES)),
+        "// %-12s %s  score" % ("candidate", " ".join("%d/%d/%d/%d" % s for s in SHAPES)),
     ]
-    for name, ts in sorted(results.items(), key=lambda kv: sum(kv[1])):
-        lines.append("// %-10s %s  %.3f" % (name, " ".join("%.3f" % t for t in ts), sum(ts)))
+    for name in sorted(results, key=lambda k: score(results, k)):
+        lines.append("// %-12s %s  %.3f" % (name, " ".join("%.3f" % t for t in results[name]), score(results, name)))
     lines.append("#pragma once")
     for k, v in sorted(CANDIDATES[best].items()):
         lines.append("#define %s %d" % (k, v))
