@@ -1,9 +1,13 @@
-// zkg_poseidon2_lanes.h -- Poseidon2 with the state spread over lanes, one
-// state element per lane (widths 8 and 16).
+// zkg_poseidon2_lanes.h -- Poseidon2 with the state spread over lanes.
 //
 // Same permutation as zkg_poseidon2.h (schedule of zkf-metal/src/shaders/
-// poseidon2.metal:17-264); this form trades throughput for latency and is used
-// where few permutations are independent, at the top of a Merkle tree.
+// poseidon2.metal:17-264) in two forms that trade throughput for latency,
+// for work with few independent permutations:
+//   * one state element per lane (widths 8 and 16; BabyBear in Montgomery
+//     form, Goldilocks in the lazy form of p2_permute_gl): the top of a
+//     Merkle tree;
+//   * Goldilocks with one 4-chunk of the state per lane (further below): row
+//     hashes of short matrices and the Merkle levels between.
 //
 // Lane layout inside a 16-lane DPP row (lane bits b3 b2 b1 b0):
 //   * width 16: one permutation per row, element e on lane e;
@@ -221,27 +225,30 @@ template <class F, int W>
 __device__ __forceinline__ typename F::T p2_permute_lanes(typename F::T x, int e, const LaneConsts<F> &k,
                                                           const typename F::T *__restrict__ rc)
 {
-    if constexpr (std::is_same<F, Gl>::value) return p2_permute_lanes_gl<W>(x, e, k, rc);
-    using T = typename F::T;
-    constexpr int RP = P2Spec<F, W>::RP;
-    x = mds_light_lanes<F, W>(x);
+    if constexpr (std::is_same<F, Gl>::value) {
+        return p2_permute_lanes_gl<W>(x, e, k, rc);
+    } else {
+        using T = typename F::T;
+        constexpr int RP = P2Spec<F, W>::RP;
+        x = mds_light_lanes<F, W>(x);
 #pragma unroll
-    for (int r = 0; r < 4; r++) x = mds_light_lanes<F, W>(sbox7<F>(F::add(x, k.ext[r])));
-    // ir[RP] is the first terminal constant, so the look-ahead stays in bounds
-    const T *ir = rc + 4 * W;
-    T c = ir[0];
+        for (int r = 0; r < 4; r++) x = mds_light_lanes<F, W>(sbox7<F>(F::add(x, k.ext[r])));
+        // ir[RP] is the first terminal constant, so the look-ahead stays in bounds
+        const T *ir = rc + 4 * W;
+        T c = ir[0];
 #pragma unroll 1
-    for (int r = 0; r < RP; r++) {
-        const T next = ir[r + 1];
-        T y = sbox7<F>(F::add(x, c));
-        x = e == 0 ? y : x;
-        T sum = lanes_sum<F, W>(x);
-        x = F::add(F::mul(x, k.diag), sum);
-        c = next;
-    }
+        for (int r = 0; r < RP; r++) {
+            const T next = ir[r + 1];
+            T y = sbox7<F>(F::add(x, c));
+            x = e == 0 ? y : x;
+            T sum = lanes_sum<F, W>(x);
+            x = F::add(F::mul(x, k.diag), sum);
+            c = next;
+        }
 #pragma unroll
-    for (int r = 0; r < 4; r++) x = mds_light_lanes<F, W>(sbox7<F>(F::add(x, k.ext[4 + r])));
-    return x;
+        for (int r = 0; r < 4; r++) x = mds_light_lanes<F, W>(sbox7<F>(F::add(x, k.ext[4 + r])));
+        return x;
+    }
 }
 
 // ---------------------------------------------------------------------------
