@@ -15,8 +15,9 @@ constexpr int TW_SPLIT = 12;      // omega^e = lo[e & 4095] * hi[e >> 12]
 // Radix split of a 2^L-point in-LDS transform: NR rounds of at most 2^rmax
 // points per lane (ZKG_PLAN_RMAX32 / ZKG_PLAN_RMAX64), larger first.
 // Plan parameters chosen by the autotuner (ziros_amd/tune.py) when it has
-// written zkg_tuned.h; the defaults below otherwise.
-#if __has_include("zkg_tuned.h")
+// written zkg_tuned.h; the defaults below otherwise.  The autotuner's own
+// candidate builds (ZKG_TUNING_CANDIDATE) take only their -D parameters.
+#if !defined(ZKG_TUNING_CANDIDATE) && __has_include("zkg_tuned.h")
 #include "zkg_tuned.h"
 #endif
 #ifndef ZKG_PLAN_RMAX32

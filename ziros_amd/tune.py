@@ -43,6 +43,11 @@ CANDIDATES = {
     # Goldilocks tiles of up to 2^8 points with 2^3-point rounds: narrow small matrices fill the block
     "r6_r4s3_u8": {"ZKG_PLAN_RMAX32": 6, "ZKG_PLAN_RMAX64": 4, "ZKG_PLAN_RMAX64_SMALL": 3, "ZKG_PLAN_SMALL_L64": 8,
                    "ZKG_TILE_STORE_UNROLL": 8},
+    "r6_r4s3_u4": {"ZKG_PLAN_RMAX32": 6, "ZKG_PLAN_RMAX64": 4, "ZKG_PLAN_RMAX64_SMALL": 3, "ZKG_PLAN_SMALL_L64": 8,
+                   "ZKG_TILE_STORE_UNROLL": 4},
+    # BabyBear 2^10-point tiles in three rounds (2^4, 2^3, 2^3): fewer live registers per lane
+    "r4_r4s3_u8": {"ZKG_PLAN_RMAX32": 4, "ZKG_PLAN_RMAX64": 4, "ZKG_PLAN_RMAX64_SMALL": 3, "ZKG_PLAN_SMALL_L64": 8,
+                   "ZKG_TILE_STORE_UNROLL": 8},
 }
 
 # (field, log2 height, width, log_blowup); field 1 = BabyBear, 0 = Goldilocks
@@ -53,7 +58,8 @@ SHAPES = [
 
 
 def _defines(cfg):
-    return ["-D%s=%d" % kv for kv in sorted(cfg.items())]
+    # ZKG_TUNING_CANDIDATE: the installed zkg_tuned.h must not override the candidate
+    return ["-DZKG_TUNING_CANDIDATE=1"] + ["-D%s=%d" % kv for kv in sorted(cfg.items())]
 
 
 def build_candidates():
