@@ -43,9 +43,21 @@ __host__ __device__ constexpr int plan_nr(int rmax, int L)
 {
     return ((L + rmax - 1) / rmax < 2 && L >= 4) ? 2 : ((L + rmax - 1) / rmax < 1 ? 1 : (L + rmax - 1) / rmax);
 }
+// ZKG_PLAN_FIRST_HEAVY: a two-round plan whose second round would keep at
+// least 2^(rmax-2) points gives its first round the full 2^rmax (the even
+// split otherwise): wider tiles (columns per tile = 256 * 2^logr(0) / 2^L),
+// so longer row segments in HBM.
+#ifndef ZKG_PLAN_FIRST_HEAVY
+#define ZKG_PLAN_FIRST_HEAVY 0
+#endif
+__host__ __device__ constexpr bool plan_first_heavy(int rmax, int L)
+{
+    return ZKG_PLAN_FIRST_HEAVY && plan_nr(rmax, L) == 2 && L - rmax >= rmax - 2;
+}
 __host__ __device__ constexpr int plan_logr(int rmax, int L, int i)
 {
-    return L / plan_nr(rmax, L) + (i < L % plan_nr(rmax, L) ? 1 : 0);
+    return plan_first_heavy(rmax, L) ? (i == 0 ? rmax : L - rmax)
+                                     : L / plan_nr(rmax, L) + (i < L % plan_nr(rmax, L) ? 1 : 0);
 }
 // Inter-round twiddles of the 2^L plan, round I (not the last; at most two
 // such rounds): omega_Mi^(a q) at rtw_offset(L, I) + q (Mi / R) + a, where Mi

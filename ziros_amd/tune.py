@@ -45,6 +45,9 @@ CANDIDATES = {
                    "ZKG_TILE_STORE_UNROLL": 8},
     "r6_r4s3_u4": {"ZKG_PLAN_RMAX32": 6, "ZKG_PLAN_RMAX64": 4, "ZKG_PLAN_RMAX64_SMALL": 3, "ZKG_PLAN_SMALL_L64": 8,
                    "ZKG_TILE_STORE_UNROLL": 4},
+    # two-round plans with the full radix first (BabyBear 2^10 tiles as 2^6 x 2^4: 16-column tiles)
+    "r6h_r4s3_u8": {"ZKG_PLAN_RMAX32": 6, "ZKG_PLAN_RMAX64": 4, "ZKG_PLAN_RMAX64_SMALL": 3, "ZKG_PLAN_SMALL_L64": 8,
+                    "ZKG_TILE_STORE_UNROLL": 8, "ZKG_PLAN_FIRST_HEAVY": 1},
     # BabyBear 2^10-point tiles in three rounds (2^4, 2^3, 2^3): fewer live registers per lane
     "r4_r4s3_u8": {"ZKG_PLAN_RMAX32": 4, "ZKG_PLAN_RMAX64": 4, "ZKG_PLAN_RMAX64_SMALL": 3, "ZKG_PLAN_SMALL_L64": 8,
                    "ZKG_TILE_STORE_UNROLL": 8},
