@@ -132,10 +132,25 @@ def test_hash_rows_vs_oracle(ctx, field, p2w, width):
     assert (root.astype(np.uint64) == exp.root()).all()
 
 
+@pytest.mark.parametrize("p2w", [8, 16])
+@pytest.mark.parametrize("h", [1 << 15, (1 << 15) + 1])
+def test_hash_rows_at_the_chunk_split(ctx, p2w, h):
+    # Goldilocks row hashes of up to 2^15 rows run with W/4 lanes per row, longer ones with a row per lane
+    # (a one-matrix MMCS commit takes the same row hash, at any height)
+    p2 = O.Poseidon2(GL, p2w)
+    m = rnd(GL, h * 6, 77 + p2w).reshape(h, 6)
+    root, tree = Z.HipMerkleTreeMmcs(ctx, GL, p2w).commit([m])
+    exp = O.MerkleTree(p2, [m])
+    assert (tree.layer(0).astype(np.uint64) == exp.layers[0]).all()
+    assert (root.astype(np.uint64) == exp.root()).all()
+    tree.free()
+
+
 @pytest.mark.parametrize("field,p2w,logh,width", [(BB, 16, 0, 5), (BB, 16, 1, 5), (BB, 16, 10, 24), (GL, 16, 12, 8),
                                                   (GL, 8, 11, 8), (BB, 16, 13, 8), (BB, 16, 9, 1),
                                                   (BB, 16, 19, 8), (GL, 16, 18, 4), (GL, 8, 16, 4),
-                                                  (GL, 8, 1, 3), (GL, 8, 7, 5), (GL, 8, 18, 2)])
+                                                  (GL, 8, 1, 3), (GL, 8, 7, 5), (GL, 8, 18, 2),
+                                                  (GL, 16, 16, 3), (GL, 16, 14, 2)])
 def test_merkle_layers_and_openings(ctx, field, p2w, logh, width):
     p2 = O.Poseidon2(field, p2w)
     h = 1 << logh
