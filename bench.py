@@ -184,6 +184,45 @@ def bench_c4(ctx, steps):
                 device_ms=dev / steps, codeword_elems_per_s=(1 << log_len) / dt)
 
 
+def bench_mmcs(ctx, steps):
+    """Mixed-height MMCS commit (p3 MerkleTree::new over several matrices,
+    compress_and_inject) on device-resident matrices: a BabyBear 2^20 x 64
+    trace with a 2^18 x 32 matrix injected two layers up, two BabyBear
+    2^20 x 32 matrices hashed as one row stream, and a Goldilocks 2^18 x 16
+    trace with a 2^15 x 8 matrix injected."""
+    import numpy as np  # noqa: F811
+
+    out = {}
+    for name, field, p2w, shapes in (("babybear_injected", Z.BABYBEAR, 16, [(1 << 20, 64), (1 << 18, 32)]),
+                                     ("babybear_pair", Z.BABYBEAR, 16, [(1 << 20, 32), (1 << 20, 32)]),
+                                     ("goldilocks_injected", Z.GOLDILOCKS, 16, [(1 << 18, 16), (1 << 15, 8)])):
+        bufs = [ctx.random(field, h * w, 0x5EED0100 + k) for k, (h, w) in enumerate(shapes)]
+        n = len(bufs)
+        handles = (ctypes.c_void_p * n)(*[b.h for b in bufs])
+        hs = np.array([h for h, _ in shapes], dtype=np.uint64)
+        ws = np.array([w for _, w in shapes], dtype=np.uint32)
+
+        def once():
+            t = ctypes.c_void_p()
+            check(ctx.lib.zkg_mmcs_commit(ctx.h, field, p2w, n, handles, hs.ctypes.data_as(ctypes.c_void_p),
+                                          ws.ctypes.data_as(ctypes.c_void_p), ctypes.byref(t)))
+            return t
+
+        for _ in range(2):
+            free_tree(ctx, once())
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            free_tree(ctx, once())
+        ctx.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        cells = sum(h * w for h, w in shapes)
+        out[name] = dict(shapes=[list(x) for x in shapes], ms_per_commit=dt * 1e3, cells_per_s=cells / dt)
+        for b in bufs:
+            b.free()
+    return out
+
+
 def bench_deep(ctx, steps):
     """PCS opening, DEEP quotient over the config-2 LDE (BabyBear 2^21 x 64 rows,
     bit-reversed on the coset 31*H): opened values at one EF4 point, then the
@@ -562,6 +601,9 @@ def main():
     log("deep: PCS opening DEEP quotient over the config-2 LDE")
     out["pcs_open_deep"] = bench_deep(ctx, max(5, args.steps))
     log("deep: %s" % out["pcs_open_deep"])
+    log("mmcs: mixed-height commits")
+    out["mmcs"] = bench_mmcs(ctx, max(5, args.steps // 2))
+    log("mmcs: %s" % out["mmcs"])
     log("constraints: 32 degree-3 constraints over a Goldilocks 2^21 x 64 LDE")
     out["constraint_eval"] = bench_constraints(ctx, max(5, args.steps), peak_gl)
     log("constraints: %s" % out["constraint_eval"])
