@@ -36,6 +36,8 @@ constexpr int DPP_QUAD_ROT1 = 0x39;  // [1,2,3,0]
 constexpr int DPP_QUAD_ROT2 = 0x4E;  // [2,3,0,1]
 constexpr int DPP_QUAD_ROT3 = 0x93;  // [3,0,1,2]
 constexpr int DPP_QUAD_SWAP1 = 0xB1;  // [1,0,3,2]
+constexpr int DPP_QUAD_0TO1 = 0xE0;   // [0,0,2,3]: position 1 reads position 0
+constexpr int DPP_QUAD_1TO0 = 0xE5;   // [1,1,2,3]: position 0 reads position 1
 constexpr int DPP_ROW_ROR1 = 0x121;
 constexpr int DPP_ROW_ROR2 = 0x122;
 constexpr int DPP_ROW_ROR4 = 0x124;
@@ -207,10 +209,16 @@ __device__ __forceinline__ uint64_t p2_permute_lanes_gl(uint64_t x, int e, const
     // ir[RP] is the first terminal constant, so the look-ahead stays in bounds
     const uint64_t *ir = rc + 4 * W;
     uint64_t c = ir[0];
+    // Only element 0 takes the S-box.  Its chain a^2 -> (a^3, a^4) -> a^7 is
+    // shared with element 1's lane, in the same quad: one product forms a^3
+    // there and a^4 here, which saves the second squaring of every lane.
 #pragma unroll 1
     for (int r = 0; r < RP; r++) {
         const uint64_t next = ir[r + 1];
-        const uint64_t y = sbox7_gl(Gl::add_const_lazy(x, c));
+        const uint64_t a = Gl::add_const_lazy(x, c);
+        const uint64_t a2 = dpp64<DPP_QUAD_0TO1>(Gl::sqr_lazy(a));  // lanes of elements 0 and 1: a^2
+        const uint64_t p = Gl::mul_lazy(a2, e == 0 ? a : a2);        // element 0: a^3, element 1: a^4
+        const uint64_t y = Gl::mul_lazy(p, dpp64<DPP_QUAD_1TO0>(p));  // element 0: a^7
         x = e == 0 ? y : x;
         x = Gl::mul_add_lazy(x, k.diag, lanes_sum_gl<W>(x));
         c = next;
