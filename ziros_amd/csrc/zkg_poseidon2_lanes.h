@@ -38,6 +38,8 @@ constexpr int DPP_QUAD_ROT3 = 0x93;  // [3,0,1,2]
 constexpr int DPP_QUAD_SWAP1 = 0xB1;  // [1,0,3,2]
 constexpr int DPP_QUAD_0TO1 = 0xE0;   // [0,0,2,3]: position 1 reads position 0
 constexpr int DPP_QUAD_1TO0 = 0xE5;   // [1,1,2,3]: position 0 reads position 1
+constexpr int DPP_PAIR_0TO1 = 0xA0;   // [0,0,2,2]: odd positions read the even one before
+constexpr int DPP_PAIR_1TO0 = 0xF5;   // [1,1,3,3]: even positions read the odd one after
 constexpr int DPP_ROW_ROR1 = 0x121;
 constexpr int DPP_ROW_ROR2 = 0x122;
 constexpr int DPP_ROW_ROR4 = 0x124;
@@ -333,10 +335,15 @@ __device__ __forceinline__ void p2_permute_gl_chunk(uint64_t (&s)[4], int j, con
     // ir[RP] is the first terminal constant, so the look-ahead stays in bounds
     const uint64_t *ir = rc + 4 * W;
     uint64_t ci = ir[0];
+    // element 0's S-box chain shared with chunk 1's lane (the next lane):
+    // one product forms a^3 there and a^4 here
 #pragma unroll 1
     for (int r = 0; r < RP; r++) {
         const uint64_t next = ir[r + 1];
-        const uint64_t y = sbox7_gl(Gl::add_const_lazy(s[0], ci));
+        const uint64_t a = Gl::add_const_lazy(s[0], ci);
+        const uint64_t a2 = dpp64<DPP_PAIR_0TO1>(Gl::sqr_lazy(a));  // chunks 0 and 1: a^2
+        const uint64_t p = Gl::mul_lazy(a2, j == 0 ? a : a2);        // chunk 0: a^3, chunk 1: a^4
+        const uint64_t y = Gl::mul_lazy(p, dpp64<DPP_PAIR_1TO0>(p));  // chunk 0: a^7
         s[0] = j == 0 ? y : s[0];
         const GlWide acc = glw_add(glw_add(glw(s[0]), s[1]), glw_add(glw(s[2]), s[3]));
         const uint64_t sum = glw_fold(chunk_sum<W>(acc));
