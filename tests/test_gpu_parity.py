@@ -4,6 +4,8 @@ Every test here runs the product path through the C ABI (ziros_amd binds
 libzkg.so with ctypes) and compares with oracle/liboracle.so on the same
 seeded inputs.  Integer/field work: the bar is bit-exact equality.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -427,6 +429,30 @@ def test_errors_are_reported(ctx):
         Z.HipDft(ctx, BB).coset_lde_batch(np.zeros((3, 2), dtype=np.uint32), 1)  # not a power of two
     with pytest.raises(Z.ZkgError):
         Z.HipDft(ctx, BB).coset_lde_batch(np.zeros((4, 2), dtype=np.uint32), 3)  # blowup too large
+
+
+def test_sizes_that_wrap_are_rejected(ctx):
+    """Counts whose byte size exceeds 64 bits must fail the buffer check, not
+    wrap below the buffer size.  Every call is refused on the host, before any
+    copy or launch."""
+    EINVAL = -1
+    lib, h = ctx.lib, ctx.h
+    buf = ctx.alloc(4096)
+    out = ctypes.c_void_p()
+    src = (ctypes.c_uint8 * 16)()
+    try:
+        assert lib.zkg_buf_upload(buf.h, (1 << 64) - 8, src, 16) == EINVAL
+        assert lib.zkg_buf_download(buf.h, 4090, src, 16) == EINVAL
+        assert lib.zkg_poseidon2_permute(h, GL, 16, buf.h, 1 << 61) == EINVAL
+        assert lib.zkg_hash_rows(h, BB, 16, buf.h, 1 << 62, 8, buf.h) == EINVAL
+        assert lib.zkg_merkle_commit(h, GL, 16, buf.h, 1 << 31, (1 << 32) - 1, ctypes.byref(out)) == EINVAL
+        heights = (ctypes.c_uint64 * 1)(1 << 31)
+        widths = (ctypes.c_uint32 * 1)((1 << 32) - 1)
+        mats = (ctypes.c_void_p * 1)(buf.h.value)
+        assert lib.zkg_mmcs_commit(h, GL, 16, 1, mats, heights, widths, ctypes.byref(out)) == EINVAL
+        assert not out.value
+    finally:
+        buf.free()
 
 
 # ----------------------------------------------------------------- DEEP quotient

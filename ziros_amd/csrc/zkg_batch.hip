@@ -42,7 +42,7 @@ static int validate_job(zkg_ctx *ctx, const zkg_job &j)
     if (j.log_blowup > 2 || j.width == 0) return set_err(ZKG_EINVAL, "job blowup/width");
     if (!get_p2(ctx, j.field, j.p2_width)) return set_err(ZKG_ESTATE, "Poseidon2 constants for job not loaded");
     uint64_t es = esz(j.field);
-    if (j.height * j.width * es > j.trace->bytes || (j.height << j.log_blowup) * j.width * es > j.lde_out->bytes)
+    if (span_bytes(j.height, j.width, es) > j.trace->bytes || span_bytes(j.height << j.log_blowup, j.width, es) > j.lde_out->bytes)
         return set_err(ZKG_EINVAL, "job buffer too small");
     return ZKG_OK;
 }

@@ -858,7 +858,7 @@ extern "C" int zkg_constraint_program_quotient(zkg_constraint_program *p, const 
     int log_h = 0;
     while ((1ull << log_h) < height) log_h++;
     if ((int)log_blowup > log_h) return set_err(ZKG_EINVAL, "log_blowup %u exceeds log height %d", log_blowup, log_h);
-    if (height * p->width * 8 > lde->bytes) return set_err(ZKG_EINVAL, "LDE buffer too small");
+    if (span_bytes(height, p->width, 8) > lde->bytes) return set_err(ZKG_EINVAL, "LDE buffer too small");
     if (height * 8 > q_out->bytes) return set_err(ZKG_EINVAL, "quotient buffer too small");
     if (lde->ctx != p->ctx || q_out->ctx != p->ctx) return set_err(ZKG_EINVAL, "buffers of another context");
     if (shift % Gl::P == 0) return set_err(ZKG_EINVAL, "zero coset shift");

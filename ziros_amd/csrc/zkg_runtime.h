@@ -115,4 +115,16 @@ void ce_cache_release(zkg_ctx *ctx);
 int p2_seed_constants(int field, int width, uint64_t seed, std::vector<uint64_t> &flat);
 int p2_internal_rounds(int field);
 
+// a * b * c, or UINT64_MAX when the product does not fit in 64 bits.  Sizes
+// checked against a buffer go through this, so that a caller's huge count
+// fails the check instead of wrapping below the buffer size.
+inline uint64_t span_bytes(uint64_t a, uint64_t b, uint64_t c = 1)
+{
+    uint64_t r;
+    if (__builtin_mul_overflow(a, b, &r) || __builtin_mul_overflow(r, c, &r)) return UINT64_MAX;
+    return r;
+}
+// [off, off + bytes) lies inside a buffer of `size` bytes.
+inline bool range_fits(uint64_t off, uint64_t bytes, uint64_t size) { return off <= size && bytes <= size - off; }
+
 }  // namespace zkg

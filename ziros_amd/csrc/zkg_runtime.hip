@@ -548,7 +548,7 @@ extern "C" int zkg_buf_free(zkg_buf *b)
 extern "C" int zkg_buf_upload(zkg_buf *b, uint64_t off, const void *src, uint64_t bytes)
 {
     if (!b || (!src && bytes)) return set_err(ZKG_EINVAL, "null argument");
-    if (off + bytes > b->bytes) return set_err(ZKG_EINVAL, "upload of %llu bytes at %llu exceeds buffer of %llu",
+    if (!range_fits(off, bytes, b->bytes)) return set_err(ZKG_EINVAL, "upload of %llu bytes at %llu exceeds buffer of %llu",
                                                (unsigned long long)bytes, (unsigned long long)off,
                                                (unsigned long long)b->bytes);
     HIP_TRY(hipStreamSynchronize(b->ctx->stream), "sync");
@@ -559,7 +559,7 @@ extern "C" int zkg_buf_upload(zkg_buf *b, uint64_t off, const void *src, uint64_
 extern "C" int zkg_buf_download(const zkg_buf *b, uint64_t off, void *dst, uint64_t bytes)
 {
     if (!b || (!dst && bytes)) return set_err(ZKG_EINVAL, "null argument");
-    if (off + bytes > b->bytes) return set_err(ZKG_EINVAL, "download exceeds buffer");
+    if (!range_fits(off, bytes, b->bytes)) return set_err(ZKG_EINVAL, "download exceeds buffer");
     HIP_TRY(hipStreamSynchronize(b->ctx->stream), "sync");
     HIP_TRY(hipMemcpy(dst, (const char *)b->ptr + off, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
     return ZKG_OK;
@@ -584,7 +584,7 @@ extern "C" int zkg_fill_random(zkg_ctx *ctx, int field, zkg_buf *buf, uint64_t s
 {
     if (!ctx || !buf) return set_err(ZKG_EINVAL, "null argument");
     if (int rc = check_field(field)) return rc;
-    if (n * esize(field) > buf->bytes) return set_err(ZKG_EINVAL, "buffer too small");
+    if (span_bytes(n, esize(field)) > buf->bytes) return set_err(ZKG_EINVAL, "buffer too small");
     HIP_TRY(launch_gen(field, buf->ptr, seed, start, n, ctx->stream), "gen");
     return ZKG_OK;
 }
@@ -646,7 +646,7 @@ extern "C" int zkg_ntt_batch(zkg_ctx *ctx, int field, zkg_buf *buf, uint64_t hei
     if (!ctx || !buf) return set_err(ZKG_EINVAL, "null argument");
     if (int rc = check_field(field)) return rc;
     if (!is_pow2(height) || ilog2u(height) > 22) return set_err(ZKG_EINVAL, "height %llu not a power of two <= 2^22", (unsigned long long)height);
-    if (height * width * esize(field) > buf->bytes) return set_err(ZKG_EINVAL, "buffer too small");
+    if (span_bytes(height, width, esize(field)) > buf->bytes) return set_err(ZKG_EINVAL, "buffer too small");
     if (height == 1 || width == 0) return ZKG_OK;
     uint64_t tb = ntt_tmp_bytes(field, height, width);
     void *tmp = nullptr;
@@ -688,7 +688,7 @@ extern "C" int zkg_coset_lde_batch(zkg_ctx *ctx, int field, const zkg_buf *in, u
     if (!ctx || !in || !out) return set_err(ZKG_EINVAL, "null argument");
     if (int rc = check_lde_shape(field, height, width, log_blowup)) return rc;
     uint64_t es = esize(field);
-    if (height * width * es > in->bytes || (height << log_blowup) * width * es > out->bytes)
+    if (span_bytes(height, width, es) > in->bytes || span_bytes(height << log_blowup, width, es) > out->bytes)
         return set_err(ZKG_EINVAL, "buffer too small");
     if (shift >= (field == FIELD_GL ? Gl::P : (uint64_t)Bb::P) || shift == 0)
         return set_err(ZKG_EINVAL, "shift must be a nonzero canonical element");
@@ -712,7 +712,7 @@ extern "C" int zkg_poseidon2_permute(zkg_ctx *ctx, int field, int width, zkg_buf
     if (!ctx || !states) return set_err(ZKG_EINVAL, "null argument");
     const P2Set *p2 = get_p2(ctx, field, width);
     if (!p2) return set_err(ZKG_ESTATE, "Poseidon2 constants for field %d width %d not loaded", field, width);
-    if (n * width * esize(field) > states->bytes) return set_err(ZKG_EINVAL, "buffer too small");
+    if (span_bytes(n, width, esize(field)) > states->bytes) return set_err(ZKG_EINVAL, "buffer too small");
     HIP_TRY(launch_p2_permute(field, width, *p2, states->ptr, n, ctx->stream), "permute");
     HIP_TRY(hipStreamSynchronize(ctx->stream), "sync");
     return ZKG_OK;
@@ -756,7 +756,7 @@ extern "C" int zkg_hash_rows(zkg_ctx *ctx, int field, int p2_width, const zkg_bu
     const P2Set *p2 = get_p2(ctx, field, p2_width);
     if (!p2) return set_err(ZKG_ESTATE, "Poseidon2 constants for field %d width %d not loaded", field, p2_width);
     uint64_t es = esize(field);
-    if (height * mat_width * es > mat->bytes || height * (p2_width / 2) * es > digests->bytes)
+    if (span_bytes(height, mat_width, es) > mat->bytes || span_bytes(height, p2_width / 2, es) > digests->bytes)
         return set_err(ZKG_EINVAL, "buffer too small");
     HIP_TRY(launch_hash_rows(field, p2_width, *p2, mat->ptr, height, mat_width, digests->ptr, ctx->stream), "hash rows");
     HIP_TRY(hipStreamSynchronize(ctx->stream), "sync");
@@ -813,7 +813,7 @@ extern "C" int zkg_merkle_commit(zkg_ctx *ctx, int field, int p2_width, const zk
     if (!get_p2(ctx, field, p2_width))
         return set_err(ZKG_ESTATE, "Poseidon2 constants for field %d width %d not loaded", field, p2_width);
     if (!is_pow2(height) || ilog2u(height) > 31) return set_err(ZKG_EINVAL, "height must be a power of two");
-    if (height * width * esize(field) > mat->bytes) return set_err(ZKG_EINVAL, "buffer too small");
+    if (span_bytes(height, width, esize(field)) > mat->bytes) return set_err(ZKG_EINVAL, "buffer too small");
     zkg_tree *t;
     int rc = tree_alloc(ctx, field, p2_width, height, mat->ptr, width, &t);
     if (rc) return rc;
@@ -949,7 +949,7 @@ extern "C" int zkg_mmcs_commit(zkg_ctx *ctx, int field, int p2_width, uint32_t n
         if (!mats[i]) return set_err(ZKG_EINVAL, "matrix %u is null", i);
         if (!heights[i] || heights[i] > (1ull << 31)) return set_err(ZKG_EINVAL, "matrix %u: height %llu", i,
                                                                     (unsigned long long)heights[i]);
-        if (heights[i] * widths[i] * es > mats[i]->bytes) return set_err(ZKG_EINVAL, "matrix %u: buffer too small", i);
+        if (span_bytes(heights[i], widths[i], es) > mats[i]->bytes) return set_err(ZKG_EINVAL, "matrix %u: buffer too small", i);
     }
     std::vector<uint32_t> ord(n_mats);
     for (uint32_t i = 0; i < n_mats; i++) ord[i] = i;
@@ -1275,7 +1275,7 @@ int deep_check(zkg_ctx *ctx, int field, const zkg_buf *lde, uint64_t height, uin
     j.field = field;
     j.height = height;
     j.width = width;
-    if (height * width * esize(field) > lde->bytes) return set_err(ZKG_EINVAL, "buffer too small");
+    if (span_bytes(height, width, esize(field)) > lde->bytes) return set_err(ZKG_EINVAL, "buffer too small");
     if (field == FIELD_GL) {
         const uint64_t zz = *(const uint64_t *)z;
         if (zz >= Gl::P) return set_err(ZKG_EINVAL, "z is not canonical");
@@ -1348,7 +1348,7 @@ int deep_values(zkg_ctx *ctx, const zkg_buf *lde, uint32_t log_blowup, DeepJob &
 int deep_reduce(zkg_ctx *ctx, const zkg_buf *lde, const void *alpha, uint64_t alpha_offset, const void *ys,
                 zkg_buf *ro, DeepJob &j)
 {
-    if (j.height * j.es() > ro->bytes) return set_err(ZKG_EINVAL, "ro buffer too small");
+    if (span_bytes(j.height, j.es()) > ro->bytes) return set_err(ZKG_EINVAL, "ro buffer too small");
     void *d_coef = nullptr;
     hipError_t e = hipSuccess;
     if (j.field == FIELD_GL) {
@@ -1481,7 +1481,7 @@ static int commit_common(zkg_ctx *ctx, int field, int p2_width, const zkg_buf *t
     if (!get_p2(ctx, field, p2_width))
         return set_err(ZKG_ESTATE, "Poseidon2 constants for field %d width %d not loaded", field, p2_width);
     uint64_t es = esize(field), big = height << log_blowup;
-    if (height * width * es > trace->bytes || big * width * es > lde_out->bytes)
+    if (span_bytes(height, width, es) > trace->bytes || span_bytes(big, width, es) > lde_out->bytes)
         return set_err(ZKG_EINVAL, "buffer too small");
     uint64_t tb = plain ? height * width * es : coset_lde_tmp_bytes(field, height, width, log_blowup);
     void *tmp = nullptr;
@@ -1543,7 +1543,7 @@ extern "C" int zkg_fri_fold(zkg_ctx *ctx, int field, const zkg_buf *evals, uint6
     if (!ctx || !evals || !itw || !out) return set_err(ZKG_EINVAL, "null argument");
     if (int rc = check_field(field)) return rc;
     uint64_t es = esize(field);
-    if (2 * n_out * es > evals->bytes || n_out * es > itw->bytes || n_out * es > out->bytes)
+    if (span_bytes(n_out, 2, es) > evals->bytes || span_bytes(n_out, es) > itw->bytes || span_bytes(n_out, es) > out->bytes)
         return set_err(ZKG_EINVAL, "buffer too small");
     if (alpha >= (field == FIELD_GL ? Gl::P : (uint64_t)Bb::P)) return set_err(ZKG_EINVAL, "alpha not canonical");
     HIP_TRY(launch_fri_fold_explicit(field, evals->ptr, n_out, alpha, itw->ptr, out->ptr, ctx->stream), "fri fold");
@@ -1559,7 +1559,7 @@ extern "C" int zkg_fri_fold_matrix(zkg_ctx *ctx, int ext, int field, const zkg_b
     if (int rc = check_field(field)) return rc;
     if (!is_pow2(n_out) || ilog2u(n_out) + 1 > LOG_TW_TOP) return set_err(ZKG_EINVAL, "n_out must be a power of two <= 2^23");
     uint64_t es = ext ? 16 : esize(field);
-    if (2 * n_out * es > evals->bytes || n_out * es > out->bytes) return set_err(ZKG_EINVAL, "buffer too small");
+    if (span_bytes(n_out, 2, es) > evals->bytes || span_bytes(n_out, es) > out->bytes) return set_err(ZKG_EINVAL, "buffer too small");
     uint32_t br[4] = {0, 0, 0, 0};
     if (ext) {
         for (int i = 0; i < 4; i++) {
