@@ -336,25 +336,25 @@ constexpr int lanes_fuse()
     return W == 16 ? 5 : 6;
 }
 
+// Block `block` of one tree's launch; lds holds 256 / W digests.
 template <class F, int W>
-__global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_lanes(const typename F::T *__restrict__ in,
-                                                                 LayerPtrs<typename F::T> out, int k,
-                                                                 const typename F::T *__restrict__ rc,
-                                                                 const typename F::T *__restrict__ diag)
+__device__ __forceinline__ void merkle_lanes_block(const typename F::T *__restrict__ in,
+                                                   const LayerPtrs<typename F::T> &out, int k, uint32_t block,
+                                                   const typename F::T *__restrict__ rc,
+                                                   const typename F::T *__restrict__ diag, typename F::T *lds)
 {
     using T = typename F::T;
     constexpr int D = W / 2;
     using M = LaneMap<W>;
     static_assert(M::PER_BLOCK * W == MERKLE_THREADS && (2 << (lanes_fuse<W>() - 1)) == 2 * M::PER_BLOCK,
                   "a block's first level holds 256 / W compressions");
-    __shared__ T lds[M::PER_BLOCK * D];
     const int g = M::slot(threadIdx.x), e = M::elem(threadIdx.x);
     LaneConsts<F> kc;
     load_lane_consts<F, W>(kc, e, rc, diag);
     int nodes = 1 << (k - 1);
     T x = 0;
     if (g < nodes) {
-        const uint64_t i = (uint64_t)blockIdx.x * nodes + g;
+        const uint64_t i = (uint64_t)block * nodes + g;
         x = p2_permute_lanes<F, W>(F::from_canon(in[2 * i * D + e]), e, kc, rc);
         if (e < D) {
             out.p[0][i * D + e] = F::to_canon(x);
@@ -367,7 +367,7 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_lanes(const typename 
         if (g < nodes) x = lds[2 * g * D + e];  // elements D..W-1 read the right child
         __syncthreads();
         if (g < nodes) {
-            const uint64_t i = (uint64_t)blockIdx.x * nodes + g;
+            const uint64_t i = (uint64_t)block * nodes + g;
             x = p2_permute_lanes<F, W>(x, e, kc, rc);
             if (e < D) {
                 out.p[lvl][i * D + e] = F::to_canon(x);
@@ -375,6 +375,41 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_lanes(const typename 
             }
         }
     }
+}
+
+template <class F, int W>
+__global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_lanes(const typename F::T *__restrict__ in,
+                                                                 LayerPtrs<typename F::T> out, int k,
+                                                                 const typename F::T *__restrict__ rc,
+                                                                 const typename F::T *__restrict__ diag)
+{
+    __shared__ typename F::T lds[LaneMap<W>::PER_BLOCK * (W / 2)];
+    merkle_lanes_block<F, W>(in, out, k, blockIdx.x, rc, diag, lds);
+}
+
+// The tops of several trees in one launch: tree t takes blocks [first[t],
+// first[t + 1]) and compresses its level `in` into k[t] levels, as one
+// k_merkle_lanes launch would.  Small trees that are ready at the same time
+// then cost one launch per lanes_fuse<W>() levels together, not each.
+constexpr int MAX_TOPS = 16;
+template <class T>
+struct TopJobs {
+    const T *in[MAX_TOPS];
+    LayerPtrs<T> out[MAX_TOPS];
+    int k[MAX_TOPS];
+    uint32_t first[MAX_TOPS + 1];
+    int n;
+};
+
+template <class F, int W>
+__global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_lanes_multi(TopJobs<typename F::T> jobs,
+                                                                       const typename F::T *__restrict__ rc,
+                                                                       const typename F::T *__restrict__ diag)
+{
+    __shared__ typename F::T lds[LaneMap<W>::PER_BLOCK * (W / 2)];
+    int t = 0;
+    while (t + 1 < jobs.n && blockIdx.x >= jobs.first[t + 1]) t++;
+    merkle_lanes_block<F, W>(jobs.in[t], jobs.out[t], jobs.k[t], blockIdx.x - jobs.first[t], rc, diag, lds);
 }
 
 // Middle levels of a Goldilocks tree: level fusion as in k_merkle_fused with
@@ -711,6 +746,45 @@ hipError_t merkle_impl(const P2Set &p2, void *const *layers, int nlayers, uint64
     return hipSuccess;
 }
 
+// Whole trees in the W-lane form, up to MAX_TOPS per launch: each launch
+// takes the next lanes_fuse<W>() levels of every tree that has levels left.
+template <class F, int W>
+hipError_t merkle_tops_impl(const P2Set &p2, const MerkleTop *trees, int ntrees, hipStream_t st)
+{
+    using T = typename F::T;
+    for (int i = 0; i < ntrees; i++)
+        if (trees[i].nlayers < 1 || trees[i].nlayers > 40 || (1ull << (trees[i].nlayers - 1)) != trees[i].n_leaves)
+            return hipErrorInvalidValue;
+    for (int g0 = 0; g0 < ntrees; g0 += MAX_TOPS) {
+        const int gn = ntrees - g0 < MAX_TOPS ? ntrees - g0 : MAX_TOPS;
+        int lvl[MAX_TOPS] = {};
+        for (;;) {
+            TopJobs<T> j{};
+            uint32_t blocks = 0;
+            for (int i = 0; i < gn; i++) {
+                const MerkleTop &tr = trees[g0 + i];
+                const int left = tr.nlayers - 1 - lvl[i];
+                if (left <= 0) continue;
+                const int k = left < lanes_fuse<W>() ? left : lanes_fuse<W>();
+                j.in[j.n] = (const T *)tr.layers[lvl[i]];
+                for (int m = 0; m < MAX_FUSE; m++) j.out[j.n].p[m] = m < k ? (T *)tr.layers[lvl[i] + 1 + m] : nullptr;
+                j.k[j.n] = k;
+                j.first[j.n] = blocks;
+                blocks += (uint32_t)((tr.n_leaves >> lvl[i]) >> k);
+                j.n++;
+                lvl[i] += k;
+            }
+            if (!j.n) break;
+            j.first[j.n] = blocks;
+            hipLaunchKernelGGL((k_merkle_lanes_multi<F, W>), dim3(blocks), dim3(MERKLE_THREADS), 0, st, j,
+                               (const T *)p2.rc, (const T *)p2.diag);
+            hipError_t e = hipGetLastError();
+            if (e) return e;
+        }
+    }
+    return hipSuccess;
+}
+
 // ---------------------------------------------------------------- mixed-height MMCS
 // PaddingFreeSponge over row `row` of every matrix of a group, concatenated in
 // commit order (p3 first_digest_layer / compress_and_inject hash of
@@ -1035,6 +1109,20 @@ hipError_t launch_merkle_tree(int field, int width, const P2Set &p2, void *const
     if (field == FIELD_GL && width == 8)
         return merkle_impl<Gl, 8>(p2, layers, nlayers, n_leaves, fuse_levels, st, sync_each);
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_merkle_tops(int field, int width, const P2Set &p2, const MerkleTop *trees, int ntrees,
+                              hipStream_t st)
+{
+    if (field == FIELD_BB && width == 16) return merkle_tops_impl<Bb, 16>(p2, trees, ntrees, st);
+    return hipErrorInvalidValue;
+}
+
+// 2^17: the BabyBear trees that merkle_impl builds in the W-lane form alone.
+uint64_t fri_tops_max_leaves()
+{
+    static const uint64_t v = env_pow2("ZKG_FRI_TOPS_LOG2", 1ull << 17);
+    return v;
 }
 
 hipError_t launch_gen(int field, void *out, uint64_t seed, uint64_t start, uint64_t n, hipStream_t st)

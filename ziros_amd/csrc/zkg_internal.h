@@ -131,6 +131,20 @@ hipError_t launch_hash_rows(int field, int width, const P2Set &p2, const void *m
 hipError_t launch_merkle_tree(int field, int width, const P2Set &p2, void *const *layers,
                               int nlayers, uint64_t n_leaves, int fuse_levels, hipStream_t st,
                               bool sync_each);
+// Several small trees built together (BabyBear width 16, the FRI layer
+// trees), W lanes per compression: one launch per 5 levels of all of them.
+// Tree t: levels 1.. over layers[0] (n_leaves digests), nlayers =
+// log2(n_leaves) + 1.
+struct MerkleTop {
+    void *const *layers;
+    int nlayers;
+    uint64_t n_leaves;
+};
+hipError_t launch_merkle_tops(int field, int width, const P2Set &p2, const MerkleTop *trees, int ntrees,
+                              hipStream_t st);
+// FRI layer trees of at most this many leaves are built together after the
+// fold chain (ZKG_FRI_TOPS_LOG2 overrides the default 2^17).
+uint64_t fri_tops_max_leaves();
 hipError_t launch_gen(int field, void *out, uint64_t seed, uint64_t start, uint64_t n,
                       hipStream_t st);
 // Mixed-height MMCS.  Leaves: digest i = sponge over row i of every matrix of

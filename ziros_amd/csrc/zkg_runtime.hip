@@ -1621,18 +1621,15 @@ extern "C" int zkg_fri_commit_phase(zkg_ctx *ctx, const zkg_buf *cw, uint32_t lo
     stage_mark(ctx, st, 0);
     if (!rc) {
         // The fold chain runs on the context stream; layer r's tree (which
-        // needs only that layer's leaf digests) runs on a side stream, so
-        // the latency-bound tree tops overlap the later folds.
+        // needs only that layer's leaf digests, event r) runs on a side
+        // stream, so the latency-bound tree tops overlap the later folds.
+        // The whole chain is enqueued before any tree: with the trees'
+        // launches between its folds, the host's enqueue rate paced the
+        // short folds of the small layers.
         hipError_t e = launch_hash_rows(FIELD_BB, 16, *p2, f->codewords[0], 1ull << (log_len - 1), 8,
                                         f->trees[0]->layers[0], st);
         if (e == hipSuccess) e = hipEventRecord(ctx->fri_ev[0], st);
         for (int r = 0; r < nl && e == hipSuccess; r++) {
-            zkg_tree *t = f->trees[r];
-            hipStream_t side = ctx->side[r & 1];
-            e = hipStreamWaitEvent(side, ctx->fri_ev[r], 0);
-            if (e == hipSuccess)
-                e = launch_merkle_tree(FIELD_BB, 16, *p2, t->layers.data(), t->nlayers, t->n_leaves, 9, side, false);
-            if (e != hipSuccess) break;
             uint32_t br[4];
             for (int i = 0; i < 4; i++) {
                 uint32_t v = betas[4 * r + i];
@@ -1644,6 +1641,22 @@ extern "C" int zkg_fri_commit_phase(zkg_ctx *ctx, const zkg_buf *cw, uint32_t lo
                                 next_digests, st);
             if (e == hipSuccess) e = hipEventRecord(ctx->fri_ev[r + 1], st);
         }
+        // Trees of small layers are latency-bound launch chains; they are
+        // built together on the context stream once the chain has produced
+        // all their leaf digests, one launch per 5 levels of all of them.
+        std::vector<MerkleTop> tops;
+        for (int r = 0; r < nl && e == hipSuccess; r++) {
+            zkg_tree *t = f->trees[r];
+            if (t->n_leaves <= fri_tops_max_leaves()) {
+                tops.push_back(MerkleTop{t->layers.data(), t->nlayers, t->n_leaves});
+                continue;
+            }
+            hipStream_t side = ctx->side[r & 1];
+            e = hipStreamWaitEvent(side, ctx->fri_ev[r], 0);
+            if (e == hipSuccess)
+                e = launch_merkle_tree(FIELD_BB, 16, *p2, t->layers.data(), t->nlayers, t->n_leaves, 9, side, false);
+        }
+        if (e == hipSuccess && !tops.empty()) e = launch_merkle_tops(FIELD_BB, 16, *p2, tops.data(), (int)tops.size(), st);
         for (int i = 0; i < 2; i++) {  // join the side streams
             hipError_t e2 = hipEventRecord(ctx->side_join[i], ctx->side[i]);
             if (e2 == hipSuccess) e2 = hipStreamWaitEvent(st, ctx->side_join[i], 0);
