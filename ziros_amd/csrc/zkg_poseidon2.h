@@ -220,11 +220,14 @@ __host__ __device__ __forceinline__ void p2_permute(typename F::T (&s)[W], const
 // Bounds (p < 2^31): the input state is first reduced to |s| <= p/2 + 1
 // (capacity lanes carry unreduced values between sponge absorptions); MDS-light
 // multiplies a bound by at most 35, S-box outputs are < 1.6 p, so external
-// rounds see |s| < 57 p < 2^37; internal rounds reduce the sum and the lanes
-// with integer |V_i| >= 2 before use, scale the V = +-2^-k lanes exactly
+// rounds see |s| < 57 p < 2^37; internal rounds reduce the sum every round and
+// the lanes with integer |V_i| >= 2 every fourth round (they stay below 171 p <
+// 2^39, see internal_round_bf), scale the V = +-2^-k lanes exactly
 // (Bf::mul_inv_pow2: |x| 2^-k + 2^31, so they shrink) and the V = 1 lane grows
-// by < p/2 per round, so every product input stays below 2^37 (the product
-// routine needs < 2^41) and every sum below 2^41 (exactness needs < 2^53).
+// by < p/2 per round.  The last internal round reduces, so the terminal
+// external rounds start from the same bounds; every product input stays below
+// 2^37 (the product routine needs < 2^41) and every sum below 2^42 (exactness
+// needs < 2^53).
 __host__ __device__ __forceinline__ void mds_light_bf(double (&s)[16])
 {
 #pragma unroll
@@ -251,6 +254,38 @@ __host__ __device__ __forceinline__ double sbox7_bf(double x)
     return Bf::mul(x3, x4);
 }
 
+// One internal round: s_i = V_i s_i + sum, V = [-2, 1, 2, 1/2, 3, 4, -1/2, -3,
+// -4, 2^-8, 1/4, 1/8, 2^-27, -2^-8, -1/16, -2^-27].  With REDUCE the lanes with
+// integer |V_i| >= 2 are reduced before they are scaled (to 2.5 p at most after
+// the round); without it they grow to at most 4 |s| + p/2, so three rounds
+// without a reduction after one with it leave them below 171 p < 2^39.
+template <bool REDUCE>
+__host__ __device__ __forceinline__ void internal_round_bf(double (&s)[16], double c)
+{
+    s[0] = sbox7_bf(s[0] + c);
+    double sum = s[0];
+#pragma unroll
+    for (int i = 1; i < 16; i++) sum += s[i];
+    sum = Bf::red(sum);
+    auto lazy = [](double x) { return REDUCE ? Bf::red(x) : x; };
+    s[0] = __builtin_fma(-2.0, s[0], sum);
+    s[1] = s[1] + sum;
+    s[2] = __builtin_fma(2.0, lazy(s[2]), sum);
+    s[3] = Bf::mul_inv_pow2<1, false>(s[3]) + sum;
+    s[4] = __builtin_fma(3.0, lazy(s[4]), sum);
+    s[5] = __builtin_fma(4.0, lazy(s[5]), sum);
+    s[6] = Bf::mul_inv_pow2<1, true>(s[6]) + sum;
+    s[7] = __builtin_fma(-3.0, lazy(s[7]), sum);
+    s[8] = __builtin_fma(-4.0, lazy(s[8]), sum);
+    s[9] = Bf::mul_inv_pow2<8, false>(s[9]) + sum;
+    s[10] = Bf::mul_inv_pow2<2, false>(s[10]) + sum;
+    s[11] = Bf::mul_inv_pow2<3, false>(s[11]) + sum;
+    s[12] = Bf::mul_inv_pow2<27, false>(s[12]) + sum;
+    s[13] = Bf::mul_inv_pow2<8, true>(s[13]) + sum;
+    s[14] = Bf::mul_inv_pow2<4, true>(s[14]) + sum;
+    s[15] = Bf::mul_inv_pow2<27, true>(s[15]) + sum;
+}
+
 // rc: flat [initial 4x16][internal 13][terminal 4x16], canonical, as doubles.
 __host__ __device__ __forceinline__ void p2_permute_bf(double (&s)[16], const double *__restrict__ rc)
 {
@@ -264,32 +299,15 @@ __host__ __device__ __forceinline__ void p2_permute_bf(double (&s)[16], const do
         mds_light_bf(s);
     }
     const double *ir = rc + 64;
+    // 13 internal rounds; the integer-V lanes are reduced in rounds 0, 4, 8, 12
 #pragma unroll 1
-    for (int r = 0; r < 13; r++) {
-        s[0] = sbox7_bf(s[0] + ir[r]);
-        double sum = s[0];
-#pragma unroll
-        for (int i = 1; i < 16; i++) sum += s[i];
-        sum = Bf::red(sum);
-        // s_i = V_i s_i + sum, V = [-2, 1, 2, 1/2, 3, 4, -1/2, -3, -4, 2^-8,
-        // 1/4, 1/8, 2^-27, -2^-8, -1/16, -2^-27]
-        s[0] = __builtin_fma(-2.0, s[0], sum);
-        s[1] = s[1] + sum;
-        s[2] = __builtin_fma(2.0, Bf::red(s[2]), sum);
-        s[3] = Bf::mul_inv_pow2<1, false>(s[3]) + sum;
-        s[4] = __builtin_fma(3.0, Bf::red(s[4]), sum);
-        s[5] = __builtin_fma(4.0, Bf::red(s[5]), sum);
-        s[6] = Bf::mul_inv_pow2<1, true>(s[6]) + sum;
-        s[7] = __builtin_fma(-3.0, Bf::red(s[7]), sum);
-        s[8] = __builtin_fma(-4.0, Bf::red(s[8]), sum);
-        s[9] = Bf::mul_inv_pow2<8, false>(s[9]) + sum;
-        s[10] = Bf::mul_inv_pow2<2, false>(s[10]) + sum;
-        s[11] = Bf::mul_inv_pow2<3, false>(s[11]) + sum;
-        s[12] = Bf::mul_inv_pow2<27, false>(s[12]) + sum;
-        s[13] = Bf::mul_inv_pow2<8, true>(s[13]) + sum;
-        s[14] = Bf::mul_inv_pow2<4, true>(s[14]) + sum;
-        s[15] = Bf::mul_inv_pow2<27, true>(s[15]) + sum;
+    for (int r = 0; r < 12; r += 4) {
+        internal_round_bf<true>(s, ir[r]);
+        internal_round_bf<false>(s, ir[r + 1]);
+        internal_round_bf<false>(s, ir[r + 2]);
+        internal_round_bf<false>(s, ir[r + 3]);
     }
+    internal_round_bf<true>(s, ir[12]);
     const double *tr = ir + 13;
 #pragma unroll 1
     for (int r = 0; r < 4; r++) {
