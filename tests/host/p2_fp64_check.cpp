@@ -73,19 +73,29 @@ int main()
     for (int t = 0; t < 100000; t++) {
         uint64_t st[16];
         double s[16];
-        const int mode = t % 4;
+        const int mode = t % 6;
         for (int i = 0; i < 16; i++) {
             uint64_t v = g() % (uint64_t)P;
             if (mode == 1) v = (uint64_t)P - 1 - (g() % 3);
             if (mode == 2) v = (g() & 1) ? (uint64_t)P - 1 : 0;
-            st[i] = v;
+            if (mode == 5) v = 0xFFFFFFFFull - (g() % 3);  // fresh but not canonical
+            st[i] = v % (uint64_t)P;
             s[i] = (double)v;
         }
         or_p2_permute(&p, st);
-        p2_permute_bf(s, rc);
+        if (mode == 4) p2_permute_bf(s, rc);  // every lane reduced on entry
+        else p2_permute_bf<16>(s, rc);       // fresh lanes, no reduction
         for (int i = 0; i < 16; i++) n++, bad_perm += Bf::canon(s[i]) != st[i];
-        if (mode == 3) {  // a second absorption over the unreduced capacity, as in a sponge
+        if (mode == 3 || mode == 5) {  // second absorptions over the unreduced capacity, as in a sponge
             for (int i = 0; i < 8; i++) {
+                uint64_t v = g() % (uint64_t)P;
+                st[i] = v;
+                s[i] = (double)v;
+            }
+            or_p2_permute(&p, st);
+            p2_permute_bf<8>(s, rc);  // full-rate absorption: capacity reduced
+            for (int i = 0; i < 16; i++) n++, bad_perm += Bf::canon(s[i]) != st[i];
+            for (int i = 0; i < 3; i++) {  // partial absorption: lanes 3..7 keep earlier outputs
                 uint64_t v = g() % (uint64_t)P;
                 st[i] = v;
                 s[i] = (double)v;

@@ -116,7 +116,7 @@ __global__ __launch_bounds__(HASH_THREADS) void k_p2_permute(typename F::T *__re
         double s[16];
 #pragma unroll
         for (int k = 0; k < 16; k++) s[k] = Bf::from_u32(states[i * 16 + k]);
-        p2_permute_bf(s, rcf);
+        p2_permute_bf<16>(s, rcf);
 #pragma unroll
         for (int k = 0; k < 16; k++) states[i * 16 + k] = Bf::canon(s[k]);
         return;
@@ -153,7 +153,7 @@ __global__ __launch_bounds__(HASH_THREADS) void k_hash_rows(const typename F::T 
         for (; pos + 8 <= width; pos += 8) {
 #pragma unroll
             for (int k = 0; k < 8; k++) s[k] = Bf::from_u32(src[pos + k]);
-            p2_permute_bf(s, rcf);
+            p2_permute_bf<8>(s, rcf);
         }
         if (pos < width) {
             uint32_t take = width - pos;
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_fused(const typename 
             uint64_t i = (uint64_t)blockIdx.x * nodes + t;
 #pragma unroll
             for (int e = 0; e < 16; e++) s[e] = Bf::from_u32(in[2 * i * 8 + e]);
-            p2_permute_bf(s, rcf);
+            p2_permute_bf<16>(s, rcf);
 #pragma unroll
             for (int e = 0; e < 8; e++) {
                 uint32_t v = Bf::canon(s[e]);
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_fused(const typename 
             }
             __syncthreads();
             if (t < nodes) {
-                p2_permute_bf(s, rcf);
+                p2_permute_bf<16>(s, rcf);
                 uint64_t i = (uint64_t)blockIdx.x * nodes + t;
 #pragma unroll
                 for (int e = 0; e < 8; e++) {
@@ -600,7 +600,7 @@ __global__ __launch_bounds__(HASH_THREADS) void k_fri_fold_ef4(const uint32_t *_
         }
 #pragma unroll
         for (int e = 8; e < 16; e++) s[e] = 0.0;
-        p2_permute_bf(s, rcf);
+        p2_permute_bf<16>(s, rcf);
 #pragma unroll
         for (int e = 0; e < 8; e++) digests[j * 8 + e] = Bf::canon(s[e]);
     }
@@ -904,7 +904,7 @@ __global__ __launch_bounds__(HASH_THREADS) void k_mmcs_layer(const typename F::T
         double s[16];
 #pragma unroll
         for (int e = 0; e < 16; e++) s[e] = Bf::from_u32(prev[2 * i * 8 + e]);
-        p2_permute_bf(s, rcf);
+        p2_permute_bf<16>(s, rcf);
         if (n_inj) {
             double r[16];
             if (i < inj_len) {

@@ -217,10 +217,11 @@ __host__ __device__ __forceinline__ void p2_permute(typename F::T (&s)[W], const
 
 // ---------------------------------------------------------------------------
 // BabyBear width-16 permutation in exact FP64 (see Bf in zkg_field.h).
-// Bounds (p < 2^31): the input state is first reduced to |s| <= p/2 + 1
-// (capacity lanes carry unreduced values between sponge absorptions); MDS-light
-// multiplies a bound by at most 35, S-box outputs are < 1.6 p, so external
-// rounds see |s| < 57 p < 2^37; internal rounds reduce the sum every round and
+// Bounds (p < 2^31): input lanes are fresh u32 values (< 2^32) or reduced to
+// |s| <= p/2 + 1 (capacity lanes carry unreduced values between sponge
+// absorptions); MDS-light multiplies a bound by at most 35, so the first
+// S-box inputs are < 35 2^32 + p < 2^37.2; S-box outputs are < 1.6 p, so the
+// later external rounds see |s| < 57 p < 2^37; internal rounds reduce the sum every round and
 // the lanes with integer |V_i| >= 2 every fourth round (they stay below 171 p <
 // 2^39, see internal_round_bf), scale the V = +-2^-k lanes exactly
 // (Bf::mul_inv_pow2: |x| 2^-k + 2^31, so they shrink) and the V = 1 lane grows
@@ -287,10 +288,16 @@ __host__ __device__ __forceinline__ void internal_round_bf(double (&s)[16], doub
 }
 
 // rc: flat [initial 4x16][internal 13][terminal 4x16], canonical, as doubles.
+// Lanes [RED_FROM, 16) are reduced on entry, as they may carry an unreduced
+// output of an earlier permutation (a sponge's capacity); the lanes below
+// RED_FROM must hold fresh values in [0, 2^32), canonical or not.  16: every
+// lane is fresh (Merkle nodes, single permutations); 8: a full-rate absorption.
+template <int RED_FROM = 0>
 __host__ __device__ __forceinline__ void p2_permute_bf(double (&s)[16], const double *__restrict__ rc)
 {
+    static_assert(RED_FROM >= 0 && RED_FROM <= 16, "lane range");
 #pragma unroll
-    for (int i = 0; i < 16; i++) s[i] = Bf::red(s[i]);
+    for (int i = RED_FROM; i < 16; i++) s[i] = Bf::red(s[i]);
     mds_light_bf(s);
 #pragma unroll 1
     for (int r = 0; r < 4; r++) {
