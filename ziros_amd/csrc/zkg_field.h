@@ -291,7 +291,13 @@ struct Bf {
         static_assert(K >= 1 && K <= 27, "shift range");
         constexpr double c = (NEG ? -1.0 : 1.0) / (double)(1u << K);
         double y = x * c;
+        // y - floor(y): at most K fraction bits, so exact, and never rounded up
+        // to 1; on the device one v_fract_f64
+#if defined(__HIP_DEVICE_COMPILE__)
+        double f = __builtin_amdgcn_fract(y);
+#else
         double f = y - __builtin_floor(y);
+#endif
         return __builtin_fma(-f, P, y);
     }
     // canonical u32 in [0, p)
