@@ -241,23 +241,23 @@ struct LayerPtrs {
 // Block b compresses input digests [2 b nb1, 2 (b+1) nb1) of `in` into k
 // levels (nb1 >= 2^(k-1) nodes at the first, halving per level); level l
 // node i goes to out.p[l-1][i].
+// Block `block` of one tree's launch; lds holds W/2 x 256 elements.
 template <class F, int W>
-__global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_fused(const typename F::T *__restrict__ in,
-                                                                 LayerPtrs<typename F::T> out, int k, int nb1,
-                                                                 const typename F::T *__restrict__ rc,
-                                                                 const typename F::T *__restrict__ diag,
-                                                                 const double *__restrict__ rcf)
+__device__ __forceinline__ void merkle_fused_block(const typename F::T *__restrict__ in,
+                                                   const LayerPtrs<typename F::T> &out, int k, int nb1,
+                                                   uint32_t block, const typename F::T *__restrict__ rc,
+                                                   const typename F::T *__restrict__ diag,
+                                                   const double *__restrict__ rcf, typename F::T *lds)
 {
     using T = typename F::T;
     constexpr int D = W / 2;
-    __shared__ T lds[D * MERKLE_THREADS];  // element-major: lds[e * 256 + node]
     const int t = threadIdx.x;
     int nodes = nb1;
     if constexpr (IS_BB16<F, W>) {
         // canonical digests in HBM and LDS, FP64 state in registers
         double s[16];
         if (t < nodes) {
-            uint64_t i = (uint64_t)blockIdx.x * nodes + t;
+            uint64_t i = (uint64_t)block * nodes + t;
 #pragma unroll
             for (int e = 0; e < 16; e++) s[e] = Bf::from_u32(in[2 * i * 8 + e]);
             p2_permute_bf<16>(s, rcf);
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_fused(const typename 
             __syncthreads();
             if (t < nodes) {
                 p2_permute_bf<16>(s, rcf);
-                uint64_t i = (uint64_t)blockIdx.x * nodes + t;
+                uint64_t i = (uint64_t)block * nodes + t;
 #pragma unroll
                 for (int e = 0; e < 8; e++) {
                     uint32_t v = Bf::canon(s[e]);
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_fused(const typename 
     }
     T s[W];
     if (t < nodes) {
-        uint64_t i = (uint64_t)blockIdx.x * nodes + t;
+        uint64_t i = (uint64_t)block * nodes + t;
 #pragma unroll
         for (int e = 0; e < W; e++) s[e] = F::from_canon(in[2 * i * D + e]);
         p2_permute<F, W>(s, rc, diag);
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_fused(const typename 
         __syncthreads();
         if (t < nodes) {
             p2_permute<F, W>(s, rc, diag);
-            uint64_t i = (uint64_t)blockIdx.x * nodes + t;
+            uint64_t i = (uint64_t)block * nodes + t;
 #pragma unroll
             for (int e = 0; e < D; e++) {
                 out.p[lvl][i * D + e] = F::to_canon(s[e]);
@@ -325,6 +325,17 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_fused(const typename 
             }
         }
     }
+}
+
+template <class F, int W>
+__global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_fused(const typename F::T *__restrict__ in,
+                                                                 LayerPtrs<typename F::T> out, int k, int nb1,
+                                                                 const typename F::T *__restrict__ rc,
+                                                                 const typename F::T *__restrict__ diag,
+                                                                 const double *__restrict__ rcf)
+{
+    __shared__ typename F::T lds[(W / 2) * MERKLE_THREADS];  // element-major: lds[e * 256 + node]
+    merkle_fused_block<F, W>(in, out, k, nb1, blockIdx.x, rc, diag, rcf, lds);
 }
 
 // Tree top: the same level fusion with each compression spread over W lanes
@@ -410,6 +421,32 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_lanes_multi(TopJobs<t
     int t = 0;
     while (t + 1 < jobs.n && blockIdx.x >= jobs.first[t + 1]) t++;
     merkle_lanes_block<F, W>(jobs.in[t], jobs.out[t], jobs.k[t], blockIdx.x - jobs.first[t], rc, diag, lds);
+}
+
+// The first levels of several larger trees in one launch, one compression per
+// lane as in k_merkle_fused: tree t takes blocks [first[t], first[t + 1]) of
+// nb1[t] first-level compressions each and k[t] levels.
+template <class T>
+struct FusedJobs {
+    const T *in[MAX_TOPS];
+    LayerPtrs<T> out[MAX_TOPS];
+    int k[MAX_TOPS];
+    int nb1[MAX_TOPS];
+    uint32_t first[MAX_TOPS + 1];
+    int n;
+};
+
+template <class F, int W>
+__global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_fused_multi(FusedJobs<typename F::T> jobs,
+                                                                       const typename F::T *__restrict__ rc,
+                                                                       const typename F::T *__restrict__ diag,
+                                                                       const double *__restrict__ rcf)
+{
+    __shared__ typename F::T lds[(W / 2) * MERKLE_THREADS];
+    int t = 0;
+    while (t + 1 < jobs.n && blockIdx.x >= jobs.first[t + 1]) t++;
+    merkle_fused_block<F, W>(jobs.in[t], jobs.out[t], jobs.k[t], jobs.nb1[t], blockIdx.x - jobs.first[t], rc, diag,
+                             rcf, lds);
 }
 
 // Middle levels of a Goldilocks tree: level fusion as in k_merkle_fused with
@@ -746,8 +783,17 @@ hipError_t merkle_impl(const P2Set &p2, void *const *layers, int nlayers, uint64
     return hipSuccess;
 }
 
-// Whole trees in the W-lane form, up to MAX_TOPS per launch: each launch
-// takes the next lanes_fuse<W>() levels of every tree that has levels left.
+// Up to MAX_TOPS trees per launch.  Levels of more than tops_fused_above()
+// digests first go through k_merkle_fused_multi (one compression per lane, 3
+// levels per launch), where the W-lane form would repeat every internal S-box
+// 16 times over enough compressions to fill the chip; then each launch takes
+// the next lanes_fuse<W>() levels of every tree that has levels left.
+uint64_t tops_fused_above()
+{
+    static const uint64_t v = env_pow2("ZKG_TOPS_FUSED_LOG2", 1ull << 14);
+    return v;
+}
+
 template <class F, int W>
 hipError_t merkle_tops_impl(const P2Set &p2, const MerkleTop *trees, int ntrees, hipStream_t st)
 {
@@ -758,6 +804,32 @@ hipError_t merkle_tops_impl(const P2Set &p2, const MerkleTop *trees, int ntrees,
     for (int g0 = 0; g0 < ntrees; g0 += MAX_TOPS) {
         const int gn = ntrees - g0 < MAX_TOPS ? ntrees - g0 : MAX_TOPS;
         int lvl[MAX_TOPS] = {};
+        for (;;) {
+            FusedJobs<T> j{};
+            uint32_t blocks = 0;
+            for (int i = 0; i < gn; i++) {
+                const MerkleTop &tr = trees[g0 + i];
+                const uint64_t level_n = tr.n_leaves >> lvl[i];
+                const int left = tr.nlayers - 1 - lvl[i];
+                if (left <= 0 || level_n <= tops_fused_above()) continue;
+                const int k = left < 3 ? left : 3;
+                const uint64_t nb1 = level_n / 2 < MERKLE_THREADS ? level_n / 2 : MERKLE_THREADS;
+                j.in[j.n] = (const T *)tr.layers[lvl[i]];
+                for (int m = 0; m < MAX_FUSE; m++) j.out[j.n].p[m] = m < k ? (T *)tr.layers[lvl[i] + 1 + m] : nullptr;
+                j.k[j.n] = k;
+                j.nb1[j.n] = (int)nb1;
+                j.first[j.n] = blocks;
+                blocks += (uint32_t)(level_n / 2 / nb1);
+                j.n++;
+                lvl[i] += k;
+            }
+            if (!j.n) break;
+            j.first[j.n] = blocks;
+            hipLaunchKernelGGL((k_merkle_fused_multi<F, W>), dim3(blocks), dim3(MERKLE_THREADS), 0, st, j,
+                               (const T *)p2.rc, (const T *)p2.diag, (const double *)p2.rc_f64);
+            hipError_t e = hipGetLastError();
+            if (e) return e;
+        }
         for (;;) {
             TopJobs<T> j{};
             uint32_t blocks = 0;
