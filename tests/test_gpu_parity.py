@@ -281,7 +281,8 @@ def test_fri_fold_matrix_vs_oracle(ctx, logn):
         assert (fri.fold_matrix_base(field, e, b).astype(np.uint64) == O.fri_fold_matrix_base(field, e, b)).all()
 
 
-@pytest.mark.parametrize("log_len,log_final", [(12, 4), (9, 8), (14, 8), (19, 12)])
+# (18, 1): 17 layer trees of at most 2^17 leaves, built together in two groups of at most 16
+@pytest.mark.parametrize("log_len,log_final", [(12, 4), (9, 8), (14, 8), (19, 12), (18, 1)])
 def test_fri_commit_phase_vs_oracle(ctx, log_len, log_final):
     p2 = O.Poseidon2(BB, 16)
     cw = rnd(BB, (1 << log_len) * 4, 60 + log_len).reshape(-1, 4)
