@@ -701,6 +701,17 @@ __global__ void k_gather_open(const T *__restrict__ mat, uint32_t mat_width, Ope
     }
 }
 
+// Levels per k_merkle_fused launch (ZKG_MERKLE_FUSED_K overrides the 3).
+int fused_levels()
+{
+    static const int v = [] {
+        const char *e = getenv("ZKG_MERKLE_FUSED_K");
+        int k = 0;
+        return (e && sscanf(e, "%d", &k) == 1 && k >= 1 && k <= MAX_FUSE) ? k : 3;
+    }();
+    return v;
+}
+
 template <class F, int W>
 hipError_t merkle_impl(const P2Set &p2, void *const *layers, int nlayers, uint64_t n, int fuse, hipStream_t st,
                        bool sync_each)
@@ -732,7 +743,7 @@ hipError_t merkle_impl(const P2Set &p2, void *const *layers, int nlayers, uint64
         const uint64_t level_n = n >> lvl;
         const bool lanes = level_n <= split;
         const bool chunks = !lanes && level_n <= csplit;
-        int k = lanes ? lanes_fuse<W>() : 3;
+        int k = lanes ? lanes_fuse<W>() : fused_levels();
         if (chunks) {
             // stop where the W-lane form takes over: a fused level with few
             // compressions costs the chunk form's longer chain
