@@ -76,14 +76,27 @@ def probe_copy(nbytes=1 << 30):
     return v.value
 
 
+# VALU issue ceiling: 256 CUs x 4 SIMDs, one wave64 instruction per 4 cycles each, 2.4 GHz peak engine clock.
+VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4
+
+
+def valu_issue(insts, ms):
+    if not insts or not ms:
+        return None
+    rate = insts / (ms * 1e-3)
+    return {"achieved": rate / 1e9, "peak": VALU_ISSUE_PEAK / 1e9, "unit": "G wave-instr/s",
+            "frac": rate / VALU_ISSUE_PEAK, "insts_per_launch": insts}
+
+
 def profiled_traffic():
     """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
-    (calibrated against a 1 GiB copy), read from the committed summary."""
+    (calibrated against a 1 GiB copy) and VALU wave-instructions per launch
+    (SQ_INSTS_VALU), read from the committed summary."""
     out = {}
     try:
         for line in open(os.path.join(ROOT, "profiles", "commit_bb_2p20x64_traffic.md")):
             k, _, v = line.partition(":")
-            if k.startswith("traffic_") and v.strip().isdigit():
+            if k.startswith(("traffic_", "valu_")) and v.strip().isdigit():
                 out[k] = int(v)
     except OSError:
         pass
@@ -576,9 +589,14 @@ def main():
         "roofline": {"kernel": "k_hash_rows<Bb,16>", "bound": "valu", "achieved": hash_muls / (hash_ms * 1e-3) / 1e9,
                      "peak": peak_bb / 1e9, "unit": "Gmodmul/s", "frac": hash_muls / (hash_ms * 1e-3) / peak_bb,
                      "traffic": traffic.get("traffic_hash_rows_bytes"),
+                     "valu_issue": valu_issue(traffic.get("valu_insts_hash_rows"), hash_ms),
                      "note": "772 BabyBear products per permutation x %d permutations per launch; peak = the "
                              "faster measured BabyBear product rate (exact FP64 or Montgomery), 8 independent "
-                             "chains per lane over the whole chip" % (leaves * ((w + 7) // 8))},
+                             "chains per lane over the whole chip. 208 of the 772 are diagonal products by "
+                             "+-2^-k and small integers, which cost 1-3 FP64 operations instead of a general "
+                             "product's 6, so frac can pass 1; valu_issue is the issue-rate roofline (profiled "
+                             "VALU wave-instructions per launch over this launch's duration)"
+                             % (leaves * ((w + 7) // 8))},
         "roofline_lde": {"kernels": "k_fs_pass1 + k_lde_mid + k_lde_final", "bound": "hbm",
                          "achieved": lde_bytes / (lde_ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                          "frac": lde_bytes / (lde_ms * 1e-3) / HBM_PEAK,
