@@ -33,7 +33,10 @@ constexpr int HASH_THREADS = 256;
 constexpr int MERKLE_THREADS = 256;
 constexpr int MAX_FUSE = 9;  // 2 * MERKLE_THREADS leaves -> 1 root
 
-// Goldilocks work-form thresholds (log2; see lanes_split_max below).
+// Work-form thresholds (log2; see lanes_split_max below).
+#ifndef ZKG_BB16_LANES_LOG2
+#define ZKG_BB16_LANES_LOG2 15
+#endif
 #ifndef ZKG_GL16_LANES_LOG2
 #define ZKG_GL16_LANES_LOG2 13
 #endif
@@ -75,7 +78,7 @@ uint64_t env_pow2(const char *name, uint64_t dflt)
 template <class F, int W>
 uint64_t lanes_split_max()
 {
-    constexpr int dflt = std::is_same<F, Bb>::value ? 17 : (W == 16 ? ZKG_GL16_LANES_LOG2 : ZKG_GL8_LANES_LOG2);
+    constexpr int dflt = std::is_same<F, Bb>::value ? ZKG_BB16_LANES_LOG2 : (W == 16 ? ZKG_GL16_LANES_LOG2 : ZKG_GL8_LANES_LOG2);
     static const uint64_t v = env_pow2("ZKG_MERKLE_LANES_LOG2", 1ull << dflt);
     return v;
 }
@@ -1201,7 +1204,7 @@ hipError_t launch_merkle_tops(int field, int width, const P2Set &p2, const Merkl
     return hipErrorInvalidValue;
 }
 
-// 2^17: the BabyBear trees that merkle_impl builds in the W-lane form alone.
+// 2^17: measured against 2^15, 2^16 and 2^18 on the 2^22 -> 2^8 commit phase.
 uint64_t fri_tops_max_leaves()
 {
     static const uint64_t v = env_pow2("ZKG_FRI_TOPS_LOG2", 1ull << 17);
