@@ -1204,10 +1204,10 @@ hipError_t launch_merkle_tops(int field, int width, const P2Set &p2, const Merkl
     return hipErrorInvalidValue;
 }
 
-// 2^17: measured against 2^15, 2^16 and 2^18 on the 2^22 -> 2^8 commit phase.
+// 2^16: measured against 2^15, 2^17 and 2^18 on the 2^22 -> 2^8 commit phase.
 uint64_t fri_tops_max_leaves()
 {
-    static const uint64_t v = env_pow2("ZKG_FRI_TOPS_LOG2", 1ull << 17);
+    static const uint64_t v = env_pow2("ZKG_FRI_TOPS_LOG2", 1ull << 16);
     return v;
 }
 

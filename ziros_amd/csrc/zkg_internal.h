@@ -143,7 +143,7 @@ struct MerkleTop {
 hipError_t launch_merkle_tops(int field, int width, const P2Set &p2, const MerkleTop *trees, int ntrees,
                               hipStream_t st);
 // FRI layer trees of at most this many leaves are built together after the
-// fold chain (ZKG_FRI_TOPS_LOG2 overrides the default 2^17).
+// fold chain (ZKG_FRI_TOPS_LOG2 overrides the default 2^16).
 uint64_t fri_tops_max_leaves();
 hipError_t launch_gen(int field, void *out, uint64_t seed, uint64_t start, uint64_t n,
                       hipStream_t st);
