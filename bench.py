@@ -40,14 +40,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import ziros_amd as Z  # noqa: E402
+from ziros_amd import workloads  # noqa: E402
 from ziros_amd._lib import check  # noqa: E402
 
 HBM_PEAK = 8.0e12
 BB_PERM_MULS = 772
 GL16_PERM_MULS = 952
 SEED_C2 = 0x5EED0002
-SEED_C3 = 0x5EED0003
-SEED_C5 = 0x5EED0005
+SEED_C3 = workloads.SEED_BATCH
+SEED_C5 = workloads.SEED_SERVE
 
 
 def log(*a):
@@ -342,10 +343,7 @@ def bench_constraints(ctx, steps, peak_gl):
 
 
 def c3_shapes():
-    rng = np.random.default_rng(SEED_C3)
-    logs = rng.integers(12, 21, 48)
-    widths = rng.integers(1, 33, 48) * 8
-    return [(int(l), int(wd)) for l, wd in zip(logs, widths)]
+    return workloads.batch_shapes(48, SEED_C3)
 
 
 def bench_c3(ctx, steps, warmup):
@@ -404,15 +402,7 @@ def bench_c5(ctx, n_req, load, slots=2, max_batch=8):
     `load` times the plain path's capacity (requests/s), so load > 1 is a
     stream the plain path cannot keep up with.
     """
-    rng = np.random.default_rng(SEED_C5)
-    reqs = []
-    for k in range(n_req):
-        field = Z.BABYBEAR if rng.random() < 0.6 else Z.GOLDILOCKS
-        lg = int(rng.integers(14, 23))
-        w = int(rng.integers(16, 129))
-        lb = int(rng.integers(1, 3))
-        reqs.append((field, lg, w, lb))
-    unit_gaps = rng.exponential(1.0, n_req)
+    reqs, unit_gaps = workloads.serve_requests(n_req, SEED_C5)
     bufs, jobs, cells = [], [], 0
     for k, (field, lg, w, lb) in enumerate(reqs):
         n = 1 << lg

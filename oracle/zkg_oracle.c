@@ -30,18 +30,23 @@ typedef unsigned __int128 u128;
  * Follows zkf-metal/src/shaders/field_goldilocks.metal:1-56 and the CPU twin
  * zkf-core/src/acceleration.rs:311-380 (reduce 2^64 = 2^32 - 1, 2^96 = -1).
  * ====================================================================== */
+/* The conditional corrections below are written as masks (m = 0 or all
+ * ones) rather than branches: on random data the branches mispredict and cost
+ * more than the arithmetic.  The values are the same either way. */
+static inline uint64_t mask_if(int c) { return (uint64_t)0 - (uint64_t)(c != 0); }
+
 uint64_t or_gl_add(uint64_t a, uint64_t b)
 {
     uint64_t s = a + b;
-    if (s < a) s += GL_EPS;         /* wrapped past 2^64: add 2^64 mod p */
-    if (s >= OR_GL_P) s -= OR_GL_P;
+    s += GL_EPS & mask_if(s < a);             /* wrapped past 2^64: add 2^64 mod p */
+    s -= OR_GL_P & mask_if(s >= OR_GL_P);
     return s;
 }
 
 uint64_t or_gl_sub(uint64_t a, uint64_t b)
 {
     uint64_t d = a - b;
-    if (a < b) d -= GL_EPS;         /* borrowed 2^64: subtract 2^64 mod p */
+    d -= GL_EPS & mask_if(a < b);             /* borrowed 2^64: subtract 2^64 mod p */
     return d;
 }
 
@@ -51,11 +56,11 @@ static inline uint64_t gl_reduce128(u128 x)
     uint64_t lo = (uint64_t)x, hi = (uint64_t)(x >> 64);
     uint64_t hh = hi >> 32, hl = hi & GL_EPS;
     uint64_t t0 = lo - hh;
-    if (lo < hh) t0 -= GL_EPS;
+    t0 -= GL_EPS & mask_if(lo < hh);
     uint64_t t1 = hl * GL_EPS;
     uint64_t r = t0 + t1;
-    if (r < t0) r += GL_EPS;
-    if (r >= OR_GL_P) r -= OR_GL_P;
+    r += GL_EPS & mask_if(r < t0);
+    r -= OR_GL_P & mask_if(r >= OR_GL_P);
     return r;
 }
 
@@ -80,9 +85,9 @@ uint64_t or_gl_pow(uint64_t a, uint64_t e)
 static inline uint64_t bb_add(uint64_t a, uint64_t b)
 {
     uint64_t s = a + b;
-    return s >= OR_BB_P ? s - OR_BB_P : s;
+    return s - (OR_BB_P & mask_if(s >= OR_BB_P));
 }
-static inline uint64_t bb_sub(uint64_t a, uint64_t b) { return a >= b ? a - b : a + OR_BB_P - b; }
+static inline uint64_t bb_sub(uint64_t a, uint64_t b) { return a - b + (OR_BB_P & mask_if(a < b)); }
 uint64_t or_bb_mul(uint64_t a, uint64_t b) { return (a * b) % OR_BB_P; }
 uint64_t or_bb_pow(uint64_t a, uint64_t e)
 {
@@ -305,7 +310,7 @@ size_t or_p2_flat_constants(const or_p2 *p, uint64_t *out)
     return n;
 }
 
-static inline uint64_t sbox7(int f, uint64_t x)
+static inline __attribute__((always_inline)) uint64_t sbox7(int f, uint64_t x)
 {
     uint64_t x2 = or_f_mul(f, x, x);
     uint64_t x3 = or_f_mul(f, x2, x);
@@ -313,7 +318,7 @@ static inline uint64_t sbox7(int f, uint64_t x)
     return or_f_mul(f, x3, x4);
 }
 
-static void mds_light(int f, int W, uint64_t *s)
+static inline __attribute__((always_inline)) void mds_light(int f, int W, uint64_t *s)
 {
     for (int c = 0; c < W; c += 4) {
         uint64_t a = s[c], b = s[c + 1], cc = s[c + 2], d = s[c + 3];
@@ -333,15 +338,19 @@ static void mds_light(int f, int W, uint64_t *s)
     for (int i = 0; i < W; i++) s[i] = or_f_add(f, s[i], sum[i & 3]);
 }
 
-static void external_round(int f, int W, uint64_t *s, const uint64_t *rc)
+static inline __attribute__((always_inline)) void external_round(int f, int W, uint64_t *s, const uint64_t *rc)
 {
     for (int i = 0; i < W; i++) s[i] = sbox7(f, or_f_add(f, s[i], rc[i]));
     mds_light(f, W, s);
 }
 
-void or_p2_permute(const or_p2 *p, uint64_t *s)
+/* The permutation body with the field and width as arguments that the two
+ * call sites below fix, so the compiler specialises the arithmetic (no
+ * per-operation field dispatch); the semantics are the ones written here. */
+static inline __attribute__((always_inline)) void p2_permute_body(const or_p2 *p, uint64_t *s, const int f,
+                                                                   const int W)
 {
-    const int f = p->field, W = p->width, half = p->rounds_f / 2;
+    const int half = p->rounds_f / 2;
     const uint64_t *init = p->rc, *internal = p->rc + half * W, *term = internal + p->rounds_p;
     mds_light(f, W, s);
     for (int r = 0; r < half; r++) external_round(f, W, s, init + r * W);
@@ -352,6 +361,16 @@ void or_p2_permute(const or_p2 *p, uint64_t *s)
         for (int i = 0; i < W; i++) s[i] = or_f_add(f, or_f_mul(f, s[i], p->diag[i]), sum);
     }
     for (int r = 0; r < half; r++) external_round(f, W, s, term + r * W);
+}
+
+void or_p2_permute(const or_p2 *p, uint64_t *s)
+{
+    if (p->field == OR_GOLDILOCKS && p->width == 16)
+        p2_permute_body(p, s, OR_GOLDILOCKS, 16);
+    else if (p->field == OR_GOLDILOCKS)
+        p2_permute_body(p, s, OR_GOLDILOCKS, 8);
+    else
+        p2_permute_body(p, s, OR_BABYBEAR, 16);
 }
 
 /* PaddingFreeSponge<P, W, RATE=W/2, OUT=W/2>::hash_iter (p3-symmetric 0.4.2,

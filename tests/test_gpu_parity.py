@@ -59,9 +59,6 @@ def test_batch_permute_explicit_constants(ctx):
         got = fn(st.copy(), rc, 8, p2.p.rounds_p)
         exp = np.stack([p2.permute(st[i]) for i in range(64)])
         assert (got.reshape(64, 16).astype(np.uint64) == exp).all()
-    # restore the session constants
-    ctx.load_constants(GL, 16, 42)
-    ctx.load_constants(BB, 16, 42)
 
 
 def test_generator_matches_oracle(ctx):

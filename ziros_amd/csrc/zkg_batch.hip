@@ -348,7 +348,7 @@ static void serve_loop(zkg_ctx *ctx, ServeState *s)
                 while (!s->queue.empty() && take.size() < s->max_batch) {
                     uint64_t t = s->queue.front();
                     const zkg_job &j = s->reqs[t].job;
-                    if (ctx->hbm_used + job_extra_bytes(j) > ctx->hbm_budget && !take.empty()) break;
+                    if (hbm_in_use(ctx) + job_extra_bytes(j) > ctx->hbm_budget && !take.empty()) break;
                     take.push_back(t);
                     jobs.push_back(j);
                     s->queue.pop_front();

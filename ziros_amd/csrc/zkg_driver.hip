@@ -105,6 +105,10 @@ int run_poseidon_batch(zkg_bridge *b, const zkg_node *nd, int field, uint64_t *i
     std::vector<T> in;
     if (int rc = read_slot(b, nd->in_slot, in)) return rc;
     const uint64_t count = nd->size, w = std::max<uint32_t>(nd->width, 1);
+    // the states live in host and device memory: a count whose states could
+    // not fit the context's HBM budget is refused before anything is sized
+    if (span_bytes(count, 16, sizeof(T)) > b->ctx->hbm_budget)
+        return set_err(ZKG_EINVAL, "PoseidonBatch count %llu exceeds the HBM budget", (unsigned long long)count);
     std::vector<T> states(count * 16, 0);
     for (uint64_t p = 0; p < count; p++)
         for (uint64_t i = 0; i < nd->width && p * w + i < in.size(); i++) states[p * 16 + i] = in[p * w + i];
@@ -113,6 +117,10 @@ int run_poseidon_batch(zkg_bridge *b, const zkg_node *nd, int field, uint64_t *i
     if (nd->aux_slot && slot_buf(b, nd->aux_slot)) {
         std::vector<T> rc;
         if (int e = read_slot(b, nd->aux_slot, rc)) return e;
+        // the flat table [initial][internal][terminal] of width-16 rounds
+        if (rc.size() != (uint64_t)n_ext * 16 + n_int)
+            return set_err(ZKG_EINVAL, "PoseidonBatch constants slot holds %zu elements, rounds (%u, %u) need %llu",
+                           rc.size(), n_ext, n_int, (unsigned long long)((uint64_t)n_ext * 16 + n_int));
         rc64.assign(rc.begin(), rc.end());
     } else {
         if (int e = p2_seed_constants(field, 16, 42, rc64)) return e;
@@ -148,7 +156,9 @@ int run_merkle_layer(zkg_bridge *b, const zkg_node *nd, int field, uint64_t *in_
     if (int rc = read_slot(b, nd->in_slot, in)) return rc;
     const uint64_t leaves = nd->size;
     if (in.empty() || !leaves) return set_err(ZKG_EFALLBACK, "MerkleLayer missing %s leaves", field_name(field));
-    const uint64_t lw = std::max<uint64_t>(in.size() / leaves, 1), parents = (leaves + 1) / 2;
+    const uint64_t lw = std::max<uint64_t>(in.size() / leaves, 1), parents = leaves / 2 + (leaves & 1);
+    if (span_bytes(parents, 16, sizeof(T)) > b->ctx->hbm_budget)
+        return set_err(ZKG_EINVAL, "MerkleLayer leaf_count %llu exceeds the HBM budget", (unsigned long long)leaves);
     const uint64_t ow = nd->out_bytes ? std::min<uint64_t>(std::max<uint64_t>(nd->out_bytes / sizeof(T) / parents, 1), 8)
                                       : 1;
     std::vector<uint64_t> rc64;

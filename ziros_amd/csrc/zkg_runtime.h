@@ -39,7 +39,7 @@ struct zkg_ctx {
     uint64_t hbm_budget = 0;
     uint64_t hbm_used = 0;
     hipStream_t stream = nullptr;
-    std::mutex mem_mu, table_mu;
+    mutable std::mutex mem_mu, table_mu;
     std::unordered_map<void *, uint64_t> alloc_sizes;  // live blocks
     std::multimap<uint64_t, void *> cache;              // freed blocks by size
     uint64_t cached_bytes = 0;
@@ -98,6 +98,8 @@ int hip_err(hipError_t e, const char *what);
 int dev_alloc(zkg_ctx *ctx, uint64_t bytes, void **out);
 void dev_free(zkg_ctx *ctx, void *p);
 void cache_release(zkg_ctx *ctx);
+// Bytes of live device blocks (read under the allocator's lock).
+uint64_t hbm_in_use(const zkg_ctx *ctx);
 int get_coset(zkg_ctx *ctx, int field, int log_n, int log_b, uint64_t shift, const CosetSet **out);
 const P2Set *get_p2(zkg_ctx *ctx, int field, int width);
 int run_lde(zkg_ctx *ctx, hipStream_t st, int field, const void *in, uint64_t height, uint32_t width,

@@ -258,6 +258,12 @@ void zkg::dev_free(zkg_ctx *ctx, void *p)
     ctx->cached_bytes += sz;
 }
 
+uint64_t zkg::hbm_in_use(const zkg_ctx *ctx)
+{
+    std::lock_guard<std::mutex> g(ctx->mem_mu);
+    return ctx->hbm_used;
+}
+
 void zkg::cache_release(zkg_ctx *ctx)
 {
     std::lock_guard<std::mutex> g(ctx->mem_mu);
@@ -384,18 +390,16 @@ const P2Set *zkg::get_p2(zkg_ctx *ctx, int field, int width)
 
 namespace {
 
+// Uploads one constant set (register form; BabyBear also as canonical doubles
+// for the FP64 permutation) into `p`, which the caller owns.
 template <class F>
-int load_p2(zkg_ctx *ctx, int idx, int width, const std::vector<uint64_t> &flat_canon, const uint64_t *diag_canon)
+int p2_build(zkg_ctx *ctx, int width, const std::vector<uint64_t> &flat_canon, const uint64_t *diag_canon, P2Set &p)
 {
     using T = typename F::T;
     std::vector<T> rc(flat_canon.size()), dg(width);
     for (size_t i = 0; i < flat_canon.size(); i++) rc[i] = F::from_canon((T)flat_canon[i]);
     for (int i = 0; i < width; i++) dg[i] = F::from_canon((T)diag_canon[i]);
-    P2Set &p = ctx->p2[idx];
-    dev_free(ctx, p.rc);
-    dev_free(ctx, p.diag);
-    dev_free(ctx, p.rc_f64);
-    p.rc = p.diag = p.rc_f64 = nullptr;
+    p = P2Set();
     int r;
     if ((r = upload_vec<F>(ctx, rc, &p.rc))) return r;
     if ((r = upload_vec<F>(ctx, dg, &p.diag))) return r;
@@ -407,6 +411,14 @@ int load_p2(zkg_ctx *ctx, int idx, int width, const std::vector<uint64_t> &flat_
     }
     p.loaded = true;
     return ZKG_OK;
+}
+
+void p2_release(zkg_ctx *ctx, P2Set &p)
+{
+    dev_free(ctx, p.rc);
+    dev_free(ctx, p.diag);
+    dev_free(ctx, p.rc_f64);
+    p = P2Set();
 }
 
 void bb_diag16(uint64_t *d)
@@ -423,15 +435,41 @@ void bb_diag16(uint64_t *d)
     memcpy(d, v, sizeof v);
 }
 
-int load_p2_flat(zkg_ctx *ctx, int field, int width, const std::vector<uint64_t> &flat)
+// A constant set for (field, width) from a flat table, diagonal per instance.
+int p2_build_flat(zkg_ctx *ctx, int field, int width, const std::vector<uint64_t> &flat, P2Set &out)
 {
-    int idx = p2_index(field, width);
     uint64_t diag[16];
     if (field == FIELD_GL)
         memcpy(diag, width == 16 ? GL_DIAG16 : GL_DIAG8, width * sizeof(uint64_t));
     else
         bb_diag16(diag);
-    return field == FIELD_GL ? load_p2<Gl>(ctx, idx, width, flat, diag) : load_p2<Bb>(ctx, idx, width, flat, diag);
+    int rc = field == FIELD_GL ? p2_build<Gl>(ctx, width, flat, diag, out) : p2_build<Bb>(ctx, width, flat, diag, out);
+    if (rc) p2_release(ctx, out);
+    return rc;
+}
+
+// Replaces the context's resident set for (field, width).  The new tables are
+// uploaded first; the old ones are freed only once nothing can still read
+// them: every entry point has drained its streams when it returns, and the
+// cached batch graph, whose kernels hold the old table addresses, is dropped.
+// Refused while the serving loop runs (its slots hold launches that read the
+// resident tables).
+int p2_replace(zkg_ctx *ctx, int field, int width, const std::vector<uint64_t> &flat)
+{
+    if (ctx->serve) return set_err(ZKG_ESTATE, "Poseidon2 constants cannot change while the serving loop runs");
+    P2Set fresh;
+    if (int rc = p2_build_flat(ctx, field, width, flat, fresh)) return rc;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i < 2 && e == hipSuccess; i++) e = hipStreamSynchronize(ctx->side[i]);
+    if (e != hipSuccess) {
+        p2_release(ctx, fresh);
+        return hip_err(e, "sync before constant reload");
+    }
+    batch_release(ctx);
+    P2Set &cur = ctx->p2[p2_index(field, width)];
+    p2_release(ctx, cur);
+    cur = fresh;
+    return ZKG_OK;
 }
 
 }  // namespace
@@ -505,7 +543,7 @@ extern "C" int zkg_ctx_synchronize(zkg_ctx *ctx)
 extern "C" int zkg_ctx_hbm_used(const zkg_ctx *ctx, uint64_t *bytes)
 {
     if (!ctx || !bytes) return set_err(ZKG_EINVAL, "null argument");
-    *bytes = ctx->hbm_used;
+    *bytes = hbm_in_use(ctx);
     return ZKG_OK;
 }
 
@@ -616,28 +654,37 @@ extern "C" int zkg_constants_load(zkg_ctx *ctx, int field, int width, uint64_t s
     if (!ctx) return set_err(ZKG_EINVAL, "null ctx");
     std::vector<uint64_t> flat;
     if (int rc = p2_seed_constants(field, width, seed, flat)) return rc;
-    HIP_TRY(hipStreamSynchronize(ctx->stream), "sync");
-    int rc = load_p2_flat(ctx, field, width, flat);
+    int rc = p2_replace(ctx, field, width, flat);
     if (!rc) ctx->p2[p2_index(field, width)].seed = seed;
     return rc;
 }
 
-extern "C" int zkg_constants_set(zkg_ctx *ctx, int field, int width, const void *flat, uint32_t n_ext, uint32_t n_int)
+// An explicit flat table [initial][internal][terminal] checked against the
+// compiled schedule (8 external rounds, the field's internal rounds) and for
+// canonical values.
+static int p2_flat_from_caller(int field, int width, const void *flat, uint32_t n_ext, uint32_t n_int,
+                               std::vector<uint64_t> &v)
 {
-    if (!ctx || !flat) return set_err(ZKG_EINVAL, "null argument");
     if (p2_index(field, width) < 0) return set_err(ZKG_EINVAL, "unsupported Poseidon2 field %d width %d", field, width);
     if (n_ext != 8 || (int)n_int != rounds_p_of(field))
         return set_err(ZKG_EINVAL, "rounds (%u, %u) differ from the compiled schedule (8, %d)", n_ext, n_int,
                        rounds_p_of(field));
     size_t n = (size_t)n_ext * width + n_int;
-    std::vector<uint64_t> v(n);
+    v.resize(n);
     for (size_t i = 0; i < n; i++) {
         uint64_t x = field == FIELD_GL ? ((const uint64_t *)flat)[i] : ((const uint32_t *)flat)[i];
         if (x >= (field == FIELD_GL ? Gl::P : (uint64_t)Bb::P)) return set_err(ZKG_EINVAL, "constant %zu not canonical", i);
         v[i] = x;
     }
-    HIP_TRY(hipStreamSynchronize(ctx->stream), "sync");
-    return load_p2_flat(ctx, field, width, v);
+    return ZKG_OK;
+}
+
+extern "C" int zkg_constants_set(zkg_ctx *ctx, int field, int width, const void *flat, uint32_t n_ext, uint32_t n_int)
+{
+    if (!ctx || !flat) return set_err(ZKG_EINVAL, "null argument");
+    std::vector<uint64_t> v;
+    if (int rc = p2_flat_from_caller(field, width, flat, n_ext, n_int, v)) return rc;
+    return p2_replace(ctx, field, width, v);
 }
 
 // ------------------------------------------------------------------ transforms
@@ -718,21 +765,32 @@ extern "C" int zkg_poseidon2_permute(zkg_ctx *ctx, int field, int width, zkg_buf
     return ZKG_OK;
 }
 
+// The trait's explicit-constant form: the caller's table goes into a set of
+// its own for this call only, so the context's resident constants (which
+// commits, MMCS and FRI use) are left as they were.
 template <class T>
 static int permute_host(zkg_ctx *ctx, int field, T *states, uint64_t n, const T *rc, uint32_t n_ext, uint32_t n_int)
 {
     if (!ctx || (!states && n) || !rc) return set_err(ZKG_EINVAL, "null argument");
-    int r = zkg_constants_set(ctx, field, 16, rc, n_ext, n_int);
+    std::vector<uint64_t> flat;
+    int r = p2_flat_from_caller(field, 16, rc, n_ext, n_int, flat);
     if (r) return r;
     if (!n) return ZKG_OK;
+    const uint64_t bytes = span_bytes(n, 16, sizeof(T));
+    if (bytes > ctx->hbm_budget) return set_err(ZKG_EINVAL, "%llu states exceed the HBM budget", (unsigned long long)n);
+    P2Set once;
+    if ((r = p2_build_flat(ctx, field, 16, flat, once))) return r;
     void *d;
-    uint64_t bytes = n * 16 * sizeof(T);
-    if ((r = dev_alloc(ctx, bytes, &d))) return r;
+    if ((r = dev_alloc(ctx, bytes, &d))) {
+        p2_release(ctx, once);
+        return r;
+    }
     hipError_t e = hipMemcpy(d, states, bytes, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = launch_p2_permute(field, 16, *get_p2(ctx, field, 16), d, n, ctx->stream);
+    if (e == hipSuccess) e = launch_p2_permute(field, 16, once, d, n, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess) e = hipMemcpy(states, d, bytes, hipMemcpyDeviceToHost);
     dev_free(ctx, d);
+    p2_release(ctx, once);
     if (e != hipSuccess) return hip_err(e, "batch permute");
     return ZKG_OK;
 }

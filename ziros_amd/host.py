@@ -65,7 +65,16 @@ class Context:
         return v.value
 
     def load_constants(self, field, width=16, seed=42):
+        """Constants of Poseidon2<field, width> derived from SmallRng(seed)."""
         check(self.lib.zkg_constants_load(self.h, field, width, seed))
+
+    def set_constants(self, field, width, flat, n_external_rounds=8, n_internal_rounds=None):
+        """An explicit flat [initial][internal][terminal] table, as p3's
+        flatten_round_constants hands it over (zkf-metal/src/poseidon2/
+        babybear.rs:76-111, goldilocks.rs:42-83)."""
+        n_int = n_internal_rounds if n_internal_rounds is not None else (22 if field == GOLDILOCKS else 13)
+        a = np.ascontiguousarray(flat, dtype=DTYPE[field])
+        check(self.lib.zkg_constants_set(self.h, field, width, _ptr(a), n_external_rounds, n_int))
 
     def set_timing(self, on=True):
         check(self.lib.zkg_ctx_set_timing(self.h, 1 if on else 0))
