@@ -91,6 +91,17 @@ int zkg_constants_set(zkg_ctx *ctx, int field, int width, const void *flat, uint
  * natural order in and out, inverse scaled by 1/height.  Replaces MetalDft
  * (zkf-metal/src/ntt/p3_adapter.rs:69-75,78-106). */
 int zkg_ntt_batch(zkg_ctx *ctx, int field, zkg_buf *buf, uint64_t height, uint32_t width, int inverse);
+/* No counterpart in the reference: zkg_ntt_batch for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The four-step transposition buffer
+ * (height x width elements for heights above 2^11) lives in a fixed device
+ * array of ZKG_NOALLOC_SCRATCH_BYTES per GPU instead of an allocation; larger
+ * transforms are ZKG_EINVAL.  Stream-ordered: returns once enqueued on the
+ * context's stream (zkg_buf_download and zkg_ctx_synchronize wait for it).
+ * All calls on one GPU share the array: calls on different contexts of one
+ * GPU that overlap in time race on it. */
+#define ZKG_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int zkg_ntt_batch_noalloc(zkg_ctx *ctx, int field, zkg_buf *buf, uint64_t height, uint32_t width, int inverse);
 /* Coset LDE as TwoAdicFriPcs::commit uses RuntimeDft
  * (zkf-backends/src/plonky3.rs:101-257): out (height << log_blowup rows) =
  * bit_reverse_rows(coset_lde_batch(in, log_blowup, shift)).  in is unchanged. */

@@ -113,6 +113,9 @@ void stage_collect(zkg_ctx *ctx, int nmarks);
 void batch_release(zkg_ctx *ctx);
 void serve_shutdown(zkg_ctx *ctx);
 void ce_cache_release(zkg_ctx *ctx);
+// Looks up this device's address of the no-allocation scratch array
+// (zkg_noalloc.hip); called once per context with its device current.
+int noalloc_setup(int device);
 // Poseidon2 constants drawn from SmallRng(seed), flat [initial][internal][terminal].
 int p2_seed_constants(int field, int width, uint64_t seed, std::vector<uint64_t> &flat);
 int p2_internal_rounds(int field);

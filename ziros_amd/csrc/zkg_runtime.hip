@@ -482,6 +482,7 @@ extern "C" int zkg_ctx_create(int device, uint64_t budget, zkg_ctx **out)
     HIP_TRY(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
     if (device < 0 || device >= ndev) return set_err(ZKG_EINVAL, "device %d not present (%d devices)", device, ndev);
     HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    if (int rc = noalloc_setup(device)) return rc;
     zkg_ctx *c = new zkg_ctx();
     c->device = device;
     c->hbm_budget = budget ? budget : (288ull << 30);
