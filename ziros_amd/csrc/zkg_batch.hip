@@ -79,31 +79,27 @@ static int prepare_job(zkg_ctx *ctx, const zkg_job &j, Prepared &p)
     return ZKG_OK;
 }
 
-// One job's pipeline on one stream: LDE, row hash, tree.
-static int enqueue_one(zkg_ctx *ctx, hipStream_t st, Prepared &p)
-{
-    int rc = run_lde(ctx, st, p.job.field, p.job.trace->ptr, p.job.height, p.job.width, p.job.log_blowup,
-                     p.job.field == FIELD_GL ? 7 : 31, p.job.lde_out->ptr, p.tmp);
-    if (rc) return rc;
-    const P2Set *p2 = get_p2(ctx, p.job.field, p.job.p2_width);
-    hipError_t e = launch_hash_rows(p.job.field, p.job.p2_width, *p2, p.tree->mat, p.tree->n_leaves, p.job.width,
-                                    p.tree->layers[0], st);
-    if (e != hipSuccess) return hip_err(e, "batch hash");
-    e = launch_merkle_tree(p.job.field, p.job.p2_width, *p2, p.tree->layers.data(), p.tree->nlayers,
-                           p.tree->n_leaves, 9, st, false);
-    if (e != hipSuccess) return hip_err(e, "batch merkle");
-    return ZKG_OK;
-}
+// ---------------------------------------------------------------- shared launches
+// A job set runs as:
+//   * the coset LDE of every job, dealt (largest first, grouped by field and
+//     Poseidon2 width) round-robin over K branch streams, so small transforms
+//     overlap large ones;
+//   * per (field, width) group, on a stream of its own once the group's LDEs
+//     are done: ONE persistent row-hash launch over a table of every member's
+//     rows, then one Merkle launch per round, where round r holds the r-th
+//     launch of every member's tree (k_merkle_segs maps its blocks to the
+//     trees by prefix offsets).
+// Under stream capture the branches and group streams become parallel paths
+// of the graph.
+constexpr int MAX_BRANCHES = 8, MAX_GROUPS = 3;
 
-// Fork/join over K side streams: jobs, largest first, are dealt round-robin
-// to the branches so their pipelines overlap on the device (small jobs'
-// latency-bound phases fill in around large jobs).  Under stream capture the
-// branches become parallel paths of the graph.
 struct Fork {
     int k = 0;
-    hipStream_t s[8] = {};
+    hipStream_t s[MAX_BRANCHES] = {};  // LDE branches
+    hipStream_t gs[MAX_GROUPS] = {};   // row hash + Merkle of one group
     hipEvent_t start = nullptr;
-    hipEvent_t join[8] = {};
+    hipEvent_t join[MAX_BRANCHES] = {}, gjoin[MAX_GROUPS] = {};
+    hipEvent_t lde_done[MAX_GROUPS][MAX_BRANCHES] = {};
 };
 
 static int fork_init(Fork &f, int k)
@@ -113,6 +109,12 @@ static int fork_init(Fork &f, int k)
     for (int i = 0; i < k && e == hipSuccess; i++) {
         e = hipStreamCreateWithFlags(&f.s[i], hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&f.join[i], hipEventDisableTiming);
+        for (int g = 0; g < MAX_GROUPS && e == hipSuccess; g++)
+            e = hipEventCreateWithFlags(&f.lde_done[g][i], hipEventDisableTiming);
+    }
+    for (int g = 0; g < MAX_GROUPS && e == hipSuccess; g++) {
+        e = hipStreamCreateWithFlags(&f.gs[g], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&f.gjoin[g], hipEventDisableTiming);
     }
     return e == hipSuccess ? ZKG_OK : hip_err(e, "fork streams");
 }
@@ -125,6 +127,15 @@ static void fork_destroy(Fork &f)
             (void)hipStreamDestroy(f.s[i]);
         }
         if (f.join[i]) (void)hipEventDestroy(f.join[i]);
+        for (int g = 0; g < MAX_GROUPS; g++)
+            if (f.lde_done[g][i]) (void)hipEventDestroy(f.lde_done[g][i]);
+    }
+    for (int g = 0; g < MAX_GROUPS; g++) {
+        if (f.gs[g]) {
+            (void)hipStreamSynchronize(f.gs[g]);
+            (void)hipStreamDestroy(f.gs[g]);
+        }
+        if (f.gjoin[g]) (void)hipEventDestroy(f.gjoin[g]);
     }
     if (f.start) (void)hipEventDestroy(f.start);
     f = Fork();
@@ -132,27 +143,158 @@ static void fork_destroy(Fork &f)
 
 static uint64_t job_cells(const zkg_job &j) { return (j.height << j.log_blowup) * j.width * (j.field == FIELD_GL ? 4 : 1); }
 
-static int enqueue_jobs(zkg_ctx *ctx, hipStream_t st, std::vector<Prepared> &ps, Fork *f)
+// The job tables of one job set, laid out in one block of device memory:
+// per group its HashJob table, its counter pair and one MerkleSeg table per
+// round.  `host` is the image uploaded to `dev` (the counters zero).
+struct GroupPlan {
+    int field = 0, width = 0;
+    std::vector<size_t> members;  // indices into the prepared jobs, LDE order
+    int njobs = 0;
+    uint32_t ntiles = 0;
+    uint64_t hash_off = 0, ctr_off = 0;
+    std::vector<uint64_t> round_off;
+    std::vector<int> round_segs;
+    std::vector<uint32_t> round_blocks;
+};
+
+struct SharedPlan {
+    std::vector<GroupPlan> groups;
+    std::vector<uint8_t> host;
+    void *dev = nullptr;
+};
+
+static void plan_release(zkg_ctx *ctx, SharedPlan &sp)
 {
-    if (!f || f->k <= 1 || ps.size() <= 1) {
-        for (auto &p : ps)
-            if (int rc = enqueue_one(ctx, st, p)) return rc;
-        return ZKG_OK;
+    dev_free(ctx, sp.dev);
+    sp = SharedPlan();
+}
+
+static uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+static int plan_build(zkg_ctx *ctx, std::vector<Prepared> &ps, SharedPlan &sp)
+{
+    sp = SharedPlan();
+    for (size_t i = 0; i < ps.size(); i++) {
+        const zkg_job &j = ps[i].job;
+        GroupPlan *g = nullptr;
+        for (auto &x : sp.groups)
+            if (x.field == j.field && x.width == j.p2_width) g = &x;
+        if (!g) {
+            if (sp.groups.size() == MAX_GROUPS) return set_err(ZKG_EINVAL, "more than %d Poseidon2 instances", MAX_GROUPS);
+            sp.groups.push_back(GroupPlan());
+            g = &sp.groups.back();
+            g->field = j.field;
+            g->width = j.p2_width;
+        }
+        g->members.push_back(i);
     }
-    std::vector<size_t> order(ps.size());
-    for (size_t i = 0; i < ps.size(); i++) order[i] = i;
-    std::sort(order.begin(), order.end(),
-              [&](size_t a, size_t b) { return job_cells(ps[a].job) > job_cells(ps[b].job); });
-    const int k = (int)std::min<size_t>((size_t)f->k, ps.size());
-    hipError_t e = hipEventRecord(f->start, st);
-    for (int i = 0; i < k && e == hipSuccess; i++) e = hipStreamWaitEvent(f->s[i], f->start, 0);
+    uint64_t off = 0;
+    std::vector<std::vector<HashJob>> hash(sp.groups.size());
+    std::vector<std::vector<std::vector<MerkleSeg>>> rounds(sp.groups.size());
+    for (size_t gi = 0; gi < sp.groups.size(); gi++) {
+        GroupPlan &g = sp.groups[gi];
+        // LDEs largest first; row-hash tiles heaviest (most permutations per
+        // row) first, so the persistent grid takes the long tiles early
+        std::sort(g.members.begin(), g.members.end(),
+                  [&](size_t a, size_t b) { return job_cells(ps[a].job) > job_cells(ps[b].job); });
+        std::vector<size_t> by_cost = g.members;
+        const uint32_t rate = (uint32_t)g.width / 2;
+        std::stable_sort(by_cost.begin(), by_cost.end(), [&](size_t a, size_t b) {
+            return (ps[a].job.width + rate - 1) / rate > (ps[b].job.width + rate - 1) / rate;
+        });
+        uint64_t tiles = 0;
+        for (size_t m : by_cost) {
+            const zkg_tree *t = ps[m].tree;
+            HashJob h{t->mat, t->layers[0], t->n_leaves, ps[m].job.width, (uint32_t)tiles};
+            hash[gi].push_back(h);
+            tiles += (t->n_leaves + HASH_TILE_ROWS - 1) / HASH_TILE_ROWS;
+        }
+        if (tiles > UINT32_MAX / 2) return set_err(ZKG_EINVAL, "job set too large for one launch");
+        g.ntiles = (uint32_t)tiles;
+        g.njobs = (int)hash[gi].size();
+        g.hash_off = off = align_up(off, 256);
+        off += hash[gi].size() * sizeof(HashJob);
+        g.ctr_off = off = align_up(off, 256);
+        off += 2 * sizeof(uint32_t);
+        for (size_t m : g.members) {
+            zkg_tree *t = ps[m].tree;
+            std::vector<MerkleSeg> segs;
+            if (merkle_plan(g.field, g.width, t->layers.data(), t->nlayers, t->n_leaves, segs))
+                return set_err(ZKG_EINVAL, "Poseidon2 field %d width %d", g.field, g.width);
+            for (size_t r = 0; r < segs.size(); r++) {
+                if (rounds[gi].size() <= r) rounds[gi].emplace_back();
+                rounds[gi][r].push_back(segs[r]);
+            }
+        }
+        for (auto &rd : rounds[gi]) {
+            uint64_t first = 0;
+            for (auto &sg : rd) {
+                sg.first = (uint32_t)first;
+                first += sg.blocks;
+            }
+            if (first > UINT32_MAX / 2) return set_err(ZKG_EINVAL, "job set too large for one launch");
+            g.round_off.push_back(off = align_up(off, 256));
+            g.round_segs.push_back((int)rd.size());
+            g.round_blocks.push_back((uint32_t)first);
+            off += rd.size() * sizeof(MerkleSeg);
+        }
+    }
+    sp.host.assign(align_up(off, 256), 0);
+    for (size_t gi = 0; gi < sp.groups.size(); gi++) {
+        const GroupPlan &g = sp.groups[gi];
+        memcpy(sp.host.data() + g.hash_off, hash[gi].data(), hash[gi].size() * sizeof(HashJob));
+        for (size_t r = 0; r < rounds[gi].size(); r++)
+            memcpy(sp.host.data() + g.round_off[r], rounds[gi][r].data(), rounds[gi][r].size() * sizeof(MerkleSeg));
+    }
+    return dev_alloc(ctx, sp.host.size(), &sp.dev);
+}
+
+static int enqueue_shared(zkg_ctx *ctx, hipStream_t st, std::vector<Prepared> &ps, Fork &f, const SharedPlan &sp)
+{
+    const int k = std::max(1, std::min<int>(f.k, (int)ps.size()));
+    hipError_t e = hipEventRecord(f.start, st);
+    for (int i = 0; i < k && e == hipSuccess; i++) e = hipStreamWaitEvent(f.s[i], f.start, 0);
     if (e != hipSuccess) return hip_err(e, "fork");
     int rc = ZKG_OK;
-    for (size_t j = 0; j < order.size() && !rc; j++) rc = enqueue_one(ctx, f->s[j % k], ps[order[j]]);
+    // LDEs, group by group, dealt over the branches
+    int next = 0;
+    for (size_t gi = 0; gi < sp.groups.size() && !rc; gi++) {
+        for (size_t m : sp.groups[gi].members) {
+            Prepared &p = ps[m];
+            rc = run_lde(ctx, f.s[next], p.job.field, p.job.trace->ptr, p.job.height, p.job.width, p.job.log_blowup,
+                         p.job.field == FIELD_GL ? 7 : 31, p.job.lde_out->ptr, p.tmp);
+            next = (next + 1) % k;
+            if (rc) break;
+        }
+        for (int i = 0; i < k && !rc; i++) {
+            e = hipEventRecord(f.lde_done[gi][i], f.s[i]);
+            if (e != hipSuccess) rc = hip_err(e, "lde event");
+        }
+    }
+    // per group: the shared row hash, then the Merkle rounds
+    uint8_t *dev = (uint8_t *)sp.dev;
+    for (size_t gi = 0; gi < sp.groups.size() && !rc; gi++) {
+        const GroupPlan &g = sp.groups[gi];
+        hipStream_t gs = f.gs[gi];
+        for (int i = 0; i < k && e == hipSuccess; i++) e = hipStreamWaitEvent(gs, f.lde_done[gi][i], 0);
+        const P2Set *p2 = get_p2(ctx, g.field, g.width);
+        if (e == hipSuccess)
+            e = launch_hash_jobs(g.field, g.width, *p2, (const HashJob *)(dev + g.hash_off), g.njobs, g.ntiles,
+                                 (uint32_t *)(dev + g.ctr_off), gs);
+        for (size_t r = 0; r < g.round_off.size() && e == hipSuccess; r++)
+            e = launch_merkle_segs(g.field, g.width, *p2, (const MerkleSeg *)(dev + g.round_off[r]), g.round_segs[r],
+                                   g.round_blocks[r], gs);
+        if (e != hipSuccess) rc = hip_err(e, "shared launches");
+    }
     // always join, so the origin stream (or capture) is left consistent
     for (int i = 0; i < k; i++) {
-        hipError_t e2 = hipEventRecord(f->join[i], f->s[i]);
-        if (e2 == hipSuccess) e2 = hipStreamWaitEvent(st, f->join[i], 0);
+        hipError_t e2 = hipEventRecord(f.join[i], f.s[i]);
+        if (e2 == hipSuccess) e2 = hipStreamWaitEvent(st, f.join[i], 0);
+        if (e2 != hipSuccess && !rc) rc = hip_err(e2, "join");
+    }
+    for (size_t gi = 0; gi < sp.groups.size(); gi++) {
+        hipError_t e2 = hipEventRecord(f.gjoin[gi], f.gs[gi]);
+        if (e2 == hipSuccess) e2 = hipStreamWaitEvent(st, f.gjoin[gi], 0);
         if (e2 != hipSuccess && !rc) rc = hip_err(e2, "join");
     }
     return rc;
@@ -163,6 +305,7 @@ struct BatchState {
     std::vector<uint64_t> key;
     hipGraphExec_t exec = nullptr;
     std::vector<Prepared> prepared;
+    SharedPlan plan;
     Fork fork;
 };
 
@@ -183,6 +326,7 @@ void batch_release(zkg_ctx *ctx)
     if (!b) return;
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     for (auto &p : b->prepared) release_prepared(ctx, p, false);
+    plan_release(ctx, b->plan);
     fork_destroy(b->fork);
     delete b;
     ctx->batch = nullptr;
@@ -230,22 +374,32 @@ extern "C" int zkg_commit_batch(zkg_ctx *ctx, const zkg_job *jobs, uint32_t n, z
             return rc;
         }
         if (!rc) rc = fork_init(b->fork, 4);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-        if (!rc && e != hipSuccess) rc = hip_err(e, "begin capture");
+        // the job tables (counters zero) are uploaded once, before capture:
+        // the graph holds kernel launches only
+        if (!rc) rc = plan_build(ctx, b->prepared, b->plan);
         if (!rc) {
-            rc = enqueue_jobs(ctx, st, b->prepared, &b->fork);
-            e = hipStreamEndCapture(st, &g);
-            if (!rc && e != hipSuccess) rc = hip_err(e, "end capture");
+            hipError_t e = hipMemcpy(b->plan.dev, b->plan.host.data(), b->plan.host.size(), hipMemcpyHostToDevice);
+            if (e != hipSuccess) rc = hip_err(e, "upload job tables");
+        }
+        hipGraph_t g = nullptr;
+        if (!rc) {
+            hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) rc = hip_err(e, "begin capture");
+            if (!rc) {
+                rc = enqueue_shared(ctx, st, b->prepared, b->fork, b->plan);
+                e = hipStreamEndCapture(st, &g);
+                if (!rc && e != hipSuccess) rc = hip_err(e, "end capture");
+            }
         }
         if (!rc) {
-            e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+            hipError_t e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
             if (e != hipSuccess) rc = hip_err(e, "graph instantiate");
         }
         if (g) (void)hipGraphDestroy(g);
         if (rc) {
             for (auto &p : b->prepared) release_prepared(ctx, p, false);
             if (b->exec) (void)hipGraphExecDestroy(b->exec);
+            plan_release(ctx, b->plan);
             fork_destroy(b->fork);
             delete b;
             return rc;
@@ -296,6 +450,7 @@ struct Slot {
     Fork fork;
     bool busy = false;
     std::vector<Prepared> prepared;
+    SharedPlan plan;  // its host image stays alive until the slot's upload has run
     std::vector<uint64_t> tickets;
 };
 
@@ -322,6 +477,7 @@ static void complete_slot(zkg_ctx *ctx, ServeState *s, Slot &sl)
         r.finish = now;
         release_prepared(ctx, sl.prepared[i], true);
     }
+    plan_release(ctx, sl.plan);
     sl.prepared.clear();
     sl.tickets.clear();
     sl.busy = false;
@@ -384,13 +540,24 @@ static void serve_loop(zkg_ctx *ctx, ServeState *s)
                     continue;
                 }
             }
-            rc = enqueue_jobs(ctx, sl.stream, ps, &sl.fork);
+            // the admitted requests share launches: one row hash per
+            // (field, width) group and one Merkle launch per round
+            rc = plan_build(ctx, ps, sl.plan);
+            if (rc == ZKG_OK) {
+                hipError_t e = hipMemcpyAsync(sl.plan.dev, sl.plan.host.data(), sl.plan.host.size(),
+                                              hipMemcpyHostToDevice, sl.stream);
+                if (e != hipSuccess) rc = hip_err(e, "upload job tables");
+            }
+            if (rc == ZKG_OK) rc = enqueue_shared(ctx, sl.stream, ps, sl.fork, sl.plan);
             if (rc == ZKG_OK) {
                 hipError_t e = hipEventRecord(sl.done, sl.stream);
                 if (e != hipSuccess) rc = hip_err(e, "event record");
             }
             if (rc != ZKG_OK) {
                 (void)hipStreamSynchronize(sl.stream);
+                for (int i = 0; i < sl.fork.k; i++) (void)hipStreamSynchronize(sl.fork.s[i]);
+                for (int g = 0; g < MAX_GROUPS; g++) (void)hipStreamSynchronize(sl.fork.gs[g]);
+                plan_release(ctx, sl.plan);
                 std::lock_guard<std::mutex> g(s->mu);
                 for (size_t i = 0; i < ps.size(); i++) {
                     release_prepared(ctx, ps[i], false);

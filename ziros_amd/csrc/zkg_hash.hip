@@ -137,16 +137,12 @@ __global__ __launch_bounds__(HASH_THREADS) void k_p2_permute(typename F::T *__re
 // elements, permute; a partial last chunk leaves the other rate lanes as they
 // were.  A zero-width row hashes to the zero digest.
 template <class F, int W>
-__global__ __launch_bounds__(HASH_THREADS) void k_hash_rows(const typename F::T *__restrict__ mat, uint64_t height,
-                                                            uint32_t width, typename F::T *__restrict__ digests,
-                                                            const typename F::T *__restrict__ rc,
-                                                            const typename F::T *__restrict__ diag,
-                                                            const double *__restrict__ rcf)
+__device__ __forceinline__ void hash_one_row(const typename F::T *__restrict__ mat, uint32_t width, uint64_t row,
+                                             typename F::T *__restrict__ digests, const typename F::T *__restrict__ rc,
+                                             const typename F::T *__restrict__ diag, const double *__restrict__ rcf)
 {
     using T = typename F::T;
     constexpr int R = W / 2;
-    uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= height) return;
     if constexpr (IS_BB16<F, W>) {
         double s[16];
 #pragma unroll
@@ -188,6 +184,18 @@ __global__ __launch_bounds__(HASH_THREADS) void k_hash_rows(const typename F::T 
     }
 #pragma unroll
     for (int k = 0; k < R; k++) digests[row * R + k] = F::to_canon(s[k]);
+}
+
+template <class F, int W>
+__global__ __launch_bounds__(HASH_THREADS) void k_hash_rows(const typename F::T *__restrict__ mat, uint64_t height,
+                                                            uint32_t width, typename F::T *__restrict__ digests,
+                                                            const typename F::T *__restrict__ rc,
+                                                            const typename F::T *__restrict__ diag,
+                                                            const double *__restrict__ rcf)
+{
+    uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= height) return;
+    hash_one_row<F, W>(mat, width, row, digests, rc, diag, rcf);
 }
 
 // The same sponge for Goldilocks with one row per L = W/4 lanes (one 4-chunk
@@ -464,15 +472,13 @@ constexpr int chunks_fuse()
 }
 
 template <int W>
-__global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_chunks(const uint64_t *__restrict__ in,
-                                                                  LayerPtrs<uint64_t> out, int k, int nb1,
-                                                                  const uint64_t *__restrict__ rc,
-                                                                  const uint64_t *__restrict__ diag)
+__device__ __forceinline__ void merkle_chunks_block(const uint64_t *__restrict__ in, const LayerPtrs<uint64_t> &out,
+                                                    int k, int nb1, uint32_t block, const uint64_t *__restrict__ rc,
+                                                    const uint64_t *__restrict__ diag, uint64_t *lds)
 {
-    constexpr int D = W / 2, L = ChunkMap<W>::L, PER = ChunkMap<W>::PER_BLOCK;
-    static_assert((2 << (chunks_fuse<W>() - 1)) == 2 * PER && chunks_fuse<W>() <= MAX_FUSE,
+    constexpr int D = W / 2, L = ChunkMap<W>::L;
+    static_assert((2 << (chunks_fuse<W>() - 1)) == 2 * ChunkMap<W>::PER_BLOCK && chunks_fuse<W>() <= MAX_FUSE,
                   "a block's first level holds 256 / L compressions");
-    __shared__ uint64_t lds[PER * D];
     const int j = threadIdx.x & (L - 1), g = threadIdx.x / L;
     const bool digest_lane = 4 * j < D;
     uint64_t d[4], s[4];
@@ -480,7 +486,7 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_chunks(const uint64_t
     for (int e = 0; e < 4; e++) d[e] = diag[4 * j + e];
     int nodes = nb1;
     if (g < nodes) {
-        const uint64_t i = (uint64_t)blockIdx.x * nodes + g;
+        const uint64_t i = (uint64_t)block * nodes + g;
 #pragma unroll
         for (int e = 0; e < 4; e++) s[e] = in[2 * i * D + 4 * j + e];
         p2_permute_gl_chunk<W>(s, j, d, rc);
@@ -501,7 +507,7 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_chunks(const uint64_t
         }
         __syncthreads();
         if (g < nodes) {
-            const uint64_t i = (uint64_t)blockIdx.x * nodes + g;
+            const uint64_t i = (uint64_t)block * nodes + g;
             p2_permute_gl_chunk<W>(s, j, d, rc);
             if (digest_lane) {
 #pragma unroll
@@ -512,6 +518,16 @@ __global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_chunks(const uint64_t
             }
         }
     }
+}
+
+template <int W>
+__global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_chunks(const uint64_t *__restrict__ in,
+                                                                  LayerPtrs<uint64_t> out, int k, int nb1,
+                                                                  const uint64_t *__restrict__ rc,
+                                                                  const uint64_t *__restrict__ diag)
+{
+    __shared__ uint64_t lds[ChunkMap<W>::PER_BLOCK * (W / 2)];
+    merkle_chunks_block<W>(in, out, k, nb1, blockIdx.x, rc, diag, lds);
 }
 
 // One level, one compression per lane (the unfused per-level path).
@@ -531,6 +547,88 @@ __global__ __launch_bounds__(HASH_THREADS) void k_merkle_level(const typename F:
     p2_permute<F, W>(s, rc, diag);
 #pragma unroll
     for (int e = 0; e < D; e++) out[i * D + e] = F::to_canon(s[e]);
+}
+
+// ---------------------------------------------------------------- shared launches
+// Row hashes of every job of a table, one persistent grid: each block takes the
+// next tile of HASH_TILE_ROWS rows from ctr[0] until none is left, finds its job
+// by binary search over the tiles' first indices, and hashes one row per lane.
+// The last block to finish (ctr[1]) puts both counters back to zero.
+static_assert(HASH_TILE_ROWS == HASH_THREADS, "a tile is one row per lane of a block");
+static_assert(SEG_MAX_LEVELS == MAX_FUSE, "a segment holds at most MAX_FUSE levels");
+
+template <class F, int W>
+__global__ __launch_bounds__(HASH_THREADS) void k_hash_jobs(const HashJob *__restrict__ jobs, int njobs,
+                                                            uint32_t ntiles, uint32_t *__restrict__ ctr,
+                                                            const typename F::T *__restrict__ rc,
+                                                            const typename F::T *__restrict__ diag,
+                                                            const double *__restrict__ rcf)
+{
+    using T = typename F::T;
+    __shared__ uint32_t s_tile;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(&ctr[0], 1u);
+        __syncthreads();
+        const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);
+        __syncthreads();  // every lane has read s_tile before it is written again
+        if (tile >= ntiles) break;
+        int lo = 0, hi = njobs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].tile0 <= tile)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const HashJob &j = jobs[lo];
+        const uint64_t row = (uint64_t)(tile - j.tile0) * HASH_TILE_ROWS + threadIdx.x;
+        if (row < j.rows) hash_one_row<F, W>((const T *)j.mat, j.width, row, (T *)j.digests, rc, diag, rcf);
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&ctr[1], 1u) == gridDim.x - 1) {  // every other block is past its last ctr[0] access
+            atomicExch(&ctr[0], 0u);
+            atomicExch(&ctr[1], 0u);
+        }
+    }
+}
+
+// One launch over the segments of several trees: block b runs block
+// b - first of the segment that holds it, in that segment's form.
+template <class F, int W>
+__global__ __launch_bounds__(MERKLE_THREADS) void k_merkle_segs(const MerkleSeg *__restrict__ segs, int nsegs,
+                                                                const typename F::T *__restrict__ rc,
+                                                                const typename F::T *__restrict__ diag,
+                                                                const double *__restrict__ rcf)
+{
+    using T = typename F::T;
+    __shared__ T lds[(W / 2) * MERKLE_THREADS];  // the largest form's (fused) LDS
+    static_assert(LaneMap<W>::PER_BLOCK * (W / 2) <= (W / 2) * MERKLE_THREADS, "lanes form fits");
+    int lo = 0, hi = nsegs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].first <= blockIdx.x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const MerkleSeg &sg = segs[lo];
+    LayerPtrs<T> lp;
+#pragma unroll
+    for (int i = 0; i < MAX_FUSE; i++) lp.p[i] = (T *)sg.out[i];
+    const uint32_t block = blockIdx.x - sg.first;
+    if (sg.form == SEG_LANES) {
+        merkle_lanes_block<F, W>((const T *)sg.in, lp, sg.k, block, rc, diag, lds);
+        return;
+    }
+    if constexpr (std::is_same<F, Gl>::value) {
+        static_assert(ChunkMap<W>::PER_BLOCK * (W / 2) <= (W / 2) * MERKLE_THREADS, "chunks form fits");
+        if (sg.form == SEG_CHUNKS) {
+            merkle_chunks_block<W>((const uint64_t *)sg.in, lp, sg.k, sg.nb1, block, rc, diag, lds);
+            return;
+        }
+    }
+    merkle_fused_block<F, W>((const T *)sg.in, lp, sg.k, sg.nb1, block, rc, diag, rcf, lds);
 }
 
 // ---------------------------------------------------------------- generator
@@ -715,6 +813,57 @@ int fused_levels()
     return v;
 }
 
+// The launches of one tree after its level 0.  Large levels: one compression
+// per lane, fused_levels() levels per launch (every level keeps full waves).
+// Goldilocks levels of at most chunks_split_max digests: W/4 lanes per
+// compression, down to the lanes split.  Levels of at most lanes_split_max
+// digests are latency-bound and finish with W-lane compressions,
+// lanes_fuse<W>() levels per launch.  No launch takes more than `fuse` levels.
+template <class F, int W>
+void plan_impl(void *const *layers, int nlayers, uint64_t n, int fuse, std::vector<MerkleSeg> &segs)
+{
+    const uint64_t split = lanes_split_max<F, W>();
+    constexpr bool GL = std::is_same<F, Gl>::value;
+    const uint64_t csplit = GL ? chunks_split_max<W>() : 0;
+    int lvl = 0;
+    while (lvl < nlayers - 1) {
+        const int left = nlayers - 1 - lvl;
+        const uint64_t level_n = n >> lvl;
+        const bool lanes = level_n <= split;
+        const bool chunks = !lanes && level_n <= csplit;
+        int k = lanes ? lanes_fuse<W>() : fused_levels();
+        if (chunks) {
+            // stop where the W-lane form takes over: a fused level with few
+            // compressions costs the chunk form's longer chain
+            k = chunks_fuse<W>();
+            int to_split = 0;
+            while ((level_n >> to_split) > split) to_split++;
+            if (k > to_split) k = to_split;
+        }
+        if (k > left) k = left;
+        if (k > fuse) k = fuse;
+        MerkleSeg sg{};
+        sg.in = layers[lvl];
+        for (int i = 0; i < k; i++) sg.out[i] = layers[lvl + 1 + i];
+        sg.k = k;
+        if (lanes) {
+            // a block = 256 / W compressions = up to 2^k inputs
+            sg.form = SEG_LANES;
+            sg.blocks = (uint32_t)(level_n >> k);
+        } else {
+            // a block = nb1 first-level compressions (up to one per lane, or
+            // 256 / (W/4) in the chunk form)
+            const uint64_t per = chunks ? (uint64_t)ChunkMap<W>::PER_BLOCK : (uint64_t)MERKLE_THREADS;
+            const uint64_t nb1 = level_n / 2 < per ? level_n / 2 : per;
+            sg.form = chunks ? SEG_CHUNKS : SEG_FUSED;
+            sg.nb1 = (int)nb1;
+            sg.blocks = (uint32_t)(level_n / 2 / nb1);
+        }
+        segs.push_back(sg);
+        lvl += k;
+    }
+}
+
 template <class F, int W>
 hipError_t merkle_impl(const P2Set &p2, void *const *layers, int nlayers, uint64_t n, int fuse, hipStream_t st,
                        bool sync_each)
@@ -733,66 +882,26 @@ hipError_t merkle_impl(const P2Set &p2, void *const *layers, int nlayers, uint64
         }
         return hipSuccess;
     }
-    // Large levels: one compression per lane, 3 levels per launch (every
-    // level keeps full waves).  Goldilocks levels of at most chunks_split_max
-    // digests: W/4 lanes per compression, down to the lanes split.  Levels of
-    // at most lanes_split_max digests are latency-bound and finish with W-lane
-    // compressions, lanes_fuse<W>() levels per launch.
-    const uint64_t split = lanes_split_max<F, W>();
-    constexpr bool GL = std::is_same<F, Gl>::value;
-    const uint64_t csplit = GL ? chunks_split_max<W>() : 0;
-    while (lvl < nlayers - 1) {
-        const int left = nlayers - 1 - lvl;
-        const uint64_t level_n = n >> lvl;
-        const bool lanes = level_n <= split;
-        const bool chunks = !lanes && level_n <= csplit;
-        int k = lanes ? lanes_fuse<W>() : fused_levels();
-        if (chunks) {
-            // stop where the W-lane form takes over: a fused level with few
-            // compressions costs the chunk form's longer chain
-            k = chunks_fuse<W>();
-            int to_split = 0;
-            while ((level_n >> to_split) > split) to_split++;
-            if (k > to_split) k = to_split;
-        }
-        if (k > left) k = left;
-        if (k > fuse) k = fuse;
+    std::vector<MerkleSeg> segs;
+    plan_impl<F, W>(layers, nlayers, n, fuse, segs);
+    for (const MerkleSeg &sg : segs) {
         LayerPtrs<T> lp;
-        for (int i = 0; i < MAX_FUSE; i++) lp.p[i] = i < k ? (T *)layers[lvl + 1 + i] : nullptr;
-        if constexpr (GL) {
-            if (chunks) {
-                // a block = nb1 <= 256 / (W/4) first-level compressions
-                const uint64_t per = ChunkMap<W>::PER_BLOCK;
-                const uint64_t nb1 = level_n / 2 < per ? level_n / 2 : per;
-                hipLaunchKernelGGL((k_merkle_chunks<W>), dim3((unsigned)(level_n / 2 / nb1)), dim3(MERKLE_THREADS), 0,
-                                   st, (const uint64_t *)layers[lvl], lp, k, (int)nb1, (const uint64_t *)p2.rc,
+        for (int i = 0; i < MAX_FUSE; i++) lp.p[i] = (T *)sg.out[i];
+        if (sg.form == SEG_LANES) {
+            hipLaunchKernelGGL((k_merkle_lanes<F, W>), dim3(sg.blocks), dim3(MERKLE_THREADS), 0, st, (const T *)sg.in,
+                               lp, sg.k, (const T *)p2.rc, (const T *)p2.diag);
+        } else if (sg.form == SEG_CHUNKS) {
+            if constexpr (std::is_same<F, Gl>::value)
+                hipLaunchKernelGGL((k_merkle_chunks<W>), dim3(sg.blocks), dim3(MERKLE_THREADS), 0, st,
+                                   (const uint64_t *)sg.in, lp, sg.k, sg.nb1, (const uint64_t *)p2.rc,
                                    (const uint64_t *)p2.diag);
-                hipError_t e = hipGetLastError();
-                if (e == hipSuccess && sync_each) e = hipStreamSynchronize(st);
-                if (e) return e;
-                lvl += k;
-                continue;
-            }
+        } else {
+            hipLaunchKernelGGL((k_merkle_fused<F, W>), dim3(sg.blocks), dim3(MERKLE_THREADS), 0, st, (const T *)sg.in,
+                               lp, sg.k, sg.nb1, (const T *)p2.rc, (const T *)p2.diag, (const double *)p2.rc_f64);
         }
-        if (lanes) {
-            // a block = 256 / W compressions = up to 2^k inputs
-            hipLaunchKernelGGL((k_merkle_lanes<F, W>), dim3((unsigned)(level_n >> k)), dim3(MERKLE_THREADS), 0, st,
-                               (const T *)layers[lvl], lp, k, (const T *)p2.rc, (const T *)p2.diag);
-            hipError_t e = hipGetLastError();
-            if (e == hipSuccess && sync_each) e = hipStreamSynchronize(st);
-            if (e) return e;
-            lvl += k;
-            continue;
-        }
-        // a block = nb1 first-level nodes (up to one per lane)
-        const uint64_t nb1 = level_n / 2 < MERKLE_THREADS ? level_n / 2 : MERKLE_THREADS;
-        hipLaunchKernelGGL((k_merkle_fused<F, W>), dim3((unsigned)(level_n / 2 / nb1)), dim3(MERKLE_THREADS), 0, st,
-                           (const T *)layers[lvl], lp, k, (int)nb1, (const T *)p2.rc, (const T *)p2.diag,
-                           (const double *)p2.rc_f64);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess && sync_each) e = hipStreamSynchronize(st);
         if (e) return e;
-        lvl += k;
     }
     return hipSuccess;
 }
@@ -1202,6 +1311,74 @@ hipError_t launch_merkle_tops(int field, int width, const P2Set &p2, const Merkl
 {
     if (field == FIELD_BB && width == 16) return merkle_tops_impl<Bb, 16>(p2, trees, ntrees, st);
     return hipErrorInvalidValue;
+}
+
+int merkle_plan(int field, int width, void *const *layers, int nlayers, uint64_t n_leaves, std::vector<MerkleSeg> &segs)
+{
+    if (field == FIELD_BB && width == 16)
+        plan_impl<Bb, 16>(layers, nlayers, n_leaves, MAX_FUSE, segs);
+    else if (field == FIELD_GL && width == 16)
+        plan_impl<Gl, 16>(layers, nlayers, n_leaves, MAX_FUSE, segs);
+    else if (field == FIELD_GL && width == 8)
+        plan_impl<Gl, 8>(layers, nlayers, n_leaves, MAX_FUSE, segs);
+    else
+        return -1;
+    return 0;
+}
+
+// Blocks of a persistent launch: as many as stay resident on the whole chip.
+template <class K>
+static uint32_t resident_blocks(K kernel, int threads)
+{
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                  hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || cus <= 0 || per <= 0) {
+        (void)hipGetLastError();
+        return 1024;
+    }
+    return (uint32_t)cus * (uint32_t)per;
+}
+
+template <class F, int W>
+static hipError_t hash_jobs_impl(const P2Set &p2, const HashJob *jobs, int njobs, uint32_t ntiles, uint32_t *ctr,
+                                 hipStream_t st)
+{
+    using T = typename F::T;
+    static const uint32_t resident = resident_blocks(k_hash_jobs<F, W>, HASH_THREADS);
+    const uint32_t grid = ntiles < resident ? ntiles : resident;
+    hipLaunchKernelGGL((k_hash_jobs<F, W>), dim3(grid), dim3(HASH_THREADS), 0, st, jobs, njobs, ntiles, ctr,
+                       (const T *)p2.rc, (const T *)p2.diag, (const double *)p2.rc_f64);
+    return hipGetLastError();
+}
+
+hipError_t launch_hash_jobs(int field, int width, const P2Set &p2, const HashJob *jobs, int njobs, uint32_t ntiles,
+                            uint32_t *ctr, hipStream_t st)
+{
+    if (!ntiles || njobs <= 0) return hipSuccess;
+    if (field == FIELD_BB && width == 16) return hash_jobs_impl<Bb, 16>(p2, jobs, njobs, ntiles, ctr, st);
+    if (field == FIELD_GL && width == 16) return hash_jobs_impl<Gl, 16>(p2, jobs, njobs, ntiles, ctr, st);
+    if (field == FIELD_GL && width == 8) return hash_jobs_impl<Gl, 8>(p2, jobs, njobs, ntiles, ctr, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_merkle_segs(int field, int width, const P2Set &p2, const MerkleSeg *segs, int nsegs,
+                              uint32_t nblocks, hipStream_t st)
+{
+    if (!nblocks || nsegs <= 0) return hipSuccess;
+    dim3 g(nblocks), b(MERKLE_THREADS);
+    if (field == FIELD_BB && width == 16)
+        hipLaunchKernelGGL((k_merkle_segs<Bb, 16>), g, b, 0, st, segs, nsegs, (const uint32_t *)p2.rc,
+                           (const uint32_t *)p2.diag, (const double *)p2.rc_f64);
+    else if (field == FIELD_GL && width == 16)
+        hipLaunchKernelGGL((k_merkle_segs<Gl, 16>), g, b, 0, st, segs, nsegs, (const uint64_t *)p2.rc,
+                           (const uint64_t *)p2.diag, nullptr);
+    else if (field == FIELD_GL && width == 8)
+        hipLaunchKernelGGL((k_merkle_segs<Gl, 8>), g, b, 0, st, segs, nsegs, (const uint64_t *)p2.rc,
+                           (const uint64_t *)p2.diag, nullptr);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 // 2^16: measured against 2^15, 2^17 and 2^18 on the 2^22 -> 2^8 commit phase.

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace zkg {
 
 enum FieldId : int { FIELD_GL = 0, FIELD_BB = 1 };
@@ -147,6 +149,46 @@ hipError_t launch_merkle_tops(int field, int width, const P2Set &p2, const Merkl
 uint64_t fri_tops_max_leaves();
 hipError_t launch_gen(int field, void *out, uint64_t seed, uint64_t start, uint64_t n,
                       hipStream_t st);
+
+// --- Shared launches over job tables (zkg_hash.hip) -------------------------
+// Row hashes of many matrices in one persistent launch: job j hashes `rows`
+// rows of `mat` (row-major, `width` elements per row) into `digests`; its
+// tiles of HASH_TILE_ROWS rows are numbered tile0 .. tile0 + ceil(rows /
+// HASH_TILE_ROWS) - 1, tile0 ascending in table order.  A grid sized to the
+// chip takes tiles in that order from a device counter pair (next tile, blocks
+// done), which the launch's last block resets to zero, so the same launch can
+// run again (a graph replay) without clearing it.
+constexpr int HASH_TILE_ROWS = 256;
+struct HashJob {
+    const void *mat;
+    void *digests;
+    uint64_t rows;
+    uint32_t width;
+    uint32_t tile0;
+};
+// One launch's worth of a tree's levels, as the single-tree commit plans them:
+// `form` SEG_FUSED (one compression per lane, nb1 first-level compressions per
+// block), SEG_CHUNKS (Goldilocks, one 4-chunk of the state per lane) or
+// SEG_LANES (W lanes per compression); k levels from `in` into out[0..k-1];
+// `blocks` blocks, the first of them `first` in a shared launch.
+constexpr int SEG_MAX_LEVELS = 9;
+enum SegForm : int32_t { SEG_FUSED = 0, SEG_CHUNKS = 1, SEG_LANES = 2 };
+struct MerkleSeg {
+    const void *in;
+    void *out[SEG_MAX_LEVELS];
+    uint32_t first, blocks;
+    int32_t form, k, nb1, unused;
+};
+// The launches of one power-of-two tree (levels 1.. over layers[0] of n_leaves
+// digests), in order: segment r needs segment r - 1 complete.
+int merkle_plan(int field, int width, void *const *layers, int nlayers, uint64_t n_leaves,
+                std::vector<MerkleSeg> &segs);
+// jobs, ctr: device memory (ctr: two uint32_t, zero before the first launch).
+hipError_t launch_hash_jobs(int field, int width, const P2Set &p2, const HashJob *jobs, int njobs, uint32_t ntiles,
+                            uint32_t *ctr, hipStream_t st);
+// segs: device memory, `first` ascending, nblocks = sum of blocks.
+hipError_t launch_merkle_segs(int field, int width, const P2Set &p2, const MerkleSeg *segs, int nsegs,
+                              uint32_t nblocks, hipStream_t st);
 // Mixed-height MMCS.  Leaves: digest i = sponge over row i of every matrix of
 // the group (device array of n MatDesc), rows concatenated.  Layer: node i of
 // the next layer (i < half) = compress(prev[2i], prev[2i+1]), then, if n_inj,
