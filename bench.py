@@ -1,31 +1,33 @@
 """Benchmark of the STARK trace-commitment path on one MI355X.
 
-Headline workload: BabyBear trace of 2^20 rows x 64 columns, coset LDE at
-blowup 2 (2^21 rows, bit-reversed), Poseidon2-16 row hashes, Merkle tree to
-one root.  A step is one full commit of that trace with the inputs already in
-HBM.  value = trace cells (rows x columns before the LDE) committed per second.
+Headline workload: the batched commit.  48 BabyBear traces (a seeded draw of
+heights 2^U{12..20} and widths U{8, 16, ..., 256}, 1.59e9 cells) each get a
+coset LDE at blowup 2, Poseidon2-16 row hashes and a Merkle tree, in one
+zkg_commit_batch call (a replayed hipGraph: LDEs on parallel streams, one
+persistent row-hash launch over all traces, one Merkle launch per round of
+levels).  A step is one such batch with the traces already in HBM; value =
+trace cells (rows x columns before the LDE) committed per second.
 
 Also reported on the same line:
-  roofline      the dominant kernel (Poseidon2 row hashing, VALU-bound):
-                algorithmic field products (772 per BabyBear-16 permutation)
-                per second against the measured Montgomery-product ceiling;
-  roofline_lde  the LDE kernels (HBM-bound): N W 4 (1 + B) bytes per commit
-                against 8 TB/s;
-  plain         the same commit through the unfused, reference-structured
-                path (one global pass per NTT stage, per-level Merkle
-                launches, a synchronisation after every launch);
-  batched       48 BabyBear traces (heights 2^U{12..20}, widths U{8..256}),
-                one hipGraph-replayed batch vs one plain call per trace;
+  batched_vs_plain  one plain call per trace (reference-structured: one
+                global pass per NTT stage, per-level Merkle launches, a
+                synchronisation after every launch) on the same traces, and
+                batched_vs_single_fused against one fused zkg_commit per trace;
+  roofline      the dominant kernel, the batch's row-hash launch (VALU bound):
+                profiled VALU wave-instructions per launch over its live
+                HIP-event duration against the issue ceiling, and 772 BabyBear
+                products per permutation against the measured product ceiling;
+  c2            one BabyBear 2^20 x 64 commit, with the row-hash and LDE
+                rooflines (the LDE against 8 TB/s);
+  c1, c4_fri, pcs_open_deep, mmcs, constraint_eval  the other configurations
+                and the opening-side kernels;
   served        continuous-batching stream (60% BabyBear / 40% Goldilocks,
                 heights 2^U{14..22}, widths U{16..128}, blowup 2 or 4,
                 Poisson arrivals): p50/p99 latency and cells/s vs one plain
                 call per request;
-  constraint_eval  the reference's constraint bytecode (32 degree-3
-                constraints) over a Goldilocks 2^21 x 64 LDE, interpreted and
-                compiled to a straight-line kernel, against HBM and the
-                measured Goldilocks product ceiling;
-  cpu_baseline  the CPU oracle (C restatement of the reference CPU path),
-                multithreaded over the host cores, on a bounded sample.
+  cpu_baseline  the CPU oracle (C restatement of the reference CPU path) on a
+                bounded sample of the batched draw, on this GPU's share of the
+                host cores and on one core.
 """
 import argparse
 import ctypes
@@ -89,13 +91,17 @@ def valu_issue(insts, ms):
             "frac": rate / VALU_ISSUE_PEAK, "insts_per_launch": insts}
 
 
-def profiled_traffic():
-    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
-    (calibrated against a 1 GiB copy) and VALU wave-instructions per launch
-    (SQ_INSTS_VALU), read from the committed summary."""
+BATCH_COUNTERS = "batch_bb_48traces_counters.md"
+C2_COUNTERS = "commit_bb_2p20x64_traffic.md"
+
+
+def profiled_counters(name):
+    """Per-launch figures measured with rocprofv3 and committed under profiles/:
+    HBM bytes (FETCH_SIZE / WRITE_SIZE, calibrated against a 1 GiB copy) and VALU
+    wave-instructions (SQ_INSTS_VALU), as `key: integer` lines."""
     out = {}
     try:
-        for line in open(os.path.join(ROOT, "profiles", "commit_bb_2p20x64_traffic.md")):
+        for line in open(os.path.join(ROOT, "profiles", name)):
             k, _, v = line.partition(":")
             if k.startswith(("traffic_", "valu_")) and v.strip().isdigit():
                 out[k] = int(v)
@@ -347,6 +353,11 @@ def c3_shapes():
 
 
 def bench_c3(ctx, steps, warmup):
+    """The batched workload: 48 BabyBear traces committed by one zkg_commit_batch
+    call (graph replay).  A step is one whole batch.  Also: one plain
+    (reference-structured) call per trace, one fused zkg_commit per trace, and
+    a timed pass (the same launches straight from the streams, bracketed by HIP
+    events) that gives each stage's device time."""
     shapes = c3_shapes()
     bufs, jobs, cells = [], [], 0
     for k, (lg, w) in enumerate(shapes):
@@ -372,6 +383,15 @@ def bench_c3(ctx, steps, warmup):
         run()
     ctx.synchronize()
     dt = (time.perf_counter() - t0) / steps
+    # stage device times: the same launches, timed with events on their streams
+    ctx.set_timing(True)
+    stages = np.zeros(4)
+    reps = 3
+    for _ in range(reps):
+        run()
+        stages += ctx.stage_ms()
+    ctx.set_timing(False)
+    stages /= reps
     # plain: one synchronous reference-structured call per trace
     t0 = time.perf_counter()
     for (tr, lde), (lg, w) in zip(bufs, shapes):
@@ -387,9 +407,14 @@ def bench_c3(ctx, steps, warmup):
     for tr, lde in bufs:
         tr.free()
         lde.free()
-    return dict(traces=len(jobs), cells=cells, batched_cells_per_s=cells / dt, plain_cells_per_s=cells / dt_plain,
-                single_fused_cells_per_s=cells / dt_single, batched_vs_plain=dt_plain / dt,
-                batch_ms=dt * 1e3)
+    perms = sum((1 << (lg + 1)) * ((w + 7) // 8) for lg, w in shapes)
+    lde_bytes = sum((1 << lg) * w * 4 * 3 for lg, w in shapes)
+    return dict(traces=len(jobs), cells=cells, sec_per_step=dt, batched_cells_per_s=cells / dt,
+                plain_cells_per_s=cells / dt_plain, single_fused_cells_per_s=cells / dt_single,
+                batched_vs_plain=dt_plain / dt, batched_vs_single_fused=dt_single / dt, batch_ms=dt * 1e3,
+                plain_ms=dt_plain * 1e3, single_fused_ms=dt_single * 1e3,
+                stage_ms=dict(lde=stages[0], row_hash=stages[1], merkle=stages[2], timed_batch=stages[3]),
+                row_hash_permutations=perms, lde_algorithmic_bytes=lde_bytes)
 
 
 def bench_c5(ctx, n_req, load, slots=2, max_batch=8):
@@ -471,39 +496,65 @@ def bench_c5(ctx, n_req, load, slots=2, max_batch=8):
                 slots=slots, max_batch=max_batch)
 
 
-def cpu_baseline(log_rows=20, width=64):
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle as O
-
-    n = 1 << log_rows
-    tr = O.gen(O.BB, SEED_C2, n * width).reshape(n, width)
-    p2 = O.Poseidon2(O.BB, 16, 42)
-    threads = min(len(os.sched_getaffinity(0)), 16)  # one GPU's share of the host cores
-    O.set_threads(threads)
-    t0 = time.perf_counter()
-    O.commit(p2, tr, 1)
-    dt = time.perf_counter() - t0
-    cpu = "unknown"
+def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):
-                cpu = line.split(":", 1)[1].strip()
-                break
+                return line.split(":", 1)[1].strip()
     except OSError:
         pass
+    return "unknown"
+
+
+def cpu_baseline(max_log_multi=16, max_log_one=13):
+    """The oracle (C restatement of the p3 CPU path: coset LDE, Poseidon2-16
+    sponge rows, Merkle) on a bounded sample of the batched workload: the draw's
+    traces of at most 2^max_log_multi rows, on every core this process may run
+    on, and those of at most 2^max_log_one rows on one core."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle as O
+
+    p2 = O.Poseidon2(O.BB, 16, 42)
+    shapes = c3_shapes()
+
+    def run(max_log, threads):
+        O.set_threads(threads)
+        sample = [(k, lg, w) for k, (lg, w) in enumerate(shapes) if lg <= max_log]
+        cells, secs = 0, 0.0
+        for k, lg, w in sample:
+            n = 1 << lg
+            tr = O.gen(O.BB, SEED_C3 + k, n * w).reshape(n, w)
+            t0 = time.perf_counter()
+            O.commit(p2, tr, 1)
+            secs += time.perf_counter() - t0
+            cells += n * w
+        return cells, secs, len(sample), O.get_threads()
+
+    # every core of this GPU's share of the host: the affinity set, capped at
+    # the box's per-GPU CPU share (OMP_NUM_THREADS there)
+    affinity = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS") or affinity)
+    cores = max(1, min(affinity, share))
+    cells, secs, nt, used = run(max_log_multi, cores)
+    cells1, secs1, nt1, _ = run(max_log_one, 1)
     # constraint evaluation: the oracle's stack machine on 1/8 of the LDE rows
+    O.set_threads(cores)
     ce_rows = (CE_SHAPE[0] << CE_SHAPE[2]) // 8
     bc, consts, n_c = air_program(CE_SHAPE[1], CE_SHAPE[3], SEED_C2 + 7)
     ce_tr = O.gen(O.GL, SEED_C2 + 9, ce_rows * CE_SHAPE[1]).reshape(ce_rows, CE_SHAPE[1])
     t1 = time.perf_counter()
     O.constraint_eval_gl(ce_tr, bc, consts, n_c)
     ce_dt = time.perf_counter() - t1
-    cores = O.get_threads()
-    share = "the whole config-2 workload" if n == 1 << 20 else "1/%d of the config-2 workload" % ((1 << 20) // n)
-    return dict(value=n * width / dt, unit="cells/s", cores=cores, kind="port",
-                sample="one BabyBear %d x %d commit (coset LDE blowup 2, Poseidon2-16 rows, Merkle) = %s; %s; "
-                       "%.1f s wall, about %.0f core-seconds" % (n, width, share, cpu, dt, dt * cores),
-                constraint_eval=dict(rows_per_s=ce_rows / ce_dt, cores=cores, kind="port",
+    cpu = _cpu_model()
+    return dict(value=cells / secs, unit="cells/s", cores=used, kind="port",
+                sample="the %d traces of at most 2^%d rows of the 48-trace batched draw (%d cells, blowup 2, "
+                       "Poseidon2-16 rows, Merkle), committed one after another by the oracle on %d threads; "
+                       "%s; %.1f s wall" % (nt, max_log_multi, cells, used, cpu, secs),
+                one_core=dict(value=cells1 / secs1, unit="cells/s", cores=1,
+                              sample="the %d traces of at most 2^%d rows (%d cells) on one thread; %.1f s wall"
+                                     % (nt1, max_log_one, cells1, secs1)),
+                host_threads_visible=os.cpu_count(), affinity=affinity,
+                constraint_eval=dict(rows_per_s=ce_rows / ce_dt, cores=used, kind="port",
                                      sample="%d of the %d LDE rows x %d columns, %d constraints (%d instructions); "
                                             "%.2f s wall" % (ce_rows, CE_SHAPE[0] << CE_SHAPE[2], CE_SHAPE[1], n_c,
                                                              len(bc), ce_dt)))
@@ -512,15 +563,19 @@ def cpu_baseline(log_rows=20, width=64):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=20, help="timed batches (one batch = the 48-trace commit)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--skip-legs", action="store_true",
+                    help="only the batched headline (with its plain ratio), no single-trace / FRI / DEEP / MMCS / "
+                         "constraint legs")
     ap.add_argument("--only-constraints", action="store_true",
                     help="run only the constraint-evaluation leg (for profiling) and print its JSON")
     ap.add_argument("--only-c1", action="store_true",
                     help="run only the Goldilocks 2^14 x 8 single-commit leg (for profiling) and print its JSON")
+    ap.add_argument("--only-c2", action="store_true",
+                    help="run only the BabyBear 2^20 x 64 single-commit leg (for profiling) and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-load", type=float, default=1.5,
                     help="Poisson arrival rate as a multiple of the plain path's capacity")
@@ -549,85 +604,119 @@ def main():
         print(json.dumps({"c1": bench_c1(ctx, max(5, args.steps))}))
         ctx.close()
         return
-    log("c2: %d warmup + %d timed steps" % (args.warmup, args.steps))
-    c2 = bench_c2(ctx, args.steps, args.warmup)
-    n, w, lb = c2["n"], c2["w"], c2["lb"]
-    cells = n * w
-    value = cells / c2["sec_per_step"]
-    lde_ms, hash_ms, merkle_ms = c2["stage_ms"][0], c2["stage_ms"][1], c2["stage_ms"][2]
-    leaves = n << lb
-    hash_muls = leaves * ((w + 7) // 8) * BB_PERM_MULS
-    lde_bytes = n * w * 4 * (1 + (1 << lb))
-    log("c2: %.3f ms/step, stages %s" % (c2["sec_per_step"] * 1e3, c2["stage_ms"]))
-    traffic = profiled_traffic()
+    if args.only_c2:
+        print(json.dumps({"c2": c2_leg(ctx, args.steps, args.warmup, peak_bb)}))
+        ctx.close()
+        return
+
+    log("batched: 48 BabyBear traces, %d warmup + %d timed batches" % (args.warmup, args.steps))
+    c3 = bench_c3(ctx, args.steps, args.warmup)
+    log("batched: %s" % c3)
+    counters = profiled_counters(BATCH_COUNTERS)
+    hash_s = c3["stage_ms"]["row_hash"] * 1e-3
+    products = c3["row_hash_permutations"] * BB_PERM_MULS
+    insts = counters.get("valu_insts_hash_jobs")
+    roof = {"kernel": "k_hash_jobs<Bb,16>", "bound": "valu",
+            "achieved": insts / hash_s / 1e9 if insts else None, "peak": VALU_ISSUE_PEAK / 1e9,
+            "unit": "G VALU wave-instructions/s", "frac": insts / hash_s / VALU_ISSUE_PEAK if insts else None,
+            "traffic": counters.get("traffic_hash_jobs_bytes"),
+            "launch_ms": hash_s * 1e3, "valu_insts_per_launch": insts,
+            "product_ceiling": {"achieved": products / hash_s / 1e9, "peak": peak_bb / 1e9, "unit": "Gmodmul/s",
+                                "frac": products / hash_s / peak_bb},
+            "note": "the batch's one row-hash launch (%d Poseidon2-16 permutations over the 48 LDEs); achieved = "
+                    "profiled VALU wave-instructions per launch over the launch's HIP-event duration in this run, "
+                    "peak = 1024 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction; product_ceiling: 772 "
+                    "BabyBear products per permutation against the measured product rate (exact FP64 or "
+                    "Montgomery, 8 independent chains per lane over the whole chip)" % c3["row_hash_permutations"]}
     out = {
         "metric": "trace cells committed/sec, batched vs plain per-trace GPU; p99 commit latency",
-        "value": value,
+        "value": c3["batched_cells_per_s"],
         "unit": "cells/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": c2["sec_per_step"] * 1e3,
+        "ms_per_step": c3["batch_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
-        "config": {"workload": "BabyBear commit: 2^20 x 64 trace, coset LDE blowup 2 (2^21 rows, shift 31), "
-                               "Poseidon2-16 row sponge (8+13 rounds, rate 8), Merkle to one root",
-                   "height": n, "width": w, "log_blowup": lb, "field": "babybear", "poseidon2_width": 16},
-        "roofline": {"kernel": "k_hash_rows<Bb,16>", "bound": "valu", "achieved": hash_muls / (hash_ms * 1e-3) / 1e9,
-                     "peak": peak_bb / 1e9, "unit": "Gmodmul/s", "frac": hash_muls / (hash_ms * 1e-3) / peak_bb,
-                     "traffic": traffic.get("traffic_hash_rows_bytes"),
-                     "valu_issue": valu_issue(traffic.get("valu_insts_hash_rows"), hash_ms),
-                     "note": "772 BabyBear products per permutation x %d permutations per launch; peak = the "
-                             "faster measured BabyBear product rate (exact FP64 or Montgomery), 8 independent "
-                             "chains per lane over the whole chip. 208 of the 772 are diagonal products by "
-                             "+-2^-k and small integers, which cost 1-3 FP64 operations instead of a general "
-                             "product's 6, so frac can pass 1; valu_issue is the issue-rate roofline (profiled "
-                             "VALU wave-instructions per launch over this launch's duration)"
-                             % (leaves * ((w + 7) // 8))},
-        "roofline_lde": {"kernels": "k_fs_pass1 + k_lde_mid + k_lde_final", "bound": "hbm",
-                         "achieved": lde_bytes / (lde_ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                         "frac": lde_bytes / (lde_ms * 1e-3) / HBM_PEAK,
-                         "traffic": traffic.get("traffic_lde_bytes"), "algorithmic_bytes": lde_bytes,
-                         "note": "N*W*4*(1+B) algorithmic bytes over the summed duration of the three passes; "
-                                 "the passes are VALU bound (56-69% VALU busy in the profile), not HBM bound"},
-        "stage_ms": {"lde": lde_ms, "row_hash": hash_ms, "merkle": merkle_ms},
-        "plain": {"cells_per_s": cells / c2["plain_sec"], "ms_per_commit": c2["plain_sec"] * 1e3,
-                  "fused_vs_plain": c2["plain_sec"] / c2["sec_per_step"]},
+        "config": {"workload": "batched commit of 48 BabyBear traces, heights 2^U{12..20}, widths U{8,16..256} "
+                               "(seeded draw, %d cells): coset LDE blowup 2 (shift 31), Poseidon2-16 row sponge "
+                               "(8+13 rounds, rate 8), Merkle to one root each; one zkg_commit_batch call "
+                               "(hipGraph replay) per step" % c3["cells"],
+                   "traces": c3["traces"], "cells": c3["cells"], "log_blowup": 1, "field": "babybear",
+                   "poseidon2_width": 16},
+        "batched_vs_plain": c3["batched_vs_plain"],
+        "batched_vs_single_fused": c3["batched_vs_single_fused"],
+        "plain_cells_per_s": c3["plain_cells_per_s"],
+        "single_fused_cells_per_s": c3["single_fused_cells_per_s"],
+        "stage_ms": c3["stage_ms"],
+        "roofline": roof,
         "probe": {"bb_product_ceiling_per_s": peak_bb, "bb_fp64_products_per_s": peak_bbf,
-                  "gl_products_per_s": peak_gl,
-                  "mul_lo_u32_per_s": peak_mul32, "copy_1GiB_GBps": copy_gbps},
+                  "gl_products_per_s": peak_gl, "mul_lo_u32_per_s": peak_mul32, "copy_1GiB_GBps": copy_gbps},
     }
-    log("c1: goldilocks 2^14 x 8")
-    out["c1"] = bench_c1(ctx, max(5, args.steps))
-    log("c1: %s" % out["c1"])
-    log("c4: fri commit phase 2^22 -> 2^8")
-    out["c4_fri"] = bench_c4(ctx, max(3, args.steps // 2))
-    log("c4: %s" % out["c4_fri"])
-    log("deep: PCS opening DEEP quotient over the config-2 LDE")
-    out["pcs_open_deep"] = bench_deep(ctx, max(5, args.steps))
-    log("deep: %s" % out["pcs_open_deep"])
-    log("mmcs: mixed-height commits")
-    out["mmcs"] = bench_mmcs(ctx, max(5, args.steps // 2))
-    log("mmcs: %s" % out["mmcs"])
-    log("constraints: 32 degree-3 constraints over a Goldilocks 2^21 x 64 LDE")
-    out["constraint_eval"] = bench_constraints(ctx, max(5, args.steps), peak_gl)
-    log("constraints: %s" % out["constraint_eval"])
-    if not args.skip_batch:
-        log("c3: batched 48 traces")
-        out["batched"] = bench_c3(ctx, max(1, args.steps // 4), 1)
-        log("c3: %s" % out["batched"])
+    if not args.skip_legs:
+        log("c2: single BabyBear 2^20 x 64 commit")
+        out["c2"] = c2_leg(ctx, max(5, args.steps // 2), args.warmup, peak_bb)
+        log("c2: %s" % out["c2"])
+        log("c1: goldilocks 2^14 x 8")
+        out["c1"] = bench_c1(ctx, max(5, args.steps))
+        log("c1: %s" % out["c1"])
+        log("c4: fri commit phase 2^22 -> 2^8")
+        out["c4_fri"] = bench_c4(ctx, max(3, args.steps // 2))
+        log("c4: %s" % out["c4_fri"])
+        log("deep: PCS opening DEEP quotient over the config-2 LDE")
+        out["pcs_open_deep"] = bench_deep(ctx, max(5, args.steps))
+        log("deep: %s" % out["pcs_open_deep"])
+        log("mmcs: mixed-height commits")
+        out["mmcs"] = bench_mmcs(ctx, max(5, args.steps // 2))
+        log("mmcs: %s" % out["mmcs"])
+        log("constraints: 32 degree-3 constraints over a Goldilocks 2^21 x 64 LDE")
+        out["constraint_eval"] = bench_constraints(ctx, max(5, args.steps), peak_gl)
+        log("constraints: %s" % out["constraint_eval"])
     if not args.skip_serve:
-        log("c5: served stream")
+        log("served: continuous batching")
         out["served"] = bench_c5(ctx, args.serve_requests, args.serve_load)
-        log("c5: %s" % out["served"])
+        log("served: %s" % out["served"])
     ctx.close()
     if not args.skip_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline()
+        log("cpu baseline: %s" % out["cpu_baseline"])
     print(json.dumps(out))
+
+
+def c2_leg(ctx, steps, warmup, peak_bb):
+    """Single large commit (BabyBear 2^20 x 64, blowup 2) with its row-hash and
+    LDE rooflines."""
+    c2 = bench_c2(ctx, steps, warmup)
+    n, w, lb = c2["n"], c2["w"], c2["lb"]
+    lde_ms, hash_ms, merkle_ms = c2["stage_ms"][0], c2["stage_ms"][1], c2["stage_ms"][2]
+    leaves = n << lb
+    hash_muls = leaves * ((w + 7) // 8) * BB_PERM_MULS
+    lde_bytes = n * w * 4 * (1 + (1 << lb))
+    counters = profiled_counters(C2_COUNTERS)
+    return {
+        "workload": "BabyBear commit: 2^20 x 64 trace, coset LDE blowup 2 (2^21 rows, shift 31), Poseidon2-16 "
+                    "row sponge, Merkle to one root",
+        "cells_per_s": n * w / c2["sec_per_step"], "ms_per_commit": c2["sec_per_step"] * 1e3,
+        "stage_ms": {"lde": lde_ms, "row_hash": hash_ms, "merkle": merkle_ms},
+        "plain": {"cells_per_s": n * w / c2["plain_sec"], "ms_per_commit": c2["plain_sec"] * 1e3,
+                  "fused_vs_plain": c2["plain_sec"] / c2["sec_per_step"]},
+        "roofline_row_hash": {"kernel": "k_hash_rows<Bb,16>", "bound": "valu",
+                              "valu_issue": valu_issue(counters.get("valu_insts_hash_rows"), hash_ms),
+                              "product_ceiling": {"achieved": hash_muls / (hash_ms * 1e-3) / 1e9,
+                                                  "peak": peak_bb / 1e9, "unit": "Gmodmul/s",
+                                                  "frac": hash_muls / (hash_ms * 1e-3) / peak_bb},
+                              "traffic": counters.get("traffic_hash_rows_bytes")},
+        "roofline_lde": {"kernels": "k_fs_pass1 + k_lde_mid + k_lde_final", "bound": "hbm",
+                         "achieved": lde_bytes / (lde_ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                         "frac": lde_bytes / (lde_ms * 1e-3) / HBM_PEAK, "algorithmic_bytes": lde_bytes,
+                         "traffic": counters.get("traffic_lde_bytes"),
+                         "valu_issue": valu_issue(counters.get("valu_insts_lde"), lde_ms),
+                         "note": "N*W*4*(1+B) algorithmic bytes over the summed duration of the three passes"},
+    }
 
 
 if __name__ == "__main__":

@@ -333,10 +333,16 @@ typedef struct zkg_job {
     zkg_buf *lde_out;     /* (height << log_blowup) x width */
 } zkg_job;
 
-/* Commits n traces together: each stage is one launch over the tiles of every
- * job (a persistent grid walking a job table), and the whole LDE -> hash ->
- * Merkle pipeline is captured once as a hipGraph per job-set shape and
- * replayed.  trees_out[i] receives job i's tree. */
+/* Commits n traces together (replaces BatchProver::prove_all's thread per
+ * job, zkf-metal/src/batch_prover.rs:167-274).  The coset LDEs run on parallel
+ * streams; then, per Poseidon2 instance, ONE persistent row-hash launch walks a
+ * device table of every job's rows (a grid sized to the chip takes 256-row
+ * tiles, heaviest first, from a device counter), and each round of Merkle
+ * levels is one launch over all the trees.  The whole pipeline is captured
+ * once as a hipGraph per job set (shapes and buffers) and replayed; with
+ * timing on (zkg_ctx_set_timing) the same launches run straight from the
+ * streams and zkg_ctx_stage_ms reports them.  trees_out[i] receives job i's
+ * tree (a copy: later batches never write it). */
 int zkg_commit_batch(zkg_ctx *ctx, const zkg_job *jobs, uint32_t n, zkg_tree **trees_out);
 
 /* ---- serving -------------------------------------------------------------- */
@@ -411,8 +417,11 @@ int zkg_driver_execute(zkg_ctx *ctx, const zkg_node *node, zkg_bridge *bridge, z
 int zkg_driver_should_try(const zkg_node *node);
 
 /* ---- telemetry ------------------------------------------------------------- */
-/* Per-stage device time of the most recent commit / batch on the context's
- * stream, from HIP events (stages: 0 LDE, 1 row hash, 2 Merkle, 3 FRI). */
+/* Per-stage device time of the most recent commit / FRI commit phase / batch,
+ * from HIP events on the streams the kernels run on.  Commit: 0 LDE, 1 row
+ * hash, 2 Merkle; FRI: 3 the whole phase; batch: 0 LDE (start to the last LDE
+ * stream's end), 1 row hash and 2 Merkle (summed over Poseidon2 instances),
+ * 3 the whole batch. */
 int zkg_ctx_set_timing(zkg_ctx *ctx, int enable);
 int zkg_ctx_stage_ms(const zkg_ctx *ctx, double *ms4);
 

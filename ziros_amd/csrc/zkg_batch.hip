@@ -100,6 +100,9 @@ struct Fork {
     hipEvent_t start = nullptr;
     hipEvent_t join[MAX_BRANCHES] = {}, gjoin[MAX_GROUPS] = {};
     hipEvent_t lde_done[MAX_GROUPS][MAX_BRANCHES] = {};
+    // timed runs: end of each branch's LDEs; per group, row hash start / end
+    // and Merkle end
+    hipEvent_t t_lde[MAX_BRANCHES] = {}, t_grp[MAX_GROUPS][3] = {};
 };
 
 static int fork_init(Fork &f, int k)
@@ -111,10 +114,12 @@ static int fork_init(Fork &f, int k)
         if (e == hipSuccess) e = hipEventCreateWithFlags(&f.join[i], hipEventDisableTiming);
         for (int g = 0; g < MAX_GROUPS && e == hipSuccess; g++)
             e = hipEventCreateWithFlags(&f.lde_done[g][i], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreate(&f.t_lde[i]);
     }
     for (int g = 0; g < MAX_GROUPS && e == hipSuccess; g++) {
         e = hipStreamCreateWithFlags(&f.gs[g], hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&f.gjoin[g], hipEventDisableTiming);
+        for (int t = 0; t < 3 && e == hipSuccess; t++) e = hipEventCreate(&f.t_grp[g][t]);
     }
     return e == hipSuccess ? ZKG_OK : hip_err(e, "fork streams");
 }
@@ -129,6 +134,7 @@ static void fork_destroy(Fork &f)
         if (f.join[i]) (void)hipEventDestroy(f.join[i]);
         for (int g = 0; g < MAX_GROUPS; g++)
             if (f.lde_done[g][i]) (void)hipEventDestroy(f.lde_done[g][i]);
+        if (f.t_lde[i]) (void)hipEventDestroy(f.t_lde[i]);
     }
     for (int g = 0; g < MAX_GROUPS; g++) {
         if (f.gs[g]) {
@@ -136,6 +142,8 @@ static void fork_destroy(Fork &f)
             (void)hipStreamDestroy(f.gs[g]);
         }
         if (f.gjoin[g]) (void)hipEventDestroy(f.gjoin[g]);
+        for (int t = 0; t < 3; t++)
+            if (f.t_grp[g][t]) (void)hipEventDestroy(f.t_grp[g][t]);
     }
     if (f.start) (void)hipEventDestroy(f.start);
     f = Fork();
@@ -249,7 +257,9 @@ static int plan_build(zkg_ctx *ctx, std::vector<Prepared> &ps, SharedPlan &sp)
     return dev_alloc(ctx, sp.host.size(), &sp.dev);
 }
 
-static int enqueue_shared(zkg_ctx *ctx, hipStream_t st, std::vector<Prepared> &ps, Fork &f, const SharedPlan &sp)
+// timed: also records the Fork's timing events (never under stream capture).
+static int enqueue_shared(zkg_ctx *ctx, hipStream_t st, std::vector<Prepared> &ps, Fork &f, const SharedPlan &sp,
+                          bool timed = false)
 {
     const int k = std::max(1, std::min<int>(f.k, (int)ps.size()));
     hipError_t e = hipEventRecord(f.start, st);
@@ -271,6 +281,7 @@ static int enqueue_shared(zkg_ctx *ctx, hipStream_t st, std::vector<Prepared> &p
             if (e != hipSuccess) rc = hip_err(e, "lde event");
         }
     }
+    for (int i = 0; i < k && timed && !rc; i++) (void)hipEventRecord(f.t_lde[i], f.s[i]);
     // per group: the shared row hash, then the Merkle rounds
     uint8_t *dev = (uint8_t *)sp.dev;
     for (size_t gi = 0; gi < sp.groups.size() && !rc; gi++) {
@@ -278,12 +289,15 @@ static int enqueue_shared(zkg_ctx *ctx, hipStream_t st, std::vector<Prepared> &p
         hipStream_t gs = f.gs[gi];
         for (int i = 0; i < k && e == hipSuccess; i++) e = hipStreamWaitEvent(gs, f.lde_done[gi][i], 0);
         const P2Set *p2 = get_p2(ctx, g.field, g.width);
+        if (timed && e == hipSuccess) e = hipEventRecord(f.t_grp[gi][0], gs);
         if (e == hipSuccess)
             e = launch_hash_jobs(g.field, g.width, *p2, (const HashJob *)(dev + g.hash_off), g.njobs, g.ntiles,
                                  (uint32_t *)(dev + g.ctr_off), gs);
+        if (timed && e == hipSuccess) e = hipEventRecord(f.t_grp[gi][1], gs);
         for (size_t r = 0; r < g.round_off.size() && e == hipSuccess; r++)
             e = launch_merkle_segs(g.field, g.width, *p2, (const MerkleSeg *)(dev + g.round_off[r]), g.round_segs[r],
                                    g.round_blocks[r], gs);
+        if (timed && e == hipSuccess) e = hipEventRecord(f.t_grp[gi][2], gs);
         if (e != hipSuccess) rc = hip_err(e, "shared launches");
     }
     // always join, so the origin stream (or capture) is left consistent
@@ -406,12 +420,21 @@ extern "C" int zkg_commit_batch(zkg_ctx *ctx, const zkg_job *jobs, uint32_t n, z
         }
         ctx->batch = b;
     }
+    // With timing on, the same launches run straight from the streams (not
+    // from the graph) so that events can bracket the stages on the streams
+    // the kernels run on.
+    BatchState *b = ctx->batch;
     stage_mark(ctx, st, 0);
-    hipError_t e = hipGraphLaunch(ctx->batch->exec, st);
-    if (e != hipSuccess) return hip_err(e, "graph launch");
+    hipError_t e = hipSuccess;
+    if (ctx->timing) {
+        if (int rc = enqueue_shared(ctx, st, b->prepared, b->fork, b->plan, true)) return rc;
+    } else {
+        e = hipGraphLaunch(b->exec, st);
+        if (e != hipSuccess) return hip_err(e, "graph launch");
+    }
     stage_mark(ctx, st, 1);
     for (uint32_t i = 0; i < n; i++) {
-        int rc = clone_tree(ctx, st, ctx->batch->prepared[i].tree, &trees_out[i]);
+        int rc = clone_tree(ctx, st, b->prepared[i].tree, &trees_out[i]);
         if (rc) {
             for (uint32_t k = 0; k < i; k++) zkg_tree_free(trees_out[k]);
             return rc;
@@ -420,10 +443,19 @@ extern "C" int zkg_commit_batch(zkg_ctx *ctx, const zkg_job *jobs, uint32_t n, z
     e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_err(e, "batch sync");
     if (ctx->timing) {
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+        // stages: 0 LDE (start to the last branch's end), 1 row hash and 2
+        // Merkle (summed over the Poseidon2 groups), 3 the whole batch
         for (int i = 0; i < 4; i++) ctx->stage_ms[i] = 0;
-        ctx->stage_ms[0] = ms;
+        float ms = 0;
+        const int k = std::max(1, std::min<int>(b->fork.k, (int)b->prepared.size()));
+        for (int i = 0; i < k; i++)
+            if (hipEventElapsedTime(&ms, ctx->ev[0], b->fork.t_lde[i]) == hipSuccess)
+                ctx->stage_ms[0] = std::max<double>(ctx->stage_ms[0], ms);
+        for (size_t g = 0; g < b->plan.groups.size(); g++) {
+            if (hipEventElapsedTime(&ms, b->fork.t_grp[g][0], b->fork.t_grp[g][1]) == hipSuccess) ctx->stage_ms[1] += ms;
+            if (hipEventElapsedTime(&ms, b->fork.t_grp[g][1], b->fork.t_grp[g][2]) == hipSuccess) ctx->stage_ms[2] += ms;
+        }
+        if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess) ctx->stage_ms[3] = ms;
     }
     return ZKG_OK;
 }
