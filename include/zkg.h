@@ -205,6 +205,25 @@ int zkg_deep_reduce(zkg_ctx *ctx, int field, const zkg_buf *lde, uint64_t height
 int zkg_deep_open(zkg_ctx *ctx, int field, const zkg_buf *lde, uint64_t height, uint32_t width, uint32_t log_blowup,
                   uint64_t shift, const void *z, const void *alpha, uint64_t alpha_offset, void *ys_out, zkg_buf *ro);
 
+/* ---- polynomial operations (Goldilocks) -------------------------------------
+ * Replace PolyOpsAccelerator (zkf-core/src/acceleration.rs:173-192; Metal
+ * zkf-metal/src/poly.rs:37-246).  Host memory in and out, canonical values
+ * (ZKG_EINVAL otherwise); each call returns when `out` is written.  Unlike the
+ * Metal accelerator there is no minimum size (it declines fewer than 1024
+ * points or evaluations).
+ * batch_eval: out[i] = sum_k coeffs[k] points[i]^k (Horner). */
+int zkg_poly_eval_goldilocks(zkg_ctx *ctx, const uint64_t *coeffs, uint64_t n_coeffs, const uint64_t *points,
+                             uint64_t n_points, uint64_t *out);
+/* coset_eval: out[i] = sum_k coeffs[k] (shift w^i)^k for i < 2^log_n, w the
+ * two-adic generator of order 2^log_n (natural order; 2^log_n coefficients,
+ * log_n <= 22). */
+int zkg_poly_coset_eval_goldilocks(zkg_ctx *ctx, const uint64_t *coeffs, uint32_t log_n, uint64_t shift,
+                                   uint64_t *out);
+/* quotient: out[i] = (evals[i] - f_z) / (w^i - z) for i < n (a power of two,
+ * w of order n), and 0 where w^i = z. */
+int zkg_poly_quotient_goldilocks(zkg_ctx *ctx, const uint64_t *evals, uint64_t n, uint64_t z, uint64_t f_z,
+                                 uint64_t *out);
+
 /* ---- constraint evaluation ---------------------------------------------------
  * Replaces ConstraintEvalAccelerator::eval_trace_goldilocks (zkf-core/src/
  * acceleration.rs:219-242; Metal body zkf-metal/src/constraint_eval.rs:115-190

@@ -1081,6 +1081,43 @@ int or_constraint_quotient_gl(const uint64_t *lde, uint64_t height, uint32_t wid
     return rc;
 }
 
+/* ======================================================================
+ * PolyOpsAccelerator, Goldilocks.  poly_eval_goldilocks
+ * (zkf-metal/src/shaders/poly_ops.metal:22-41): Horner from the top
+ * coefficient.  coset_eval_goldilocks (zkf-metal/src/poly.rs:159-189):
+ * poly_coset_shift_goldilocks (poly_ops.metal:85-95) then dft_batch of one
+ * column.  quotient_goldilocks (poly.rs:191-246, poly_ops.metal:67-82):
+ * x = generator^i, (e_i - f_z) * inverse(x - z), 0 for a zero denominator.
+ * ====================================================================== */
+void or_poly_eval_gl(const uint64_t *coeffs, uint64_t n_coeffs, const uint64_t *points, uint64_t n_points,
+                     uint64_t *out)
+{
+#pragma omp parallel for schedule(static) if (n_points > 4096)
+    for (uint64_t i = 0; i < n_points; i++) {
+        uint64_t r = 0;
+        for (uint64_t k = n_coeffs; k-- > 0;) r = or_gl_add(or_gl_mul(r, points[i]), coeffs[k]);
+        out[i] = r;
+    }
+}
+
+void or_poly_coset_eval_gl(const uint64_t *coeffs, uint32_t log_n, uint64_t shift, uint64_t *out)
+{
+    uint64_t n = 1ULL << log_n;
+    for (uint64_t k = 0; k < n; k++) out[k] = or_gl_mul(coeffs[k], or_gl_pow(shift, k));
+    or_dft_batch(OR_GOLDILOCKS, out, n, 1, 0);
+}
+
+void or_poly_quotient_gl(const uint64_t *evals, uint64_t n, uint64_t z, uint64_t f_z, uint64_t *out)
+{
+    uint64_t g = or_f_two_adic_generator(OR_GOLDILOCKS, log2_exact(n));
+#pragma omp parallel for schedule(static) if (n > 4096)
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t x = or_gl_pow(g, i);
+        uint64_t num = or_gl_sub(evals[i], f_z), den = or_gl_sub(x, z);
+        out[i] = den == 0 ? 0 : or_gl_mul(num, or_f_inv(OR_GOLDILOCKS, den));
+    }
+}
+
 void or_set_threads(int n)
 {
 #ifdef _OPENMP

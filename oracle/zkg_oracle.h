@@ -183,6 +183,18 @@ int or_constraint_quotient_gl(const uint64_t *lde, uint64_t height, uint32_t wid
                               uint64_t shift, const uint32_t *bytecode, uint32_t n_instr, const uint64_t *constants,
                               uint32_t n_constants, uint32_t n_constraints, const uint64_t *coef, uint64_t *q);
 
+/* ---- PolyOpsAccelerator, Goldilocks (zkf-core/src/acceleration.rs:173-192;
+ * Metal semantics zkf-metal/src/poly.rs:37-246, shaders/poly_ops.metal) ----
+ * eval: out[i] = sum_k coeffs[k] points[i]^k by Horner (poly_eval_goldilocks).
+ * coset_eval: coeffs[k] *= shift^k, then the forward DFT in natural order
+ * (n = 2^log_n coefficients; MetalPolyOps::coset_eval_goldilocks).
+ * quotient: out[i] = (evals[i] - f_z) / (g^i - z), g = two_adic_generator
+ * (log2 n), 0 where the denominator vanishes (poly_quotient_goldilocks). */
+void or_poly_eval_gl(const uint64_t *coeffs, uint64_t n_coeffs, const uint64_t *points, uint64_t n_points,
+                     uint64_t *out);
+void or_poly_coset_eval_gl(const uint64_t *coeffs, uint32_t log_n, uint64_t shift, uint64_t *out);
+void or_poly_quotient_gl(const uint64_t *evals, uint64_t n, uint64_t z, uint64_t f_z, uint64_t *out);
+
 /* ---- threading ---- */
 void or_set_threads(int n);
 int or_get_threads(void);

@@ -107,6 +107,9 @@ def lib():
         L.or_constraint_quotient_gl.restype = c_int
         L.or_constraint_quotient_gl.argtypes = [c_vp, c_u64, c_u32, c_u32, c_u64, c_vp, c_u32, c_vp, c_u32, c_u32,
                                                 c_vp, c_vp]
+        L.or_poly_eval_gl.argtypes = [c_vp, c_u64, c_vp, c_u64, c_vp]
+        L.or_poly_coset_eval_gl.argtypes = [c_vp, c_u32, c_u64, c_vp]
+        L.or_poly_quotient_gl.argtypes = [c_vp, c_u64, c_u64, c_u64, c_vp]
         L.or_set_threads.argtypes = [c_int]
         L.or_get_threads.restype = c_int
         _lib = L
@@ -227,6 +230,29 @@ def constraint_quotient_gl(lde, log_blowup, shift, bytecode, constants, n_constr
     rc = lib().or_constraint_quotient_gl(_p(t), rows, width, log_blowup, shift, _p(bc), bc.size, _p(cs),
                                          len(constants), n_constraints, _p(cf), _p(q))
     return None if rc else q
+
+
+# ---------------------------------------------------------------- PolyOps (Goldilocks)
+def poly_eval_gl(coeffs, points):
+    c, x = u64(coeffs), u64(points)
+    out = np.empty(x.size, dtype=np.uint64)
+    lib().or_poly_eval_gl(_p(c) if c.size else None, c.size, _p(x), x.size, _p(out))
+    return out
+
+
+def poly_coset_eval_gl(coeffs, shift, log_n):
+    c = u64(coeffs)
+    assert c.size == 1 << log_n
+    out = np.empty(c.size, dtype=np.uint64)
+    lib().or_poly_coset_eval_gl(_p(c), log_n, shift, _p(out))
+    return out
+
+
+def poly_quotient_gl(evals, z, f_z):
+    e = u64(evals)
+    out = np.empty(e.size, dtype=np.uint64)
+    lib().or_poly_quotient_gl(_p(e), e.size, z, f_z, _p(out))
+    return out
 
 
 # ---------------------------------------------------------------- generator

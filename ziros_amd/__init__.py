@@ -25,6 +25,7 @@ from .host import (  # noqa: F401
     HipDft,
     HipFri,
     HipMerkleTreeMmcs,
+    HipPolyOps,
     HipPoseidon2,
     MerkleTree,
     Server,

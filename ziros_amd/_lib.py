@@ -77,6 +77,9 @@ SIGNATURES = {
     "zkg_constants_set": (c_int, [c_vp, c_int, c_int, c_vp, c_u32, c_u32]),
     "zkg_ntt_batch": (c_int, [c_vp, c_int, c_vp, c_u64, c_u32, c_int]),
     "zkg_ntt_batch_noalloc": (c_int, [c_vp, c_int, c_vp, c_u64, c_u32, c_int]),
+    "zkg_poly_eval_goldilocks": (c_int, [c_vp, c_vp, c_u64, c_vp, c_u64, c_vp]),
+    "zkg_poly_coset_eval_goldilocks": (c_int, [c_vp, c_vp, c_u32, c_u64, c_vp]),
+    "zkg_poly_quotient_goldilocks": (c_int, [c_vp, c_vp, c_u64, c_u64, c_u64, c_vp]),
     "zkg_coset_lde_batch": (c_int, [c_vp, c_int, c_vp, c_u64, c_u32, c_u32, c_u64, c_vp]),
     "zkg_poseidon2_permute": (c_int, [c_vp, c_int, c_int, c_vp, c_u64]),
     "zkg_poseidon2_batch_permute_goldilocks": (c_int, [c_vp, c_vp, c_u64, c_vp, c_u32, c_u32]),

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libzkg.so")
 PROBE_LIB = os.path.join(HERE, "libzkg_probe.so")
 SOURCES = ["zkg_ntt.hip", "zkg_hash.hip", "zkg_deep.hip", "zkg_constraints.hip", "zkg_runtime.hip",
-           "zkg_batch.hip", "zkg_driver.hip", "zkg_noalloc.hip"]
+           "zkg_batch.hip", "zkg_driver.hip", "zkg_noalloc.hip", "zkg_poly.hip"]
 PROBE_SOURCES = ["zkg_probe.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

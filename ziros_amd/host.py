@@ -202,6 +202,46 @@ class HipPoseidon2:
         return out
 
 
+class HipPolyOps:
+    """PolyOpsAccelerator (zkf-core/src/acceleration.rs:173-192) over libzkg:
+    batch_eval_goldilocks, coset_eval_goldilocks, quotient_goldilocks."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def name(self):
+        return "hip-gfx950-poly-ops"
+
+    def is_available(self):
+        return True
+
+    def min_batch_size(self):
+        return 1024
+
+    def batch_eval_goldilocks(self, coeffs, points):
+        c = np.ascontiguousarray(coeffs, dtype=np.uint64)
+        x = np.ascontiguousarray(points, dtype=np.uint64)
+        out = np.empty(x.size, dtype=np.uint64)
+        check(self.ctx.lib.zkg_poly_eval_goldilocks(self.ctx.h, _ptr(c) if c.size else None, c.size,
+                                                    _ptr(x) if x.size else None, x.size,
+                                                    _ptr(out) if out.size else None))
+        return out
+
+    def coset_eval_goldilocks(self, coeffs, shift, log_n):
+        c = np.ascontiguousarray(coeffs, dtype=np.uint64)
+        if c.size != 1 << log_n:
+            raise ValueError("coset_eval needs 2^log_n coefficients")
+        out = np.empty(c.size, dtype=np.uint64)
+        check(self.ctx.lib.zkg_poly_coset_eval_goldilocks(self.ctx.h, _ptr(c), log_n, int(shift), _ptr(out)))
+        return out
+
+    def quotient_goldilocks(self, evals, z, f_z):
+        e = np.ascontiguousarray(evals, dtype=np.uint64)
+        out = np.empty(e.size, dtype=np.uint64)
+        check(self.ctx.lib.zkg_poly_quotient_goldilocks(self.ctx.h, _ptr(e), e.size, int(z), int(f_z), _ptr(out)))
+        return out
+
+
 class MerkleTree:
     """Prover data of one commitment (zkg_tree)."""
 
