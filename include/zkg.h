@@ -435,6 +435,50 @@ int zkg_driver_execute(zkg_ctx *ctx, const zkg_node *node, zkg_bridge *bridge, z
 /* should_try_on_either: 1 if the node is large enough to be worth a launch. */
 int zkg_driver_should_try(const zkg_node *node);
 
+/* ---- tuning ------------------------------------------------------------------
+ * After the reference's device tuning (zkf-metal/src/tuning.rs:84-111,276-365:
+ * a runtime override beats learned values, which beat the static device
+ * profile; learned values from zkf-runtime/src/adaptive_tuning.rs).  Two
+ * process-wide settings resolve that way:
+ *   - the tile plan of every column transform (NTT, coset LDE), per shape
+ *     (field, log2 height, width class = ceil(log2 width), log2 blowup, 0 for
+ *     a plain NTT).  Plans differ in radix split, tile store unrolling and
+ *     pass-B occupancy; all give identical values.  Static layer: the table
+ *     ziros_amd/tune.py measured on gfx950, nearest measured shape; then plan 0;
+ *   - the work-form splits of the row hash and Merkle levels per (field,
+ *     Poseidon2 width): levels of at most 2^merkle_lanes_log2 digests use W
+ *     lanes per compression, Goldilocks levels of at most
+ *     2^merkle_chunks_log2 one 4-chunk of the state per lane, Goldilocks row
+ *     hashes of at most 2^hash_chunks_log2 rows a 4-chunk per lane, and
+ *     fused_levels levels per one-compression-per-lane launch.  The
+ *     environment variables ZKG_MERKLE_LANES_LOG2, ZKG_MERKLE_CHUNKS_LOG2,
+ *     ZKG_HASH_CHUNKS_LOG2 and ZKG_MERKLE_FUSED_K seed the override layer.
+ * Override and learned entries match exactly; a field of -1 matches anything
+ * (split values of -1 fall through to the next layer).  Setting a layer
+ * replaces it; n = 0 clears it.  A change applies to launches made after it
+ * (a cached batch graph keeps the plans it was captured with). */
+enum zkg_tune_layer { ZKG_TUNE_OVERRIDE = 0, ZKG_TUNE_LEARNED = 1 };
+typedef struct zkg_plan_entry {
+    int32_t field, log_height, width_class, log_blowup;
+    int32_t plan;  /* 0 .. zkg_lde_plan_count() - 1 */
+} zkg_plan_entry;
+typedef struct zkg_split_entry {
+    int32_t field, p2_width;
+    int32_t merkle_lanes_log2, merkle_chunks_log2, hash_chunks_log2, fused_levels;
+} zkg_split_entry;
+int zkg_lde_plan_count(void);
+const char *zkg_lde_plan_name(int plan);
+int zkg_lde_plan_set(int layer, const zkg_plan_entry *entries, uint32_t n);
+/* The plan a transform of this shape runs now; *source = 0 override,
+ * 1 learned, 2 static table, 3 built-in default (source may be NULL). */
+int zkg_lde_plan_for(int field, uint64_t height, uint32_t width, uint32_t log_blowup, int *plan, int *source);
+/* Times every plan on this shape on the context's device (random input,
+ * median of three) and records the fastest in the learned layer. */
+int zkg_lde_plan_learn(zkg_ctx *ctx, int field, uint64_t height, uint32_t width, uint32_t log_blowup, int *plan);
+int zkg_split_set(int layer, const zkg_split_entry *entries, uint32_t n);
+/* The effective split values for (field, p2_width). */
+int zkg_split_get(int field, int p2_width, zkg_split_entry *out);
+
 /* ---- telemetry ------------------------------------------------------------- */
 /* Per-stage device time of the most recent commit / FRI commit phase / batch,
  * from HIP events on the streams the kernels run on.  Commit: 0 LDE, 1 row

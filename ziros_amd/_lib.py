@@ -59,6 +59,21 @@ class Job(ctypes.Structure):
     ]
 
 
+class PlanEntry(ctypes.Structure):
+    """zkg_plan_entry"""
+
+    _fields_ = [("field", ctypes.c_int32), ("log_height", ctypes.c_int32), ("width_class", ctypes.c_int32),
+                ("log_blowup", ctypes.c_int32), ("plan", ctypes.c_int32)]
+
+
+class SplitEntry(ctypes.Structure):
+    """zkg_split_entry"""
+
+    _fields_ = [("field", ctypes.c_int32), ("p2_width", ctypes.c_int32), ("merkle_lanes_log2", ctypes.c_int32),
+                ("merkle_chunks_log2", ctypes.c_int32), ("hash_chunks_log2", ctypes.c_int32),
+                ("fused_levels", ctypes.c_int32)]
+
+
 # name -> (restype, argtypes); every entry point declared in include/zkg.h
 SIGNATURES = {
     "zkg_last_error": (ctypes.c_char_p, []),
@@ -77,6 +92,13 @@ SIGNATURES = {
     "zkg_constants_set": (c_int, [c_vp, c_int, c_int, c_vp, c_u32, c_u32]),
     "zkg_ntt_batch": (c_int, [c_vp, c_int, c_vp, c_u64, c_u32, c_int]),
     "zkg_ntt_batch_noalloc": (c_int, [c_vp, c_int, c_vp, c_u64, c_u32, c_int]),
+    "zkg_lde_plan_count": (c_int, []),
+    "zkg_lde_plan_name": (ctypes.c_char_p, [c_int]),
+    "zkg_lde_plan_set": (c_int, [c_int, c_vp, c_u32]),
+    "zkg_lde_plan_for": (c_int, [c_int, c_u64, c_u32, c_u32, P(c_int), P(c_int)]),
+    "zkg_lde_plan_learn": (c_int, [c_vp, c_int, c_u64, c_u32, c_u32, P(c_int)]),
+    "zkg_split_set": (c_int, [c_int, c_vp, c_u32]),
+    "zkg_split_get": (c_int, [c_int, c_int, c_vp]),
     "zkg_poly_eval_goldilocks": (c_int, [c_vp, c_vp, c_u64, c_vp, c_u64, c_vp]),
     "zkg_poly_coset_eval_goldilocks": (c_int, [c_vp, c_vp, c_u32, c_u64, c_vp]),
     "zkg_poly_quotient_goldilocks": (c_int, [c_vp, c_vp, c_u64, c_u64, c_u64, c_vp]),

@@ -14,9 +14,14 @@ CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libzkg.so")
 PROBE_LIB = os.path.join(HERE, "libzkg_probe.so")
-SOURCES = ["zkg_ntt.hip", "zkg_hash.hip", "zkg_deep.hip", "zkg_constraints.hip", "zkg_runtime.hip",
-           "zkg_batch.hip", "zkg_driver.hip", "zkg_noalloc.hip", "zkg_poly.hip"]
-PROBE_SOURCES = ["zkg_probe.hip"]
+# (source, object name, extra defines): the column transforms are compiled once
+# per tile plan (zkg_plans.h), each into its own namespace.
+LDE_PLANS = 4
+UNITS = [("zkg_ntt.hip", "zkg_ntt_p%d.o" % k, ["-DZKG_LDE_VARIANT=%d" % k]) for k in range(LDE_PLANS)] + [
+    (src, src.replace(".hip", ".o"), [])
+    for src in ("zkg_hash.hip", "zkg_deep.hip", "zkg_constraints.hip", "zkg_runtime.hip", "zkg_batch.hip",
+                "zkg_driver.hip", "zkg_noalloc.hip", "zkg_poly.hip", "zkg_tuning.hip")]
+PROBE_UNITS = [("zkg_probe.hip", "zkg_probe.o", [])]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
@@ -52,13 +57,14 @@ def write_jit_prelude():
             f.write(body)
 
 
-def _compile(src, obj_dir=None, defines=()):
-    obj = os.path.join(obj_dir or os.path.join(CSRC, "_obj"), src.replace(".hip", ".o"))
+def _compile(unit, obj_dir=None):
+    src, obj_name, defines = unit
+    obj = os.path.join(obj_dir or os.path.join(CSRC, "_obj"), obj_name)
     cmd = [HIPCC, *FLAGS, *EXTRA.get(src, []), *defines, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     msgs = [l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l]
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed for %s:\n%s" % (src, "\n".join(msgs)))
+        raise RuntimeError("hipcc failed for %s %s:\n%s" % (src, " ".join(defines), "\n".join(msgs)))
     return obj, [m for m in msgs if m.strip()]
 
 
@@ -75,29 +81,17 @@ def build(verbose=False):
     """Compile every translation unit and link libzkg.so and libzkg_probe.so."""
     os.makedirs(os.path.join(CSRC, "_obj"), exist_ok=True)
     write_jit_prelude()
+    # longest units first so the pool stays busy
     with ThreadPoolExecutor(max_workers=8) as ex:
-        results = list(ex.map(_compile, SOURCES + PROBE_SOURCES))
+        results = list(ex.map(_compile, UNITS + PROBE_UNITS))
     if verbose:
         for _, msgs in results:
             for m in msgs:
                 print(m)
     objs = [o for o, _ in results]
-    _link(objs[: len(SOURCES)], LIB)
-    _link(objs[len(SOURCES):], PROBE_LIB)
+    _link(objs[: len(UNITS)], LIB)
+    _link(objs[len(UNITS):], PROBE_LIB)
     return LIB
-
-
-def build_variant(out_dir, defines):
-    """libzkg.so with extra -D definitions (tuning candidates) into out_dir."""
-    os.makedirs(out_dir, exist_ok=True)
-    write_jit_prelude()
-    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        results = list(ex.map(lambda src: _compile(src, out_dir, defines), SOURCES))
-    lib = os.path.join(out_dir, "libzkg.so")
-    _link([o for o, _ in results], lib)
-    for o, _ in results:
-        os.remove(o)
-    return lib
 
 
 if __name__ == "__main__":

@@ -33,29 +33,6 @@ constexpr int HASH_THREADS = 256;
 constexpr int MERKLE_THREADS = 256;
 constexpr int MAX_FUSE = 9;  // 2 * MERKLE_THREADS leaves -> 1 root
 
-// Work-form thresholds (log2; see lanes_split_max below).
-#ifndef ZKG_BB16_LANES_LOG2
-#define ZKG_BB16_LANES_LOG2 15
-#endif
-#ifndef ZKG_GL16_LANES_LOG2
-#define ZKG_GL16_LANES_LOG2 13
-#endif
-#ifndef ZKG_GL8_LANES_LOG2
-#define ZKG_GL8_LANES_LOG2 14
-#endif
-#ifndef ZKG_GL16_CHUNKS_LOG2
-#define ZKG_GL16_CHUNKS_LOG2 16
-#endif
-#ifndef ZKG_GL8_CHUNKS_LOG2
-#define ZKG_GL8_CHUNKS_LOG2 16
-#endif
-#ifndef ZKG_GL16_HASH_LOG2
-#define ZKG_GL16_HASH_LOG2 15
-#endif
-#ifndef ZKG_GL8_HASH_LOG2
-#define ZKG_GL8_HASH_LOG2 15
-#endif
-
 // A tuning override from the environment: 2^k for NAME=<k>, 0 <= k <= 30,
 // else `dflt`.
 uint64_t env_pow2(const char *name, uint64_t dflt)
@@ -73,26 +50,22 @@ uint64_t env_pow2(const char *name, uint64_t dflt)
 // matrices of at most hash_chunks_max rows run with W/4 lanes per row.  Below
 // these sizes one compression (or row) per lane leaves the chip mostly idle,
 // while the W-lane form repeats the internal S-box in every lane and has the
-// most work per compression.  ZKG_MERKLE_LANES_LOG2, ZKG_MERKLE_CHUNKS_LOG2
-// and ZKG_HASH_CHUNKS_LOG2 are tuning overrides (for every field and width).
+// most work per compression.  The sizes come from the tuning layers
+// (zkg_tuning.hip: override > learned > static table > built-in).
 template <class F, int W>
 uint64_t lanes_split_max()
 {
-    constexpr int dflt = std::is_same<F, Bb>::value ? ZKG_BB16_LANES_LOG2 : (W == 16 ? ZKG_GL16_LANES_LOG2 : ZKG_GL8_LANES_LOG2);
-    static const uint64_t v = env_pow2("ZKG_MERKLE_LANES_LOG2", 1ull << dflt);
-    return v;
+    return 1ull << split_value(std::is_same<F, Bb>::value ? FIELD_BB : FIELD_GL, W, SPLIT_LANES);
 }
 template <int W>
 uint64_t chunks_split_max()
 {
-    static const uint64_t v = env_pow2("ZKG_MERKLE_CHUNKS_LOG2", 1ull << (W == 16 ? ZKG_GL16_CHUNKS_LOG2 : ZKG_GL8_CHUNKS_LOG2));
-    return v;
+    return 1ull << split_value(FIELD_GL, W, SPLIT_CHUNKS);
 }
 template <int W>
 uint64_t hash_chunks_max()
 {
-    static const uint64_t v = env_pow2("ZKG_HASH_CHUNKS_LOG2", 1ull << (W == 16 ? ZKG_GL16_HASH_LOG2 : ZKG_GL8_HASH_LOG2));
-    return v;
+    return 1ull << split_value(FIELD_GL, W, SPLIT_HASH_CHUNKS);
 }
 
 // BabyBear width-16 permutations run in exact FP64 (Bf); the Montgomery form
@@ -802,15 +775,11 @@ __global__ void k_gather_open(const T *__restrict__ mat, uint32_t mat_width, Ope
     }
 }
 
-// Levels per k_merkle_fused launch (ZKG_MERKLE_FUSED_K overrides the 3).
+// Levels per k_merkle_fused launch (tuning layers; built-in 3).
+template <class F, int W>
 int fused_levels()
 {
-    static const int v = [] {
-        const char *e = getenv("ZKG_MERKLE_FUSED_K");
-        int k = 0;
-        return (e && sscanf(e, "%d", &k) == 1 && k >= 1 && k <= MAX_FUSE) ? k : 3;
-    }();
-    return v;
+    return split_value(std::is_same<F, Bb>::value ? FIELD_BB : FIELD_GL, W, SPLIT_FUSED);
 }
 
 // The launches of one tree after its level 0.  Large levels: one compression
@@ -831,7 +800,7 @@ void plan_impl(void *const *layers, int nlayers, uint64_t n, int fuse, std::vect
         const uint64_t level_n = n >> lvl;
         const bool lanes = level_n <= split;
         const bool chunks = !lanes && level_n <= csplit;
-        int k = lanes ? lanes_fuse<W>() : fused_levels();
+        int k = lanes ? lanes_fuse<W>() : fused_levels<F, W>();
         if (chunks) {
             // stop where the W-lane form takes over: a fused level with few
             // compressions costs the chunk form's longer chain

@@ -6,6 +6,8 @@
 
 #include <vector>
 
+#include "zkg_plans.h"
+
 namespace zkg {
 
 enum FieldId : int { FIELD_GL = 0, FIELD_BB = 1 };
@@ -14,53 +16,6 @@ constexpr int LOG_TW_TILE = 11;   // largest in-LDS transform: 2^11 points
 constexpr int LOG_TW_TOP = 24;    // four-step / coset tables cover orders <= 2^24
 constexpr int TW_SPLIT = 12;      // omega^e = lo[e & 4095] * hi[e >> 12]
 
-// Radix split of a 2^L-point in-LDS transform: NR rounds of at most 2^rmax
-// points per lane (ZKG_PLAN_RMAX32 / ZKG_PLAN_RMAX64), larger first.
-// Plan parameters chosen by the autotuner (ziros_amd/tune.py) when it has
-// written zkg_tuned.h; the defaults below otherwise.  The autotuner's own
-// candidate builds (ZKG_TUNING_CANDIDATE) take only their -D parameters.
-#if !defined(ZKG_TUNING_CANDIDATE) && __has_include("zkg_tuned.h")
-#include "zkg_tuned.h"
-#endif
-#ifndef ZKG_PLAN_RMAX32
-#define ZKG_PLAN_RMAX32 6
-#endif
-#ifndef ZKG_PLAN_RMAX64
-#define ZKG_PLAN_RMAX64 4
-#endif
-// Goldilocks tiles of at most 2^ZKG_PLAN_SMALL_L64 points use radix rounds of
-// at most 2^ZKG_PLAN_RMAX64_SMALL points per lane: more lanes per column, so
-// narrow matrices fill the block (columns per tile = 256 * radix / 2^L).
-#ifndef ZKG_PLAN_RMAX64_SMALL
-#define ZKG_PLAN_RMAX64_SMALL ZKG_PLAN_RMAX64
-#endif
-#ifndef ZKG_PLAN_SMALL_L64
-#define ZKG_PLAN_SMALL_L64 0
-#endif
-__host__ __device__ constexpr int plan_rmax(int elem_bytes, int L)
-{
-    return elem_bytes == 4 ? ZKG_PLAN_RMAX32 : (L <= ZKG_PLAN_SMALL_L64 ? ZKG_PLAN_RMAX64_SMALL : ZKG_PLAN_RMAX64);
-}
-__host__ __device__ constexpr int plan_nr(int rmax, int L)
-{
-    return ((L + rmax - 1) / rmax < 2 && L >= 4) ? 2 : ((L + rmax - 1) / rmax < 1 ? 1 : (L + rmax - 1) / rmax);
-}
-// ZKG_PLAN_FIRST_HEAVY: a two-round plan whose second round would keep at
-// least 2^(rmax-2) points gives its first round the full 2^rmax (the even
-// split otherwise): wider tiles (columns per tile = 256 * 2^logr(0) / 2^L),
-// so longer row segments in HBM.
-#ifndef ZKG_PLAN_FIRST_HEAVY
-#define ZKG_PLAN_FIRST_HEAVY 0
-#endif
-__host__ __device__ constexpr bool plan_first_heavy(int rmax, int L)
-{
-    return ZKG_PLAN_FIRST_HEAVY && plan_nr(rmax, L) == 2 && L - rmax >= rmax - 2;
-}
-__host__ __device__ constexpr int plan_logr(int rmax, int L, int i)
-{
-    return plan_first_heavy(rmax, L) ? (i == 0 ? rmax : L - rmax)
-                                     : L / plan_nr(rmax, L) + (i < L % plan_nr(rmax, L) ? 1 : 0);
-}
 // Inter-round twiddles of the 2^L plan, round I (not the last; at most two
 // such rounds): omega_Mi^(a q) at rtw_offset(L, I) + q (Mi / R) + a, where Mi
 // is the round's sub-problem size and R its radix.
@@ -71,7 +26,7 @@ constexpr int RTW_ENTRIES = 1 << (LOG_TW_TILE + 2);
 struct TwiddleSet {
     void *tile_fwd = nullptr;  // omega_2048^j, j < 1024
     void *tile_inv = nullptr;  // omega_2048^-j
-    void *round_fwd = nullptr, *round_inv = nullptr;  // RTW_ENTRIES inter-round twiddles
+    void *round_fwd[LDE_PLANS] = {}, *round_inv[LDE_PLANS] = {};  // RTW_ENTRIES inter-round twiddles per plan
     void *top_lo_fwd = nullptr, *top_hi_fwd = nullptr;  // omega_2^24^i, omega_2^24^(4096 i)
     void *top_lo_inv = nullptr, *top_hi_inv = nullptr;
 };
@@ -113,8 +68,35 @@ hipError_t launch_ntt(int field, const TwiddleSet &tw, void *data, void *tmp, ui
                       uint32_t width, int inverse, hipStream_t st);
 uint64_t ntt_tmp_bytes(int field, uint64_t height, uint32_t width);
 // Fills the RTW_ENTRIES inter-round table (register form) of one direction
-// from omega_2048 of that direction (register form).
-void fill_round_twiddles(int field, uint64_t w2048, void *table);
+// and plan from omega_2048 of that direction (register form).
+void fill_round_twiddles(int plan, int field, uint64_t w2048, void *table);
+// The plan a transform of this shape runs (zkg_tuning.hip: override > learned
+// > static table > plan 0).
+int lde_plan_for(int field, int log_n, uint32_t width, uint32_t log_blowup);
+// The coset LDE with an explicit plan (the tuner's timing runs).
+hipError_t launch_coset_lde_plan(int plan, int field, const TwiddleSet &tw, const CosetSet &cs, const void *in,
+                                 void *out, void *tmp, uint64_t height, uint32_t width, uint32_t log_blowup,
+                                 hipStream_t st);
+// Work-form splits of the row hash and Merkle levels for (field, Poseidon2
+// width), through the same layers (zkg_tuning.hip): log2 sizes, or levels
+// per fused launch.
+enum SplitKind { SPLIT_LANES = 0, SPLIT_CHUNKS = 1, SPLIT_HASH_CHUNKS = 2, SPLIT_FUSED = 3 };
+int split_value(int field, int width, int which);
+// One compiled instance of the transforms per plan (zkg_ntt.hip built with
+// -DZKG_LDE_VARIANT=k into namespace lde_plan<k>).
+#define ZKG_DECLARE_PLAN(NS)                                                                                       \
+    namespace NS {                                                                                                 \
+    hipError_t coset_lde(int field, const TwiddleSet &tw, const CosetSet &cs, const void *in, void *out, void *tmp,  \
+                         uint64_t height, uint32_t width, uint32_t log_blowup, hipStream_t st);                       \
+    hipError_t ntt(int field, const TwiddleSet &tw, void *data, void *tmp, uint64_t height, uint32_t width,          \
+                   int inverse, hipStream_t st);                                                                   \
+    void fill_round_twiddles(int field, uint64_t w2048, void *table);                                             \
+    }
+ZKG_DECLARE_PLAN(lde_plan0)
+ZKG_DECLARE_PLAN(lde_plan1)
+ZKG_DECLARE_PLAN(lde_plan2)
+ZKG_DECLARE_PLAN(lde_plan3)
+static_assert(LDE_PLANS == 4, "one compiled instance per plan");
 // Unfused per-stage radix-2 path (one HBM pass per stage): the plain GPU
 // denominator used for comparison with the fused path.
 // tmp: height * width elements.

@@ -36,9 +36,51 @@
 #include "zkg_field.h"
 #include "zkg_internal.h"
 
+// This file is compiled once per tile plan (zkg_plans.h): -DZKG_LDE_VARIANT=k
+// puts plan k's kernels and entry points into namespace zkg::lde_plan<k>.  The
+// plan-independent parts (the reference-structured plain path, scratch sizes,
+// the per-shape dispatch) are compiled with plan 0 only.
+#ifndef ZKG_LDE_VARIANT
+#define ZKG_LDE_VARIANT 0
+#endif
+#define ZKG_PLAN_NS_(k) lde_plan##k
+#define ZKG_PLAN_NS(k) ZKG_PLAN_NS_(k)
+
 namespace zkg {
+namespace ZKG_PLAN_NS(ZKG_LDE_VARIANT) {
 
 namespace {
+
+// This plan's parameters.
+constexpr int P_RMAX32 = LDE_PLAN_TABLE[ZKG_LDE_VARIANT].rmax32;
+constexpr int P_RMAX64 = LDE_PLAN_TABLE[ZKG_LDE_VARIANT].rmax64;
+constexpr int P_RMAX64_SMALL = LDE_PLAN_TABLE[ZKG_LDE_VARIANT].rmax64_small;
+constexpr int P_SMALL_L64 = LDE_PLAN_TABLE[ZKG_LDE_VARIANT].small_l64;
+constexpr int P_UNROLL = LDE_PLAN_TABLE[ZKG_LDE_VARIANT].unroll;
+constexpr int P_WAVES32 = LDE_PLAN_TABLE[ZKG_LDE_VARIANT].waves32;
+constexpr int P_WAVES64 = LDE_PLAN_TABLE[ZKG_LDE_VARIANT].waves64;
+constexpr int P_FIRST_HEAVY = LDE_PLAN_TABLE[ZKG_LDE_VARIANT].first_heavy;
+
+// Radix split of a 2^L-point in-LDS transform: NR rounds of at most 2^rmax
+// points per lane, larger first (or, first-heavy, the full radix first when
+// the second round would keep at least 2^(rmax-2) points).
+__host__ __device__ constexpr int plan_rmax(int elem_bytes, int L)
+{
+    return elem_bytes == 4 ? P_RMAX32 : (L <= P_SMALL_L64 ? P_RMAX64_SMALL : P_RMAX64);
+}
+__host__ __device__ constexpr int plan_nr(int rmax, int L)
+{
+    return ((L + rmax - 1) / rmax < 2 && L >= 4) ? 2 : ((L + rmax - 1) / rmax < 1 ? 1 : (L + rmax - 1) / rmax);
+}
+__host__ __device__ constexpr bool plan_first_heavy(int rmax, int L)
+{
+    return P_FIRST_HEAVY && plan_nr(rmax, L) == 2 && L - rmax >= rmax - 2;
+}
+__host__ __device__ constexpr int plan_logr(int rmax, int L, int i)
+{
+    return plan_first_heavy(rmax, L) ? (i == 0 ? rmax : L - rmax)
+                                     : L / plan_nr(rmax, L) + (i < L % plan_nr(rmax, L) ? 1 : 0);
+}
 
 constexpr int NTT_THREADS = 256;
 
@@ -238,13 +280,8 @@ struct Geo {
 // A lane of a tile pass always handles column c = tid % C of the tile and
 // rows r0 + i * RSTEP (i < ITERS), r0 = tid / C.  Whether that column exists
 // (the last chunk of a matrix may be partial) is therefore known once per lane.
-#ifndef ZKG_TILE_STORE_UNROLL
-#define ZKG_TILE_STORE_UNROLL 8
-#endif
-#define ZKG_PRAGMA_(x) _Pragma(#x)
-#define ZKG_UNROLL(n) ZKG_PRAGMA_(unroll n)
 #define TILE_STORE_LOOP(P, row) \
-    ZKG_UNROLL(ZKG_TILE_STORE_UNROLL) \
+    _Pragma("unroll P_UNROLL") \
     for (int row = (int)(threadIdx.x >> P::LOG_C); row < P::M; row += P::RSTEP)
 
 // HBM -> LDS: tile row m is matrix row row0 + m * row_stride, columns c0 ..
@@ -405,23 +442,19 @@ __device__ __forceinline__ void load_coefficients(const typename F::T *lds, type
 // matrix.  The coset weights (by k2) and omega_N^(j2 k1) (by j2) are vectors
 // in LDS shared by the tile's columns; the weighting is fused into the first
 // forward round.
-// Occupancy of pass B: at least ZKG_LDE_MID_WAVES32 (BabyBear) or
-// ZKG_LDE_MID_WAVES64 (Goldilocks) waves per SIMD for tiles of up to 2^10
-// points.  These are autotuner knobs; 0 leaves the register budget to the
-// compiler.  2^11-point tiles hold too much LDS for more than one block per CU.
-#ifndef ZKG_LDE_MID_WAVES32
-#define ZKG_LDE_MID_WAVES32 0
-#endif
-#ifndef ZKG_LDE_MID_WAVES64
-#define ZKG_LDE_MID_WAVES64 0
-#endif
+// Occupancy of pass B: plans with waves32 / waves64 set compile it for at
+// least that many waves per SIMD (BabyBear / Goldilocks) for tiles of up to
+// 2^10 points; 0 leaves the register budget to the compiler.  2^11-point tiles
+// hold too much LDS for more than one block per CU.
 template <class F, int L>
 constexpr int lde_mid_waves()
 {
-    const int w = sizeof(typename F::T) == 4 ? ZKG_LDE_MID_WAVES32 : ZKG_LDE_MID_WAVES64;
+    const int w = sizeof(typename F::T) == 4 ? P_WAVES32 : P_WAVES64;
     return L <= 10 && w > 0 ? w : 1;
 }
-#if ZKG_LDE_MID_WAVES32 > 0 || ZKG_LDE_MID_WAVES64 > 0
+// the occupancy plan (zkg_plans.h row 2) is the one that sets waves
+static_assert((P_WAVES32 > 0 || P_WAVES64 > 0) == (ZKG_LDE_VARIANT == 2), "plan 2 carries the occupancy attribute");
+#if ZKG_LDE_VARIANT == 2
 #define ZKG_LDE_MID_OCC __attribute__((amdgpu_waves_per_eu(lde_mid_waves<F, L>())))
 #else
 #define ZKG_LDE_MID_OCC
@@ -628,7 +661,8 @@ struct Tabs {
     const typename F::T *tile_fwd, *tile_inv, *round_fwd, *round_inv, *lo_fwd, *hi_fwd, *lo_inv, *hi_inv;
     explicit Tabs(const TwiddleSet &t)
         : tile_fwd((const typename F::T *)t.tile_fwd), tile_inv((const typename F::T *)t.tile_inv),
-          round_fwd((const typename F::T *)t.round_fwd), round_inv((const typename F::T *)t.round_inv),
+          round_fwd((const typename F::T *)t.round_fwd[ZKG_LDE_VARIANT]),
+          round_inv((const typename F::T *)t.round_inv[ZKG_LDE_VARIANT]),
           lo_fwd((const typename F::T *)t.top_lo_fwd), hi_fwd((const typename F::T *)t.top_hi_fwd),
           lo_inv((const typename F::T *)t.top_lo_inv), hi_inv((const typename F::T *)t.top_hi_inv)
     {
@@ -884,17 +918,18 @@ hipError_t lde_plain_impl(const TwiddleSet &tws, const void *in, void *out, void
 
 }  // namespace
 
-uint64_t coset_lde_tmp_bytes(int field, uint64_t height, uint32_t width, uint32_t log_blowup)
+hipError_t coset_lde(int field, const TwiddleSet &tw, const CosetSet &cs, const void *in, void *out, void *tmp,
+                     uint64_t height, uint32_t width, uint32_t log_blowup, hipStream_t st)
 {
-    if (ilog2(height) <= LOG_TW_TILE) return 0;
-    uint64_t es = field == FIELD_GL ? 8 : 4;
-    return (height << log_blowup) * width * es;
+    if (field == FIELD_GL) return coset_lde_impl<Gl>(tw, cs, in, out, tmp, height, width, log_blowup, st);
+    return coset_lde_impl<Bb>(tw, cs, in, out, tmp, height, width, log_blowup, st);
 }
 
-uint64_t ntt_tmp_bytes(int field, uint64_t height, uint32_t width)
+hipError_t ntt(int field, const TwiddleSet &tw, void *data, void *tmp, uint64_t height, uint32_t width, int inverse,
+               hipStream_t st)
 {
-    if (ilog2(height) <= LOG_TW_TILE) return 0;
-    return height * width * (field == FIELD_GL ? 8 : 4);
+    if (field == FIELD_GL) return ntt_impl<Gl>(tw, data, tmp, height, width, inverse, st);
+    return ntt_impl<Bb>(tw, data, tmp, height, width, inverse, st);
 }
 
 void fill_round_twiddles(int field, uint64_t w2048, void *table)
@@ -905,25 +940,78 @@ void fill_round_twiddles(int field, uint64_t w2048, void *table)
         fill_round_twiddles_t<Bb>((uint32_t)w2048, (uint32_t *)table);
 }
 
+#if ZKG_LDE_VARIANT == 0
+hipError_t lde_plain(int field, const TwiddleSet &tw, const void *in, void *out, void *tmp, uint64_t height,
+                     uint32_t width, uint32_t log_blowup, uint64_t shift, hipStream_t st, bool sync_each)
+{
+    if (field == FIELD_GL) return lde_plain_impl<Gl>(tw, in, out, tmp, height, width, log_blowup, shift, st, sync_each);
+    return lde_plain_impl<Bb>(tw, in, out, tmp, height, width, log_blowup, shift, st, sync_each);
+}
+#endif
+
+}  // namespace lde_plan<k>
+
+#if ZKG_LDE_VARIANT == 0
+uint64_t coset_lde_tmp_bytes(int field, uint64_t height, uint32_t width, uint32_t log_blowup)
+{
+    if (lde_plan0::ilog2(height) <= LOG_TW_TILE) return 0;
+    uint64_t es = field == FIELD_GL ? 8 : 4;
+    return (height << log_blowup) * width * es;
+}
+
+uint64_t ntt_tmp_bytes(int field, uint64_t height, uint32_t width)
+{
+    if (lde_plan0::ilog2(height) <= LOG_TW_TILE) return 0;
+    return height * width * (field == FIELD_GL ? 8 : 4);
+}
+
+void fill_round_twiddles(int plan, int field, uint64_t w2048, void *table)
+{
+    switch (plan) {
+    case 1: return lde_plan1::fill_round_twiddles(field, w2048, table);
+    case 2: return lde_plan2::fill_round_twiddles(field, w2048, table);
+    case 3: return lde_plan3::fill_round_twiddles(field, w2048, table);
+    default: return lde_plan0::fill_round_twiddles(field, w2048, table);
+    }
+}
+
+hipError_t launch_coset_lde_plan(int plan, int field, const TwiddleSet &tw, const CosetSet &cs, const void *in,
+                                 void *out, void *tmp, uint64_t height, uint32_t width, uint32_t log_blowup,
+                                 hipStream_t st)
+{
+    switch (plan) {
+    case 1: return lde_plan1::coset_lde(field, tw, cs, in, out, tmp, height, width, log_blowup, st);
+    case 2: return lde_plan2::coset_lde(field, tw, cs, in, out, tmp, height, width, log_blowup, st);
+    case 3: return lde_plan3::coset_lde(field, tw, cs, in, out, tmp, height, width, log_blowup, st);
+    default: return lde_plan0::coset_lde(field, tw, cs, in, out, tmp, height, width, log_blowup, st);
+    }
+}
+
+// The plan of each transform is chosen by its shape (lde_plan_for); every
+// plan computes the same values.
 hipError_t launch_coset_lde(int field, const TwiddleSet &tw, const CosetSet &cs, const void *in, void *out, void *tmp,
                             uint64_t height, uint32_t width, uint32_t log_blowup, hipStream_t st)
 {
-    if (field == FIELD_GL) return coset_lde_impl<Gl>(tw, cs, in, out, tmp, height, width, log_blowup, st);
-    return coset_lde_impl<Bb>(tw, cs, in, out, tmp, height, width, log_blowup, st);
+    return launch_coset_lde_plan(lde_plan_for(field, lde_plan0::ilog2(height), width, log_blowup), field, tw, cs, in,
+                                 out, tmp, height, width, log_blowup, st);
 }
 
 hipError_t launch_ntt(int field, const TwiddleSet &tw, void *data, void *tmp, uint64_t height, uint32_t width,
                       int inverse, hipStream_t st)
 {
-    if (field == FIELD_GL) return ntt_impl<Gl>(tw, data, tmp, height, width, inverse, st);
-    return ntt_impl<Bb>(tw, data, tmp, height, width, inverse, st);
+    switch (lde_plan_for(field, lde_plan0::ilog2(height), width, 0)) {
+    case 1: return lde_plan1::ntt(field, tw, data, tmp, height, width, inverse, st);
+    case 2: return lde_plan2::ntt(field, tw, data, tmp, height, width, inverse, st);
+    case 3: return lde_plan3::ntt(field, tw, data, tmp, height, width, inverse, st);
+    default: return lde_plan0::ntt(field, tw, data, tmp, height, width, inverse, st);
+    }
 }
 
 hipError_t launch_lde_plain(int field, const TwiddleSet &tw, const void *in, void *out, void *tmp, uint64_t height,
                             uint32_t width, uint32_t log_blowup, uint64_t shift, hipStream_t st, bool sync_each)
 {
-    if (field == FIELD_GL) return lde_plain_impl<Gl>(tw, in, out, tmp, height, width, log_blowup, shift, st, sync_each);
-    return lde_plain_impl<Bb>(tw, in, out, tmp, height, width, log_blowup, shift, st, sync_each);
+    return lde_plan0::lde_plain(field, tw, in, out, tmp, height, width, log_blowup, shift, st, sync_each);
 }
+#endif
 
 }  // namespace zkg

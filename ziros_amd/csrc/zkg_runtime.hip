@@ -317,14 +317,16 @@ int build_twiddles(zkg_ctx *ctx, TwiddleSet &ts)
         x = F::mul(x, step);
         y = F::mul(y, stepi);
     }
-    std::vector<T> rf(RTW_ENTRIES), ri(RTW_ENTRIES);
-    fill_round_twiddles(sizeof(T) == 8 ? FIELD_GL : FIELD_BB, (uint64_t)w, rf.data());
-    fill_round_twiddles(sizeof(T) == 8 ? FIELD_GL : FIELD_BB, (uint64_t)wi, ri.data());
     int rc;
+    for (int plan = 0; plan < LDE_PLANS; plan++) {  // the inter-round twiddles depend on the plan's radix split
+        std::vector<T> rf(RTW_ENTRIES), ri(RTW_ENTRIES);
+        fill_round_twiddles(plan, sizeof(T) == 8 ? FIELD_GL : FIELD_BB, (uint64_t)w, rf.data());
+        fill_round_twiddles(plan, sizeof(T) == 8 ? FIELD_GL : FIELD_BB, (uint64_t)wi, ri.data());
+        if ((rc = upload_vec<F>(ctx, rf, &ts.round_fwd[plan]))) return rc;
+        if ((rc = upload_vec<F>(ctx, ri, &ts.round_inv[plan]))) return rc;
+    }
     if ((rc = upload_vec<F>(ctx, tf, &ts.tile_fwd))) return rc;
     if ((rc = upload_vec<F>(ctx, ti, &ts.tile_inv))) return rc;
-    if ((rc = upload_vec<F>(ctx, rf, &ts.round_fwd))) return rc;
-    if ((rc = upload_vec<F>(ctx, ri, &ts.round_inv))) return rc;
     if ((rc = upload_vec<F>(ctx, lf, &ts.top_lo_fwd))) return rc;
     if ((rc = upload_vec<F>(ctx, hf, &ts.top_hi_fwd))) return rc;
     if ((rc = upload_vec<F>(ctx, li, &ts.top_lo_inv))) return rc;

@@ -654,6 +654,52 @@ class HipDispatchDriver:
                     output_bytes=tel.output_bytes)
 
 
+# ---------------------------------------------------------------- tuning layers
+TUNE_OVERRIDE, TUNE_LEARNED = 0, 1
+PLAN_SOURCES = ("override", "learned", "static", "default")
+
+
+def lde_plans():
+    """Names of the transform tile plans, by index."""
+    lib = _lib.load()
+    return [lib.zkg_lde_plan_name(i).decode() for i in range(lib.zkg_lde_plan_count())]
+
+
+def set_lde_plans(layer, entries):
+    """Replace a tuning layer's plan entries: (field, log_height, width_class,
+    log_blowup, plan) tuples, -1 matching anything; [] clears the layer."""
+    arr = (_lib.PlanEntry * max(len(entries), 1))(*[_lib.PlanEntry(*e) for e in entries])
+    check(_lib.load().zkg_lde_plan_set(layer, arr, len(entries)))
+
+
+def lde_plan_for(field, height, width, log_blowup):
+    """(plan, source) a transform of this shape runs now."""
+    p, src = ctypes.c_int(), ctypes.c_int()
+    check(_lib.load().zkg_lde_plan_for(field, height, width, log_blowup, ctypes.byref(p), ctypes.byref(src)))
+    return p.value, PLAN_SOURCES[src.value]
+
+
+def learn_lde_plan(ctx, field, height, width, log_blowup):
+    """Time every plan on this shape and record the fastest (learned layer)."""
+    p = ctypes.c_int()
+    check(ctx.lib.zkg_lde_plan_learn(ctx.h, field, height, width, log_blowup, ctypes.byref(p)))
+    return p.value
+
+
+def set_splits(layer, entries):
+    """Replace a tuning layer's split entries: (field, p2_width, merkle_lanes_log2,
+    merkle_chunks_log2, hash_chunks_log2, fused_levels), -1 falling through."""
+    arr = (_lib.SplitEntry * max(len(entries), 1))(*[_lib.SplitEntry(*e) for e in entries])
+    check(_lib.load().zkg_split_set(layer, arr, len(entries)))
+
+
+def splits(field, p2_width):
+    e = _lib.SplitEntry()
+    check(_lib.load().zkg_split_get(field, p2_width, ctypes.byref(e)))
+    return dict(merkle_lanes_log2=e.merkle_lanes_log2, merkle_chunks_log2=e.merkle_chunks_log2,
+                hash_chunks_log2=e.hash_chunks_log2, fused_levels=e.fused_levels)
+
+
 def commit_trace(ctx, field, trace_buf, height, width, log_blowup, p2_width=16, plain=False):
     """Fused (or plain reference-structured) commit of a device-resident trace."""
     es = np.dtype(DTYPE[field]).itemsize

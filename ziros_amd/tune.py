@@ -1,147 +1,196 @@
-"""Autotuner for the LDE tile plans of libzkg.
+"""Autotuner for libzkg: writes the static tuning layer (csrc/zkg_tuned.h).
 
-The in-LDS transforms split a 2^L-point tile into radix rounds of at most
-2^ZKG_PLAN_RMAX32 (BabyBear) / 2^ZKG_PLAN_RMAX64 (Goldilocks) points per lane.
-The radix fixes the tile shape: columns per tile = 256 * radix / 2^L.  It also
-fixes the kernels' registers and LDS, and with them the blocks per CU.
-ZKG_PLAN_RMAX64_SMALL applies to Goldilocks tiles of at most
-2^ZKG_PLAN_SMALL_L64 points.  ZKG_TILE_STORE_UNROLL sets how far the tile
-store loops unroll, and ZKG_LDE_MID_WAVES32/64 the waves per SIMD (blocks per
-CU) that LDE pass B is compiled for, BabyBear and Goldilocks.  The tuner builds
-every candidate and times the coset LDE of each on shapes that cover the
-single, batched and served workloads (heights 2^13..2^22, widths 8..256,
-blowup 2 and 4).  It writes the candidate with the least score (the sum over
-shapes of its time relative to the best candidate on that shape) into
-csrc/zkg_tuned.h, which the build picks up:
+Two settings are tuned on the device, both chosen at run time through the
+tuning layers of zkg_tuning.hip (override > learned > static > built-in):
 
-    python -m ziros_amd.tune --build     # compile the candidates (no GPU needed)
-    python -m ziros_amd.tune --measure   # time them on the GPU, write zkg_tuned.h
-    python -m ziros_amd.build            # rebuild with the tuned plan
+* the tile plan of the column transforms per shape.  Every plan of
+  csrc/zkg_plans.h is compiled into the library; for each shape of a grid
+  covering the single, batched and served workloads (both fields, heights
+  2^12..2^22, widths 8..256, blowup 2 and 4, and plain NTTs) the tuner forces
+  each plan through the override layer, times the transform (median of 3 after
+  a warm-up) and keeps the fastest if it beats plan 0 by more than
+  PLAN_MARGIN.  At run time a shape takes the entry of the nearest measured
+  shape of its field.
+* the work-form splits of the row hash and Merkle levels per (field,
+  Poseidon2 width): for each candidate value the tuner times fused commits
+  of several heights and keeps the value with the least total if it beats the
+  current value by more than SPLIT_MARGIN.
+
+    python -m ziros_amd.tune --measure   # on the GPU: time, write zkg_tuned.h
+    python -m ziros_amd.build            # rebuild with the new static layer
 """
 import argparse
-import json
 import os
 import subprocess
 import sys
 import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-TUNE_DIR = os.path.join(HERE, "_tune")
 TUNED_H = os.path.join(HERE, "csrc", "zkg_tuned.h")
 
-CANDIDATES = {
-    "r6_r4_u8": {"ZKG_PLAN_RMAX32": 6, "ZKG_PLAN_RMAX64": 4, "ZKG_TILE_STORE_UNROLL": 8},
-    "r6_r5_u8": {"ZKG_PLAN_RMAX32": 6, "ZKG_PLAN_RMAX64": 5, "ZKG_TILE_STORE_UNROLL": 8},
-    "r5_r4_u8": {"ZKG_PLAN_RMAX32": 5, "ZKG_PLAN_RMAX64": 4, "ZKG_TILE_STORE_UNROLL": 8},
-    "r6_r4_u4": {"ZKG_PLAN_RMAX32": 6, "ZKG_PLAN_RMAX64": 4, "ZKG_TILE_STORE_UNROLL": 4},
-    "r6_r4_u16": {"ZKG_PLAN_RMAX32": 6, "ZKG_PLAN_RMAX64": 4, "ZKG_TILE_STORE_UNROLL": 16},
-    "r5_r5_u8": {"ZKG_PLAN_RMAX32": 5, "ZKG_PLAN_RMAX64": 5, "ZKG_TILE_STORE_UNROLL": 8},
-    # pass B held to 3 waves per SIMD (168 registers) instead of the compiler's 2
-    "r6_r4_u4_o3": {"ZKG_PLAN_RMAX32": 6, "ZKG_PLAN_RMAX64": 4, "ZKG_TILE_STORE_UNROLL": 4, "ZKG_LDE_MID_WAVES32": 3},
-    "r6_r4_u4_o33": {"ZKG_PLAN_RMAX32": 6, "ZKG_PLAN_RMAX64": 4, "ZKG_TILE_STORE_UNROLL": 4, "ZKG_LDE_MID_WAVES32": 3,
-                     "ZKG_LDE_MID_WAVES64": 3},
-    # Goldilocks tiles of up to 2^8 points with 2^3-point rounds: narrow small matrices fill the block
-    "r6_r4s3_u8": {"ZKG_PLAN_RMAX32": 6, "ZKG_PLAN_RMAX64": 4, "ZKG_PLAN_RMAX64_SMALL": 3, "ZKG_PLAN_SMALL_L64": 8,
-                   "ZKG_TILE_STORE_UNROLL": 8},
-    "r6_r4s3_u4": {"ZKG_PLAN_RMAX32": 6, "ZKG_PLAN_RMAX64": 4, "ZKG_PLAN_RMAX64_SMALL": 3, "ZKG_PLAN_SMALL_L64": 8,
-                   "ZKG_TILE_STORE_UNROLL": 4},
-    # two-round plans with the full radix first (BabyBear 2^10 tiles as 2^6 x 2^4: 16-column tiles)
-    "r6h_r4s3_u8": {"ZKG_PLAN_RMAX32": 6, "ZKG_PLAN_RMAX64": 4, "ZKG_PLAN_RMAX64_SMALL": 3, "ZKG_PLAN_SMALL_L64": 8,
-                    "ZKG_TILE_STORE_UNROLL": 8, "ZKG_PLAN_FIRST_HEAVY": 1},
-    # BabyBear 2^10-point tiles in three rounds (2^4, 2^3, 2^3): fewer live registers per lane
-    "r4_r4s3_u8": {"ZKG_PLAN_RMAX32": 4, "ZKG_PLAN_RMAX64": 4, "ZKG_PLAN_RMAX64_SMALL": 3, "ZKG_PLAN_SMALL_L64": 8,
-                   "ZKG_TILE_STORE_UNROLL": 8},
+GL, BB = 0, 1
+# (field, log2 height, width, log_blowup); blowup 0 = plain NTT
+LDE_GRID = [(f, lg, w, lb) for f in (BB, GL) for lg in (12, 14, 16, 18, 20, 22) for w in (8, 32, 128, 256)
+            for lb in (1, 2) if not (lg == 22 and w >= 128) and not (f == GL and lg == 22 and w == 32 and lb == 2)]
+LDE_GRID += [(f, lg, w, 0) for f in (BB, GL) for lg in (14, 18, 22) for w in (8, 64)]
+
+# split candidates and the commits (log2 height, width, log_blowup) that time them
+SPLIT_CANDIDATES = {
+    "merkle_lanes_log2": {(BB, 16): (13, 14, 15, 16, 17), (GL, 16): (11, 12, 13, 14, 15), (GL, 8): (12, 13, 14, 15, 16)},
+    "fused_levels": {(BB, 16): (2, 3, 4), (GL, 16): (2, 3, 4), (GL, 8): (2, 3, 4)},
+    "merkle_chunks_log2": {(GL, 16): (14, 15, 16, 17), (GL, 8): (14, 15, 16, 17)},
+    "hash_chunks_log2": {(GL, 16): (13, 14, 15, 16), (GL, 8): (13, 14, 15, 16)},
 }
-
-# (field, log2 height, width, log_blowup); field 1 = BabyBear, 0 = Goldilocks
-SHAPES = [
-    (1, 20, 64, 1), (1, 22, 16, 1), (1, 16, 128, 2), (1, 13, 256, 1), (1, 18, 48, 1),
-    (0, 20, 64, 1), (0, 18, 64, 2), (0, 16, 32, 1), (0, 21, 16, 1), (0, 14, 96, 2), (0, 14, 8, 1),
-]
+SPLIT_COMMITS = [(12, 16, 1), (14, 8, 1), (16, 16, 1), (18, 16, 1), (20, 16, 1)]
+SPLIT_ORDER = ("merkle_lanes_log2", "merkle_chunks_log2", "hash_chunks_log2", "fused_levels")
 
 
-def _defines(cfg):
-    # ZKG_TUNING_CANDIDATE: the installed zkg_tuned.h must not override the candidate
-    return ["-DZKG_TUNING_CANDIDATE=1"] + ["-D%s=%d" % kv for kv in sorted(cfg.items())]
+# A measured winner replaces the default only if it is faster by more than
+# this fraction: closer than that, the order of two candidates is run-to-run
+# noise (wall-clock timing of calls of 0.05 .. 20 ms).
+PLAN_MARGIN = 0.02
+SPLIT_MARGIN = 0.01
 
 
-def build_candidates():
-    from .build import build_variant
-
-    for name, cfg in CANDIDATES.items():
-        lib = build_variant(os.path.join(TUNE_DIR, name), _defines(cfg))
-        print("built", name, lib, flush=True)
+def _median(ts):
+    return sorted(ts)[len(ts) // 2]
 
 
-def time_shapes(reps=3):
-    """Runs in a child process whose ZKG_LIBRARY names one candidate."""
+def choose(times, default, margin):
+    """Index of the fastest entry of `times`, unless entry `default` is within
+    `margin` of it."""
+    best = min(range(len(times)), key=lambda i: times[i])
+    return default if times[default] <= (1 + margin) * times[best] else best
+
+
+def measure_plans(ctx, log=print):
     import ziros_amd as Z
+    from ziros_amd import host as H
     from ziros_amd._lib import check
 
-    ctx = Z.Context(0)
-    out = []
-    for field, lg, w, lb in SHAPES:
+    n_plans = len(H.lde_plans())
+    table = []
+    for field, lg, w, lb in LDE_GRID:
         n = 1 << lg
-        es = 8 if field == Z.GOLDILOCKS else 4
+        es = 8 if field == GL else 4
         tr = ctx.random(field, n * w, 1000 + lg)
-        lde = ctx.alloc((n << lb) * w * es)
-        shift = Z.GENERATOR[field]
-        check(ctx.lib.zkg_coset_lde_batch(ctx.h, field, tr.h, n, w, lb, shift, lde.h))
-        ctx.synchronize()
-        ts = []
-        for _ in range(reps):
-            t0 = time.perf_counter()
-            check(ctx.lib.zkg_coset_lde_batch(ctx.h, field, tr.h, n, w, lb, shift, lde.h))
-            ctx.synchronize()
-            ts.append(time.perf_counter() - t0)
-        out.append(sorted(ts)[len(ts) // 2] * 1e3)
+        out = ctx.alloc((n << lb) * w * es) if lb else None
+        times = []
+        for plan in range(n_plans):
+            H.set_lde_plans(H.TUNE_OVERRIDE, [(-1, -1, -1, -1, plan)])
+
+            def once():
+                if lb:
+                    check(ctx.lib.zkg_coset_lde_batch(ctx.h, field, tr.h, n, w, lb, Z.GENERATOR[field], out.h))
+                else:
+                    check(ctx.lib.zkg_ntt_batch(ctx.h, field, tr.h, n, w, 0))
+
+            once()
+            ts = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                once()
+                ts.append(time.perf_counter() - t0)
+            times.append(_median(ts) * 1e3)
+        H.set_lde_plans(H.TUNE_OVERRIDE, [])
+        best = choose(times, 0, PLAN_MARGIN)
+        table.append(((field, lg, w, lb), times, best))
+        log("plan %d/%d/%d/%d %s -> %d" % (field, lg, w, lb, " ".join("%.3f" % t for t in times), best))
+        tr.free()
+        if out:
+            out.free()
+    return table
+
+
+def measure_splits(ctx, log=print):
+    import ziros_amd as Z
+    from ziros_amd import host as H
+    from ziros_amd._lib import check
+    import ctypes
+
+    bufs = {}
+    for field in (BB, GL):
+        for lg, w, lb in SPLIT_COMMITS:
+            n = 1 << lg
+            es = 8 if field == GL else 4
+            bufs[(field, lg, w, lb)] = (ctx.random(field, n * w, 2000 + lg), ctx.alloc((n << lb) * w * es))
+
+    def total(field, p2w):
+        s = 0.0
+        for lg, w, lb in SPLIT_COMMITS:
+            tr, lde = bufs[(field, lg, w, lb)]
+            ts = []
+            for rep in range(4):
+                t = ctypes.c_void_p()
+                t0 = time.perf_counter()
+                check(ctx.lib.zkg_commit(ctx.h, field, p2w, tr.h, 1 << lg, w, lb, lde.h, ctypes.byref(t)))
+                dt = time.perf_counter() - t0
+                ctx.lib.zkg_tree_free(t)
+                if rep:
+                    ts.append(dt)
+            s += _median(ts)
+        return s * 1e3
+
+    chosen = {}
+    for key in ((BB, 16), (GL, 16), (GL, 8)):
+        field, p2w = key
+        cur = H.splits(field, p2w)
+        rows = []
+        for name in SPLIT_ORDER:
+            cands = SPLIT_CANDIDATES[name].get(key)
+            if not cands:
+                continue
+            res = []
+            for v in cands:
+                trial = dict(cur, **{name: v})
+                H.set_splits(H.TUNE_OVERRIDE, [(field, p2w, trial["merkle_lanes_log2"], trial["merkle_chunks_log2"],
+                                                trial["hash_chunks_log2"], trial["fused_levels"])])
+                res.append((total(field, p2w), v))
+            H.set_splits(H.TUNE_OVERRIDE, [])
+            values = [v for _, v in res]
+            best_v = values[choose([t for t, _ in res], values.index(cur[name]) if cur[name] in values else 0,
+                                   SPLIT_MARGIN)]
+            cur[name] = best_v
+            rows.append((name, res))
+            log("split %d/%d %s %s -> %d" % (field, p2w, name, " ".join("%d:%.3f" % (v, t) for t, v in res), best_v))
+        chosen[key] = (cur, rows)
+    for tr, lde in bufs.values():
         tr.free()
         lde.free()
-        print("shape", field, lg, w, lb, "%.3f ms" % out[-1], file=sys.stderr, flush=True)
-    ctx.close()
-    print(json.dumps(out), flush=True)
+    return chosen
 
 
-def measure(out_path=TUNED_H):
-    results = {}
-    for name in CANDIDATES:
-        lib = os.path.join(TUNE_DIR, name, "libzkg.so")
-        if not os.path.exists(lib):
-            raise SystemExit("candidate %s is not built: run python -m ziros_amd.tune --build" % name)
-        env = dict(os.environ, ZKG_LIBRARY=lib)
-        print("timing", name, flush=True)
-        r = subprocess.run([sys.executable, "-m", "ziros_amd.tune", "--time"], env=env, capture_output=True,
-                           text=True, timeout=300)
-        if r.returncode != 0:
-            raise SystemExit("timing %s failed:\n%s" % (name, r.stderr[-2000:]))
-        results[name] = json.loads(r.stdout.strip().splitlines()[-1])
-        print(name, " ".join("%.3f" % t for t in results[name]), flush=True)
-    best = min(results, key=lambda k: score(results, k))
-    write_header(best, results, out_path)
-    print("best", best, "->", out_path, flush=True)
+def write_header(plans, splits, out_path=TUNED_H):
+    from ziros_amd import host as H
 
-
-def score(results, name):
-    """Sum over shapes of the candidate's time relative to the best time on
-    that shape: every shape weighs the same, whatever its size."""
-    best = [min(ts[i] for ts in results.values()) for i in range(len(SHAPES))]
-    return sum(t / b for t, b in zip(results[name], best))
-
-
-def write_header(best, results, out_path):
+    names = H.lde_plans()
     lines = [
-        "// zkg_tuned.h -- LDE tile plan chosen by ziros_amd/tune.py (python -m ziros_amd.tune --measure)",
-        "// on %s.  Coset LDE milliseconds per shape (field: 1 BabyBear, 0 Goldilocks;" % _device_name(),
-        "// log2 height, width, log_blowup), median of 3.  Score = sum over shapes of the time relative to",
-        "// the best candidate on that shape; the least score wins.",
+        "// zkg_tuned.h -- static tuning layer (see zkg_tuning.hip), written by ziros_amd/tune.py --measure on",
+        "// %s." % _device_name(),
         "//",
-        "// %-12s %s  score" % ("candidate", " ".join("%d/%d/%d/%d" % s for s in SHAPES)),
+        "// Transform plans: milliseconds per call (median of 3) for each plan, per shape",
+        "// (field 1 BabyBear / 0 Goldilocks, log2 height, width, log2 blowup; blowup 0 = plain NTT).",
+        "// Plans: %s." % ", ".join("%d %s" % (i, n) for i, n in enumerate(names)),
+        "//",
     ]
-    for name in sorted(results, key=lambda k: score(results, k)):
-        lines.append("// %-12s %s  %.3f" % (name, " ".join("%.3f" % t for t in results[name]), score(results, name)))
+    for (f, lg, w, lb), times, best in plans:
+        lines.append("//   %d/%d/%d/%d  %s  -> %d" % (f, lg, w, lb, " ".join("%7.3f" % t for t in times), best))
+    lines.append("//")
+    lines.append("// Split levels: fused-commit milliseconds summed over heights %s, each value tried in turn"
+                 % ", ".join("2^%d x %d" % (lg, w) for lg, w, _ in SPLIT_COMMITS))
+    lines.append("// with the values chosen before it.")
+    for (f, p2w), (cur, rows) in splits.items():
+        for name, res in rows:
+            lines.append("//   %d/%d %s: %s" % (f, p2w, name, " ".join("%d=%.3f" % (v, t) for t, v in res)))
     lines.append("#pragma once")
-    for k, v in sorted(CANDIDATES[best].items()):
-        lines.append("#define %s %d" % (k, v))
+    ent = ", ".join("{%d, %d, %d, %d, %d}" % (f, lg, (w - 1).bit_length(), lb, best)
+                    for (f, lg, w, lb), _, best in plans)
+    lines.append("#define ZKG_TUNED_LDE_TABLE {%s}" % ent)
+    sp = ", ".join("{%d, %d, %d, %d, %d, %d}" % (f, p2w, c["merkle_lanes_log2"],
+                                                  c["merkle_chunks_log2"] if f == GL else -1,
+                                                  c["hash_chunks_log2"] if f == GL else -1, c["fused_levels"])
+                   for (f, p2w), (c, _) in splits.items())
+    lines.append("#define ZKG_TUNED_SPLIT_TABLE {%s}" % sp)
     with open(out_path, "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -159,18 +208,24 @@ def _device_name():
 
 def main():
     ap = argparse.ArgumentParser()
-    g = ap.add_mutually_exclusive_group(required=True)
-    g.add_argument("--build", action="store_true")
-    g.add_argument("--measure", action="store_true")
-    g.add_argument("--time", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--out", default=TUNED_H, help="where --measure writes the header")
+    ap.add_argument("--measure", action="store_true", required=True)
+    ap.add_argument("--out", default=TUNED_H, help="where the header is written")
+    ap.add_argument("--skip-splits", action="store_true")
     args = ap.parse_args()
-    if args.build:
-        build_candidates()
-    elif args.measure:
-        measure(args.out)
-    else:
-        time_shapes()
+    sys.path.insert(0, os.path.dirname(HERE))
+    import ziros_amd as Z
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    ctx = Z.Context(0)
+    for field, width in ((Z.BABYBEAR, 16), (Z.GOLDILOCKS, 16), (Z.GOLDILOCKS, 8)):
+        ctx.load_constants(field, width, 42)
+    plans = measure_plans(ctx, log)
+    splits = {} if args.skip_splits else measure_splits(ctx, log)
+    ctx.close()
+    write_header(plans, splits, args.out)
+    log("wrote %s" % args.out)
 
 
 if __name__ == "__main__":
