@@ -537,3 +537,26 @@ def test_deep_goldilocks_rejects_point_on_domain(ctx):
     x = 7 * pow(int(O.two_adic_generator(GL, 5)), 3, O.GL_P) % O.GL_P
     with pytest.raises(Z.ZkgError):
         Z.HipDeep(ctx, GL, buf, 32, 2, 1, 7).open_values(x)
+
+
+def test_batch_replays_interleaved_with_eager_work(ctx):
+    """The batch graph's persistent row-hash launch resets its own tile counter;
+    replays separated by other eager launches (single commits, a FRI commit
+    phase, permutations) and synchronisations must each give the oracle's roots."""
+    specs = [(BB, 16, 12, 40, 1), (GL, 16, 11, 9, 2), (BB, 16, 10, 7, 1), (GL, 8, 12, 16, 1)]
+    jobs, keep, expect = _jobs(ctx, specs)
+    other = rnd(BB, (1 << 11) * 24, 3).reshape(1 << 11, 24)
+    obuf = ctx.upload(other)
+    cw = ctx.upload(rnd(BB, (1 << 12) * 4, 4).reshape(-1, 4))
+    betas = O.fri_betas(1, 6).astype(np.uint32)
+    root_other = O.commit(O.Poseidon2(BB, 16), other, 1)[0]
+    for rep in range(5):
+        trees = Z.commit_batch(ctx, jobs)
+        for k, (t, e) in enumerate(zip(trees, expect)):
+            assert (t.root().astype(np.uint64) == e).all(), (rep, k)
+            t.free()
+        tree, _ = Z.commit_trace(ctx, BB, obuf, 1 << 11, 24, 1)
+        assert (tree.root().astype(np.uint64) == root_other).all()
+        tree.free()
+        Z.HipFri(ctx).commit_phase(cw, 12, 6, betas).free()
+        ctx.synchronize()
