@@ -52,7 +52,7 @@ def test_wildcards_match_any_shape():
 
 def test_bad_entries_are_refused():
     with pytest.raises(ZkgError):
-        H.set_lde_plans(H.TUNE_OVERRIDE, [(BB, 20, 6, 1, 4)])
+        H.set_lde_plans(H.TUNE_OVERRIDE, [(BB, 20, 6, 1, len(H.lde_plans()))])
     with pytest.raises(ZkgError):
         H.set_lde_plans(7, [])
     with pytest.raises(ZkgError):

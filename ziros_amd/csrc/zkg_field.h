@@ -41,6 +41,105 @@ __host__ __device__ inline uint64_t sub_b(uint64_t a, uint64_t b, uint32_t &bw)
     return gl_join(lo, hi);
 }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// Lazily reduced Goldilocks products on gfx950 that keep their carries in SGPR
+// lane masks: the carry-out of v_mad_u64_u32 and of the add/sub chains.  A
+// product (also one with an addend) takes 16 VALU instructions; the same
+// arithmetic written in C compiles to 22-24, with 64-bit compares standing in
+// for the carries and moves building the aligned register pairs of the 64-bit
+// addends.
+// Every asm block that reads an SGPR written by a VALU instruction waits two
+// cycles first (s_nop 1), the wait gfx950 needs between the two.
+namespace gl_asm {
+
+// a0 b1 + a1 b0 = x + 2^64 cx (cx as 0 / 1)
+__device__ __forceinline__ uint64_t cross(uint32_t a0, uint32_t a1, uint32_t b0, uint32_t b1, uint32_t &cx)
+{
+    uint64_t x, m;
+    asm("v_mad_u64_u32 %0, %1, %2, %3, 0\n\t"
+        "v_mad_u64_u32 %0, %1, %4, %5, %0"
+        : "=&v"(x), "=&s"(m)
+        : "v"(a0), "v"(b1), "v"(a1), "v"(b0));
+    asm("s_nop 1\n\tv_cndmask_b32_e64 %0, 0, 1, %1" : "=v"(cx) : "s"(m));
+    return x;
+}
+
+// lo + 2^64 h0 + 2^96 h1 == lo + h0 (2^32 - 1) - h1 (mod p), as a u64
+// representative.  d = lo - h1 with borrow b, r = h0 (2^32 - 1) + d with carry
+// c (2^64 == 2^32 - 1), result r + (c - b)(2^32 - 1).  With c alone r is below
+// (2^32 - 1)^2, so adding 2^32 - 1 cannot wrap; with b alone r >= d > 2^64 -
+// 2^32, so subtracting it cannot borrow; c with b cancel.  The carry is set
+// for about half the lanes and is always applied; the borrow needs lo < h1 <
+// 2^32 and almost never occurs, so its correction runs only when some lane of
+// the wave has it (a scalar branch on the wave's mask).
+__device__ __forceinline__ uint64_t reduce(uint64_t lo, uint32_t h0, uint32_t h1)
+{
+    uint32_t d0, d1;
+    uint64_t b;
+    asm("v_sub_co_u32_e64 %0, %2, %3, %5\n\t"
+        "s_nop 1\n\t"
+        "v_subb_co_u32_e64 %1, %2, %4, 0, %2"
+        : "=&v"(d0), "=&v"(d1), "=&s"(b)
+        : "v"((uint32_t)lo), "v"((uint32_t)(lo >> 32)), "v"(h1));
+    uint64_t r, c;
+    asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=&v"(r), "=&s"(c) : "v"(h0), "v"(((uint64_t)d1 << 32) | d0));
+    uint32_t r0 = (uint32_t)r, r1 = (uint32_t)(r >> 32), t0, t1;
+    uint64_t k;
+    asm("s_nop 1\n\t"
+        "v_cndmask_b32_e64 %2, 0, -1, %6\n\t"  // c: + (2^32 - 1)
+        "v_add_co_u32_e64 %0, %4, %0, %2\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32_e64 %1, %4, %1, 0, %4\n\t"
+        "s_cmp_lg_u64 %5, 0\n\t"
+        "s_cbranch_scc0 .Lzkg_gl_reduce%=\n\t"
+        "v_cndmask_b32_e64 %2, 0, 1, %5\n\t"  // b: + 2^64 - (2^32 - 1)
+        "v_cndmask_b32_e64 %3, 0, -1, %5\n\t"
+        "v_add_co_u32_e64 %0, %4, %0, %2\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32_e64 %1, %4, %1, %3, %4\n"
+        ".Lzkg_gl_reduce%=:"
+        : "+v"(r0), "+v"(r1), "=&v"(t0), "=&v"(t1), "=&s"(k)
+        : "s"(b), "s"(c)
+        : "scc");
+    return ((uint64_t)r1 << 32) | r0;
+}
+
+// a b for any u64 representatives.  a b = P + 2^32 (x + 2^64 cx) + 2^64 H with
+// P = a0 b0, H = a1 b1; the 128-bit value is lo + 2^64 (h0 + 2^32 h1).
+__device__ __forceinline__ uint64_t mul(uint64_t a, uint64_t b)
+{
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+    uint32_t cx, c1, c2, c3;
+    const uint64_t x = cross(a0, a1, b0, b1, cx);
+    const uint64_t p = (uint64_t)a0 * b0, h = (uint64_t)a1 * b1;
+    const uint32_t lo1 = __builtin_addc((uint32_t)(p >> 32), (uint32_t)x, 0u, &c1);
+    const uint32_t h0 = __builtin_addc((uint32_t)h, (uint32_t)(x >> 32), c1, &c2);
+    const uint32_t h1 = __builtin_addc((uint32_t)(h >> 32), cx, c2, &c3);  // the true high word fits: c3 = 0
+    return reduce(((uint64_t)lo1 << 32) | (uint32_t)p, h0, h1);
+}
+
+// a b + k for any u64 representatives: the halves of k ride in the addends of
+// a0 b0 + k0 and a0 b1 + k1, neither of which can pass 2^64 - 1.
+__device__ __forceinline__ uint64_t mul_add(uint64_t a, uint64_t b, uint64_t k)
+{
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+    uint32_t cx, c1, c2, c3;
+    uint64_t x, m;
+    asm("v_mad_u64_u32 %0, %1, %2, %3, %6\n\t"
+        "v_mad_u64_u32 %0, %1, %4, %5, %0"
+        : "=&v"(x), "=&s"(m)
+        : "v"(a0), "v"(b1), "v"(a1), "v"(b0), "v"((uint64_t)(uint32_t)(k >> 32)));
+    asm("s_nop 1\n\tv_cndmask_b32_e64 %0, 0, 1, %1" : "=v"(cx) : "s"(m));
+    const uint64_t p = (uint64_t)a0 * b0 + (uint32_t)k, h = (uint64_t)a1 * b1;
+    const uint32_t lo1 = __builtin_addc((uint32_t)(p >> 32), (uint32_t)x, 0u, &c1);
+    const uint32_t h0 = __builtin_addc((uint32_t)h, (uint32_t)(x >> 32), c1, &c2);
+    const uint32_t h1 = __builtin_addc((uint32_t)(h >> 32), cx, c2, &c3);
+    return reduce(((uint64_t)lo1 << 32) | (uint32_t)p, h0, h1);
+}
+
+}  // namespace gl_asm
+#endif
+
 struct Gl {
     using T = uint64_t;
     static constexpr uint64_t P = 0xFFFFFFFF00000001ULL;
@@ -123,25 +222,37 @@ struct Gl {
     // a^2 for any u64 representative, result a u64 representative
     __host__ __device__ static inline T sqr_lazy(T a)
     {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return gl_asm::mul(a, a);
+#else
         uint64_t lo, hi;
         sqr_wide(a, lo, hi);
         return reduce128_lazy(lo, hi);
+#endif
     }
     // a b for any u64 representatives, result a u64 representative
     __host__ __device__ static inline T mul_lazy(T a, T b)
     {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return gl_asm::mul(a, b);
+#else
         uint64_t lo, hi;
         mul_wide(a, b, lo, hi);
         return reduce128_lazy(lo, hi);
+#endif
     }
     // a b + k (k any u64): the 128-bit product plus k cannot pass 2^128 - 1
     __host__ __device__ static inline T mul_add_lazy(T a, T b, T k)
     {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return gl_asm::mul_add(a, b, k);
+#else
         uint64_t lo, hi;
         mul_wide(a, b, lo, hi);
         uint32_t c;
         lo = add_c(lo, k, c);
         return reduce128_lazy(lo, hi + c);
+#endif
     }
     // a + k for any u64 a and canonical k: a wrap leaves a + k - 2^64 < k, so
     // adding 2^64 mod p = EPS once stays below 2^64.

@@ -124,11 +124,13 @@ __host__ __device__ __forceinline__ void mds_light_gl(uint64_t (&s)[W])
 #pragma unroll
     for (int c = 0; c < W; c += 4) {
         const uint64_t a = s[c], b = s[c + 1], cc = s[c + 2], d = s[c + 3];
-        const GlWide t = glw_add(glw_add(glw(a), b), glw_add(glw(cc), d));
-        y[c] = glw_add(glw_add(t, a), glw_dbl(b));        // 2a + 3b + c + d
-        y[c + 1] = glw_add(glw_add(t, b), glw_dbl(cc));   // a + 2b + 3c + d
-        y[c + 2] = glw_add(glw_add(t, cc), glw_dbl(d));   // a + b + 2c + 3d
-        y[c + 3] = glw_add(glw_add(t, d), glw_dbl(a));    // 3a + b + c + 2d
+        const GlWide ab = glw_add(glw(a), b), cd = glw_add(glw(cc), d);
+        const GlWide t = glw_add(ab, cd);
+        const GlWide tb = glw_add(t, b), td = glw_add(t, d);
+        y[c] = glw_add(tb, ab);                 // 2a + 3b + c + d
+        y[c + 1] = glw_add(tb, glw_dbl(cc));    // a + 2b + 3c + d
+        y[c + 2] = glw_add(td, cd);             // a + b + 2c + 3d
+        y[c + 3] = glw_add(td, glw_dbl(a));     // 3a + b + c + 2d
     }
     GlWide sum[4];
 #pragma unroll
@@ -234,11 +236,13 @@ __host__ __device__ __forceinline__ void mds_light_bf(double (&s)[16])
 #pragma unroll
     for (int c = 0; c < 16; c += 4) {
         double a = s[c], b = s[c + 1], cc = s[c + 2], d = s[c + 3];
-        double quad = (a + b) + (cc + d);
-        s[c] = __builtin_fma(2.0, b, quad + a);        // 2a + 3b + c + d
-        s[c + 1] = __builtin_fma(2.0, cc, quad + b);   // a + 2b + 3c + d
-        s[c + 2] = __builtin_fma(2.0, d, quad + cc);   // a + b + 2c + 3d
-        s[c + 3] = __builtin_fma(2.0, a, quad + d);    // 3a + b + c + 2d
+        double ab = a + b, cd = cc + d;
+        double t = ab + cd;
+        double tb = t + b, td = t + d;
+        s[c] = tb + ab;                        // 2a + 3b + c + d
+        s[c + 1] = __builtin_fma(2.0, cc, tb);  // a + 2b + 3c + d
+        s[c + 2] = td + cd;                    // a + b + 2c + 3d
+        s[c + 3] = __builtin_fma(2.0, a, td);   // 3a + b + c + 2d
     }
     double sum[4];
 #pragma unroll
