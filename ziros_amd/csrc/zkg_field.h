@@ -44,10 +44,13 @@ __host__ __device__ inline uint64_t sub_b(uint64_t a, uint64_t b, uint32_t &bw)
 #if defined(__HIP_DEVICE_COMPILE__)
 // Lazily reduced Goldilocks products on gfx950 that keep their carries in SGPR
 // lane masks: the carry-out of v_mad_u64_u32 and of the add/sub chains.  A
-// product (also one with an addend) takes 16 VALU instructions; the same
-// arithmetic written in C compiles to 22-24, with 64-bit compares standing in
-// for the carries and moves building the aligned register pairs of the 64-bit
-// addends.
+// product (also one with an addend) issues 14 VALU instructions on its common
+// path, none of them a move; the same arithmetic written in C compiles to
+// 22-24, with 64-bit compares standing in for the carries and moves building
+// the aligned register pairs of the 64-bit addends.  (gfx950 pairs moves and
+// plain 32-bit adds in one issue slot, but no instruction that reads or
+// writes a carry, so the saving in issue slots is about a fifth.)  The chains
+// are longer, which costs where few waves hide latency (Gl::*_lean).
 // Every asm block that reads an SGPR written by a VALU instruction waits two
 // cycles first (s_nop 1), the wait gfx950 needs between the two.
 namespace gl_asm {
@@ -222,36 +225,52 @@ struct Gl {
     // a^2 for any u64 representative, result a u64 representative
     __host__ __device__ static inline T sqr_lazy(T a)
     {
-#if defined(__HIP_DEVICE_COMPILE__)
-        return gl_asm::mul(a, a);
-#else
         uint64_t lo, hi;
         sqr_wide(a, lo, hi);
         return reduce128_lazy(lo, hi);
-#endif
     }
     // a b for any u64 representatives, result a u64 representative
     __host__ __device__ static inline T mul_lazy(T a, T b)
     {
-#if defined(__HIP_DEVICE_COMPILE__)
-        return gl_asm::mul(a, b);
-#else
         uint64_t lo, hi;
         mul_wide(a, b, lo, hi);
         return reduce128_lazy(lo, hi);
-#endif
     }
     // a b + k (k any u64): the 128-bit product plus k cannot pass 2^128 - 1
     __host__ __device__ static inline T mul_add_lazy(T a, T b, T k)
     {
-#if defined(__HIP_DEVICE_COMPILE__)
-        return gl_asm::mul_add(a, b, k);
-#else
         uint64_t lo, hi;
         mul_wide(a, b, lo, hi);
         uint32_t c;
         lo = add_c(lo, k, c);
         return reduce128_lazy(lo, hi + c);
+    }
+    // The same three products in the fewest issue slots (gl_asm on the
+    // device): for kernels with enough waves to hide their longer dependency
+    // chains (one state per lane over many rows).  Kernels bound by latency
+    // (a state spread over lanes, short tree levels) keep the forms above.
+    __host__ __device__ static inline T sqr_lean(T a)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return gl_asm::mul(a, a);
+#else
+        return sqr_lazy(a);
+#endif
+    }
+    __host__ __device__ static inline T mul_lean(T a, T b)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return gl_asm::mul(a, b);
+#else
+        return mul_lazy(a, b);
+#endif
+    }
+    __host__ __device__ static inline T mul_add_lean(T a, T b, T k)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return gl_asm::mul_add(a, b, k);
+#else
+        return mul_add_lazy(a, b, k);
 #endif
     }
     // a + k for any u64 a and canonical k: a wrap leaves a + k - 2^64 < k, so

@@ -517,7 +517,7 @@ __global__ __launch_bounds__(HASH_THREADS) void k_merkle_level(const typename F:
     T s[W];
 #pragma unroll
     for (int e = 0; e < W; e++) s[e] = F::from_canon(in[2 * i * D + e]);
-    p2_permute<F, W>(s, rc, diag);
+    p2_permute<F, W, false>(s, rc, diag);  // latency-bound short levels: the compiler's products
 #pragma unroll
     for (int e = 0; e < D; e++) out[i * D + e] = F::to_canon(s[e]);
 }

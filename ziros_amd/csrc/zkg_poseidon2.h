@@ -143,16 +143,26 @@ __host__ __device__ __forceinline__ void mds_light_gl(uint64_t (&s)[W])
     for (int i = 0; i < W; i++) s[i] = glw_fold(glw_add(y[i], sum[i & 3]));
 }
 
+// LEAN picks the products with the fewest issue slots (Gl::*_lean), for
+// kernels with enough waves to hide their latency; the others use Gl::*_lazy.
+template <bool LEAN>
 __host__ __device__ __forceinline__ uint64_t sbox7_gl(uint64_t x)
 {
-    const uint64_t x2 = Gl::sqr_lazy(x);
-    const uint64_t x3 = Gl::mul_lazy(x2, x);
-    const uint64_t x4 = Gl::sqr_lazy(x2);
-    return Gl::mul_lazy(x3, x4);
+    if constexpr (LEAN) {
+        const uint64_t x2 = Gl::sqr_lean(x);
+        const uint64_t x3 = Gl::mul_lean(x2, x);
+        const uint64_t x4 = Gl::sqr_lean(x2);
+        return Gl::mul_lean(x3, x4);
+    } else {
+        const uint64_t x2 = Gl::sqr_lazy(x);
+        const uint64_t x3 = Gl::mul_lazy(x2, x);
+        const uint64_t x4 = Gl::sqr_lazy(x2);
+        return Gl::mul_lazy(x3, x4);
+    }
 }
 
 // rc: flat [initial 4W][internal RP][terminal 4W] and diag[W], canonical.
-template <int W>
+template <int W, bool LEAN>
 __host__ __device__ __forceinline__ void p2_permute_gl(uint64_t (&s)[W], const uint64_t *__restrict__ rc,
                                                        const uint64_t *__restrict__ diag)
 {
@@ -161,38 +171,38 @@ __host__ __device__ __forceinline__ void p2_permute_gl(uint64_t (&s)[W], const u
 #pragma unroll 1
     for (int r = 0; r < 4; r++) {
 #pragma unroll
-        for (int i = 0; i < W; i++) s[i] = sbox7_gl(Gl::add_const_lazy(s[i], rc[r * W + i]));
+        for (int i = 0; i < W; i++) s[i] = sbox7_gl<LEAN>(Gl::add_const_lazy(s[i], rc[r * W + i]));
         mds_light_gl<W>(s);
     }
     const uint64_t *ir = rc + 4 * W;
 #pragma unroll 1
     for (int r = 0; r < RP; r++) {
-        s[0] = sbox7_gl(Gl::add_const_lazy(s[0], ir[r]));
+        s[0] = sbox7_gl<LEAN>(Gl::add_const_lazy(s[0], ir[r]));
         GlWide acc = glw(s[0]);
 #pragma unroll
         for (int i = 1; i < W; i++) acc = glw_add(acc, s[i]);
         const uint64_t sum = glw_fold(acc);
 #pragma unroll
-        for (int i = 0; i < W; i++) s[i] = Gl::mul_add_lazy(s[i], diag[i], sum);
+        for (int i = 0; i < W; i++) s[i] = LEAN ? Gl::mul_add_lean(s[i], diag[i], sum) : Gl::mul_add_lazy(s[i], diag[i], sum);
     }
     const uint64_t *tr = ir + RP;
 #pragma unroll 1
     for (int r = 0; r < 4; r++) {
 #pragma unroll
-        for (int i = 0; i < W; i++) s[i] = sbox7_gl(Gl::add_const_lazy(s[i], tr[r * W + i]));
+        for (int i = 0; i < W; i++) s[i] = sbox7_gl<LEAN>(Gl::add_const_lazy(s[i], tr[r * W + i]));
         mds_light_gl<W>(s);
     }
 #pragma unroll
     for (int i = 0; i < W; i++) s[i] = Gl::canon(s[i]);
 }
 
-template <class F, int W>
+template <class F, int W, bool LEAN = true>
 __host__ __device__ __forceinline__ void p2_permute(typename F::T (&s)[W], const typename F::T *__restrict__ rc,
                                            const typename F::T *__restrict__ diag)
 {
     if constexpr (std::is_same<F, Gl>::value) {
         static_assert(P2Spec<Gl, W>::RF == 8 && P2Spec<Gl, W>::RP == 22, "Goldilocks schedule");
-        p2_permute_gl<W>(s, rc, diag);
+        p2_permute_gl<W, LEAN>(s, rc, diag);
         return;
     }
     using T = typename F::T;

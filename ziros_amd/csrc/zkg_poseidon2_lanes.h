@@ -207,7 +207,7 @@ __device__ __forceinline__ uint64_t p2_permute_lanes_gl(uint64_t x, int e, const
     constexpr int RP = P2Spec<Gl, W>::RP;
     x = mds_light_lanes_gl<W>(x);
 #pragma unroll
-    for (int r = 0; r < 4; r++) x = mds_light_lanes_gl<W>(sbox7_gl(Gl::add_const_lazy(x, k.ext[r])));
+    for (int r = 0; r < 4; r++) x = mds_light_lanes_gl<W>(sbox7_gl<false>(Gl::add_const_lazy(x, k.ext[r])));
     // ir[RP] is the first terminal constant, so the look-ahead stays in bounds
     const uint64_t *ir = rc + 4 * W;
     uint64_t c = ir[0];
@@ -226,7 +226,7 @@ __device__ __forceinline__ uint64_t p2_permute_lanes_gl(uint64_t x, int e, const
         c = next;
     }
 #pragma unroll
-    for (int r = 0; r < 4; r++) x = mds_light_lanes_gl<W>(sbox7_gl(Gl::add_const_lazy(x, k.ext[4 + r])));
+    for (int r = 0; r < 4; r++) x = mds_light_lanes_gl<W>(sbox7_gl<false>(Gl::add_const_lazy(x, k.ext[4 + r])));
     return Gl::canon(x);
 }
 
@@ -316,7 +316,7 @@ __device__ __forceinline__ void gl_chunk_external(uint64_t (&s)[4], const uint64
 #pragma unroll
         for (int k = 0; k < 4; k++) cn[k] = er[rn * W + k];
 #pragma unroll
-        for (int k = 0; k < 4; k++) s[k] = sbox7_gl(Gl::add_const_lazy(s[k], c[k]));
+        for (int k = 0; k < 4; k++) s[k] = sbox7_gl<false>(Gl::add_const_lazy(s[k], c[k]));
         mds_light_gl_chunk<W>(s);
 #pragma unroll
         for (int k = 0; k < 4; k++) c[k] = cn[k];
