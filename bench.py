@@ -175,6 +175,34 @@ def bench_c1(ctx, steps):
     return out
 
 
+def bench_gl_row_hash(ctx, steps):
+    """The Goldilocks row hash of one fused commit: 2^20 x 97 trace, blowup 2,
+    so 2^21 LDE rows of 13 permutations each, Poseidon2 width 16 and width 8
+    (the served mix's Goldilocks traces run the same kernel).  Device time of
+    the row-hash stage from the library's HIP events, median over `steps`."""
+    n, w, lb = 1 << 20, 97, 1
+    trace = ctx.random(Z.GOLDILOCKS, n * w, 0x5EED0097)
+    lde = ctx.alloc((n << lb) * w * 8)
+    out = {}
+    ctx.set_timing(True)
+    try:
+        for p2w in (16, 8):
+            rate = 8 if p2w == 16 else 4
+            perms = (n << lb) * -(-w // rate)
+            ms = []
+            for i in range(steps + 1):
+                free_tree(ctx, commit_once(ctx, ctx.lib.zkg_commit, Z.GOLDILOCKS, p2w, trace, n, w, lb, lde))
+                if i:
+                    ms.append(ctx.stage_ms()[1])
+            med = sorted(ms)[len(ms) // 2]
+            out["poseidon2_w%d" % p2w] = dict(row_hash_ms=med, permutations=perms, permutations_per_s=perms / (med * 1e-3))
+    finally:
+        ctx.set_timing(False)
+        trace.free()
+        lde.free()
+    return out
+
+
 def bench_c4(ctx, steps):
     """Config 4: FRI commit phase over EF4, codeword 2^22 (a 2^20-coefficient
     polynomial at blowup 4) folded to 2^8, one Poseidon2-16 tree per layer."""
@@ -596,6 +624,8 @@ def main():
                     help="run only the constraint-evaluation leg (for profiling) and print its JSON")
     ap.add_argument("--only-c1", action="store_true",
                     help="run only the Goldilocks 2^14 x 8 single-commit leg (for profiling) and print its JSON")
+    ap.add_argument("--only-gl-hash", action="store_true",
+                    help="run only the Goldilocks 2^21-row x 97 row-hash leg (for profiling) and print its JSON")
     ap.add_argument("--only-c2", action="store_true",
                     help="run only the BabyBear 2^20 x 64 single-commit leg (for profiling) and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=48)
@@ -620,6 +650,10 @@ def main():
     peak_bb = max(peak_bb, peak_bbf)
     if args.only_constraints:
         print(json.dumps({"constraint_eval": bench_constraints(ctx, max(2, args.steps), peak_gl)}))
+        ctx.close()
+        return
+    if args.only_gl_hash:
+        print(json.dumps({"gl_row_hash": bench_gl_row_hash(ctx, max(3, args.steps))}))
         ctx.close()
         return
     if args.only_c1:
@@ -685,6 +719,8 @@ def main():
         log("c1: goldilocks 2^14 x 8")
         out["c1"] = bench_c1(ctx, max(5, args.steps))
         log("c1: %s" % out["c1"])
+        out["gl_row_hash"] = bench_gl_row_hash(ctx, 5)
+        log("gl_row_hash: %s" % out["gl_row_hash"])
         log("c4: fri commit phase 2^22 -> 2^8")
         out["c4_fri"] = bench_c4(ctx, max(3, args.steps // 2))
         log("c4: %s" % out["c4_fri"])
