@@ -626,6 +626,8 @@ def main():
                     help="run only the Goldilocks 2^14 x 8 single-commit leg (for profiling) and print its JSON")
     ap.add_argument("--only-gl-hash", action="store_true",
                     help="run only the Goldilocks 2^21-row x 97 row-hash leg (for profiling) and print its JSON")
+    ap.add_argument("--only-c4", action="store_true",
+                    help="run only the FRI commit-phase leg (for profiling) and print its JSON")
     ap.add_argument("--only-c2", action="store_true",
                     help="run only the BabyBear 2^20 x 64 single-commit leg (for profiling) and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=48)
@@ -654,6 +656,10 @@ def main():
         return
     if args.only_gl_hash:
         print(json.dumps({"gl_row_hash": bench_gl_row_hash(ctx, max(3, args.steps))}))
+        ctx.close()
+        return
+    if args.only_c4:
+        print(json.dumps({"c4_fri": bench_c4(ctx, args.steps)}))
         ctx.close()
         return
     if args.only_c1:
