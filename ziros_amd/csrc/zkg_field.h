@@ -281,6 +281,23 @@ struct Gl {
         return s + (s < k ? EPS : 0);
     }
     __host__ __device__ static inline T sqr(T a) { return mul(a, a); }
+    // x 2^k for canonical x and 0 <= k < 96, canonical result (2^96 = -1
+    // covers the other exponents by a sign).  The small roots of unity are
+    // powers of two (omega_4 = 2^48, omega_8 = 2^120, omega_16 = 2^156 for
+    // the generator powers the transforms use), so a butterfly's twiddle inside
+    // a register sub-DFT is a shift and one reduction instead of a product.
+    // k < 64: x 2^k = (x << k) + 2^64 (x >> (64 - k)), one 128-bit reduction.
+    // k >= 64: with y = x 2^(k-64) = y0 + 2^32 Y (Y < 2^(32+k-64) < p),
+    // x 2^k = 2^64 y0 + 2^96 Y == y0 (2^32 - 1) - Y, and y0 (2^32 - 1) < p.
+    // k is a constant wherever the callers' loops unroll.
+    __host__ __device__ static inline T mul_pow2(T x, int k)
+    {
+        if (k == 0) return x;
+        if (k < 64) return reduce128(x << k, x >> (64 - k));
+        const int j = k - 64;
+        const uint64_t a = (uint64_t)(uint32_t)(x << j) * EPS;
+        return sub(a, x >> (32 - j));
+    }
     // HBM <-> register representation (identity for Goldilocks)
     __host__ __device__ static inline T from_canon(T a) { return a; }
     __host__ __device__ static inline T to_canon(T a) { return a; }

@@ -166,12 +166,33 @@ constexpr int vpad_len()
     return M + (M >> 5) + 1;
 }
 
+// Where the twiddles inside a register sub-DFT come from: the tile table
+// passed in (TW_TABLE), or, for Goldilocks, the powers of two that the roots of
+// order <= 16 are (TW_FWD / TW_INV: the forward or the inverse transform).
+enum { TW_TABLE = 0, TW_FWD = 1, TW_INV = 2 };
+template <class F>
+__host__ __device__ constexpr int tw_mode(bool inverse)
+{
+    return sizeof(typename F::T) == 8 ? (inverse ? TW_INV : TW_FWD) : TW_TABLE;
+}
+// omega_2048^e for e a multiple of 128 (a root of order <= 16) as 2^k, k < 192:
+// p3's omega_16 is 2^156 (its inverse 2^36), so k = 156 e/128 or 36 e/128 mod 192.
+template <int DIR>
+__device__ constexpr int gl_root_log2(int e)
+{
+    return ((DIR == TW_FWD ? 156 : 36) * (e / 128)) % 192;
+}
+
 // R-point DFT in registers: radix-2 DIF, natural order in, x[p] = X[rev(p)]
-// out.  omega_{2h}^t = omega_2048^(t*1024/h) = tw[t * (1024/h)].
-template <class F, int R>
+// out.  omega_{2h}^t = omega_2048^(t*1024/h) = tw[t * (1024/h)]; with a
+// power-of-two mode the product by 2^k is Gl::mul_pow2, and 2^96 = -1 turns
+// the exponents k >= 96 into a swapped difference.
+template <class F, int R, int DIR>
 __device__ __forceinline__ void reg_dft(typename F::T (&x)[R], const typename F::T *__restrict__ tw)
 {
     using T = typename F::T;
+    constexpr bool POW2 = DIR != TW_TABLE;
+    static_assert(!POW2 || (sizeof(T) == 8 && R <= 16), "power-of-two twiddles: Goldilocks roots of order <= 16");
 #pragma unroll
     for (int h = R / 2; h >= 1; h >>= 1) {
 #pragma unroll
@@ -180,7 +201,12 @@ __device__ __forceinline__ void reg_dft(typename F::T (&x)[R], const typename F:
                 T a = x[i], b = x[i + h];
                 x[i] = F::add(a, b);
                 const int t = i & (h - 1);
-                x[i + h] = t == 0 ? F::sub(a, b) : F::mul(F::sub_for_mul(a, b), tw[t * (1024 / h)]);
+                if constexpr (POW2) {
+                    const int k = gl_root_log2<DIR>(t * (1024 / h));
+                    x[i + h] = k < 96 ? F::mul_pow2(F::sub(a, b), k) : F::mul_pow2(F::sub(b, a), k - 96);
+                } else {
+                    x[i + h] = t == 0 ? F::sub(a, b) : F::mul(F::sub_for_mul(a, b), tw[t * (1024 / h)]);
+                }
             }
         }
     }
@@ -207,7 +233,7 @@ __device__ constexpr int crev(int q)
 // wt[vpad(row)], from the lane's registers src[j] (SRC_REGS) or from an LDS
 // coefficient array src[row][c] (SRC_COEF).
 enum { SRC_TILE = 0, SRC_REGS = 1, SRC_COEF = 2 };
-template <class F, int L, int I, int SHIFT, int WSRC>
+template <class F, int L, int I, int SHIFT, int DIR, int WSRC>
 __device__ __forceinline__ void tile_round(typename F::T *lds, const typename F::T *__restrict__ tw,
                                            const typename F::T *__restrict__ rtw, const typename F::T *src,
                                            const typename F::T *wt)
@@ -238,7 +264,7 @@ __device__ __forceinline__ void tile_round(typename F::T *lds, const typename F:
             else
                 x[j] = lds[P::slot(row, c)];
         }
-        reg_dft<F, R>(x, tw);
+        reg_dft<F, R, DIR>(x, tw);
         if constexpr (I + 1 < P::NR) {
             const T *r = rtw + rtw_offset(L, I) + a;
 #pragma unroll
@@ -253,20 +279,20 @@ __device__ __forceinline__ void tile_round(typename F::T *lds, const typename F:
 // weighted inputs of round 0, see tile_round; with SRC_REGS lane input j is
 // row tid / C + j 2^LOG_TC), row `pos` holds transform index
 // Plan::index_of(pos) out.  Ends with a barrier.
-template <class F, int L, int WSRC = SRC_TILE>
+template <class F, int L, int DIR, int WSRC = SRC_TILE>
 __device__ __forceinline__ void tile_dft(typename F::T *lds, const typename F::T *__restrict__ tw,
                                          const typename F::T *__restrict__ rtw, const typename F::T *src = nullptr,
                                          const typename F::T *wt = nullptr)
 {
     using P = Plan<F, L>;
-    tile_round<F, L, 0, 0, WSRC>(lds, tw, rtw, src, wt);
+    tile_round<F, L, 0, 0, DIR, WSRC>(lds, tw, rtw, src, wt);
     __syncthreads();
     if constexpr (P::NR > 1) {
-        tile_round<F, L, 1, P::logr(0), SRC_TILE>(lds, tw, rtw, nullptr, nullptr);
+        tile_round<F, L, 1, P::logr(0), DIR, SRC_TILE>(lds, tw, rtw, nullptr, nullptr);
         __syncthreads();
     }
     if constexpr (P::NR > 2) {
-        tile_round<F, L, 2, P::logr(0) + P::logr(1), SRC_TILE>(lds, tw, rtw, nullptr, nullptr);
+        tile_round<F, L, 2, P::logr(0) + P::logr(1), DIR, SRC_TILE>(lds, tw, rtw, nullptr, nullptr);
         __syncthreads();
     }
 }
@@ -319,7 +345,7 @@ __device__ __forceinline__ void tile_load(typename F::T *lds, const typename F::
 // Four-step pass 1 (also LDE pass A): group n2 reads rows n2 + N2 n1,
 // transforms over n1, multiplies by omega_N^(+-n2 k1) and stores element k1 at
 // row n2 + N2 k1.
-template <class F, int L>
+template <class F, int L, int DIR>
 __global__ __launch_bounds__(NTT_THREADS) void k_fs_pass1(const typename F::T *__restrict__ src,
                                                           typename F::T *__restrict__ dst, Geo g,
                                                           const typename F::T *__restrict__ tile_tw,
@@ -338,7 +364,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_fs_pass1(const typename F::T *_
         wk[vpad(k)] = top_pow<F>(top_lo, top_hi, (uint64_t)n2 * k, g.logN);
     tile_load<F, L>(lds, src, n2, 1ull << g.logN2, g.width, c0);
     __syncthreads();
-    tile_dft<F, L>(lds, tile_tw, round_tw);
+    tile_dft<F, L, DIR>(lds, tile_tw, round_tw);
     const int c = threadIdx.x & (P::C - 1);
     if (c0 + c >= g.width) return;
     T *d = dst + (uint64_t)n2 * g.width + c0 + c;
@@ -353,7 +379,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_fs_pass1(const typename F::T *_
 // Plain four-step pass 2: group k1 reads contiguous rows N2 k1 + n2,
 // transforms over n2 and stores element k2 at row k1 + N1 k2, times `scale`
 // (1/N for the inverse).
-template <class F, int L>
+template <class F, int L, int DIR>
 __global__ __launch_bounds__(NTT_THREADS) void k_fs_pass2(const typename F::T *__restrict__ src,
                                                           typename F::T *__restrict__ dst, Geo g,
                                                           typename F::T scale,
@@ -368,7 +394,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_fs_pass2(const typename F::T *_
     const uint32_t c0 = (tile % g.nchunks) << P::LOG_C;
     tile_load<F, L>(lds, src, (uint64_t)k1 << g.logN2, 1, g.width, c0);
     __syncthreads();
-    tile_dft<F, L>(lds, tile_tw, round_tw);
+    tile_dft<F, L, DIR>(lds, tile_tw, round_tw);
     const int c = threadIdx.x & (P::C - 1);
     if (c0 + c >= g.width) return;
     T *d = dst + (uint64_t)k1 * g.width + c0 + c;
@@ -380,7 +406,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_fs_pass2(const typename F::T *_
 }
 
 // Single-pass transform (N <= 2^11): natural in/out.
-template <class F, int L>
+template <class F, int L, int DIR>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_small(typename F::T *__restrict__ data, Geo g,
                                                            typename F::T scale,
                                                            const typename F::T *__restrict__ tile_tw,
@@ -392,7 +418,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_small(typename F::T *__rest
     const uint32_t c0 = xcd_tile(blockIdx.x, gridDim.x) << P::LOG_C;
     tile_load<F, L>(lds, data, 0, 1, g.width, c0);
     __syncthreads();
-    tile_dft<F, L>(lds, tile_tw, round_tw);
+    tile_dft<F, L, DIR>(lds, tile_tw, round_tw);
     const int c = threadIdx.x & (P::C - 1);
     if (c0 + c >= g.width) return;
     T *d = data + c0 + c;
@@ -484,7 +510,7 @@ __global__ __launch_bounds__(NTT_THREADS) ZKG_LDE_MID_OCC void k_lde_mid(const t
         wj[vpad(j)] = top_pow<F>(top_lo, top_hi, (uint64_t)j * k1, g.logN);
     tile_load<F, L>(lds, src, (uint64_t)k1 << g.logN2, 1, g.width, c0);
     __syncthreads();
-    tile_dft<F, L>(lds, tile_inv, round_inv);
+    tile_dft<F, L, tw_mode<F>(true)>(lds, tile_inv, round_inv);
     const int c = threadIdx.x & (P::C - 1);
     const bool col_ok = c0 + c < g.width;
     // the coefficients c[k1 + N1 k2] this lane feeds into every coset's first
@@ -501,7 +527,8 @@ __global__ __launch_bounds__(NTT_THREADS) ZKG_LDE_MID_OCC void k_lde_mid(const t
             wb[vpad(k2)] = F::mul(clo[k & ((1u << TW_SPLIT) - 1u)], chi[k >> TW_SPLIT]);
         }
         __syncthreads();  // creg loads (first coset) done, wb complete, previous coset's stores done
-        tile_dft<F, L, CREG ? SRC_REGS : SRC_COEF>(lds, tile_fwd, round_fwd, CREG ? creg : coef, wb);
+        tile_dft<F, L, tw_mode<F>(false), CREG ? SRC_REGS : SRC_COEF>(lds, tile_fwd, round_fwd, CREG ? creg : coef,
+                                                                  wb);
         if (col_ok) {
             T *d = dst + (((uint64_t)b << g.logN) + ((uint64_t)k1 << g.logN2)) * g.width + c0 + c;
             TILE_STORE_LOOP(P, pos)
@@ -534,7 +561,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_final(const typename F::T *
     const uint64_t base = (uint64_t)b << g.logN;
     tile_load<F, L>(lds, src, base + j2, 1ull << g.logN2, g.width, c0);
     __syncthreads();
-    tile_dft<F, L>(lds, tile_fwd, round_fwd);
+    tile_dft<F, L, tw_mode<F>(false)>(lds, tile_fwd, round_fwd);
     const uint64_t obase = base + ((uint64_t)brev(j2, g.logN2) << L);
     // tile row pos holds j1 = index_of(pos), which goes to output row
     // obase + rev_L(j1).  Walking pos in order keeps a wave's LDS reads on
@@ -565,7 +592,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_small(const typename F::T *
     const uint32_t c0 = xcd_tile(blockIdx.x, gridDim.x) << P::LOG_C;
     tile_load<F, L>(lds, in, 0, 1, g.width, c0);
     __syncthreads();
-    tile_dft<F, L>(lds, tile_inv, round_inv);
+    tile_dft<F, L, tw_mode<F>(true)>(lds, tile_inv, round_inv);
     const int c = threadIdx.x & (P::C - 1);
     const bool col_ok = c0 + c < g.width;
     constexpr bool CREG = lde_coef_in_regs<F, L>();
@@ -576,7 +603,8 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_small(const typename F::T *
         const T *clo = cos_lo + (size_t)b * (1u << TW_SPLIT);
         for (int k = threadIdx.x; k < P::M; k += NTT_THREADS) wb[vpad(k)] = clo[k];
         __syncthreads();
-        tile_dft<F, L, CREG ? SRC_REGS : SRC_COEF>(lds, tile_fwd, round_fwd, CREG ? creg : coef, wb);
+        tile_dft<F, L, tw_mode<F>(false), CREG ? SRC_REGS : SRC_COEF>(lds, tile_fwd, round_fwd, CREG ? creg : coef,
+                                                                  wb);
         if (col_ok) {
             T *d = out + ((uint64_t)b << L) * g.width + c0 + c;
             TILE_STORE_LOOP(P, pos)
@@ -755,15 +783,19 @@ hipError_t dispatch_l(int L, A &&...a)
     }
 }
 
+// The plain transform passes run in either direction: `inverse` picks the
+// kernel instance (the same one for BabyBear, whose twiddles all come from the
+// tables passed in).
 template <class F, int L>
 struct RunPass1 {
     static hipError_t run(const void *src, void *dst, Geo g, const void *tile, const void *round, const void *lo,
-                          const void *hi, hipStream_t st)
+                          const void *hi, bool inverse, hipStream_t st)
     {
         using T = typename F::T;
         g.nchunks = chunks<F, L>(g.width);
-        hipLaunchKernelGGL((k_fs_pass1<F, L>), dim3(g.nchunks << g.logN2), dim3(NTT_THREADS), 0, st, (const T *)src,
-                           (T *)dst, g, (const T *)tile, (const T *)round, (const T *)lo, (const T *)hi);
+        auto k = inverse ? k_fs_pass1<F, L, tw_mode<F>(true)> : k_fs_pass1<F, L, tw_mode<F>(false)>;
+        hipLaunchKernelGGL(k, dim3(g.nchunks << g.logN2), dim3(NTT_THREADS), 0, st, (const T *)src, (T *)dst, g,
+                           (const T *)tile, (const T *)round, (const T *)lo, (const T *)hi);
         return hipGetLastError();
     }
 };
@@ -771,24 +803,27 @@ struct RunPass1 {
 template <class F, int L>
 struct RunPass2 {
     static hipError_t run(const void *src, void *dst, Geo g, typename F::T scale, const void *tile, const void *round,
-                          hipStream_t st)
+                          bool inverse, hipStream_t st)
     {
         using T = typename F::T;
         g.nchunks = chunks<F, L>(g.width);
-        hipLaunchKernelGGL((k_fs_pass2<F, L>), dim3(g.nchunks << g.logN1), dim3(NTT_THREADS), 0, st, (const T *)src,
-                           (T *)dst, g, scale, (const T *)tile, (const T *)round);
+        auto k = inverse ? k_fs_pass2<F, L, tw_mode<F>(true)> : k_fs_pass2<F, L, tw_mode<F>(false)>;
+        hipLaunchKernelGGL(k, dim3(g.nchunks << g.logN1), dim3(NTT_THREADS), 0, st, (const T *)src, (T *)dst, g,
+                           scale, (const T *)tile, (const T *)round);
         return hipGetLastError();
     }
 };
 
 template <class F, int L>
 struct RunSmall {
-    static hipError_t run(void *data, Geo g, typename F::T scale, const void *tile, const void *round, hipStream_t st)
+    static hipError_t run(void *data, Geo g, typename F::T scale, const void *tile, const void *round, bool inverse,
+                          hipStream_t st)
     {
         using T = typename F::T;
         g.nchunks = chunks<F, L>(g.width);
-        hipLaunchKernelGGL((k_ntt_small<F, L>), dim3(g.nchunks), dim3(NTT_THREADS), 0, st, (T *)data, g, scale,
-                           (const T *)tile, (const T *)round);
+        auto k = inverse ? k_ntt_small<F, L, tw_mode<F>(true)> : k_ntt_small<F, L, tw_mode<F>(false)>;
+        hipLaunchKernelGGL(k, dim3(g.nchunks), dim3(NTT_THREADS), 0, st, (T *)data, g, scale, (const T *)tile,
+                           (const T *)round);
         return hipGetLastError();
     }
 };
@@ -846,7 +881,7 @@ hipError_t coset_lde_impl(const TwiddleSet &tws, const CosetSet &cs, const void 
     if (g.logN <= LOG_TW_TILE) return dispatch_l<RunLdeSmall, F>(g.logN, in, out, g, t, cs, st);
     // pass A: in -> out[0 .. N*W) (scratch), pass B: -> tmp (B*N*W), pass C: -> out
     hipError_t e = dispatch_l<RunPass1, F>(g.logN1, in, out, g, (const void *)t.tile_inv, (const void *)t.round_inv,
-                                           (const void *)t.lo_inv, (const void *)t.hi_inv, st);
+                                           (const void *)t.lo_inv, (const void *)t.hi_inv, true, st);
     if (e) return e;
     e = dispatch_l<RunMid, F>(g.logN2, (const void *)out, tmp, g, t, cs, st);
     if (e) return e;
@@ -864,12 +899,13 @@ hipError_t ntt_impl(const TwiddleSet &tws, void *data, void *tmp, uint64_t heigh
     T scale = inverse ? host_ninv<F>(height) : F::from_canon(1);
     const void *tile = inverse ? t.tile_inv : t.tile_fwd;
     const void *round = inverse ? t.round_inv : t.round_fwd;
-    if (g.logN <= LOG_TW_TILE) return dispatch_l<RunSmall, F>(g.logN, data, g, scale, tile, round, st);
+    const bool inv = inverse != 0;
+    if (g.logN <= LOG_TW_TILE) return dispatch_l<RunSmall, F>(g.logN, data, g, scale, tile, round, inv, st);
     hipError_t e = dispatch_l<RunPass1, F>(g.logN1, (const void *)data, tmp, g, tile, round,
-                                           (const void *)(inverse ? t.lo_inv : t.lo_fwd),
-                                           (const void *)(inverse ? t.hi_inv : t.hi_fwd), st);
+                                           (const void *)(inv ? t.lo_inv : t.lo_fwd),
+                                           (const void *)(inv ? t.hi_inv : t.hi_fwd), inv, st);
     if (e) return e;
-    return dispatch_l<RunPass2, F>(g.logN2, (const void *)tmp, data, g, scale, tile, round, st);
+    return dispatch_l<RunPass2, F>(g.logN2, (const void *)tmp, data, g, scale, tile, round, inv, st);
 }
 
 template <class F>
