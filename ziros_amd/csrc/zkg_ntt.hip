@@ -226,17 +226,14 @@ __device__ constexpr int crev(int q)
 // One radix round of the tile transform (round I of the plan).  Sub-DFT s of
 // a column covers rows b*Mi + a + Mi1*j (j < R) and writes output q, times
 // omega_Mi^(a q) unless this is the last round, to row b*Mi + q*Mi1 + a: the
-// same rows, so rounds work in place and need no barrier inside.  The
+// same rows, so rounds in LDS work in place and need no barrier inside.  The
 // inter-round twiddles come from the resident table rtw ([q][a] per round).
-// Round inputs (WSRC): SRC_TILE reads the tile; the weighted forms (round 0
-// only, one sub-DFT per lane) read input j, row `row`, times the row weight
-// wt[vpad(row)], from the lane's registers src[j] (SRC_REGS) or from an LDS
-// coefficient array src[row][c] (SRC_COEF).
-enum { SRC_TILE = 0, SRC_REGS = 1, SRC_COEF = 2 };
-template <class F, int L, int I, int SHIFT, int DIR, int WSRC>
-__device__ __forceinline__ void tile_round(typename F::T *lds, const typename F::T *__restrict__ tw,
-                                           const typename F::T *__restrict__ rtw, const typename F::T *src,
-                                           const typename F::T *wt)
+// The round reads input j of sub-DFT u of the lane through src(u, j, row)
+// and hands output q to sink(u, q, row, value); u, j and q are constants once
+// the loops unroll, so a source or sink may index a register array by them.
+template <class F, int L, int I, int SHIFT, int DIR, class Src, class Sink>
+__device__ __forceinline__ void tile_round(const typename F::T *__restrict__ tw, const typename F::T *__restrict__ rtw,
+                                           Src src, Sink sink)
 {
     using T = typename F::T;
     using P = Plan<F, L>;
@@ -245,8 +242,6 @@ __device__ __forceinline__ void tile_round(typename F::T *lds, const typename F:
     constexpr int LOG_MI = L - SHIFT;          // current sub-problem size
     constexpr int LOG_MI1 = LOG_MI - LOGR;     // stride between its points
     constexpr int NSUB = (P::M >> LOGR) >> P::LOG_TC;
-    static_assert(WSRC == SRC_TILE || (I == 0 && NSUB == 1), "weighted input is round 0, one sub-DFT per lane");
-    const int c = threadIdx.x & (P::C - 1);
     const int t = threadIdx.x >> P::LOG_C;
 #pragma unroll
     for (int u = 0; u < NSUB; u++) {
@@ -255,15 +250,7 @@ __device__ __forceinline__ void tile_round(typename F::T *lds, const typename F:
         const int base = (s >> LOG_MI1) << LOG_MI;
         T x[R];
 #pragma unroll
-        for (int j = 0; j < R; j++) {
-            const int row = base + a + (j << LOG_MI1);
-            if constexpr (WSRC == SRC_REGS)
-                x[j] = F::mul(src[j], wt[vpad(row)]);
-            else if constexpr (WSRC == SRC_COEF)
-                x[j] = F::mul(src[(row << P::LOG_C) + c], wt[vpad(row)]);
-            else
-                x[j] = lds[P::slot(row, c)];
-        }
+        for (int j = 0; j < R; j++) x[j] = src(u, j, base + a + (j << LOG_MI1));
         reg_dft<F, R, DIR>(x, tw);
         if constexpr (I + 1 < P::NR) {
             const T *r = rtw + rtw_offset(L, I) + a;
@@ -271,29 +258,38 @@ __device__ __forceinline__ void tile_round(typename F::T *lds, const typename F:
             for (int q = 1; q < R; q++) x[crev<R>(q)] = F::mul(x[crev<R>(q)], r[q << LOG_MI1]);
         }
 #pragma unroll
-        for (int q = 0; q < R; q++) lds[P::slot(base + (q << LOG_MI1) + a, c)] = x[crev<R>(q)];
+        for (int q = 0; q < R; q++) sink(u, q, base + (q << LOG_MI1) + a, x[crev<R>(q)]);
     }
 }
 
-// Full tile transform: rows of the LDS tile in natural order in (or the
-// weighted inputs of round 0, see tile_round; with SRC_REGS lane input j is
-// row tid / C + j 2^LOG_TC), row `pos` holds transform index
-// Plan::index_of(pos) out.  Ends with a barrier.
-template <class F, int L, int DIR, int WSRC = SRC_TILE>
+// Full tile transform of the lane's column c = tid % C.  Round 0 reads its
+// inputs from src0 (tile row `row` of the column: HBM, registers, or weighted
+// coefficients), the last round hands transform index Plan::index_of(pos) to
+// sinkN(u, q, pos, value); only the exchanges between rounds go through the
+// LDS tile, one barrier each.  There is no barrier at either end: a caller
+// that wrote LDS data a round reads (twiddle vectors) or that reuses the tile
+// afterwards places its own (plans of one round have none inside).
+template <class F, int L, int DIR, class Src, class Sink>
 __device__ __forceinline__ void tile_dft(typename F::T *lds, const typename F::T *__restrict__ tw,
-                                         const typename F::T *__restrict__ rtw, const typename F::T *src = nullptr,
-                                         const typename F::T *wt = nullptr)
+                                         const typename F::T *__restrict__ rtw, Src src0, Sink sinkN)
 {
+    using T = typename F::T;
     using P = Plan<F, L>;
-    tile_round<F, L, 0, 0, DIR, WSRC>(lds, tw, rtw, src, wt);
-    __syncthreads();
-    if constexpr (P::NR > 1) {
-        tile_round<F, L, 1, P::logr(0), DIR, SRC_TILE>(lds, tw, rtw, nullptr, nullptr);
+    const int c = threadIdx.x & (P::C - 1);
+    auto from_tile = [&](int, int, int row) { return lds[P::slot(row, c)]; };
+    auto to_tile = [&](int, int, int pos, T v) { lds[P::slot(pos, c)] = v; };
+    if constexpr (P::NR == 1) {
+        tile_round<F, L, 0, 0, DIR>(tw, rtw, src0, sinkN);
+    } else {
+        tile_round<F, L, 0, 0, DIR>(tw, rtw, src0, to_tile);
         __syncthreads();
-    }
-    if constexpr (P::NR > 2) {
-        tile_round<F, L, 2, P::logr(0) + P::logr(1), DIR, SRC_TILE>(lds, tw, rtw, nullptr, nullptr);
-        __syncthreads();
+        if constexpr (P::NR == 2) {
+            tile_round<F, L, 1, P::logr(0), DIR>(tw, rtw, from_tile, sinkN);
+        } else {
+            tile_round<F, L, 1, P::logr(0), DIR>(tw, rtw, from_tile, to_tile);
+            __syncthreads();
+            tile_round<F, L, 2, P::logr(0) + P::logr(1), DIR>(tw, rtw, from_tile, sinkN);
+        }
     }
 }
 
@@ -303,38 +299,41 @@ struct Geo {
     uint32_t nchunks;
 };
 
-// A lane of a tile pass always handles column c = tid % C of the tile and
-// rows r0 + i * RSTEP (i < ITERS), r0 = tid / C.  Whether that column exists
-// (the last chunk of a matrix may be partial) is therefore known once per lane.
+// A lane of a tile pass handles column c = tid % C of its tile.  Whether that
+// column exists (the last chunk of a matrix may be partial) is known once per
+// lane; a missing column loads column 0 (its transform touches no other
+// column) and stores nothing.
 #define TILE_STORE_LOOP(P, row) \
     _Pragma("unroll P_UNROLL") \
     for (int row = (int)(threadIdx.x >> P::LOG_C); row < P::M; row += P::RSTEP)
 
-// HBM -> LDS: tile row m is matrix row row0 + m * row_stride, columns c0 ..
-// c0 + C - 1 (zero beyond the width).  Loads are unconditional (a missing
-// column reads column 0 and is replaced by zero) so each lane issues a batch
-// of them before the first wait.
+// The last round's outputs through the LDS tile: the caller stores them
+// afterwards in TILE_STORE_LOOP order (row `pos` holds transform index
+// Plan::index_of(pos)), a few at a time.  Stores that also take a twiddle
+// product go this way: issued straight from the last round, with every
+// product's operands in flight at once, they cost more VGPRs than the
+// exchange saves.
 template <class F, int L>
-__device__ __forceinline__ void tile_load(typename F::T *lds, const typename F::T *__restrict__ src, uint64_t row0,
-                                          uint64_t row_stride, uint32_t width, uint32_t c0)
-{
-    using T = typename F::T;
-    using P = Plan<F, L>;
-    constexpr int BATCH = P::ITERS < 32 ? P::ITERS : 32;
-    const int c = threadIdx.x & (P::C - 1), r0 = threadIdx.x >> P::LOG_C;
-    const uint32_t col = c0 + c;
-    const bool ok = col < width;
-    const T *p = src + (row0 + (uint64_t)r0 * row_stride) * width + (ok ? col : 0u);
-    const uint64_t step = (uint64_t)P::RSTEP * row_stride * width;
-#pragma unroll
-    for (int i0 = 0; i0 < P::ITERS; i0 += BATCH) {
-        T v[BATCH];
-#pragma unroll
-        for (int i = 0; i < BATCH; i++) v[i] = p[(uint64_t)(i0 + i) * step];
-#pragma unroll
-        for (int i = 0; i < BATCH; i++) lds[P::slot(r0 + (i0 + i) * P::RSTEP, c)] = ok ? v[i] : T(0);
+struct SinkToTile {
+    typename F::T *lds;
+    int c;
+    __device__ void operator()(int, int, int pos, typename F::T v) const { lds[Plan<F, L>::slot(pos, c)] = v; }
+};
+
+// Round 0 straight from HBM: tile row `row` of the lane's column is matrix row
+// row0 + row * row_stride.  A wave reads C columns of 64 / C rows per load,
+// the same row segments an LDS staging pass would read, and every load of a
+// lane is issued before the first is waited for.
+template <class F>
+struct TileFromHbm {
+    const typename F::T *p;
+    uint64_t step;
+    __device__ TileFromHbm(const typename F::T *src, uint64_t row0, uint64_t row_stride, uint32_t width, uint32_t col)
+        : p(src + row0 * width + (col < width ? col : 0u)), step(row_stride * width)
+    {
     }
-}
+    __device__ typename F::T operator()(int, int, int row) const { return p[(uint64_t)row * step]; }
+};
 
 // Products inside the transforms are always data x constant, and the
 // constants (twiddles, coset weights, 1/N) are Montgomery form, so a product
@@ -360,13 +359,14 @@ __global__ __launch_bounds__(NTT_THREADS) void k_fs_pass1(const typename F::T *_
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t n2 = tile / g.nchunks;
     const uint32_t c0 = (tile % g.nchunks) << P::LOG_C;
+    const int c = threadIdx.x & (P::C - 1);
+    const bool col_ok = c0 + c < g.width;
     for (int k = threadIdx.x; k < P::M; k += NTT_THREADS)
         wk[vpad(k)] = top_pow<F>(top_lo, top_hi, (uint64_t)n2 * k, g.logN);
-    tile_load<F, L>(lds, src, n2, 1ull << g.logN2, g.width, c0);
+    tile_dft<F, L, DIR>(lds, tile_tw, round_tw, TileFromHbm<F>(src, n2, 1ull << g.logN2, g.width, c0 + c),
+                        SinkToTile<F, L>{lds, c});
     __syncthreads();
-    tile_dft<F, L, DIR>(lds, tile_tw, round_tw);
-    const int c = threadIdx.x & (P::C - 1);
-    if (c0 + c >= g.width) return;
+    if (!col_ok) return;
     T *d = dst + (uint64_t)n2 * g.width + c0 + c;
     const uint64_t rs = (uint64_t)g.width << g.logN2;
     TILE_STORE_LOOP(P, pos)
@@ -392,20 +392,19 @@ __global__ __launch_bounds__(NTT_THREADS) void k_fs_pass2(const typename F::T *_
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t k1 = tile / g.nchunks;
     const uint32_t c0 = (tile % g.nchunks) << P::LOG_C;
-    tile_load<F, L>(lds, src, (uint64_t)k1 << g.logN2, 1, g.width, c0);
-    __syncthreads();
-    tile_dft<F, L, DIR>(lds, tile_tw, round_tw);
     const int c = threadIdx.x & (P::C - 1);
-    if (c0 + c >= g.width) return;
+    const bool col_ok = c0 + c < g.width;
     T *d = dst + (uint64_t)k1 * g.width + c0 + c;
     const uint64_t rs = (uint64_t)g.width << g.logN1;
-    TILE_STORE_LOOP(P, pos)
-    {
-        d[P::index_of(pos) * rs] = F::mul(lds[P::slot(pos, c)], scale);
-    }
+    tile_dft<F, L, DIR>(lds, tile_tw, round_tw, TileFromHbm<F>(src, (uint64_t)k1 << g.logN2, 1, g.width, c0 + c),
+                        [&](int, int, int pos, T v) {
+                            if (col_ok) d[P::index_of(pos) * rs] = F::mul(v, scale);
+                        });
 }
 
-// Single-pass transform (N <= 2^11): natural in/out.
+// Single-pass transform (N <= 2^11): natural in/out, in place.  Every lane
+// has loaded its inputs before the round-0 barrier, and a one-round plan reads
+// and writes only the lane's own rows, so no store overtakes a load.
 template <class F, int L, int DIR>
 __global__ __launch_bounds__(NTT_THREADS) void k_ntt_small(typename F::T *__restrict__ data, Geo g,
                                                            typename F::T scale,
@@ -416,16 +415,13 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_small(typename F::T *__rest
     using P = Plan<F, L>;
     __shared__ T lds[P::LDS_ELEMS];
     const uint32_t c0 = xcd_tile(blockIdx.x, gridDim.x) << P::LOG_C;
-    tile_load<F, L>(lds, data, 0, 1, g.width, c0);
-    __syncthreads();
-    tile_dft<F, L, DIR>(lds, tile_tw, round_tw);
     const int c = threadIdx.x & (P::C - 1);
-    if (c0 + c >= g.width) return;
+    const bool col_ok = c0 + c < g.width;
     T *d = data + c0 + c;
-    TILE_STORE_LOOP(P, pos)
-    {
-        d[(uint64_t)P::index_of(pos) * g.width] = F::mul(lds[P::slot(pos, c)], scale);
-    }
+    tile_dft<F, L, DIR>(lds, tile_tw, round_tw, TileFromHbm<F>(data, 0, 1, g.width, c0 + c),
+                        [&](int, int, int pos, T v) {
+                            if (col_ok) d[(uint64_t)P::index_of(pos) * g.width] = F::mul(v, scale);
+                        });
 }
 
 // ---------------------------------------------------------------------------
@@ -439,9 +435,24 @@ constexpr bool lde_coef_in_regs()
     return (1 << Plan<F, L>::LOGRMAX) * sizeof(typename F::T) <= 128;
 }
 
-// After the inverse tile transform (row pos holds coefficient index_of(pos)):
-// registers get the coefficients of rows tid / C + j 2^LOG_TC, or the LDS
-// array gets every coefficient of the tile in natural order.
+// Where the inverse transform leaves the coefficients.  The forward round 0
+// (one sub-DFT per lane) takes input j, coefficient t + j 2^LOG_TC with
+// t = tid / C, from the lane's registers or from the LDS array.  In a
+// two-round plan the inverse's last round ends with exactly those
+// coefficients in the lane: sub-DFT u (s = t + u 2^LOG_TC) output q is
+// coefficient s + q 2^logr(0), i.e. j = u + q 2^(logr(0) - LOG_TC), so they go
+// straight to the registers.  Otherwise the inverse ends in the LDS tile and
+// the lane gathers them (load_coefficients).
+template <class F, int L>
+constexpr bool lde_coef_direct()
+{
+    return lde_coef_in_regs<F, L>() && Plan<F, L>::NR == 2;
+}
+
+// After an inverse transform that ended in the LDS tile (row pos holds
+// coefficient index_of(pos)): registers get the coefficients of rows
+// tid / C + j 2^LOG_TC, or the LDS array gets every coefficient of the tile in
+// natural order.
 template <class F, int L, bool CREG>
 __device__ __forceinline__ void load_coefficients(const typename F::T *lds, typename F::T *creg,
                                                   typename F::T *coef)
@@ -459,6 +470,47 @@ __device__ __forceinline__ void load_coefficients(const typename F::T *lds, type
         }
     }
 }
+
+// The inverse transform of an LDE tile, leaving its coefficients in creg
+// (lde_coef_in_regs) or in the LDS array coef.  Ends with a barrier when the
+// coefficients passed through LDS.
+template <class F, int L, class Src>
+__device__ __forceinline__ void lde_inverse(typename F::T *lds, const typename F::T *__restrict__ tile_inv,
+                                            const typename F::T *__restrict__ round_inv, Src src,
+                                            typename F::T *creg, typename F::T *coef)
+{
+    using T = typename F::T;
+    using P = Plan<F, L>;
+    constexpr bool CREG = lde_coef_in_regs<F, L>();
+    const int c = threadIdx.x & (P::C - 1);
+    if constexpr (lde_coef_direct<F, L>()) {
+        constexpr int QS = P::logr(0) - P::LOG_TC;
+        static_assert(QS >= 0, "two-round plans put the larger radix first");
+        tile_dft<F, L, tw_mode<F>(true)>(lds, tile_inv, round_inv, src,
+                                         [&](int u, int q, int, T v) { creg[u + (q << QS)] = v; });
+    } else {
+        tile_dft<F, L, tw_mode<F>(true)>(lds, tile_inv, round_inv, src,
+                                         [&](int, int, int pos, T v) { lds[P::slot(pos, c)] = v; });
+        __syncthreads();
+        load_coefficients<F, L, CREG>(lds, creg, coef);
+    }
+}
+
+// Round-0 input of a coset's forward transform: coefficient `row` times its
+// coset weight wt[vpad(row)], from the lane's registers or the LDS array.
+template <class F, int L, bool CREG>
+struct WeightedCoef {
+    const typename F::T *creg, *coef, *wt;
+    int c;
+    __device__ typename F::T operator()(int, int j, int row) const
+    {
+        using P = Plan<F, L>;
+        if constexpr (CREG)
+            return F::mul(creg[j], wt[vpad(row)]);
+        else
+            return F::mul(coef[(row << P::LOG_C) + c], wt[vpad(row)]);
+    }
+};
 
 // ---------------------------------------------------------------------------
 // LDE pass B: group k1 reads contiguous rows N2 k1 + n2 of pass A's output,
@@ -506,19 +558,17 @@ __global__ __launch_bounds__(NTT_THREADS) ZKG_LDE_MID_OCC void k_lde_mid(const t
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t k1 = tile / g.nchunks;
     const uint32_t c0 = (tile % g.nchunks) << P::LOG_C;
-    for (int j = threadIdx.x; j < P::M; j += NTT_THREADS)
-        wj[vpad(j)] = top_pow<F>(top_lo, top_hi, (uint64_t)j * k1, g.logN);
-    tile_load<F, L>(lds, src, (uint64_t)k1 << g.logN2, 1, g.width, c0);
-    __syncthreads();
-    tile_dft<F, L, tw_mode<F>(true)>(lds, tile_inv, round_inv);
     const int c = threadIdx.x & (P::C - 1);
     const bool col_ok = c0 + c < g.width;
+    for (int j = threadIdx.x; j < P::M; j += NTT_THREADS)
+        wj[vpad(j)] = top_pow<F>(top_lo, top_hi, (uint64_t)j * k1, g.logN);
     // the coefficients c[k1 + N1 k2] this lane feeds into every coset's first
     // forward round (k2 = tid / C + j 2^LOG_TC), kept in VGPRs across cosets
     constexpr bool CREG = lde_coef_in_regs<F, L>();
     __shared__ T coef[CREG ? 1 : (P::M << P::LOG_C)];  // c[k1 + N1 k2] at [k2][c] otherwise
     T creg[CREG ? (1 << P::LOGRMAX) : 1];
-    load_coefficients<F, L, CREG>(lds, creg, coef);
+    lde_inverse<F, L>(lds, tile_inv, round_inv, TileFromHbm<F>(src, (uint64_t)k1 << g.logN2, 1, g.width, c0 + c),
+                      creg, coef);
     for (int b = 0; b < (1 << g.logB); b++) {
         const T *clo = cos_lo + (size_t)b * (1u << TW_SPLIT);
         const T *chi = cos_hi + (size_t)b * (1u << (LOG_TW_TOP - TW_SPLIT - 2));
@@ -526,9 +576,12 @@ __global__ __launch_bounds__(NTT_THREADS) ZKG_LDE_MID_OCC void k_lde_mid(const t
             const uint32_t k = k1 + ((uint32_t)k2 << g.logN1);
             wb[vpad(k2)] = F::mul(clo[k & ((1u << TW_SPLIT) - 1u)], chi[k >> TW_SPLIT]);
         }
-        __syncthreads();  // creg loads (first coset) done, wb complete, previous coset's stores done
-        tile_dft<F, L, tw_mode<F>(false), CREG ? SRC_REGS : SRC_COEF>(lds, tile_fwd, round_fwd, CREG ? creg : coef,
-                                                                  wb);
+        // wj and wb complete; every lane is done with the tile (the inverse,
+        // or the previous coset's stores) and with the previous weights
+        __syncthreads();
+        tile_dft<F, L, tw_mode<F>(false)>(lds, tile_fwd, round_fwd, WeightedCoef<F, L, CREG>{creg, coef, wb, c},
+                                          SinkToTile<F, L>{lds, c});
+        __syncthreads();
         if (col_ok) {
             T *d = dst + (((uint64_t)b << g.logN) + ((uint64_t)k1 << g.logN2)) * g.width + c0 + c;
             TILE_STORE_LOOP(P, pos)
@@ -537,7 +590,6 @@ __global__ __launch_bounds__(NTT_THREADS) ZKG_LDE_MID_OCC void k_lde_mid(const t
                 d[(uint64_t)j2 * g.width] = F::mul(lds[P::slot(pos, c)], wj[vpad(j2)]);
             }
         }
-        __syncthreads();  // wb and the tile are rewritten by the next coset
     }
 }
 
@@ -558,21 +610,16 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_final(const typename F::T *
     const uint32_t grp = tile / g.nchunks;  // b * N2 + j2
     const uint32_t j2 = grp & ((1u << g.logN2) - 1u), b = grp >> g.logN2;
     const uint32_t c0 = chunk << P::LOG_C;
-    const uint64_t base = (uint64_t)b << g.logN;
-    tile_load<F, L>(lds, src, base + j2, 1ull << g.logN2, g.width, c0);
-    __syncthreads();
-    tile_dft<F, L, tw_mode<F>(false)>(lds, tile_fwd, round_fwd);
-    const uint64_t obase = base + ((uint64_t)brev(j2, g.logN2) << L);
-    // tile row pos holds j1 = index_of(pos), which goes to output row
-    // obase + rev_L(j1).  Walking pos in order keeps a wave's LDS reads on
-    // distinct banks; the stores are one row segment per row either way.
     const int c = threadIdx.x & (P::C - 1);
-    if (c0 + c >= g.width) return;
-    T *d = out + obase * g.width + c0 + c;
-    TILE_STORE_LOOP(P, pos)
-    {
-        d[(uint64_t)brev(P::index_of(pos), L) * g.width] = lds[P::slot(pos, c)];
-    }
+    const bool col_ok = c0 + c < g.width;
+    const uint64_t base = (uint64_t)b << g.logN;
+    // transform index j1 = index_of(pos) goes to output row obase + rev_L(j1)
+    T *d = out + (base + ((uint64_t)brev(j2, g.logN2) << L)) * g.width + c0 + c;
+    tile_dft<F, L, tw_mode<F>(false)>(lds, tile_fwd, round_fwd,
+                                      TileFromHbm<F>(src, base + j2, 1ull << g.logN2, g.width, c0 + c),
+                                      [&](int, int, int pos, T v) {
+                                          if (col_ok) d[(uint64_t)brev(P::index_of(pos), L) * g.width] = v;
+                                      });
 }
 
 // Whole LDE in one pass for N <= 2^11: one block per column chunk.
@@ -590,21 +637,19 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_small(const typename F::T *
     __shared__ T lds[P::LDS_ELEMS];
     __shared__ T wb[vpad_len<P::M>()];  // s_b^k / N by k, per coset
     const uint32_t c0 = xcd_tile(blockIdx.x, gridDim.x) << P::LOG_C;
-    tile_load<F, L>(lds, in, 0, 1, g.width, c0);
-    __syncthreads();
-    tile_dft<F, L, tw_mode<F>(true)>(lds, tile_inv, round_inv);
     const int c = threadIdx.x & (P::C - 1);
     const bool col_ok = c0 + c < g.width;
     constexpr bool CREG = lde_coef_in_regs<F, L>();
     __shared__ T coef[CREG ? 1 : (P::M << P::LOG_C)];  // coefficient k at [k][c] otherwise
     T creg[CREG ? (1 << P::LOGRMAX) : 1];
-    load_coefficients<F, L, CREG>(lds, creg, coef);
+    lde_inverse<F, L>(lds, tile_inv, round_inv, TileFromHbm<F>(in, 0, 1, g.width, c0 + c), creg, coef);
     for (int b = 0; b < (1 << g.logB); b++) {
         const T *clo = cos_lo + (size_t)b * (1u << TW_SPLIT);
         for (int k = threadIdx.x; k < P::M; k += NTT_THREADS) wb[vpad(k)] = clo[k];
         __syncthreads();
-        tile_dft<F, L, tw_mode<F>(false), CREG ? SRC_REGS : SRC_COEF>(lds, tile_fwd, round_fwd, CREG ? creg : coef,
-                                                                  wb);
+        tile_dft<F, L, tw_mode<F>(false)>(lds, tile_fwd, round_fwd, WeightedCoef<F, L, CREG>{creg, coef, wb, c},
+                                          SinkToTile<F, L>{lds, c});
+        __syncthreads();
         if (col_ok) {
             T *d = out + ((uint64_t)b << L) * g.width + c0 + c;
             TILE_STORE_LOOP(P, pos)
@@ -612,7 +657,6 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_small(const typename F::T *
                 d[(uint64_t)brev(P::index_of(pos), L) * g.width] = lds[P::slot(pos, c)];
             }
         }
-        __syncthreads();
     }
 }
 
