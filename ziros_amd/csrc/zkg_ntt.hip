@@ -613,13 +613,22 @@ __global__ __launch_bounds__(NTT_THREADS) void k_lde_final(const typename F::T *
     const int c = threadIdx.x & (P::C - 1);
     const bool col_ok = c0 + c < g.width;
     const uint64_t base = (uint64_t)b << g.logN;
-    // transform index j1 = index_of(pos) goes to output row obase + rev_L(j1)
-    T *d = out + (base + ((uint64_t)brev(j2, g.logN2) << L)) * g.width + c0 + c;
     tile_dft<F, L, tw_mode<F>(false)>(lds, tile_fwd, round_fwd,
                                       TileFromHbm<F>(src, base + j2, 1ull << g.logN2, g.width, c0 + c),
-                                      [&](int, int, int pos, T v) {
-                                          if (col_ok) d[(uint64_t)brev(P::index_of(pos), L) * g.width] = v;
-                                      });
+                                      [&](int, int, int pos, T v) { lds[P::slot(pos, c)] = v; });
+    __syncthreads();
+    if (!col_ok) return;
+    // Transform index j1 = index_of(pos) goes to output row obase + rev_L(j1).
+    // The stores go through the tile: walking pos in order, a wave writes 8
+    // adjacent output rows per store, where the last round's lanes hold rows
+    // 2^(L - r0) apart.  Stored straight from the round, the same 32-byte
+    // segments moved 4% more HBM bytes (fewer partial lines merge in L2) in
+    // the same time.
+    T *d = out + (base + ((uint64_t)brev(j2, g.logN2) << L)) * g.width + c0 + c;
+    TILE_STORE_LOOP(P, pos)
+    {
+        d[(uint64_t)brev(P::index_of(pos), L) * g.width] = lds[P::slot(pos, c)];
+    }
 }
 
 // Whole LDE in one pass for N <= 2^11: one block per column chunk.
