@@ -13,14 +13,18 @@
 //     size-N1 transforms over rows strided by N2, a twiddle omega_N^(a b), and
 //     size-N2 transforms over contiguous rows.
 //   * Each sub-transform of M = 2^L points runs on a block tile of M points x C
-//     columns: the tile is staged through LDS with row-segment loads, then
-//     transformed in 1-3 radix-2^r rounds.  In a round every lane holds one
+//     columns in 1-3 radix-2^r rounds.  In a round every lane holds one
 //     2^r-point sub-DFT in VGPRs (radix-2 DIF fully unrolled; its twiddles are
 //     compile-time indices into the resident table, i.e. wave-uniform scalar
-//     loads), applies the inter-round twiddle and writes back in place, so a
-//     round needs one barrier instead of one per radix-2 stage.  The LDS tile
-//     is padded by one row every 2^r_last rows so the last round's strided
-//     accesses spread over the banks.
+//     loads, or for Goldilocks shifts: the roots of order <= 16 are powers of
+//     two), applies the inter-round twiddle and writes to the LDS tile in
+//     place, so a round needs one barrier instead of one per radix-2 stage.
+//     Round 0 loads its inputs straight from HBM (a wave reads C-column row
+//     segments); the last round hands its outputs to the caller, which stores
+//     them to HBM, keeps them in VGPRs (LDE coefficients) or stages them
+//     through the tile (stores with a twiddle product, bit-reversed rows).  The
+//     LDS tile is padded by one row every 2^r_last rows so the last round's
+//     strided accesses spread over the banks.
 //   * The LDE output rows are in bit-reversed order.  Row b*N + q holds
 //     p(s_b omega_N^rev(q)) with s_b = shift * omega_{BN}^{rev_B(b)}, so the
 //     LDE is one inverse transform followed by B forward size-N transforms of
