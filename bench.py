@@ -20,7 +20,8 @@ Also reported on the same line:
   c2            one BabyBear 2^20 x 64 commit, with the row-hash and LDE
                 rooflines (the LDE against 8 TB/s);
   c1, c4_fri, pcs_open_deep, mmcs, constraint_eval  the other configurations
-                and the opening-side kernels;
+                and the opening-side kernels; gl_fri the Goldilocks (base-field)
+                FRI commit phase;
   served        continuous-batching stream (60% BabyBear / 40% Goldilocks,
                 heights 2^U{14..22}, widths U{16..128}, blowup 2 or 4,
                 Poisson arrivals): p50/p99 latency and cells/s vs one plain
@@ -222,6 +223,34 @@ def bench_c4(ctx, steps):
     dev = 0.0
     for _ in range(steps):
         f = fri.commit_phase(cw, log_len, log_final, betas)
+        dev += ctx.stage_ms()[3]
+        f.free()
+    dt = (time.perf_counter() - t0) / steps
+    ctx.set_timing(False)
+    coeff.free()
+    cw.free()
+    return dict(codeword=1 << log_len, layers=log_len - log_final, ms_per_commit_phase=dt * 1e3,
+                device_ms=dev / steps, codeword_elems_per_s=(1 << log_len) / dt)
+
+
+def bench_fri_gl(ctx, steps):
+    """The Goldilocks STARK's FRI commit phase (Challenge = Val): a codeword of
+    2^22 base-field values (a 2^20-coefficient polynomial at blowup 4, one
+    column) folded to 2^8, one Poseidon2-16 tree per layer."""
+    log_len, log_final = 22, 8
+    coeff = ctx.random(Z.GOLDILOCKS, 1 << 20, 0x5EED0044)
+    cw = ctx.alloc((1 << log_len) * 8)
+    check(ctx.lib.zkg_coset_lde_batch(ctx.h, Z.GOLDILOCKS, coeff.h, 1 << 20, 1, 2, 7, cw.h))
+    rng = np.random.default_rng(0x5EED0044)
+    betas = rng.integers(0, 0xFFFFFFFF00000001, log_len - log_final, dtype=np.uint64)
+    fri = Z.HipFri(ctx)
+    ctx.set_timing(True)
+    for _ in range(2):
+        fri.commit_phase_base(Z.GOLDILOCKS, cw, log_len, log_final, betas).free()
+    t0 = time.perf_counter()
+    dev = 0.0
+    for _ in range(steps):
+        f = fri.commit_phase_base(Z.GOLDILOCKS, cw, log_len, log_final, betas)
         dev += ctx.stage_ms()[3]
         f.free()
     dt = (time.perf_counter() - t0) / steps
@@ -664,7 +693,7 @@ def main():
         ctx.close()
         return
     if args.only_c4:
-        print(json.dumps({"c4_fri": bench_c4(ctx, args.steps)}))
+        print(json.dumps({"c4_fri": bench_c4(ctx, args.steps), "gl_fri": bench_fri_gl(ctx, args.steps)}))
         ctx.close()
         return
     if args.only_c1:
@@ -739,6 +768,8 @@ def main():
         log("c4: fri commit phase 2^22 -> 2^8")
         out["c4_fri"] = bench_c4(ctx, max(3, args.steps // 2))
         log("c4: %s" % out["c4_fri"])
+        out["gl_fri"] = bench_fri_gl(ctx, max(3, args.steps // 2))
+        log("gl_fri: %s" % out["gl_fri"])
         log("deep: PCS opening DEEP quotient over the config-2 LDE")
         out["pcs_open_deep"] = bench_deep(ctx, max(5, args.steps))
         log("deep: %s" % out["pcs_open_deep"])

@@ -328,15 +328,28 @@ int zkg_fri_fold_matrix(zkg_ctx *ctx, int ext, int field, const zkg_buf *evals, 
  * folded with betas[4 r .. 4 r + 3].  The codeword buffer is read, not kept. */
 int zkg_fri_commit_phase(zkg_ctx *ctx, const zkg_buf *codeword, uint32_t log_len, uint32_t log_final,
                          const uint32_t *betas, zkg_fri **out);
+/* FRI commit phase over a base field (ZKG_GOLDILOCKS, or ZKG_BABYBEAR with
+ * base-field challenges): p3-fri's commit phase with Challenge = Val, the
+ * reference's configuration (`type Challenge<F> = F`, zkf-backends/src/
+ * plonky3.rs:55, with create_test_fri_params at :597).  Each layer of
+ * 2^log_len ... 2^(log_final+1) canonical field elements is committed as a
+ * width-2 matrix under the field's Poseidon2-16 MMCS and folded (arity 2)
+ * with betas[r], a canonical element.  The codeword buffer is read, not kept. */
+int zkg_fri_commit_phase_base(zkg_ctx *ctx, int field, const zkg_buf *codeword, uint32_t log_len,
+                              uint32_t log_final, const uint64_t *betas, zkg_fri **out);
 int zkg_fri_num_layers(const zkg_fri *fri, uint32_t *n);
+/* layer root: 8 field elements (uint32_t for BabyBear, uint64_t for
+ * Goldilocks; root8 points to 32 or 64 bytes) */
 int zkg_fri_layer_root(const zkg_fri *fri, uint32_t layer, uint32_t *root8);
-/* final codeword: 2^log_final EF4 values (4 uint32_t each) */
+/* final codeword: 2^log_final elements (EF4: 4 uint32_t each; base field:
+ * one uint32_t or uint64_t each) */
 int zkg_fri_final(const zkg_fri *fri, uint32_t *out);
 /* Query openings as p3-fri's answer_query makes them: for a query index q of
  * the first codeword (q < 2^log_len), layer r opens leaf (q >> (r+1)) of its
- * width-2 matrix.  rows_out gets n * layers * 8 uint32_t (the opened pair,
- * one EF4 value per half); paths_out gets, per query, the sibling paths of
- * layers 0, 1, ... concatenated (layer r has log_len - 1 - r digests of 8). */
+ * width-2 matrix.  rows_out gets, for each query and layer, the opened pair
+ * (EF4: 8 uint32_t; Goldilocks: 2 uint64_t; base-field BabyBear: 2
+ * uint32_t); paths_out gets, per query, the sibling paths of layers 0, 1, ...
+ * concatenated (layer r has log_len - 1 - r digests of 8 field elements). */
 int zkg_fri_open(const zkg_fri *fri, const uint64_t *indices, uint32_t n, uint32_t *rows_out,
                  uint32_t *paths_out);
 int zkg_fri_free(zkg_fri *fri);

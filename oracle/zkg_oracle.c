@@ -857,6 +857,42 @@ int or_fri_commit_phase_ef4(const or_p2 *p, uint64_t *cw, uint32_t log_len, uint
     return r;
 }
 
+/* Commit phase over a base field (Challenge = Val, the reference's
+ * `type Challenge<F> = F` at zkf-backends/src/plonky3.rs:55, FRI parameters
+ * from create_test_fri_params at :597; p3-fri 0.4.2's commit phase itself is
+ * external): each layer is committed as a width-2 matrix under the field's
+ * Poseidon2 MMCS (one sponge absorption of 2 elements per leaf) and folded
+ * with fold_matrix, arity 2, beta = betas[r]. */
+int or_fri_commit_phase_base(const or_p2 *p, uint64_t *cw, uint32_t log_len, uint32_t log_final,
+                             const uint64_t *betas, uint64_t *layer_roots, uint64_t *final_out)
+{
+    if (p->width != 16 || log_final >= log_len) return -1;
+    uint64_t len = 1ULL << log_len;
+    uint64_t *cur = cw;
+    uint64_t *buf = (uint64_t *)malloc((len / 2) * sizeof(uint64_t));
+    uint64_t *nxt = buf;
+    int r = 0;
+    while (len > (1ULL << log_final)) {
+        uint64_t h = len / 2;
+        uint32_t w = 2;
+        const uint64_t *mats[1] = {cur};
+        or_tree t;
+        int rc = or_merkle_build(p, 1, mats, &h, &w, &t);
+        if (rc) { free(buf); return rc; }
+        or_merkle_root(&t, layer_roots + (size_t)r * (p->width / 2));
+        or_merkle_free(&t);
+        or_fri_fold_matrix(0, p->field, cur, h, betas + r, nxt);
+        uint64_t *t2 = cur;
+        cur = nxt;
+        nxt = t2;
+        len = h;
+        r++;
+    }
+    memcpy(final_out, cur, len * sizeof(uint64_t));
+    free(buf);
+    return r;
+}
+
 /* ======================================================================
  * PCS opening (DEEP quotient), restated from p3-fri 0.4.2
  * TwoAdicFriPcs::open and p3-interpolation 0.4.2 interpolate_coset; neither

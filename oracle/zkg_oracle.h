@@ -136,6 +136,11 @@ int or_fri_commit_phase_ef4(const or_p2 *p, uint64_t *codeword, uint32_t log_len
                             uint32_t log_final, const uint64_t *betas, uint64_t *layer_roots,
                             uint64_t *final_out, or_tree *trees_out);
 void or_gen_ef4_betas(uint64_t seed, int n, uint64_t *betas);
+/* Commit phase over the base field of p (Poseidon2 width 16): layers committed
+ * as width-2 matrices, folded with betas[r] (one element each).  layer_roots:
+ * nfolds * digest; final_out: 2^log_final.  Codeword is consumed. */
+int or_fri_commit_phase_base(const or_p2 *p, uint64_t *codeword, uint32_t log_len, uint32_t log_final,
+                             const uint64_t *betas, uint64_t *layer_roots, uint64_t *final_out);
 
 /* ---- PCS opening: DEEP quotient over BabyBear with EF4 challenges ----
  * Restates p3-fri 0.4.2 TwoAdicFriPcs::open (external to the reference; the

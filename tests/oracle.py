@@ -94,6 +94,8 @@ def lib():
         L.or_commit.argtypes = [ctypes.POINTER(P2), c_vp, c_u64, c_u32, c_u32, c_u64, c_vp, c_vp]
         L.or_fri_fold_base.argtypes = [c_int, c_vp, c_u64, c_u64, c_vp, c_vp]
         L.or_fri_fold_matrix.argtypes = [c_int, c_int, c_vp, c_u64, c_vp, c_vp]
+        L.or_fri_commit_phase_base.restype = c_int
+        L.or_fri_commit_phase_base.argtypes = [ctypes.POINTER(P2), c_vp, c_u32, c_u32, c_vp, c_vp, c_vp]
         L.or_fri_commit_phase_ef4.restype = c_int
         L.or_fri_commit_phase_ef4.argtypes = [ctypes.POINTER(P2), c_vp, c_u32, c_u32, c_vp, c_vp, c_vp, c_vp]
         L.or_gen_ef4_betas.argtypes = [c_u64, c_int, c_vp]
@@ -407,6 +409,18 @@ def fri_commit_phase_ef4(p2, codeword, log_len, log_final, betas):
                                       _p(roots), _p(final), None)
     assert r == nl, r
     return roots.reshape(nl, 8), final.reshape(-1, 4)
+
+
+def fri_commit_phase_base(p2, codeword, log_len, log_final, betas):
+    """Base-field commit phase: (layer roots (nfolds, digest), final codeword)."""
+    cw = u64(codeword).reshape(-1).copy()
+    nl = log_len - log_final
+    roots = np.empty(nl * p2.digest, dtype=np.uint64)
+    final = np.empty(1 << log_final, dtype=np.uint64)
+    r = lib().or_fri_commit_phase_base(ctypes.byref(p2.p), _p(cw), log_len, log_final, _p(u64(betas)), _p(roots),
+                                       _p(final))
+    assert r == nl, r
+    return roots.reshape(nl, p2.digest), final
 
 
 def set_threads(n):

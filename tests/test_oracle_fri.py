@@ -82,3 +82,33 @@ def test_commit_phase_layers():
     assert (roots[0] == t0.root()).all()
     g = O.fri_fold_matrix_ef4(cw, betas[0])
     assert (roots[1] == O.MerkleTree(p2, [g.reshape(-1, 8)]).root()).all()
+
+
+@pytest.mark.parametrize("f", [O.GL, O.BB])
+def test_commit_phase_base_layers(f):
+    """Base-field commit phase (Challenge = Val): layer r's root is the tree of
+    layer r's codeword as a width-2 matrix, each layer is the fold of the one
+    before, and the final codeword of a low-degree input is low degree."""
+    p2 = O.Poseidon2(f, 16)
+    lg, lf, lb = 9, 4, 2
+    coeffs = O.gen(f, 21, 1 << (lg - lb)).reshape(-1, 1)
+    cw = O.coset_lde_batch(f, coeffs, lb, O.lib().or_f_generator(f)).reshape(-1)  # bit-reversed, blowup 4
+    betas = O.gen(f, 23, lg - lf)
+    roots, final = O.fri_commit_phase_base(p2, cw, lg, lf, betas)
+    assert roots.shape == (lg - lf, 8) and final.shape == (1 << lf,)
+    layer = cw
+    for r in range(lg - lf):
+        assert (roots[r] == O.MerkleTree(p2, [layer.reshape(-1, 2)]).root()).all(), r
+        layer = O.fri_fold_matrix_base(f, layer, int(betas[r]))
+    assert (final == layer).all()
+    # degree < 2^(lg - lb) folded lg - lf times: degree < 2^(lf - lb) on 2^lf points
+    nat = np.empty_like(final)
+    for j in range(len(final)):
+        nat[_rev(j, lf)] = final[j]
+    back = O.dft_batch(f, nat.reshape(-1, 1), inverse=True).reshape(-1)
+    assert (back[1 << (lf - lb):] == 0).all() and not (back[: 1 << (lf - lb)] == 0).all()
+    # a wrong beta in one round breaks the final codeword
+    bad = betas.copy()
+    bad[1] = (int(bad[1]) + 1) % O.PRIME[f]
+    _, final_bad = O.fri_commit_phase_base(p2, cw, lg, lf, bad)
+    assert not (final_bad == final).all()

@@ -1279,6 +1279,7 @@ hipError_t launch_merkle_tops(int field, int width, const P2Set &p2, const Merkl
                               hipStream_t st)
 {
     if (field == FIELD_BB && width == 16) return merkle_tops_impl<Bb, 16>(p2, trees, ntrees, st);
+    if (field == FIELD_GL && width == 16) return merkle_tops_impl<Gl, 16>(p2, trees, ntrees, st);
     return hipErrorInvalidValue;
 }
 

@@ -86,8 +86,10 @@ struct zkg_tree {
 
 struct zkg_fri {
     zkg_ctx *ctx = nullptr;
+    int field = 1;                  // FIELD_BB: EF4 or base-field BabyBear; FIELD_GL: base field
+    uint32_t elem_bytes = 16;       // one codeword element: 16 (EF4), 8 (Goldilocks), 4 (BabyBear)
     uint32_t log_len = 0, log_final = 0;
-    std::vector<void *> codewords;  // layer r codeword (2^(log_len - r) EF4), last = final
+    std::vector<void *> codewords;  // layer r codeword (2^(log_len - r) elements), last = final
     std::vector<zkg_tree *> trees;  // layer r tree over width-2 rows of codeword r
 };
 

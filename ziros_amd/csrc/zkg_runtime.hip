@@ -1742,6 +1742,112 @@ extern "C" int zkg_fri_commit_phase(zkg_ctx *ctx, const zkg_buf *cw, uint32_t lo
     return ZKG_OK;
 }
 
+// FRI commit phase over a base field: p3-fri 0.4.2's commit phase (external;
+// the reference configures it with create_test_fri_params at
+// zkf-backends/src/plonky3.rs:597) with Challenge = Val, the reference's own
+// configuration (`type Challenge<F> = F`, plonky3.rs:55), so ExtensionMmcs is
+// the field's Poseidon2-16 MMCS.  Layer r of 2^(log_len - r) elements is
+// committed as a width-2 matrix (one sponge absorption per leaf) and folded
+// with betas[r] (p3 fold_matrix, arity 2).  The fold chain and each layer's
+// leaf hash run on the context stream; trees of more than
+// fri_tops_max_leaves() leaves go to the side streams as the EF4 phase does,
+// the smaller ones follow the chain on the context stream, built together
+// (one launch per few levels of all of them).
+extern "C" int zkg_fri_commit_phase_base(zkg_ctx *ctx, int field, const zkg_buf *cw, uint32_t log_len,
+                                         uint32_t log_final, const uint64_t *betas, zkg_fri **out)
+{
+    if (!ctx || !cw || !betas || !out) return set_err(ZKG_EINVAL, "null argument");
+    if (int rc = check_field(field)) return rc;
+    if (log_final >= log_len || log_len > 23 || log_final < 1)
+        return set_err(ZKG_EINVAL, "need 1 <= log_final < log_len <= 23");
+    const uint64_t es = esize(field);
+    if ((es << log_len) > cw->bytes) return set_err(ZKG_EINVAL, "codeword buffer too small");
+    const uint64_t p = field == FIELD_GL ? Gl::P : (uint64_t)Bb::P;
+    const int nl = (int)(log_len - log_final);
+    for (int r = 0; r < nl; r++)
+        if (betas[r] >= p) return set_err(ZKG_EINVAL, "beta %d is not a canonical field element", r);
+    const P2Set *p2 = get_p2(ctx, field, 16);
+    if (!p2) return set_err(ZKG_ESTATE, "Poseidon2-16 constants for field %d not loaded", field);
+    zkg_fri *f = new zkg_fri();
+    f->ctx = ctx;
+    f->field = field;
+    f->elem_bytes = (uint32_t)es;
+    f->log_len = log_len;
+    f->log_final = log_final;
+    int rc = ZKG_OK;
+    hipStream_t st = ctx->stream;
+    for (int r = 0; r <= nl && !rc; r++) {
+        void *ptr;
+        rc = dev_alloc(ctx, es << (log_len - r), &ptr);
+        if (!rc) f->codewords.push_back(ptr);
+    }
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(f->codewords[0], cw->ptr, es << log_len, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) rc = hip_err(e, "copy codeword");
+    }
+    for (int r = 0; r < nl && !rc; r++) {
+        zkg_tree *t;
+        rc = tree_alloc(ctx, field, 16, 1ull << (log_len - 1 - r), f->codewords[r], 2, &t);
+        if (rc) break;
+        f->trees.push_back(t);
+    }
+    stage_mark(ctx, st, 0);
+    if (!rc) {
+        hipError_t e = hipSuccess;
+        for (int r = 0; r < nl && e == hipSuccess; r++) {
+            // leaf digests of layer r, then the fold into layer r + 1
+            e = launch_hash_rows(field, 16, *p2, f->codewords[r], 1ull << (log_len - 1 - r), 2,
+                                 f->trees[r]->layers[0], st);
+            if (e == hipSuccess) e = hipEventRecord(ctx->fri_ev[r], st);
+            uint32_t br[2];
+            if (field == FIELD_GL) {
+                br[0] = (uint32_t)betas[r];
+                br[1] = (uint32_t)(betas[r] >> 32);
+            } else {
+                br[0] = Bb::from_canon((uint32_t)betas[r]);
+                br[1] = 0;
+            }
+            if (e == hipSuccess)
+                e = launch_fri_fold(0, field, ctx->tw[field], *p2, f->codewords[r], 1ull << (log_len - 1 - r), br,
+                                    f->codewords[r + 1], nullptr, st);
+        }
+        std::vector<MerkleTop> tops;
+        for (int r = 0; r < nl && e == hipSuccess; r++) {
+            zkg_tree *t = f->trees[r];
+            if (t->n_leaves <= fri_tops_max_leaves()) {
+                tops.push_back(MerkleTop{t->layers.data(), t->nlayers, t->n_leaves});
+                continue;
+            }
+            hipStream_t side = ctx->side[r & 1];
+            e = hipStreamWaitEvent(side, ctx->fri_ev[r], 0);
+            if (e == hipSuccess)
+                e = launch_merkle_tree(field, 16, *p2, t->layers.data(), t->nlayers, t->n_leaves, 9, side, false);
+        }
+        if (e == hipSuccess && !tops.empty()) e = launch_merkle_tops(field, 16, *p2, tops.data(), (int)tops.size(), st);
+        for (int i = 0; i < 2; i++) {  // join the side streams
+            hipError_t e2 = hipEventRecord(ctx->side_join[i], ctx->side[i]);
+            if (e2 == hipSuccess) e2 = hipStreamWaitEvent(st, ctx->side_join[i], 0);
+            if (e == hipSuccess) e = e2;
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rc = hip_err(e, "fri commit phase");
+    }
+    stage_mark(ctx, st, 1);
+    if (ctx->timing) {
+        (void)hipEventSynchronize(ctx->ev[1]);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+        for (int i = 0; i < 4; i++) ctx->stage_ms[i] = 0;
+        ctx->stage_ms[3] = ms;
+    }
+    if (rc) {
+        zkg_fri_free(f);
+        return rc;
+    }
+    *out = f;
+    return ZKG_OK;
+}
+
 extern "C" int zkg_fri_num_layers(const zkg_fri *f, uint32_t *n)
 {
     if (!f || !n) return set_err(ZKG_EINVAL, "null argument");
@@ -1760,7 +1866,8 @@ extern "C" int zkg_fri_final(const zkg_fri *f, uint32_t *out)
 {
     if (!f || !out) return set_err(ZKG_EINVAL, "null argument");
     HIP_TRY(hipStreamSynchronize(f->ctx->stream), "sync");
-    HIP_TRY(hipMemcpy(out, f->codewords.back(), 16ull << f->log_final, hipMemcpyDeviceToHost), "final");
+    HIP_TRY(hipMemcpy(out, f->codewords.back(), (uint64_t)f->elem_bytes << f->log_final, hipMemcpyDeviceToHost),
+            "final");
     return ZKG_OK;
 }
 
@@ -1770,21 +1877,23 @@ extern "C" int zkg_fri_open(const zkg_fri *f, const uint64_t *idx, uint32_t n, u
     const int nl = (int)f->trees.size();
     for (uint32_t q = 0; q < n; q++)
         if (idx[q] >= (1ull << f->log_len)) return set_err(ZKG_EINVAL, "query index out of range");
-    std::vector<uint64_t> li(n);
-    std::vector<uint32_t> rows(8), path;
-    size_t path_off = 0;
+    // per (query, layer): one width-2 row (2 elements) and the sibling digests
+    // (8 field elements each), in bytes
+    const size_t row_bytes = 2ull * f->elem_bytes;
+    const size_t digest_bytes = 8ull * (f->field == FIELD_GL ? 8 : 4);
+    std::vector<uint8_t> rows(row_bytes), path;
     size_t per_query_path = 0;
-    for (int r = 0; r < nl; r++) per_query_path += (size_t)(f->trees[r]->nlayers - 1) * 8;
+    for (int r = 0; r < nl; r++) per_query_path += (size_t)(f->trees[r]->nlayers - 1) * digest_bytes;
     for (uint32_t q = 0; q < n; q++) {
-        path_off = 0;
+        size_t path_off = 0;
         for (int r = 0; r < nl; r++) {
             const zkg_tree *t = f->trees[r];
             uint64_t leaf = idx[q] >> (r + 1);
-            path.resize((size_t)(t->nlayers - 1) * 8);
+            path.resize((size_t)(t->nlayers - 1) * digest_bytes);
             int rc = tree_open(t, &leaf, 1, rows.data(), path.data());
             if (rc) return rc;
-            if (rows_out) memcpy(rows_out + ((size_t)q * nl + r) * 8, rows.data(), 32);
-            if (paths_out) memcpy(paths_out + q * per_query_path + path_off, path.data(), path.size() * 4);
+            if (rows_out) memcpy((uint8_t *)rows_out + ((size_t)q * nl + r) * row_bytes, rows.data(), row_bytes);
+            if (paths_out) memcpy((uint8_t *)paths_out + q * per_query_path + path_off, path.data(), path.size());
             path_off += path.size();
         }
     }

@@ -722,7 +722,7 @@ def commit_batch(ctx, jobs):
 
 
 class HipFri:
-    """FriAccelerator counterpart plus the EF4 commit phase."""
+    """FriAccelerator counterpart plus the EF4 and base-field commit phases."""
 
     def __init__(self, ctx):
         self.ctx = ctx
@@ -767,40 +767,58 @@ class HipFri:
         return bo.download(dt, n)
 
     def commit_phase(self, codeword_buf, log_len, log_final, betas):
+        """EF4 commit phase: codeword of 2^log_len EF4 values, betas (n, 4)."""
         b = np.ascontiguousarray(betas, dtype=np.uint32)
         h = ctypes.c_void_p()
         check(self.ctx.lib.zkg_fri_commit_phase(self.ctx.h, codeword_buf.h, log_len, log_final, _ptr(b),
                                                 ctypes.byref(h)))
         return FriCommitment(self.ctx, h, log_len, log_final)
 
+    def commit_phase_base(self, field, codeword_buf, log_len, log_final, betas):
+        """Base-field commit phase (Challenge = Val): codeword of 2^log_len field
+        elements, one canonical beta per fold."""
+        b = np.ascontiguousarray(betas, dtype=np.uint64)
+        h = ctypes.c_void_p()
+        check(self.ctx.lib.zkg_fri_commit_phase_base(self.ctx.h, field, codeword_buf.h, log_len, log_final, _ptr(b),
+                                                     ctypes.byref(h)))
+        return FriCommitment(self.ctx, h, log_len, log_final, field=field)
+
 
 class FriCommitment:
-    def __init__(self, ctx, h, log_len, log_final):
+    """Layers of a FRI commit phase: EF4 (field None) or a base field."""
+
+    def __init__(self, ctx, h, log_len, log_final, field=None):
         self.ctx, self.h, self.log_len, self.log_final = ctx, h, log_len, log_final
+        self.field = field
+        # element dtype and words per element of the codeword, digest dtype
+        self.dt = np.uint32 if field is None else DTYPE[field]
+        self.per = 4 if field is None else 1
         n = ctypes.c_uint32()
         check(ctx.lib.zkg_fri_num_layers(h, ctypes.byref(n)))
         self.num_layers = n.value
 
     def layer_roots(self):
-        out = np.empty((self.num_layers, 8), dtype=np.uint32)
+        out = np.empty((self.num_layers, 8), dtype=self.dt)
         for r in range(self.num_layers):
             check(self.ctx.lib.zkg_fri_layer_root(self.h, r, _ptr(out[r])))
         return out
 
     def final(self):
-        out = np.empty((1 << self.log_final) * 4, dtype=np.uint32)
+        out = np.empty((1 << self.log_final) * self.per, dtype=self.dt)
         check(self.ctx.lib.zkg_fri_final(self.h, _ptr(out)))
-        return out.reshape(-1, 4)
+        return out.reshape(-1, self.per) if self.field is None else out
 
     def open(self, indices):
+        """(rows, paths): rows (n, layers, 2 * per) opened pairs, paths (n, sum of
+        path lengths, 8) sibling digests."""
         idx = np.ascontiguousarray(indices, dtype=np.uint64)
         n = idx.size
         nl = self.num_layers
         plen = sum(self.log_len - 1 - r for r in range(nl))
-        rows = np.empty(n * nl * 8, dtype=np.uint32)
-        paths = np.empty(n * plen * 8, dtype=np.uint32)
+        rows = np.empty(n * nl * 2 * self.per, dtype=self.dt)
+        paths = np.empty(n * plen * 8, dtype=self.dt)
         check(self.ctx.lib.zkg_fri_open(self.h, _ptr(idx), n, _ptr(rows), _ptr(paths)))
-        return rows.reshape(n, nl, 8), paths.reshape(n, plen, 8)
+        return rows.reshape(n, nl, 2 * self.per), paths.reshape(n, plen, 8)
 
     def free(self):
         if self.h:
