@@ -735,6 +735,51 @@ __global__ __launch_bounds__(HASH_THREADS) void k_fri_fold_base(const typename F
     out[i] = F::to_canon(F::add(sum, F::mul(beta, F::mul(F::sub(lo, hi), t))));
 }
 
+// The base-field fold fused with the next layer's leaf hash: each lane folds
+// output pair (2j, 2j+1) and hashes it as one 2-element row of the next
+// layer's tree (one sponge absorption, Poseidon2-16: FP64 for BabyBear).
+template <class F>
+__global__ __launch_bounds__(HASH_THREADS) void k_fri_fold_base_leaf(
+    const typename F::T *__restrict__ in, uint64_t n_out, int log_n, typename F::T *__restrict__ out,
+    typename F::T *__restrict__ digests, typename F::T beta, typename F::T half,
+    const typename F::T *__restrict__ lo_inv, const typename F::T *__restrict__ hi_inv,
+    const typename F::T *__restrict__ rc, const typename F::T *__restrict__ diag, const double *__restrict__ rcf)
+{
+    using T = typename F::T;
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_out / 2) return;
+    T r[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const uint64_t i = 2 * j + h;
+        T lo = F::from_canon(in[2 * i]), hi = F::from_canon(in[2 * i + 1]);
+        T g = inv_root_pow<F>(lo_inv, hi_inv, brev((uint32_t)i, log_n), log_n + 1);
+        T t = F::mul(g, half);
+        T sum = F::mul(F::add(lo, hi), half);
+        r[h] = F::to_canon(F::add(sum, F::mul(beta, F::mul(F::sub(lo, hi), t))));
+        out[i] = r[h];
+    }
+    if constexpr (std::is_same<F, Bb>::value) {
+        double s[16];
+        s[0] = Bf::from_u32(r[0]);
+        s[1] = Bf::from_u32(r[1]);
+#pragma unroll
+        for (int e = 2; e < 16; e++) s[e] = 0.0;
+        p2_permute_bf<16>(s, rcf);
+#pragma unroll
+        for (int e = 0; e < 8; e++) digests[j * 8 + e] = Bf::canon(s[e]);
+    } else {
+        T s[16];
+        s[0] = F::from_canon(r[0]);
+        s[1] = F::from_canon(r[1]);
+#pragma unroll
+        for (int e = 2; e < 16; e++) s[e] = 0;
+        p2_permute<F, 16>(s, rc, diag);
+#pragma unroll
+        for (int e = 0; e < 8; e++) digests[j * 8 + e] = F::to_canon(s[e]);
+    }
+}
+
 // FriAccelerator::fold_*: g[i] = (f[2i] + f[2i+1]) / 2 + alpha (f[2i] - f[2i+1]) itw[i]
 template <class F>
 __global__ void k_fri_fold_explicit(const typename F::T *__restrict__ in, uint64_t n_out,
@@ -1385,6 +1430,31 @@ hipError_t launch_fri_fold(int ext, int field, const TwiddleSet &tw, const P2Set
                            dim3(HASH_THREADS), 0, st, (const uint32_t *)in, n_out, log_n, (uint32_t *)out,
                            (uint32_t *)leaf_digests, a, (const uint32_t *)tw.top_lo_inv,
                            (const uint32_t *)tw.top_hi_inv, (const double *)p2.rc_f64);
+        return hipGetLastError();
+    }
+    // next-layer leaves are pairs of outputs
+    if (leaf_digests && (n_out & 1)) return hipErrorInvalidValue;
+    if (leaf_digests && field == FIELD_GL && n_out / 2 <= hash_chunks_max<16>()) {
+        // short Goldilocks layers: the fold, then the leaves with 4 lanes per
+        // permutation, whose chain is a third of one lane's
+        hipError_t e = launch_fri_fold(0, field, tw, p2, in, n_out, beta_reg, out, nullptr, st);
+        if (e == hipSuccess) e = launch_hash_rows(field, 16, p2, out, n_out / 2, 2, leaf_digests, st);
+        return e;
+    }
+    if (leaf_digests) {
+        dim3 g((unsigned)((n_out / 2 + HASH_THREADS - 1) / HASH_THREADS)), b(HASH_THREADS);
+        if (field == FIELD_GL)
+            hipLaunchKernelGGL(k_fri_fold_base_leaf<Gl>, g, b, 0, st, (const uint64_t *)in, n_out, log_n,
+                               (uint64_t *)out, (uint64_t *)leaf_digests,
+                               (uint64_t)beta_reg[0] | ((uint64_t)beta_reg[1] << 32), (uint64_t)(Gl::P + 1) / 2,
+                               (const uint64_t *)tw.top_lo_inv, (const uint64_t *)tw.top_hi_inv,
+                               (const uint64_t *)p2.rc, (const uint64_t *)p2.diag, nullptr);
+        else
+            hipLaunchKernelGGL(k_fri_fold_base_leaf<Bb>, g, b, 0, st, (const uint32_t *)in, n_out, log_n,
+                               (uint32_t *)out, (uint32_t *)leaf_digests, beta_reg[0],
+                               bb_to_mont_host((Bb::P + 1) / 2), (const uint32_t *)tw.top_lo_inv,
+                               (const uint32_t *)tw.top_hi_inv, (const uint32_t *)p2.rc, (const uint32_t *)p2.diag,
+                               (const double *)p2.rc_f64);
         return hipGetLastError();
     }
     dim3 g((unsigned)((n_out + HASH_THREADS - 1) / HASH_THREADS)), b(HASH_THREADS);

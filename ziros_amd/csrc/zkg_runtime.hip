@@ -1748,8 +1748,8 @@ extern "C" int zkg_fri_commit_phase(zkg_ctx *ctx, const zkg_buf *cw, uint32_t lo
 // configuration (`type Challenge<F> = F`, plonky3.rs:55), so ExtensionMmcs is
 // the field's Poseidon2-16 MMCS.  Layer r of 2^(log_len - r) elements is
 // committed as a width-2 matrix (one sponge absorption per leaf) and folded
-// with betas[r] (p3 fold_matrix, arity 2).  The fold chain and each layer's
-// leaf hash run on the context stream; trees of more than
+// with betas[r] (p3 fold_matrix, arity 2).  The fold chain, each fold fused
+// with the next layer's leaf hash, runs on the context stream; trees of more than
 // fri_tops_max_leaves() leaves go to the side streams as the EF4 phase does,
 // the smaller ones follow the chain on the context stream, built together
 // (one launch per few levels of all of them).
@@ -1793,12 +1793,12 @@ extern "C" int zkg_fri_commit_phase_base(zkg_ctx *ctx, int field, const zkg_buf 
     }
     stage_mark(ctx, st, 0);
     if (!rc) {
-        hipError_t e = hipSuccess;
+        // The chain: layer 0's leaf hash, then each fold fused with the
+        // leaf hash of the layer it produces (event r: layer r's leaves).
+        hipError_t e = launch_hash_rows(field, 16, *p2, f->codewords[0], 1ull << (log_len - 1), 2,
+                                        f->trees[0]->layers[0], st);
+        if (e == hipSuccess) e = hipEventRecord(ctx->fri_ev[0], st);
         for (int r = 0; r < nl && e == hipSuccess; r++) {
-            // leaf digests of layer r, then the fold into layer r + 1
-            e = launch_hash_rows(field, 16, *p2, f->codewords[r], 1ull << (log_len - 1 - r), 2,
-                                 f->trees[r]->layers[0], st);
-            if (e == hipSuccess) e = hipEventRecord(ctx->fri_ev[r], st);
             uint32_t br[2];
             if (field == FIELD_GL) {
                 br[0] = (uint32_t)betas[r];
@@ -1807,9 +1807,10 @@ extern "C" int zkg_fri_commit_phase_base(zkg_ctx *ctx, int field, const zkg_buf 
                 br[0] = Bb::from_canon((uint32_t)betas[r]);
                 br[1] = 0;
             }
-            if (e == hipSuccess)
-                e = launch_fri_fold(0, field, ctx->tw[field], *p2, f->codewords[r], 1ull << (log_len - 1 - r), br,
-                                    f->codewords[r + 1], nullptr, st);
+            void *next_digests = (r + 1 < nl) ? f->trees[r + 1]->layers[0] : nullptr;
+            e = launch_fri_fold(0, field, ctx->tw[field], *p2, f->codewords[r], 1ull << (log_len - 1 - r), br,
+                                f->codewords[r + 1], next_digests, st);
+            if (e == hipSuccess) e = hipEventRecord(ctx->fri_ev[r + 1], st);
         }
         std::vector<MerkleTop> tops;
         for (int r = 0; r < nl && e == hipSuccess; r++) {
