@@ -227,10 +227,21 @@ def bench_c4(ctx, steps):
         f.free()
     dt = (time.perf_counter() - t0) / steps
     ctx.set_timing(False)
+    # transcript order: each layer's root read back before its fold (zkg_fri_begin / zkg_fri_fold_next)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f = fri.begin(cw, log_len, log_final)
+        for r in range(log_len - log_final):
+            f.layer_root(r)
+            f.fold_next(betas[4 * r:4 * r + 4])
+        f.final()
+        f.free()
+    dt_step = (time.perf_counter() - t0) / steps
     coeff.free()
     cw.free()
     return dict(codeword=1 << log_len, layers=log_len - log_final, ms_per_commit_phase=dt * 1e3,
-                device_ms=dev / steps, codeword_elems_per_s=(1 << log_len) / dt)
+                device_ms=dev / steps, codeword_elems_per_s=(1 << log_len) / dt,
+                ms_per_commit_phase_transcript_order=dt_step * 1e3)
 
 
 def bench_fri_gl(ctx, steps):
@@ -255,10 +266,21 @@ def bench_fri_gl(ctx, steps):
         f.free()
     dt = (time.perf_counter() - t0) / steps
     ctx.set_timing(False)
+    # transcript order: each layer's root read back before its fold (zkg_fri_begin / zkg_fri_fold_next)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f = fri.begin(cw, log_len, log_final, field=Z.GOLDILOCKS)
+        for r in range(log_len - log_final):
+            f.layer_root(r)
+            f.fold_next(int(betas[r]))
+        f.final()
+        f.free()
+    dt_step = (time.perf_counter() - t0) / steps
     coeff.free()
     cw.free()
     return dict(codeword=1 << log_len, layers=log_len - log_final, ms_per_commit_phase=dt * 1e3,
-                device_ms=dev / steps, codeword_elems_per_s=(1 << log_len) / dt)
+                device_ms=dev / steps, codeword_elems_per_s=(1 << log_len) / dt,
+                ms_per_commit_phase_transcript_order=dt_step * 1e3)
 
 
 def bench_mmcs(ctx, steps):

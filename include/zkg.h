@@ -337,6 +337,21 @@ int zkg_fri_commit_phase(zkg_ctx *ctx, const zkg_buf *codeword, uint32_t log_len
  * with betas[r], a canonical element.  The codeword buffer is read, not kept. */
 int zkg_fri_commit_phase_base(zkg_ctx *ctx, int field, const zkg_buf *codeword, uint32_t log_len,
                               uint32_t log_final, const uint64_t *betas, zkg_fri **out);
+/* The commit phase one layer at a time, in p3-fri's transcript order (p3-fri
+ * 0.4.2 commit_phase: commit layer r, observe its root, sample beta_r, fold),
+ * which the two calls above cannot follow because they take every beta at
+ * once.  zkg_fri_begin commits layer 0 of the codeword (ext = 1: EF4 over
+ * ZKG_BABYBEAR; ext = 0: the base field `field`); each zkg_fri_fold_next folds
+ * the newest layer with beta (ext = 1: 4 canonical uint32_t; ext = 0: one
+ * canonical element as uint64_t, as zkg_fri_fold_matrix takes it) and commits
+ * the layer it produces, until log_len - log_final calls leave the final
+ * codeword.  Layer r's root can be read once r folds are done; zkg_fri_final
+ * and zkg_fri_open need the whole phase.  Refused (ZKG_ESTATE) after the
+ * Poseidon2 constants are reloaded mid-phase.  The result equals the one-call
+ * commit phase with the same betas. */
+int zkg_fri_begin(zkg_ctx *ctx, int ext, int field, const zkg_buf *codeword, uint32_t log_len, uint32_t log_final,
+                  zkg_fri **out);
+int zkg_fri_fold_next(zkg_fri *fri, const void *beta);
 int zkg_fri_num_layers(const zkg_fri *fri, uint32_t *n);
 /* layer root: 8 field elements (uint32_t for BabyBear, uint64_t for
  * Goldilocks; root8 points to 32 or 64 bytes) */

@@ -139,6 +139,8 @@ SIGNATURES = {
     "zkg_fri_fold_matrix": (c_int, [c_vp, c_int, c_int, c_vp, c_u64, c_vp, c_vp]),
     "zkg_fri_commit_phase": (c_int, [c_vp, c_vp, c_u32, c_u32, c_vp, P(c_vp)]),
     "zkg_fri_commit_phase_base": (c_int, [c_vp, c_int, c_vp, c_u32, c_u32, c_vp, P(c_vp)]),
+    "zkg_fri_begin": (c_int, [c_vp, c_int, c_int, c_vp, c_u32, c_u32, P(c_vp)]),
+    "zkg_fri_fold_next": (c_int, [c_vp, c_vp]),
     "zkg_fri_num_layers": (c_int, [c_vp, P(c_u32)]),
     "zkg_fri_layer_root": (c_int, [c_vp, c_u32, c_vp]),
     "zkg_fri_final": (c_int, [c_vp, c_vp]),

@@ -45,6 +45,7 @@ struct zkg_ctx {
     uint64_t cached_bytes = 0;
     zkg::TwiddleSet tw[2];
     zkg::P2Set p2[3];  // GL-16, GL-8, BB-16
+    uint64_t p2_epoch = 0;  // bumped by every constant reload
     std::map<zkg::CosetKey, zkg::CosetSet> cosets;
     bool timing = false;
     hipEvent_t ev[8] = {};
@@ -91,6 +92,10 @@ struct zkg_fri {
     uint32_t log_len = 0, log_final = 0;
     std::vector<void *> codewords;  // layer r codeword (2^(log_len - r) elements), last = final
     std::vector<zkg_tree *> trees;  // layer r tree over width-2 rows of codeword r
+    int ext = 1;                    // EF4 challenges (1) or base-field challenges (0)
+    int folds = 0;                  // folds done: layers 0..folds are committed (all once folds = n layers)
+    bool stepwise = false;          // built by zkg_fri_begin / zkg_fri_fold_next
+    uint64_t p2_epoch = 0;          // the context's constant epoch at zkg_fri_begin
 };
 
 namespace zkg {

@@ -471,6 +471,7 @@ int p2_replace(zkg_ctx *ctx, int field, int width, const std::vector<uint64_t> &
     P2Set &cur = ctx->p2[p2_index(field, width)];
     p2_release(ctx, cur);
     cur = fresh;
+    ctx->p2_epoch++;
     return ZKG_OK;
 }
 
@@ -1738,6 +1739,7 @@ extern "C" int zkg_fri_commit_phase(zkg_ctx *ctx, const zkg_buf *cw, uint32_t lo
         zkg_fri_free(f);
         return rc;
     }
+    f->folds = (int)f->trees.size();
     *out = f;
     return ZKG_OK;
 }
@@ -1845,7 +1847,113 @@ extern "C" int zkg_fri_commit_phase_base(zkg_ctx *ctx, int field, const zkg_buf 
         zkg_fri_free(f);
         return rc;
     }
+    f->folds = (int)f->trees.size();
     *out = f;
+    return ZKG_OK;
+}
+
+// Canonical challenge words to the fold kernels' register form: EF4, four
+// uint32_t to Montgomery form; base field, one uint64_t (Goldilocks: its two
+// halves; BabyBear: Montgomery form).
+static int fri_beta_words(int ext, int field, const void *beta, uint32_t br[4])
+{
+    if (ext) {
+        const uint32_t *b = (const uint32_t *)beta;
+        for (int i = 0; i < 4; i++) {
+            if (b[i] >= Bb::P) return set_err(ZKG_EINVAL, "beta is not canonical");
+            br[i] = Bb::from_canon(b[i]);
+        }
+        return ZKG_OK;
+    }
+    const uint64_t v = *(const uint64_t *)beta;
+    if (v >= (field == FIELD_GL ? Gl::P : (uint64_t)Bb::P)) return set_err(ZKG_EINVAL, "beta is not canonical");
+    br[0] = field == FIELD_GL ? (uint32_t)v : Bb::from_canon((uint32_t)v);
+    br[1] = field == FIELD_GL ? (uint32_t)(v >> 32) : 0;
+    br[2] = br[3] = 0;
+    return ZKG_OK;
+}
+
+// The commit phase one layer at a time, in p3-fri's transcript order
+// (p3-fri 0.4.2 commit_phase, external; the reference's prover runs it through
+// TwoAdicFriPcs, zkf-backends/src/plonky3.rs:68,597): zkg_fri_begin commits
+// layer 0; each zkg_fri_fold_next(beta_r) folds layer r (p3 fold_matrix,
+// arity 2) into layer r + 1 and commits that layer, the fold fused with its
+// leaf hash, until the last call leaves the final codeword.  Every step runs
+// on the context stream; the caller reads layer r's root (zkg_fri_layer_root)
+// before sampling beta_r.  All layer storage is allocated by zkg_fri_begin.
+extern "C" int zkg_fri_begin(zkg_ctx *ctx, int ext, int field, const zkg_buf *cw, uint32_t log_len,
+                             uint32_t log_final, zkg_fri **out)
+{
+    if (!ctx || !cw || !out) return set_err(ZKG_EINVAL, "null argument");
+    if (ext && field != FIELD_BB) return set_err(ZKG_EINVAL, "EF4 challenges need the BabyBear field");
+    if (int rc = check_field(field)) return rc;
+    if (log_final >= log_len || log_len > 23 || log_final < 1)
+        return set_err(ZKG_EINVAL, "need 1 <= log_final < log_len <= 23");
+    const uint64_t es = ext ? 16 : esize(field);
+    if ((es << log_len) > cw->bytes) return set_err(ZKG_EINVAL, "codeword buffer too small");
+    const P2Set *p2 = get_p2(ctx, field, 16);
+    if (!p2) return set_err(ZKG_ESTATE, "Poseidon2-16 constants for field %d not loaded", field);
+    const int nl = (int)(log_len - log_final);
+    const uint32_t row_width = ext ? 8 : 2;  // a leaf: two codeword elements in base-field words
+    zkg_fri *f = new zkg_fri();
+    f->ctx = ctx;
+    f->field = field;
+    f->ext = ext;
+    f->elem_bytes = (uint32_t)es;
+    f->log_len = log_len;
+    f->log_final = log_final;
+    f->stepwise = true;
+    f->p2_epoch = ctx->p2_epoch;
+    int rc = ZKG_OK;
+    hipStream_t st = ctx->stream;
+    for (int r = 0; r <= nl && !rc; r++) {
+        void *ptr;
+        rc = dev_alloc(ctx, es << (log_len - r), &ptr);
+        if (!rc) f->codewords.push_back(ptr);
+    }
+    for (int r = 0; r < nl && !rc; r++) {
+        zkg_tree *t;
+        rc = tree_alloc(ctx, field, 16, 1ull << (log_len - 1 - r), f->codewords[r], row_width, &t);
+        if (!rc) f->trees.push_back(t);
+    }
+    if (!rc) {
+        zkg_tree *t = f->trees[0];
+        hipError_t e = hipMemcpyAsync(f->codewords[0], cw->ptr, es << log_len, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess) e = launch_hash_rows(field, 16, *p2, f->codewords[0], t->n_leaves, row_width, t->layers[0], st);
+        if (e == hipSuccess) e = launch_merkle_tree(field, 16, *p2, t->layers.data(), t->nlayers, t->n_leaves, 9, st, false);
+        if (e != hipSuccess) rc = hip_err(e, "fri begin");
+    }
+    if (rc) {
+        zkg_fri_free(f);
+        return rc;
+    }
+    *out = f;
+    return ZKG_OK;
+}
+
+extern "C" int zkg_fri_fold_next(zkg_fri *f, const void *beta)
+{
+    if (!f || !beta) return set_err(ZKG_EINVAL, "null argument");
+    if (!f->stepwise) return set_err(ZKG_ESTATE, "this commit phase was built in one call");
+    const int nl = (int)f->trees.size();
+    if (f->folds >= nl) return set_err(ZKG_ESTATE, "the commit phase is complete");
+    zkg_ctx *ctx = f->ctx;
+    if (ctx->p2_epoch != f->p2_epoch)
+        return set_err(ZKG_ESTATE, "Poseidon2 constants were reloaded during the commit phase");
+    const P2Set *p2 = get_p2(ctx, f->field, 16);
+    if (!p2) return set_err(ZKG_ESTATE, "Poseidon2-16 constants not loaded");
+    uint32_t br[4];
+    if (int rc = fri_beta_words(f->ext, f->field, beta, br)) return rc;
+    const int r = f->folds;
+    hipStream_t st = ctx->stream;
+    zkg_tree *next = r + 1 < nl ? f->trees[r + 1] : nullptr;
+    hipError_t e = launch_fri_fold(f->ext, f->field, ctx->tw[f->field], *p2, f->codewords[r],
+                                   1ull << (f->log_len - 1 - r), br, f->codewords[r + 1],
+                                   next ? next->layers[0] : nullptr, st);
+    if (e == hipSuccess && next)
+        e = launch_merkle_tree(f->field, 16, *p2, next->layers.data(), next->nlayers, next->n_leaves, 9, st, false);
+    if (e != hipSuccess) return hip_err(e, "fri fold");
+    f->folds++;
     return ZKG_OK;
 }
 
@@ -1860,12 +1968,14 @@ extern "C" int zkg_fri_layer_root(const zkg_fri *f, uint32_t layer, uint32_t *ro
 {
     if (!f || !root8) return set_err(ZKG_EINVAL, "null argument");
     if (layer >= f->trees.size()) return set_err(ZKG_EINVAL, "layer out of range");
+    if ((int)layer > f->folds) return set_err(ZKG_ESTATE, "layer %u is not committed yet", layer);
     return zkg_tree_root(f->trees[layer], root8);
 }
 
 extern "C" int zkg_fri_final(const zkg_fri *f, uint32_t *out)
 {
     if (!f || !out) return set_err(ZKG_EINVAL, "null argument");
+    if (f->folds < (int)f->trees.size()) return set_err(ZKG_ESTATE, "the commit phase is not complete");
     HIP_TRY(hipStreamSynchronize(f->ctx->stream), "sync");
     HIP_TRY(hipMemcpy(out, f->codewords.back(), (uint64_t)f->elem_bytes << f->log_final, hipMemcpyDeviceToHost),
             "final");
@@ -1876,6 +1986,7 @@ extern "C" int zkg_fri_open(const zkg_fri *f, const uint64_t *idx, uint32_t n, u
 {
     if (!f || (!idx && n)) return set_err(ZKG_EINVAL, "null argument");
     const int nl = (int)f->trees.size();
+    if (f->folds < nl) return set_err(ZKG_ESTATE, "the commit phase is not complete");
     for (uint32_t q = 0; q < n; q++)
         if (idx[q] >= (1ull << f->log_len)) return set_err(ZKG_EINVAL, "query index out of range");
     // per (query, layer): one width-2 row (2 elements) and the sibling digests

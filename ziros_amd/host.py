@@ -783,6 +783,16 @@ class HipFri:
                                                      ctypes.byref(h)))
         return FriCommitment(self.ctx, h, log_len, log_final, field=field)
 
+    def begin(self, codeword_buf, log_len, log_final, field=None):
+        """The commit phase one layer at a time, in transcript order: EF4
+        (field None) or a base field.  Layer 0 is committed on return; each
+        FriCommitment.fold_next(beta) folds and commits the next layer."""
+        h = ctypes.c_void_p()
+        check(self.ctx.lib.zkg_fri_begin(self.ctx.h, 1 if field is None else 0,
+                                         BABYBEAR if field is None else field, codeword_buf.h, log_len,
+                                         log_final, ctypes.byref(h)))
+        return FriCommitment(self.ctx, h, log_len, log_final, field=field)
+
 
 class FriCommitment:
     """Layers of a FRI commit phase: EF4 (field None) or a base field."""
@@ -796,6 +806,20 @@ class FriCommitment:
         n = ctypes.c_uint32()
         check(ctx.lib.zkg_fri_num_layers(h, ctypes.byref(n)))
         self.num_layers = n.value
+
+    def layer_root(self, r):
+        out = np.empty(8, dtype=self.dt)
+        check(self.ctx.lib.zkg_fri_layer_root(self.h, r, _ptr(out)))
+        return out
+
+    def fold_next(self, beta):
+        """Fold the newest layer with beta (EF4: 4 canonical words; base field:
+        one canonical element) and commit the layer it produces."""
+        if self.field is None:
+            b = np.ascontiguousarray(np.asarray(beta, dtype=np.uint64).reshape(4), dtype=np.uint32)
+        else:
+            b = np.array([int(beta)], dtype=np.uint64)
+        check(self.ctx.lib.zkg_fri_fold_next(self.h, _ptr(b)))
 
     def layer_roots(self):
         out = np.empty((self.num_layers, 8), dtype=self.dt)
