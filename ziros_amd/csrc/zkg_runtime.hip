@@ -1773,6 +1773,7 @@ extern "C" int zkg_fri_commit_phase_base(zkg_ctx *ctx, int field, const zkg_buf 
     zkg_fri *f = new zkg_fri();
     f->ctx = ctx;
     f->field = field;
+    f->ext = 0;
     f->elem_bytes = (uint32_t)es;
     f->log_len = log_len;
     f->log_final = log_final;
