@@ -3,7 +3,9 @@
 Config 2 (BabyBear 2^20 x 64, blowup 2, Poseidon2-16): the GPU root equals the
 oracle's root bit for bit, 64 random authentication paths verify, and the
 commit is deterministic.  Config 4 (FRI over EF4, 2^22 -> 2^8): every layer
-root and the final codeword equal the oracle's, 80 query openings verify.
+root and the final codeword equal the oracle's, 80 query openings verify.  The
+same phase over Goldilocks with base-field challenges (the reference STARK's
+configuration), in one call and in transcript order.
 Inputs come from the seeded generator on the GPU; the oracle regenerates the
 same stream on the CPU.
 """
@@ -65,3 +67,41 @@ def test_config4_fri_commit_phase(ctx):
             assert O.verify(p2, roots_o[r], [1 << (log_len - 1 - r)], [8], leaf, [rows[k, r]],
                             paths[k, off:off + plen])
             off += plen
+
+
+@pytest.mark.parametrize("stepwise", [False, True])
+def test_goldilocks_fri_commit_phase_full_size(ctx, stepwise):
+    log_len, log_final = 22, 8
+    seed = 0x5EED0044
+    nl = log_len - log_final
+    # 2^20 Goldilocks coefficients at blowup 4 (coset LDE of one column)
+    coeff = ctx.random(Z.GOLDILOCKS, 1 << 20, seed)
+    cw = ctx.alloc((1 << log_len) * 8)
+    assert ctx.lib.zkg_coset_lde_batch(ctx.h, Z.GOLDILOCKS, coeff.h, 1 << 20, 1, 2, 7, cw.h) == 0
+    betas = O.gen(Z.GOLDILOCKS, seed + 1, nl)
+    fri = Z.HipFri(ctx)
+    if stepwise:
+        fc = fri.begin(cw, log_len, log_final, field=Z.GOLDILOCKS)
+        for r in range(nl):
+            fc.fold_next(int(betas[r]))
+    else:
+        fc = fri.commit_phase_base(Z.GOLDILOCKS, cw, log_len, log_final, betas)
+    p2 = O.Poseidon2(O.GL, 16)
+    cw_o = O.coset_lde_batch(O.GL, O.gen(O.GL, seed, 1 << 20).reshape(-1, 1), 2, 7).reshape(-1)
+    assert (cw.download(np.uint64, 1 << log_len) == cw_o).all()
+    roots_o, final_o = O.fri_commit_phase_base(p2, cw_o, log_len, log_final, betas)
+    assert (fc.layer_roots() == roots_o).all()
+    assert (fc.final() == final_o).all()
+    q = np.random.default_rng(seed).integers(0, 1 << log_len, 80).astype(np.uint64)
+    rows, paths = fc.open(q)
+    for k in range(80):
+        off = 0
+        for r in range(nl):
+            plen = log_len - 1 - r
+            leaf = int(q[k]) >> (r + 1)
+            assert O.verify(p2, roots_o[r], [1 << (log_len - 1 - r)], [2], leaf, [rows[k, r]],
+                            paths[k, off:off + plen])
+            off += plen
+    fc.free()
+    cw.free()
+    coeff.free()
