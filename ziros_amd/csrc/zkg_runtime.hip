@@ -15,6 +15,7 @@
 #include "zkg_runtime.h"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -1613,246 +1614,6 @@ extern "C" int zkg_fri_fold(zkg_ctx *ctx, int field, const zkg_buf *evals, uint6
     return ZKG_OK;
 }
 
-extern "C" int zkg_fri_fold_matrix(zkg_ctx *ctx, int ext, int field, const zkg_buf *evals, uint64_t n_out,
-                                   const void *beta, zkg_buf *out)
-{
-    if (!ctx || !evals || !beta || !out) return set_err(ZKG_EINVAL, "null argument");
-    if (ext) field = FIELD_BB;
-    if (int rc = check_field(field)) return rc;
-    if (!is_pow2(n_out) || ilog2u(n_out) + 1 > LOG_TW_TOP) return set_err(ZKG_EINVAL, "n_out must be a power of two <= 2^23");
-    uint64_t es = ext ? 16 : esize(field);
-    if (span_bytes(n_out, 2, es) > evals->bytes || span_bytes(n_out, es) > out->bytes) return set_err(ZKG_EINVAL, "buffer too small");
-    uint32_t br[4] = {0, 0, 0, 0};
-    if (ext) {
-        for (int i = 0; i < 4; i++) {
-            uint32_t v = ((const uint32_t *)beta)[i];
-            if (v >= Bb::P) return set_err(ZKG_EINVAL, "beta not canonical");
-            br[i] = Bb::from_canon(v);
-        }
-    } else if (field == FIELD_GL) {
-        uint64_t v = *(const uint64_t *)beta;
-        if (v >= Gl::P) return set_err(ZKG_EINVAL, "beta not canonical");
-        br[0] = (uint32_t)v;
-        br[1] = (uint32_t)(v >> 32);
-    } else {
-        uint64_t v = *(const uint64_t *)beta;
-        if (v >= Bb::P) return set_err(ZKG_EINVAL, "beta not canonical");
-        br[0] = Bb::from_canon((uint32_t)v);
-    }
-    P2Set none;
-    HIP_TRY(launch_fri_fold(ext, field, ctx->tw[field], none, evals->ptr, n_out, br, out->ptr, nullptr, ctx->stream),
-            "fold");
-    HIP_TRY(hipStreamSynchronize(ctx->stream), "sync");
-    return ZKG_OK;
-}
-
-extern "C" int zkg_fri_commit_phase(zkg_ctx *ctx, const zkg_buf *cw, uint32_t log_len, uint32_t log_final,
-                                    const uint32_t *betas, zkg_fri **out)
-{
-    if (!ctx || !cw || !betas || !out) return set_err(ZKG_EINVAL, "null argument");
-    if (log_final >= log_len || log_len > 23 || log_final < 1)
-        return set_err(ZKG_EINVAL, "need 1 <= log_final < log_len <= 23");
-    if ((16ull << log_len) > cw->bytes) return set_err(ZKG_EINVAL, "codeword buffer too small");
-    const P2Set *p2 = get_p2(ctx, FIELD_BB, 16);
-    if (!p2) return set_err(ZKG_ESTATE, "BabyBear Poseidon2-16 constants not loaded");
-    const int nl = (int)(log_len - log_final);
-    zkg_fri *f = new zkg_fri();
-    f->ctx = ctx;
-    f->log_len = log_len;
-    f->log_final = log_final;
-    int rc = ZKG_OK;
-    hipStream_t st = ctx->stream;
-    // codewords of layers 1..nl (layer 0 is the caller's buffer, copied so
-    // openings stay valid after the caller reuses it)
-    for (int r = 0; r <= nl && !rc; r++) {
-        void *p;
-        rc = dev_alloc(ctx, 16ull << (log_len - r), &p);
-        if (!rc) f->codewords.push_back(p);
-    }
-    if (!rc) {
-        hipError_t e = hipMemcpyAsync(f->codewords[0], cw->ptr, 16ull << log_len, hipMemcpyDeviceToDevice, st);
-        if (e != hipSuccess) rc = hip_err(e, "copy codeword");
-    }
-    for (int r = 0; r < nl && !rc; r++) {
-        zkg_tree *t;
-        uint64_t leaves = 1ull << (log_len - 1 - r);
-        rc = tree_alloc(ctx, FIELD_BB, 16, leaves, f->codewords[r], 8, &t);
-        if (rc) break;
-        f->trees.push_back(t);
-    }
-    stage_mark(ctx, st, 0);
-    if (!rc) {
-        // The fold chain runs on the context stream; layer r's tree (which
-        // needs only that layer's leaf digests, event r) runs on a side
-        // stream, so the latency-bound tree tops overlap the later folds.
-        // The whole chain is enqueued before any tree: with the trees'
-        // launches between its folds, the host's enqueue rate paced the
-        // short folds of the small layers.
-        hipError_t e = launch_hash_rows(FIELD_BB, 16, *p2, f->codewords[0], 1ull << (log_len - 1), 8,
-                                        f->trees[0]->layers[0], st);
-        if (e == hipSuccess) e = hipEventRecord(ctx->fri_ev[0], st);
-        for (int r = 0; r < nl && e == hipSuccess; r++) {
-            uint32_t br[4];
-            for (int i = 0; i < 4; i++) {
-                uint32_t v = betas[4 * r + i];
-                br[i] = Bb::from_canon(v >= Bb::P ? v - Bb::P : v);
-            }
-            uint64_t n_out = 1ull << (log_len - 1 - r);
-            void *next_digests = (r + 1 < nl) ? f->trees[r + 1]->layers[0] : nullptr;
-            e = launch_fri_fold(1, FIELD_BB, ctx->tw[FIELD_BB], *p2, f->codewords[r], n_out, br, f->codewords[r + 1],
-                                next_digests, st);
-            if (e == hipSuccess) e = hipEventRecord(ctx->fri_ev[r + 1], st);
-        }
-        // Trees of small layers are latency-bound launch chains; they are
-        // built together on the context stream once the chain has produced
-        // all their leaf digests, one launch per 5 levels of all of them.
-        std::vector<MerkleTop> tops;
-        for (int r = 0; r < nl && e == hipSuccess; r++) {
-            zkg_tree *t = f->trees[r];
-            if (t->n_leaves <= fri_tops_max_leaves()) {
-                tops.push_back(MerkleTop{t->layers.data(), t->nlayers, t->n_leaves});
-                continue;
-            }
-            hipStream_t side = ctx->side[r & 1];
-            e = hipStreamWaitEvent(side, ctx->fri_ev[r], 0);
-            if (e == hipSuccess)
-                e = launch_merkle_tree(FIELD_BB, 16, *p2, t->layers.data(), t->nlayers, t->n_leaves, 9, side, false);
-        }
-        if (e == hipSuccess && !tops.empty()) e = launch_merkle_tops(FIELD_BB, 16, *p2, tops.data(), (int)tops.size(), st);
-        for (int i = 0; i < 2; i++) {  // join the side streams
-            hipError_t e2 = hipEventRecord(ctx->side_join[i], ctx->side[i]);
-            if (e2 == hipSuccess) e2 = hipStreamWaitEvent(st, ctx->side_join[i], 0);
-            if (e == hipSuccess) e = e2;
-        }
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess) rc = hip_err(e, "fri commit phase");
-    }
-    stage_mark(ctx, st, 1);
-    if (ctx->timing) {
-        (void)hipEventSynchronize(ctx->ev[1]);
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
-        for (int i = 0; i < 4; i++) ctx->stage_ms[i] = 0;
-        ctx->stage_ms[3] = ms;
-    }
-    if (rc) {
-        zkg_fri_free(f);
-        return rc;
-    }
-    f->folds = (int)f->trees.size();
-    *out = f;
-    return ZKG_OK;
-}
-
-// FRI commit phase over a base field: p3-fri 0.4.2's commit phase (external;
-// the reference configures it with create_test_fri_params at
-// zkf-backends/src/plonky3.rs:597) with Challenge = Val, the reference's own
-// configuration (`type Challenge<F> = F`, plonky3.rs:55), so ExtensionMmcs is
-// the field's Poseidon2-16 MMCS.  Layer r of 2^(log_len - r) elements is
-// committed as a width-2 matrix (one sponge absorption per leaf) and folded
-// with betas[r] (p3 fold_matrix, arity 2).  The fold chain, each fold fused
-// with the next layer's leaf hash, runs on the context stream; trees of more than
-// fri_tops_max_leaves() leaves go to the side streams as the EF4 phase does,
-// the smaller ones follow the chain on the context stream, built together
-// (one launch per few levels of all of them).
-extern "C" int zkg_fri_commit_phase_base(zkg_ctx *ctx, int field, const zkg_buf *cw, uint32_t log_len,
-                                         uint32_t log_final, const uint64_t *betas, zkg_fri **out)
-{
-    if (!ctx || !cw || !betas || !out) return set_err(ZKG_EINVAL, "null argument");
-    if (int rc = check_field(field)) return rc;
-    if (log_final >= log_len || log_len > 23 || log_final < 1)
-        return set_err(ZKG_EINVAL, "need 1 <= log_final < log_len <= 23");
-    const uint64_t es = esize(field);
-    if ((es << log_len) > cw->bytes) return set_err(ZKG_EINVAL, "codeword buffer too small");
-    const uint64_t p = field == FIELD_GL ? Gl::P : (uint64_t)Bb::P;
-    const int nl = (int)(log_len - log_final);
-    for (int r = 0; r < nl; r++)
-        if (betas[r] >= p) return set_err(ZKG_EINVAL, "beta %d is not a canonical field element", r);
-    const P2Set *p2 = get_p2(ctx, field, 16);
-    if (!p2) return set_err(ZKG_ESTATE, "Poseidon2-16 constants for field %d not loaded", field);
-    zkg_fri *f = new zkg_fri();
-    f->ctx = ctx;
-    f->field = field;
-    f->ext = 0;
-    f->elem_bytes = (uint32_t)es;
-    f->log_len = log_len;
-    f->log_final = log_final;
-    int rc = ZKG_OK;
-    hipStream_t st = ctx->stream;
-    for (int r = 0; r <= nl && !rc; r++) {
-        void *ptr;
-        rc = dev_alloc(ctx, es << (log_len - r), &ptr);
-        if (!rc) f->codewords.push_back(ptr);
-    }
-    if (!rc) {
-        hipError_t e = hipMemcpyAsync(f->codewords[0], cw->ptr, es << log_len, hipMemcpyDeviceToDevice, st);
-        if (e != hipSuccess) rc = hip_err(e, "copy codeword");
-    }
-    for (int r = 0; r < nl && !rc; r++) {
-        zkg_tree *t;
-        rc = tree_alloc(ctx, field, 16, 1ull << (log_len - 1 - r), f->codewords[r], 2, &t);
-        if (rc) break;
-        f->trees.push_back(t);
-    }
-    stage_mark(ctx, st, 0);
-    if (!rc) {
-        // The chain: layer 0's leaf hash, then each fold fused with the
-        // leaf hash of the layer it produces (event r: layer r's leaves).
-        hipError_t e = launch_hash_rows(field, 16, *p2, f->codewords[0], 1ull << (log_len - 1), 2,
-                                        f->trees[0]->layers[0], st);
-        if (e == hipSuccess) e = hipEventRecord(ctx->fri_ev[0], st);
-        for (int r = 0; r < nl && e == hipSuccess; r++) {
-            uint32_t br[2];
-            if (field == FIELD_GL) {
-                br[0] = (uint32_t)betas[r];
-                br[1] = (uint32_t)(betas[r] >> 32);
-            } else {
-                br[0] = Bb::from_canon((uint32_t)betas[r]);
-                br[1] = 0;
-            }
-            void *next_digests = (r + 1 < nl) ? f->trees[r + 1]->layers[0] : nullptr;
-            e = launch_fri_fold(0, field, ctx->tw[field], *p2, f->codewords[r], 1ull << (log_len - 1 - r), br,
-                                f->codewords[r + 1], next_digests, st);
-            if (e == hipSuccess) e = hipEventRecord(ctx->fri_ev[r + 1], st);
-        }
-        std::vector<MerkleTop> tops;
-        for (int r = 0; r < nl && e == hipSuccess; r++) {
-            zkg_tree *t = f->trees[r];
-            if (t->n_leaves <= fri_tops_max_leaves()) {
-                tops.push_back(MerkleTop{t->layers.data(), t->nlayers, t->n_leaves});
-                continue;
-            }
-            hipStream_t side = ctx->side[r & 1];
-            e = hipStreamWaitEvent(side, ctx->fri_ev[r], 0);
-            if (e == hipSuccess)
-                e = launch_merkle_tree(field, 16, *p2, t->layers.data(), t->nlayers, t->n_leaves, 9, side, false);
-        }
-        if (e == hipSuccess && !tops.empty()) e = launch_merkle_tops(field, 16, *p2, tops.data(), (int)tops.size(), st);
-        for (int i = 0; i < 2; i++) {  // join the side streams
-            hipError_t e2 = hipEventRecord(ctx->side_join[i], ctx->side[i]);
-            if (e2 == hipSuccess) e2 = hipStreamWaitEvent(st, ctx->side_join[i], 0);
-            if (e == hipSuccess) e = e2;
-        }
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess) rc = hip_err(e, "fri commit phase");
-    }
-    stage_mark(ctx, st, 1);
-    if (ctx->timing) {
-        (void)hipEventSynchronize(ctx->ev[1]);
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
-        for (int i = 0; i < 4; i++) ctx->stage_ms[i] = 0;
-        ctx->stage_ms[3] = ms;
-    }
-    if (rc) {
-        zkg_fri_free(f);
-        return rc;
-    }
-    f->folds = (int)f->trees.size();
-    *out = f;
-    return ZKG_OK;
-}
-
 // Canonical challenge words to the fold kernels' register form: EF4, four
 // uint32_t to Montgomery form; base field, one uint64_t (Goldilocks: its two
 // halves; BabyBear: Montgomery form).
@@ -1874,28 +1635,43 @@ static int fri_beta_words(int ext, int field, const void *beta, uint32_t br[4])
     return ZKG_OK;
 }
 
-// The commit phase one layer at a time, in p3-fri's transcript order
-// (p3-fri 0.4.2 commit_phase, external; the reference's prover runs it through
-// TwoAdicFriPcs, zkf-backends/src/plonky3.rs:68,597): zkg_fri_begin commits
-// layer 0; each zkg_fri_fold_next(beta_r) folds layer r (p3 fold_matrix,
-// arity 2) into layer r + 1 and commits that layer, the fold fused with its
-// leaf hash, until the last call leaves the final codeword.  Every step runs
-// on the context stream; the caller reads layer r's root (zkg_fri_layer_root)
-// before sampling beta_r.  All layer storage is allocated by zkg_fri_begin.
-extern "C" int zkg_fri_begin(zkg_ctx *ctx, int ext, int field, const zkg_buf *cw, uint32_t log_len,
-                             uint32_t log_final, zkg_fri **out)
+extern "C" int zkg_fri_fold_matrix(zkg_ctx *ctx, int ext, int field, const zkg_buf *evals, uint64_t n_out,
+                                   const void *beta, zkg_buf *out)
 {
-    if (!ctx || !cw || !out) return set_err(ZKG_EINVAL, "null argument");
-    if (ext && field != FIELD_BB) return set_err(ZKG_EINVAL, "EF4 challenges need the BabyBear field");
+    if (!ctx || !evals || !beta || !out) return set_err(ZKG_EINVAL, "null argument");
+    if (ext) field = FIELD_BB;
     if (int rc = check_field(field)) return rc;
+    if (!is_pow2(n_out) || ilog2u(n_out) + 1 > LOG_TW_TOP) return set_err(ZKG_EINVAL, "n_out must be a power of two <= 2^23");
+    uint64_t es = ext ? 16 : esize(field);
+    if (span_bytes(n_out, 2, es) > evals->bytes || span_bytes(n_out, es) > out->bytes) return set_err(ZKG_EINVAL, "buffer too small");
+    uint32_t br[4];
+    if (int rc = fri_beta_words(ext, field, beta, br)) return rc;
+    P2Set none;
+    HIP_TRY(launch_fri_fold(ext, field, ctx->tw[field], none, evals->ptr, n_out, br, out->ptr, nullptr, ctx->stream),
+            "fold");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "sync");
+    return ZKG_OK;
+}
+
+// The commit phase of p3-fri 0.4.2 (external; the reference configures it
+// with create_test_fri_params at zkf-backends/src/plonky3.rs:597): layer r of
+// 2^(log_len - r) codeword elements is committed as a width-2 matrix (two
+// elements per leaf, one sponge absorption: 8 BabyBear words for EF4, 2 field
+// elements over a base field) and folded with beta_r (p3 fold_matrix, arity
+// 2) into layer r + 1; the last fold leaves the final codeword.
+//
+// fri_setup checks the shape and allocates a phase's storage: the codewords of
+// layers 0 .. n (layer 0 a copy of the caller's buffer, enqueued on the
+// context stream, so openings stay valid after the caller reuses it) and the
+// tree of every committed layer.
+static int fri_setup(zkg_ctx *ctx, int ext, int field, const zkg_buf *cw, uint32_t log_len, uint32_t log_final,
+                     zkg_fri **out)
+{
     if (log_final >= log_len || log_len > 23 || log_final < 1)
         return set_err(ZKG_EINVAL, "need 1 <= log_final < log_len <= 23");
     const uint64_t es = ext ? 16 : esize(field);
     if ((es << log_len) > cw->bytes) return set_err(ZKG_EINVAL, "codeword buffer too small");
-    const P2Set *p2 = get_p2(ctx, field, 16);
-    if (!p2) return set_err(ZKG_ESTATE, "Poseidon2-16 constants for field %d not loaded", field);
     const int nl = (int)(log_len - log_final);
-    const uint32_t row_width = ext ? 8 : 2;  // a leaf: two codeword elements in base-field words
     zkg_fri *f = new zkg_fri();
     f->ctx = ctx;
     f->field = field;
@@ -1903,10 +1679,7 @@ extern "C" int zkg_fri_begin(zkg_ctx *ctx, int ext, int field, const zkg_buf *cw
     f->elem_bytes = (uint32_t)es;
     f->log_len = log_len;
     f->log_final = log_final;
-    f->stepwise = true;
-    f->p2_epoch = ctx->p2_epoch;
     int rc = ZKG_OK;
-    hipStream_t st = ctx->stream;
     for (int r = 0; r <= nl && !rc; r++) {
         void *ptr;
         rc = dev_alloc(ctx, es << (log_len - r), &ptr);
@@ -1914,17 +1687,146 @@ extern "C" int zkg_fri_begin(zkg_ctx *ctx, int ext, int field, const zkg_buf *cw
     }
     for (int r = 0; r < nl && !rc; r++) {
         zkg_tree *t;
-        rc = tree_alloc(ctx, field, 16, 1ull << (log_len - 1 - r), f->codewords[r], row_width, &t);
+        rc = tree_alloc(ctx, field, 16, 1ull << (log_len - 1 - r), f->codewords[r], ext ? 8 : 2, &t);
         if (!rc) f->trees.push_back(t);
     }
     if (!rc) {
-        zkg_tree *t = f->trees[0];
-        hipError_t e = hipMemcpyAsync(f->codewords[0], cw->ptr, es << log_len, hipMemcpyDeviceToDevice, st);
-        if (e == hipSuccess) e = launch_hash_rows(field, 16, *p2, f->codewords[0], t->n_leaves, row_width, t->layers[0], st);
-        if (e == hipSuccess) e = launch_merkle_tree(field, 16, *p2, t->layers.data(), t->nlayers, t->n_leaves, 9, st, false);
-        if (e != hipSuccess) rc = hip_err(e, "fri begin");
+        hipError_t e = hipMemcpyAsync(f->codewords[0], cw->ptr, es << log_len, hipMemcpyDeviceToDevice, ctx->stream);
+        if (e != hipSuccess) rc = hip_err(e, "copy codeword");
     }
     if (rc) {
+        zkg_fri_free(f);
+        return rc;
+    }
+    *out = f;
+    return ZKG_OK;
+}
+
+// The whole phase in one call, every beta given (br[r]: fold r's challenge in
+// the fold kernels' register form).  The fold chain runs on the context
+// stream: layer 0's leaf hash, then each fold fused with the leaf hash of the
+// layer it produces (event r: layer r's leaves ready).  Layer r's tree needs
+// only those leaves, so trees of more than fri_tops_max_leaves() leaves run on
+// the side streams beside the later folds.  The whole chain is enqueued before
+// any tree: with the trees' launches between its folds, the host's enqueue
+// rate paced the short folds of the small layers.  The trees of the small
+// layers are latency-bound launch chains; they are built together on the
+// context stream once the chain has produced all their leaves, one launch per
+// few levels of all of them.
+static int fri_one_call(zkg_ctx *ctx, int ext, int field, const zkg_buf *cw, uint32_t log_len, uint32_t log_final,
+                        const std::vector<std::array<uint32_t, 4>> &br, zkg_fri **out)
+{
+    const P2Set *p2 = get_p2(ctx, field, 16);
+    if (!p2) return set_err(ZKG_ESTATE, "Poseidon2-16 constants for field %d not loaded", field);
+    zkg_fri *f;
+    if (int rc = fri_setup(ctx, ext, field, cw, log_len, log_final, &f)) return rc;
+    const int nl = (int)f->trees.size();
+    hipStream_t st = ctx->stream;
+    stage_mark(ctx, st, 0);
+    zkg_tree *t0 = f->trees[0];
+    hipError_t e = launch_hash_rows(field, 16, *p2, f->codewords[0], t0->n_leaves, ext ? 8 : 2, t0->layers[0], st);
+    if (e == hipSuccess) e = hipEventRecord(ctx->fri_ev[0], st);
+    for (int r = 0; r < nl && e == hipSuccess; r++) {
+        void *next_digests = (r + 1 < nl) ? f->trees[r + 1]->layers[0] : nullptr;
+        e = launch_fri_fold(ext, field, ctx->tw[field], *p2, f->codewords[r], 1ull << (log_len - 1 - r), br[r].data(),
+                            f->codewords[r + 1], next_digests, st);
+        if (e == hipSuccess) e = hipEventRecord(ctx->fri_ev[r + 1], st);
+    }
+    std::vector<MerkleTop> tops;
+    for (int r = 0; r < nl && e == hipSuccess; r++) {
+        zkg_tree *t = f->trees[r];
+        if (t->n_leaves <= fri_tops_max_leaves()) {
+            tops.push_back(MerkleTop{t->layers.data(), t->nlayers, t->n_leaves});
+            continue;
+        }
+        hipStream_t side = ctx->side[r & 1];
+        e = hipStreamWaitEvent(side, ctx->fri_ev[r], 0);
+        if (e == hipSuccess)
+            e = launch_merkle_tree(field, 16, *p2, t->layers.data(), t->nlayers, t->n_leaves, 9, side, false);
+    }
+    if (e == hipSuccess && !tops.empty()) e = launch_merkle_tops(field, 16, *p2, tops.data(), (int)tops.size(), st);
+    for (int i = 0; i < 2; i++) {  // join the side streams
+        hipError_t e2 = hipEventRecord(ctx->side_join[i], ctx->side[i]);
+        if (e2 == hipSuccess) e2 = hipStreamWaitEvent(st, ctx->side_join[i], 0);
+        if (e == hipSuccess) e = e2;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    stage_mark(ctx, st, 1);
+    if (ctx->timing) {
+        (void)hipEventSynchronize(ctx->ev[1]);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+        for (int i = 0; i < 4; i++) ctx->stage_ms[i] = 0;
+        ctx->stage_ms[3] = ms;
+    }
+    if (e != hipSuccess) {
+        int rc = hip_err(e, "fri commit phase");
+        zkg_fri_free(f);
+        return rc;
+    }
+    f->folds = nl;
+    *out = f;
+    return ZKG_OK;
+}
+
+// EF4 over BabyBear: betas[4 r .. 4 r + 3] (reduced mod p).
+extern "C" int zkg_fri_commit_phase(zkg_ctx *ctx, const zkg_buf *cw, uint32_t log_len, uint32_t log_final,
+                                    const uint32_t *betas, zkg_fri **out)
+{
+    if (!ctx || !cw || !betas || !out) return set_err(ZKG_EINVAL, "null argument");
+    if (log_final >= log_len || log_len > 23 || log_final < 1)
+        return set_err(ZKG_EINVAL, "need 1 <= log_final < log_len <= 23");
+    std::vector<std::array<uint32_t, 4>> br(log_len - log_final);
+    for (size_t r = 0; r < br.size(); r++)
+        for (int i = 0; i < 4; i++) {
+            const uint32_t v = betas[4 * r + i];
+            br[r][i] = Bb::from_canon(v >= Bb::P ? v - Bb::P : v);
+        }
+    return fri_one_call(ctx, 1, FIELD_BB, cw, log_len, log_final, br, out);
+}
+
+// A base field, Challenge = Val: the reference's own configuration
+// (`type Challenge<F> = F`, zkf-backends/src/plonky3.rs:55), whose
+// ExtensionMmcs is then the field's Poseidon2-16 MMCS; betas[r] canonical.
+extern "C" int zkg_fri_commit_phase_base(zkg_ctx *ctx, int field, const zkg_buf *cw, uint32_t log_len,
+                                         uint32_t log_final, const uint64_t *betas, zkg_fri **out)
+{
+    if (!ctx || !cw || !betas || !out) return set_err(ZKG_EINVAL, "null argument");
+    if (int rc = check_field(field)) return rc;
+    if (log_final >= log_len || log_len > 23 || log_final < 1)
+        return set_err(ZKG_EINVAL, "need 1 <= log_final < log_len <= 23");
+    std::vector<std::array<uint32_t, 4>> br(log_len - log_final);
+    for (size_t r = 0; r < br.size(); r++)
+        if (int rc = fri_beta_words(0, field, &betas[r], br[r].data()))
+            return set_err(rc, "beta %zu is not a canonical field element", r);
+    return fri_one_call(ctx, 0, field, cw, log_len, log_final, br, out);
+}
+
+// The commit phase one layer at a time, in p3-fri's transcript order (the
+// reference's prover runs it through TwoAdicFriPcs, plonky3.rs:68,597):
+// zkg_fri_begin commits layer 0; each zkg_fri_fold_next(beta_r) folds layer r
+// into layer r + 1 and commits that layer, the fold fused with its leaf hash,
+// until the last call leaves the final codeword.  Every step runs on the
+// context stream; the caller reads layer r's root (zkg_fri_layer_root) before
+// sampling beta_r.  All layer storage is allocated by zkg_fri_begin.
+extern "C" int zkg_fri_begin(zkg_ctx *ctx, int ext, int field, const zkg_buf *cw, uint32_t log_len,
+                             uint32_t log_final, zkg_fri **out)
+{
+    if (!ctx || !cw || !out) return set_err(ZKG_EINVAL, "null argument");
+    if (ext && field != FIELD_BB) return set_err(ZKG_EINVAL, "EF4 challenges need the BabyBear field");
+    if (int rc = check_field(field)) return rc;
+    const P2Set *p2 = get_p2(ctx, field, 16);
+    if (!p2) return set_err(ZKG_ESTATE, "Poseidon2-16 constants for field %d not loaded", field);
+    zkg_fri *f;
+    if (int rc = fri_setup(ctx, ext, field, cw, log_len, log_final, &f)) return rc;
+    f->stepwise = true;
+    f->p2_epoch = ctx->p2_epoch;
+    hipStream_t st = ctx->stream;
+    zkg_tree *t = f->trees[0];
+    hipError_t e = launch_hash_rows(field, 16, *p2, f->codewords[0], t->n_leaves, ext ? 8 : 2, t->layers[0], st);
+    if (e == hipSuccess) e = launch_merkle_tree(field, 16, *p2, t->layers.data(), t->nlayers, t->n_leaves, 9, st, false);
+    if (e != hipSuccess) {
+        int rc = hip_err(e, "fri begin");
         zkg_fri_free(f);
         return rc;
     }
