@@ -64,6 +64,7 @@ constexpr int P_UNROLL = LDE_PLAN_TABLE[ZKG_LDE_VARIANT].unroll;
 constexpr int P_WAVES32 = LDE_PLAN_TABLE[ZKG_LDE_VARIANT].waves32;
 constexpr int P_WAVES64 = LDE_PLAN_TABLE[ZKG_LDE_VARIANT].waves64;
 constexpr int P_FIRST_HEAVY = LDE_PLAN_TABLE[ZKG_LDE_VARIANT].first_heavy;
+constexpr int P_LEAN64 = LDE_PLAN_TABLE[ZKG_LDE_VARIANT].lean64;
 
 // Radix split of a 2^L-point in-LDS transform: NR rounds of at most 2^rmax
 // points per lane, larger first (or, first-heavy, the full radix first when
@@ -102,12 +103,24 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nblocks)
     return x * q + (x < r ? x : r) + k;
 }
 
+// A product inside the tile passes.  With lean64, Goldilocks takes the
+// fewest-slot lazy product and one canonicalisation: fewer issue slots, a
+// longer dependency chain; which wins depends on the shape.
+template <class F>
+__device__ __forceinline__ typename F::T tmul(typename F::T a, typename F::T b)
+{
+    if constexpr (sizeof(typename F::T) == 8 && P_LEAN64)
+        return Gl::canon(Gl::mul_lean(a, b));
+    else
+        return F::mul(a, b);
+}
+
 template <class F>
 __device__ __forceinline__ typename F::T top_pow(const typename F::T *__restrict__ lo,
                                                  const typename F::T *__restrict__ hi, uint64_t e, int logN)
 {
     uint32_t ep = (uint32_t)(e << (LOG_TW_TOP - logN)) & ((1u << LOG_TW_TOP) - 1u);
-    return F::mul(lo[ep & ((1u << TW_SPLIT) - 1u)], hi[ep >> TW_SPLIT]);
+    return tmul<F>(lo[ep & ((1u << TW_SPLIT) - 1u)], hi[ep >> TW_SPLIT]);
 }
 
 // ---------------------------------------------------------------------------
@@ -259,7 +272,7 @@ __device__ __forceinline__ void tile_round(const typename F::T *__restrict__ tw,
         if constexpr (I + 1 < P::NR) {
             const T *r = rtw + rtw_offset(L, I) + a;
 #pragma unroll
-            for (int q = 1; q < R; q++) x[crev<R>(q)] = F::mul(x[crev<R>(q)], r[q << LOG_MI1]);
+            for (int q = 1; q < R; q++) x[crev<R>(q)] = tmul<F>(x[crev<R>(q)], r[q << LOG_MI1]);
         }
 #pragma unroll
         for (int q = 0; q < R; q++) sink(u, q, base + (q << LOG_MI1) + a, x[crev<R>(q)]);
@@ -376,7 +389,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_fs_pass1(const typename F::T *_
     TILE_STORE_LOOP(P, pos)
     {
         const uint32_t k1 = P::index_of(pos);
-        d[k1 * rs] = F::mul(lds[P::slot(pos, c)], wk[vpad(k1)]);
+        d[k1 * rs] = tmul<F>(lds[P::slot(pos, c)], wk[vpad(k1)]);
     }
 }
 
@@ -402,7 +415,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_fs_pass2(const typename F::T *_
     const uint64_t rs = (uint64_t)g.width << g.logN1;
     tile_dft<F, L, DIR>(lds, tile_tw, round_tw, TileFromHbm<F>(src, (uint64_t)k1 << g.logN2, 1, g.width, c0 + c),
                         [&](int, int, int pos, T v) {
-                            if (col_ok) d[P::index_of(pos) * rs] = F::mul(v, scale);
+                            if (col_ok) d[P::index_of(pos) * rs] = tmul<F>(v, scale);
                         });
 }
 
@@ -424,7 +437,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_ntt_small(typename F::T *__rest
     T *d = data + c0 + c;
     tile_dft<F, L, DIR>(lds, tile_tw, round_tw, TileFromHbm<F>(data, 0, 1, g.width, c0 + c),
                         [&](int, int, int pos, T v) {
-                            if (col_ok) d[(uint64_t)P::index_of(pos) * g.width] = F::mul(v, scale);
+                            if (col_ok) d[(uint64_t)P::index_of(pos) * g.width] = tmul<F>(v, scale);
                         });
 }
 
@@ -510,9 +523,9 @@ struct WeightedCoef {
     {
         using P = Plan<F, L>;
         if constexpr (CREG)
-            return F::mul(creg[j], wt[vpad(row)]);
+            return tmul<F>(creg[j], wt[vpad(row)]);
         else
-            return F::mul(coef[(row << P::LOG_C) + c], wt[vpad(row)]);
+            return tmul<F>(coef[(row << P::LOG_C) + c], wt[vpad(row)]);
     }
 };
 
@@ -578,7 +591,7 @@ __global__ __launch_bounds__(NTT_THREADS) ZKG_LDE_MID_OCC void k_lde_mid(const t
         const T *chi = cos_hi + (size_t)b * (1u << (LOG_TW_TOP - TW_SPLIT - 2));
         for (int k2 = threadIdx.x; k2 < P::M; k2 += NTT_THREADS) {
             const uint32_t k = k1 + ((uint32_t)k2 << g.logN1);
-            wb[vpad(k2)] = F::mul(clo[k & ((1u << TW_SPLIT) - 1u)], chi[k >> TW_SPLIT]);
+            wb[vpad(k2)] = tmul<F>(clo[k & ((1u << TW_SPLIT) - 1u)], chi[k >> TW_SPLIT]);
         }
         // wj and wb complete; every lane is done with the tile (the inverse,
         // or the previous coset's stores) and with the previous weights
@@ -591,7 +604,7 @@ __global__ __launch_bounds__(NTT_THREADS) ZKG_LDE_MID_OCC void k_lde_mid(const t
             TILE_STORE_LOOP(P, pos)
             {
                 const uint32_t j2 = P::index_of(pos);
-                d[(uint64_t)j2 * g.width] = F::mul(lds[P::slot(pos, c)], wj[vpad(j2)]);
+                d[(uint64_t)j2 * g.width] = tmul<F>(lds[P::slot(pos, c)], wj[vpad(j2)]);
             }
         }
     }
