@@ -49,6 +49,7 @@ from ziros_amd._lib import check  # noqa: E402
 HBM_PEAK = 8.0e12
 BB_PERM_MULS = 772
 GL16_PERM_MULS = 952
+SEED_C1 = 0x5EED0001
 SEED_C2 = 0x5EED0002
 SEED_C3 = workloads.SEED_BATCH
 SEED_C5 = workloads.SEED_SERVE
@@ -156,7 +157,7 @@ def bench_c1(ctx, steps):
     digest 4) as configured, and width 16 (rate 8, digest 8) as the reference
     STARK uses.  Small: latency of one synchronous commit."""
     n, w, lb = 1 << 14, 8, 1
-    trace = ctx.random(Z.GOLDILOCKS, n * w, 0x5EED0001)
+    trace = ctx.random(Z.GOLDILOCKS, n * w, SEED_C1)
     lde = ctx.alloc((n << lb) * w * 8)
     out = {}
     for p2w in (8, 16):
@@ -649,6 +650,19 @@ def cpu_baseline(max_log_multi=16, max_log_one=13):
     t1 = time.perf_counter()
     O.constraint_eval_gl(ce_tr, bc, consts, n_c)
     ce_dt = time.perf_counter() - t1
+    # the single Goldilocks commit (2^14 x 8, blowup 2, Poseidon2-8): the
+    # reference's CPU-runnable case, whole, on every core and on one
+    p2_gl8 = O.Poseidon2(O.GL, 8, 42)
+    tr_c1 = O.gen(O.GL, SEED_C1, (1 << 14) * 8).reshape(1 << 14, 8)
+    c1_ms = {}
+    for threads in (cores, 1):
+        O.set_threads(threads)
+        O.commit(p2_gl8, tr_c1, 1)
+        t1 = time.perf_counter()
+        for _ in range(3):
+            O.commit(p2_gl8, tr_c1, 1)
+        c1_ms[threads] = (time.perf_counter() - t1) / 3 * 1e3
+    O.set_threads(cores)
     cpu = _cpu_model()
     return dict(value=cells / secs, unit="cells/s", cores=used, kind="port",
                 sample="the %d traces of at most 2^%d rows of the 48-trace batched draw (%d cells, blowup 2, "
@@ -658,6 +672,8 @@ def cpu_baseline(max_log_multi=16, max_log_one=13):
                               sample="the %d traces of at most 2^%d rows (%d cells) on one thread; %.1f s wall"
                                      % (nt1, max_log_one, cells1, secs1)),
                 host_threads_visible=os.cpu_count(), affinity=affinity,
+                c1=dict(workload="Goldilocks 2^14 x 8 trace, coset LDE blowup 2, Poseidon2-8 rows, Merkle",
+                        ms_per_commit=c1_ms[cores], cores=cores, ms_per_commit_one_core=c1_ms[1], kind="port"),
                 constraint_eval=dict(rows_per_s=ce_rows / ce_dt, cores=used, kind="port",
                                      sample="%d of the %d LDE rows x %d columns, %d constraints (%d instructions); "
                                             "%.2f s wall" % (ce_rows, CE_SHAPE[0] << CE_SHAPE[2], CE_SHAPE[1], n_c,
