@@ -238,11 +238,28 @@ def bench_c4(ctx, steps):
         f.final()
         f.free()
     dt_step = (time.perf_counter() - t0) / steps
+    opening = fri_open_times(fri.commit_phase(cw, log_len, log_final, betas), log_len)
     coeff.free()
     cw.free()
     return dict(codeword=1 << log_len, layers=log_len - log_final, ms_per_commit_phase=dt * 1e3,
                 device_ms=dev / steps, codeword_elems_per_s=(1 << log_len) / dt,
-                ms_per_commit_phase_transcript_order=dt_step * 1e3)
+                ms_per_commit_phase_transcript_order=dt_step * 1e3, query_openings=opening)
+
+
+def fri_open_times(f, log_len, n=80):
+    """zkg_fri_open of n seeded query indices (rows and sibling paths of every
+    layer), all in one call and one call per query."""
+    q = np.random.default_rng(0x5EED0080).integers(0, 1 << log_len, n).astype(np.uint64)
+    f.open(q)
+    t0 = time.perf_counter()
+    f.open(q)
+    t_all = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for k in range(n):
+        f.open(q[k:k + 1])
+    t_one = time.perf_counter() - t0
+    f.free()
+    return dict(queries=n, ms_all_in_one_call=t_all * 1e3, ms_one_call_per_query=t_one * 1e3)
 
 
 def bench_fri_gl(ctx, steps):
@@ -277,11 +294,12 @@ def bench_fri_gl(ctx, steps):
         f.final()
         f.free()
     dt_step = (time.perf_counter() - t0) / steps
+    opening = fri_open_times(fri.commit_phase_base(Z.GOLDILOCKS, cw, log_len, log_final, betas), log_len)
     coeff.free()
     cw.free()
     return dict(codeword=1 << log_len, layers=log_len - log_final, ms_per_commit_phase=dt * 1e3,
                 device_ms=dev / steps, codeword_elems_per_s=(1 << log_len) / dt,
-                ms_per_commit_phase_transcript_order=dt_step * 1e3)
+                ms_per_commit_phase_transcript_order=dt_step * 1e3, query_openings=opening)
 
 
 def bench_mmcs(ctx, steps):

@@ -1892,25 +1892,31 @@ extern "C" int zkg_fri_open(const zkg_fri *f, const uint64_t *idx, uint32_t n, u
     if (f->folds < nl) return set_err(ZKG_ESTATE, "the commit phase is not complete");
     for (uint32_t q = 0; q < n; q++)
         if (idx[q] >= (1ull << f->log_len)) return set_err(ZKG_EINVAL, "query index out of range");
+    if (!n) return ZKG_OK;
     // per (query, layer): one width-2 row (2 elements) and the sibling digests
-    // (8 field elements each), in bytes
+    // (8 field elements each), in bytes; each layer opens every query's leaf
+    // in one gather and the results are laid out query by query
     const size_t row_bytes = 2ull * f->elem_bytes;
     const size_t digest_bytes = 8ull * (f->field == FIELD_GL ? 8 : 4);
-    std::vector<uint8_t> rows(row_bytes), path;
     size_t per_query_path = 0;
     for (int r = 0; r < nl; r++) per_query_path += (size_t)(f->trees[r]->nlayers - 1) * digest_bytes;
-    for (uint32_t q = 0; q < n; q++) {
-        size_t path_off = 0;
-        for (int r = 0; r < nl; r++) {
-            const zkg_tree *t = f->trees[r];
-            uint64_t leaf = idx[q] >> (r + 1);
-            path.resize((size_t)(t->nlayers - 1) * digest_bytes);
-            int rc = tree_open(t, &leaf, 1, rows.data(), path.data());
-            if (rc) return rc;
-            if (rows_out) memcpy((uint8_t *)rows_out + ((size_t)q * nl + r) * row_bytes, rows.data(), row_bytes);
-            if (paths_out) memcpy((uint8_t *)paths_out + q * per_query_path + path_off, path.data(), path.size());
-            path_off += path.size();
+    std::vector<uint64_t> leaves(n);
+    std::vector<uint8_t> rows, paths;
+    size_t path_off = 0;
+    for (int r = 0; r < nl; r++) {
+        const zkg_tree *t = f->trees[r];
+        const size_t plen = (size_t)(t->nlayers - 1) * digest_bytes;
+        for (uint32_t q = 0; q < n; q++) leaves[q] = idx[q] >> (r + 1);
+        rows.resize((size_t)n * row_bytes);
+        paths.resize((size_t)n * plen);
+        if (int rc = tree_open(t, leaves.data(), n, rows.data(), paths.data())) return rc;
+        for (uint32_t q = 0; q < n; q++) {
+            if (rows_out)
+                memcpy((uint8_t *)rows_out + ((size_t)q * nl + r) * row_bytes, rows.data() + q * row_bytes, row_bytes);
+            if (paths_out && plen)
+                memcpy((uint8_t *)paths_out + q * per_query_path + path_off, paths.data() + q * plen, plen);
         }
+        path_off += plen;
     }
     return ZKG_OK;
 }
