@@ -336,6 +336,19 @@ def bench_mmcs(ctx, steps):
         dt = (time.perf_counter() - t0) / steps
         cells = sum(h * w for h, w in shapes)
         out[name] = dict(shapes=[list(x) for x in shapes], ms_per_commit=dt * 1e3, cells_per_s=cells / dt)
+        # p3 open_batch of 80 seeded query indices: every matrix's row and the
+        # sibling path, one zkg_mmcs_open call
+        t = once()
+        q = np.random.default_rng(0x5EED0081).integers(0, shapes[0][0], 80).astype(np.uint64)
+        es = 8 if field == Z.GOLDILOCKS else 4
+        rows = np.empty(80 * sum(w for _, w in shapes) * es, dtype=np.uint8)
+        paths = np.empty(80 * (shapes[0][0].bit_length() - 1) * (p2w // 2) * es, dtype=np.uint8)
+        qp, rp, pp = (a.ctypes.data_as(ctypes.c_void_p) for a in (q, rows, paths))
+        check(ctx.lib.zkg_mmcs_open(t, qp, 80, rp, pp))
+        t0 = time.perf_counter()
+        check(ctx.lib.zkg_mmcs_open(t, qp, 80, rp, pp))
+        out[name]["ms_open_80_queries"] = (time.perf_counter() - t0) * 1e3
+        free_tree(ctx, t)
         for b in bufs:
             b.free()
     return out
@@ -720,6 +733,8 @@ def main():
                     help="run only the BabyBear 2^20 x 64 single-commit leg (for profiling) and print its JSON")
     ap.add_argument("--only-served", action="store_true",
                     help="run only the served-stream leg (for profiling) and print its JSON")
+    ap.add_argument("--only-mmcs", action="store_true",
+                    help="run only the mixed-height MMCS leg (commits and openings) and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-load", type=float, default=1.5,
                     help="Poisson arrival rate as a multiple of the plain path's capacity")
@@ -750,6 +765,10 @@ def main():
         return
     if args.only_c4:
         print(json.dumps({"c4_fri": bench_c4(ctx, args.steps), "gl_fri": bench_fri_gl(ctx, args.steps)}))
+        ctx.close()
+        return
+    if args.only_mmcs:
+        print(json.dumps({"mmcs": bench_mmcs(ctx, max(5, args.steps // 2))}))
         ctx.close()
         return
     if args.only_c1:

@@ -820,6 +820,13 @@ __global__ void k_gather_open(const T *__restrict__ mat, uint32_t mat_width, Ope
     }
 }
 
+// Block s copies piece s of an opening.
+__global__ void k_gather_segs(const GatherSeg *__restrict__ segs, uint32_t n)
+{
+    const GatherSeg g = segs[blockIdx.x];
+    for (uint32_t w = threadIdx.x; w < g.words; w += blockDim.x) g.dst[w] = g.src[w];
+}
+
 // Levels per k_merkle_fused launch (tuning layers; built-in 3).
 template <class F, int W>
 int fused_levels()
@@ -1501,6 +1508,13 @@ hipError_t launch_gather_open(int field, int digest, const void *mat, uint32_t m
         hipLaunchKernelGGL(k_gather_open<uint32_t>, dim3(n), dim3(64), 0, st, (const uint32_t *)mat, mat_width, ol,
                            nlayers, digest, d_indices, (uint32_t *)rows_out, (uint32_t *)paths_out);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_segs(const GatherSeg *segs, uint32_t n, hipStream_t st)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_segs, dim3(n), dim3(64), 0, st, segs, n);
     return hipGetLastError();
 }
 

@@ -115,8 +115,8 @@ hipError_t launch_hash_rows(int field, int width, const P2Set &p2, const void *m
 hipError_t launch_merkle_tree(int field, int width, const P2Set &p2, void *const *layers,
                               int nlayers, uint64_t n_leaves, int fuse_levels, hipStream_t st,
                               bool sync_each);
-// Several small trees built together (BabyBear width 16, the FRI layer
-// trees), W lanes per compression: one launch per 5 levels of all of them.
+// Several small trees built together (width 16, the FRI layer trees), W
+// lanes per compression: one launch per 5 levels of all of them.
 // Tree t: levels 1.. over layers[0] (n_leaves digests), nlayers =
 // log2(n_leaves) + 1.
 struct MerkleTop {
@@ -194,6 +194,15 @@ hipError_t launch_fri_fold_explicit(int field, const void *in, uint64_t n_out, u
 hipError_t launch_gather_open(int field, int digest, const void *mat, uint32_t mat_width,
                               void *const *layers, int nlayers, const uint64_t *d_indices,
                               uint32_t n, void *rows_out, void *paths_out, hipStream_t st);
+// One piece of an opening (a matrix row, a sibling digest): `words` 32-bit
+// words from src to dst, both device memory.
+struct GatherSeg {
+    const uint32_t *src;
+    uint32_t *dst;
+    uint32_t words, pad;
+};
+// segs: device memory, n pieces, one block each.
+hipError_t launch_gather_segs(const GatherSeg *segs, uint32_t n, hipStream_t st);
 
 // --- DEEP quotient (zkg_deep.hip), BabyBear LDE, EF4 challenges -------------
 // EF4 arguments are 4 words; "mont" = Montgomery form, otherwise canonical.

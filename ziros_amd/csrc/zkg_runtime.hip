@@ -1115,6 +1115,43 @@ static int log2_ceil_u64(uint64_t x)
     return l;
 }
 
+// The pieces of a batch of openings (matrix rows, sibling digests) gathered
+// on the device in one launch into one block [0, na + nb), whose two parts are
+// then copied to a and b.  Piece: `bytes` (a multiple of 4) from device
+// address src to offset off of the block.
+struct OpenPiece {
+    const void *src;
+    uint64_t off;
+    uint32_t bytes;
+};
+
+static int gather_openings(zkg_ctx *ctx, const std::vector<OpenPiece> &pieces, uint64_t na, void *a, uint64_t nb,
+                           void *b)
+{
+    if (pieces.empty() || na + nb == 0) return ZKG_OK;
+    void *d_out = nullptr, *d_segs = nullptr;
+    int rc = dev_alloc(ctx, na + nb, &d_out);
+    if (!rc) rc = dev_alloc(ctx, pieces.size() * sizeof(GatherSeg), &d_segs);
+    hipError_t e = hipSuccess;
+    if (!rc) {
+        std::vector<GatherSeg> segs(pieces.size());
+        for (size_t i = 0; i < pieces.size(); i++)
+            segs[i] = GatherSeg{(const uint32_t *)pieces[i].src, (uint32_t *)((uint8_t *)d_out + pieces[i].off),
+                                pieces[i].bytes / 4, 0};
+        e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) e = hipMemcpy(d_segs, segs.data(), segs.size() * sizeof(GatherSeg), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = launch_gather_segs((const GatherSeg *)d_segs, (uint32_t)segs.size(), ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess && na) e = hipMemcpy(a, d_out, na, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && nb) e = hipMemcpy(b, (uint8_t *)d_out + na, nb, hipMemcpyDeviceToHost);
+    }
+    if (d_out) dev_free(ctx, d_out);
+    if (d_segs) dev_free(ctx, d_segs);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_err(e, "gather openings");
+    return ZKG_OK;
+}
+
 // Mmcs::open_batch (vendor/p3-merkle-tree-gpu/src/mmcs.rs:179-210): matrix j
 // opens row index >> (log2_ceil(max_h) - log2_ceil(h_j)); the proof is the
 // sibling digest of every layer below the root.
@@ -1131,36 +1168,39 @@ extern "C" int zkg_mmcs_open(const zkg_tree *t, const uint64_t *indices, uint32_
     const int logmax = log2_ceil_u64(maxh);
     uint64_t row_elems = 0;
     for (auto &m : mats) row_elems += m.width;
-    HIP_TRY(hipStreamSynchronize(t->ctx->stream), "sync");
     for (uint32_t q = 0; q < n; q++) {
         const uint64_t index = indices[q];
         if (index >= maxh)
             return set_err(ZKG_EINVAL, "index %llu out of range %llu", (unsigned long long)index,
                            (unsigned long long)maxh);
-        if (rows_out) {
-            char *dst = (char *)rows_out + q * row_elems * es;
-            for (size_t j = 0; j < mats.size(); j++) {
-                const uint64_t r = index >> (logmax - log2_ceil_u64(hs[j]));
-                if (r >= hs[j])  // a padding position of a shorter matrix (p3 would panic on the row)
-                    return set_err(ZKG_EINVAL, "index %llu has no row in matrix %zu (height %llu)",
-                                   (unsigned long long)index, j, (unsigned long long)hs[j]);
-                const uint64_t rb = mats[j].width * es;
-                if (rb) HIP_TRY(hipMemcpy(dst, (const char *)mats[j].ptr + r * rb, rb, hipMemcpyDeviceToHost), "row");
-                dst += rb;
-            }
+        for (size_t j = 0; j < mats.size() && rows_out; j++)
+            if ((index >> (logmax - log2_ceil_u64(hs[j]))) >= hs[j])  // a padding position of a shorter matrix
+                return set_err(ZKG_EINVAL, "index %llu has no row in matrix %zu (height %llu)",
+                               (unsigned long long)index, j, (unsigned long long)hs[j]);
+    }
+    if (!n || (!rows_out && !paths_out)) return ZKG_OK;
+    // every piece of every opening in one gather: [rows | paths]
+    const uint64_t row_bytes = row_elems * es, path_bytes = (uint64_t)logmax * t->digest * es;
+    const uint64_t rows_total = rows_out ? n * row_bytes : 0, paths_total = paths_out ? n * path_bytes : 0;
+    std::vector<OpenPiece> pieces;
+    pieces.reserve((size_t)n * (mats.size() + logmax));
+    for (uint32_t q = 0; q < n; q++) {
+        const uint64_t index = indices[q];
+        uint64_t off = q * row_bytes;
+        for (size_t j = 0; j < mats.size() && rows_out; j++) {
+            const uint64_t r = index >> (logmax - log2_ceil_u64(hs[j]));
+            const uint64_t rb = mats[j].width * es;
+            if (rb) pieces.push_back(OpenPiece{(const uint8_t *)mats[j].ptr + r * rb, off, (uint32_t)rb});
+            off += rb;
         }
-        if (paths_out) {
-            char *dst = (char *)paths_out + (uint64_t)q * logmax * t->digest * es;
-            for (int i = 0; i < logmax; i++) {
-                const uint64_t sib = (index >> i) ^ 1;
-                HIP_TRY(hipMemcpy(dst + (uint64_t)i * t->digest * es,
-                                  (const char *)t->layers[i] + sib * t->digest * es, t->digest * es,
-                                  hipMemcpyDeviceToHost),
-                        "path");
-            }
+        for (int i = 0; i < logmax && paths_out; i++) {
+            const uint64_t sib = (index >> i) ^ 1;
+            pieces.push_back(OpenPiece{(const uint8_t *)t->layers[i] + sib * t->digest * es,
+                                       rows_total + q * path_bytes + (uint64_t)i * t->digest * es,
+                                       (uint32_t)(t->digest * es)});
         }
     }
-    return ZKG_OK;
+    return gather_openings(t->ctx, pieces, rows_total, rows_out, paths_total, paths_out);
 }
 
 // Mmcs::verify_batch (vendor/p3-merkle-tree-gpu/src/mmcs.rs:210-330) for n
@@ -1892,33 +1932,31 @@ extern "C" int zkg_fri_open(const zkg_fri *f, const uint64_t *idx, uint32_t n, u
     if (f->folds < nl) return set_err(ZKG_ESTATE, "the commit phase is not complete");
     for (uint32_t q = 0; q < n; q++)
         if (idx[q] >= (1ull << f->log_len)) return set_err(ZKG_EINVAL, "query index out of range");
-    if (!n) return ZKG_OK;
     // per (query, layer): one width-2 row (2 elements) and the sibling digests
-    // (8 field elements each), in bytes; each layer opens every query's leaf
-    // in one gather and the results are laid out query by query
-    const size_t row_bytes = 2ull * f->elem_bytes;
-    const size_t digest_bytes = 8ull * (f->field == FIELD_GL ? 8 : 4);
-    size_t per_query_path = 0;
-    for (int r = 0; r < nl; r++) per_query_path += (size_t)(f->trees[r]->nlayers - 1) * digest_bytes;
-    std::vector<uint64_t> leaves(n);
-    std::vector<uint8_t> rows, paths;
-    size_t path_off = 0;
-    for (int r = 0; r < nl; r++) {
-        const zkg_tree *t = f->trees[r];
-        const size_t plen = (size_t)(t->nlayers - 1) * digest_bytes;
-        for (uint32_t q = 0; q < n; q++) leaves[q] = idx[q] >> (r + 1);
-        rows.resize((size_t)n * row_bytes);
-        paths.resize((size_t)n * plen);
-        if (int rc = tree_open(t, leaves.data(), n, rows.data(), paths.data())) return rc;
-        for (uint32_t q = 0; q < n; q++) {
+    // (8 field elements each); every piece of every query in one gather
+    const uint64_t row_bytes = 2ull * f->elem_bytes;
+    const uint64_t digest_bytes = 8ull * (f->field == FIELD_GL ? 8 : 4);
+    uint64_t per_query_path = 0;
+    for (int r = 0; r < nl; r++) per_query_path += (uint64_t)(f->trees[r]->nlayers - 1) * digest_bytes;
+    const uint64_t rows_total = rows_out ? n * nl * row_bytes : 0, paths_total = paths_out ? n * per_query_path : 0;
+    std::vector<OpenPiece> pieces;
+    for (uint32_t q = 0; q < n; q++) {
+        uint64_t path_off = rows_total + q * per_query_path;
+        for (int r = 0; r < nl; r++) {
+            const zkg_tree *t = f->trees[r];
+            const uint64_t leaf = idx[q] >> (r + 1);
             if (rows_out)
-                memcpy((uint8_t *)rows_out + ((size_t)q * nl + r) * row_bytes, rows.data() + q * row_bytes, row_bytes);
-            if (paths_out && plen)
-                memcpy((uint8_t *)paths_out + q * per_query_path + path_off, paths.data() + q * plen, plen);
+                pieces.push_back(OpenPiece{(const uint8_t *)f->codewords[r] + leaf * row_bytes,
+                                           ((uint64_t)q * nl + r) * row_bytes, (uint32_t)row_bytes});
+            for (int i = 0; i + 1 < t->nlayers && paths_out; i++) {
+                const uint64_t sib = (leaf >> i) ^ 1;
+                pieces.push_back(OpenPiece{(const uint8_t *)t->layers[i] + sib * digest_bytes, path_off,
+                                           (uint32_t)digest_bytes});
+                path_off += digest_bytes;
+            }
         }
-        path_off += plen;
     }
-    return ZKG_OK;
+    return gather_openings(f->ctx, pieces, rows_total, rows_out, paths_total, paths_out);
 }
 
 extern "C" int zkg_fri_free(zkg_fri *f)
