@@ -380,8 +380,31 @@ def bench_deep(ctx, steps):
     alg = h * w * 4 + n * w * 4 + h * 32
     for b in (tr, lde, ro):
         b.free()
-    return dict(lde_rows=h, width=w, ms_per_open=dt * 1e3, algorithmic_bytes=alg,
-                achieved_GBps=alg / dt / 1e9, hbm_frac=alg / dt / HBM_PEAK)
+    out = dict(lde_rows=h, width=w, ms_per_open=dt * 1e3, algorithmic_bytes=alg,
+               achieved_GBps=alg / dt / 1e9, hbm_frac=alg / dt / HBM_PEAK)
+    # the reference STARK's configuration: Goldilocks with base-field challenges,
+    # a 2^20 x 64 trace at blowup 2 (2^21 x 64 LDE rows of u64)
+    tr = ctx.random(Z.GOLDILOCKS, n * w, SEED_C2 + 2)
+    lde = ctx.alloc(h * w * 8)
+    check(ctx.lib.zkg_coset_lde_batch(ctx.h, Z.GOLDILOCKS, tr.h, n, w, lb, 7, lde.h))
+    ro = ctx.upload(np.zeros(h, dtype=np.uint64))
+    zg = int(rng.integers(0, 0xFFFFFFFF00000001, dtype=np.uint64))
+    ag = int(rng.integers(0, 0xFFFFFFFF00000001, dtype=np.uint64))
+    deep = Z.HipDeep(ctx, Z.GOLDILOCKS, lde, h, w, lb, 7)
+    for _ in range(2):
+        deep.open(zg, ag, 0, ro)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        deep.open(zg, ag, 0, ro)
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    alg = h * w * 8 + n * w * 8 + h * 16
+    for b in (tr, lde, ro):
+        b.free()
+    out["goldilocks"] = dict(lde_rows=h, width=w, ms_per_open=dt * 1e3, algorithmic_bytes=alg,
+                             achieved_GBps=alg / dt / 1e9, hbm_frac=alg / dt / HBM_PEAK)
+    return out
 
 
 def air_program(width, n_c, seed):
@@ -733,6 +756,8 @@ def main():
                     help="run only the BabyBear 2^20 x 64 single-commit leg (for profiling) and print its JSON")
     ap.add_argument("--only-served", action="store_true",
                     help="run only the served-stream leg (for profiling) and print its JSON")
+    ap.add_argument("--only-deep", action="store_true",
+                    help="run only the PCS-opening (DEEP quotient) leg and print its JSON")
     ap.add_argument("--only-mmcs", action="store_true",
                     help="run only the mixed-height MMCS leg (commits and openings) and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=48)
@@ -765,6 +790,10 @@ def main():
         return
     if args.only_c4:
         print(json.dumps({"c4_fri": bench_c4(ctx, args.steps), "gl_fri": bench_fri_gl(ctx, args.steps)}))
+        ctx.close()
+        return
+    if args.only_deep:
+        print(json.dumps({"pcs_open_deep": bench_deep(ctx, max(5, args.steps))}))
         ctx.close()
         return
     if args.only_mmcs:
