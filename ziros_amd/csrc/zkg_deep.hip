@@ -133,45 +133,88 @@ __global__ __launch_bounds__(DEEP_THREADS) void k_deep_inv(uint32_t *__restrict_
 // partial[blk][col] = sum over the block's rows i of e_ik * x_i * inv_i
 // (canonical EF4).  Rows [COLSUM_ROWS blk, ...) of the first n rows; grid.y
 // takes 64-column chunks.  Each wave walks every fourth row, lane = column, so
-// a load is 64 consecutive columns of one row; the four waves' sums meet in LDS.
+// a load is 64 consecutive columns of one row; the four waves' sums meet in
+// LDS.  The row weights w_i = x_i inv_i are wave-uniform: the block writes
+// them to LDS, in halves of COLSUM_HALF rows, as the 16-bit halves of their
+// canonical coefficients in doubles, and every element adds e_ik times them
+// with 8 exact FP64 fmas (each product < 2^47; the sums are reduced with
+// Bf::red after every COLSUM_BATCH rows, so they stay below 2^53).
+constexpr int COLSUM_HALF = COLSUM_ROWS / 2;
+constexpr int COLSUM_BATCH = 16;
+
 __global__ __launch_bounds__(DEEP_THREADS) void k_deep_colsum(const uint32_t *__restrict__ lde, uint32_t width,
                                                               uint64_t n, int log_h, const uint32_t *__restrict__ inv,
                                                               DeepConsts k, const uint32_t *__restrict__ lo,
                                                               const uint32_t *__restrict__ hi,
                                                               uint32_t *__restrict__ partial)
 {
-    __shared__ uint4 w[COLSUM_ROWS];  // x_i inv_i, Montgomery
+    __shared__ double w[COLSUM_HALF][8];  // [lo0..lo3, hi0..hi3] of canonical x_i inv_i
     __shared__ uint4 red[DEEP_THREADS];
     const uint64_t base = (uint64_t)blockIdx.x * COLSUM_ROWS;
     const int rows = n - base < (uint64_t)COLSUM_ROWS ? (int)(n - base) : COLSUM_ROWS;
-    for (int r = threadIdx.x; r < rows; r += DEEP_THREADS) {
-        const uint32_t x = Bb::mul(k.g, root_pow(lo, hi, brev((uint32_t)(base + r), log_h), log_h));
-        const Ef4 v = ef4_mul_base(ef4_ld(inv + (base + r) * 4), x);
-        w[r] = make_uint4(v.c[0], v.c[1], v.c[2], v.c[3]);
-    }
-    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t col = blockIdx.y * 64 + lane;
-    Ef4 acc{{0u, 0u, 0u, 0u}};
-    if (col < width) {
-        const uint32_t *e = lde + base * width + col;
-        constexpr int STEP = DEEP_THREADS / 64, BATCH = 16;
-        int r = wave;
-        // full batches: all BATCH loads issue before the first use
-        for (; r + STEP * (BATCH - 1) < rows; r += STEP * BATCH) {
-            uint32_t v[BATCH];
+    constexpr int STEP = DEEP_THREADS / 64;
+    double al[4] = {0, 0, 0, 0}, ah[4] = {0, 0, 0, 0};
+    for (int h0 = 0; h0 < rows; h0 += COLSUM_HALF) {
+        const int hr = rows - h0 < COLSUM_HALF ? rows - h0 : COLSUM_HALF;
+        __syncthreads();  // the previous half's reads are done
+        for (int r = threadIdx.x; r < hr; r += DEEP_THREADS) {
+            const uint64_t i = base + h0 + r;
+            const uint32_t x = Bb::mul(k.g, root_pow(lo, hi, brev((uint32_t)i, log_h), log_h));
+            const Ef4 v = ef4_mul_base(ef4_ld(inv + i * 4), x);
 #pragma unroll
-            for (int u = 0; u < BATCH; u++) v[u] = e[(uint64_t)(r + STEP * u) * width];
-#pragma unroll
-            for (int u = 0; u < BATCH; u++) {
-                const uint4 q = w[r + STEP * u];
-                acc = ef4_add(acc, ef4_mul_base(Ef4{{q.x, q.y, q.z, q.w}}, v[u]));
+            for (int j = 0; j < 4; j++) {
+                const uint32_t c = Bb::to_canon(v.c[j]);
+                w[r][j] = (double)(c & 0xFFFFu);
+                w[r][4 + j] = (double)(c >> 16);
             }
         }
-        for (; r < rows; r += STEP) {
-            const uint4 q = w[r];
-            acc = ef4_add(acc, ef4_mul_base(Ef4{{q.x, q.y, q.z, q.w}}, e[(uint64_t)r * width]));
+        __syncthreads();
+        if (col < width) {
+            const uint32_t *e = lde + (base + h0) * width + col;
+            int r = wave;
+            // full batches: all COLSUM_BATCH loads issue before the first use
+            for (; r + STEP * (COLSUM_BATCH - 1) < hr; r += STEP * COLSUM_BATCH) {
+                uint32_t v[COLSUM_BATCH];
+#pragma unroll
+                for (int u = 0; u < COLSUM_BATCH; u++) v[u] = e[(uint64_t)(r + STEP * u) * width];
+#pragma unroll
+                for (int u = 0; u < COLSUM_BATCH; u++) {
+                    const double ev = (double)v[u];
+                    const double *wr = w[r + STEP * u];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        al[j] = __builtin_fma(ev, wr[j], al[j]);
+                        ah[j] = __builtin_fma(ev, wr[4 + j], ah[j]);
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    al[j] = Bf::red(al[j]);
+                    ah[j] = Bf::red(ah[j]);
+                }
+            }
+            // the tail: fewer than COLSUM_BATCH rows
+            for (; r < hr; r += STEP) {
+                const double ev = (double)e[(uint64_t)r * width];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    al[j] = __builtin_fma(ev, w[r][j], al[j]);
+                    ah[j] = __builtin_fma(ev, w[r][4 + j], ah[j]);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                al[j] = Bf::red(al[j]);
+                ah[j] = Bf::red(ah[j]);
+            }
         }
+    }
+    Ef4 acc{{0u, 0u, 0u, 0u}};
+    if (col < width) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc.c[j] = Bf::canon(__builtin_fma(ah[j], 65536.0, al[j]));
     }
     red[threadIdx.x] = make_uint4(acc.c[0], acc.c[1], acc.c[2], acc.c[3]);
     __syncthreads();
